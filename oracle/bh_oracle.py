@@ -1,0 +1,253 @@
+"""ctypes front end of the CPU oracle (liboracle ``libbh_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path. Only the tests, the
+smoke check and bench.py's CPU-baseline leg import this module; the product
+package never does.
+
+Arrays are numpy: bodies ``(n, 8) float32`` (the reference's ``Body`` std430
+layout), nodes ``(n, 20) uint32`` (the 80-byte ``Node`` std430 layout; use
+:func:`node_fields` to split them).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbh_oracle.so")
+EMPTY = 0xFFFFFFFF
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Params(C.Structure):
+    _fields_ = [("theta", C.c_float), ("dt", C.c_float), ("softening", C.c_float),
+                ("elasticity", C.c_float), ("restitution", C.c_float),
+                ("merging", C.c_uint32), ("dynamic", C.c_uint32), ("wrap_around", C.c_uint32)]
+
+
+class WalkStats(C.Structure):
+    _fields_ = [("opened", C.c_uint64), ("interactions", C.c_uint64), ("mac_tests", C.c_uint64),
+                ("out_of_bounds", C.c_uint32), ("max_stack", C.c_uint32),
+                ("n_tasks", C.c_uint32), ("task_overflow", C.c_uint32)]
+
+
+class Sim(C.Structure):
+    _fields_ = [("numBodies", C.c_uint32), ("initialNumBodies", C.c_uint32), ("justDied", C.c_uint32),
+                ("merged", C.c_uint32), ("outOfBounds", C.c_uint32), ("relativeTo", C.c_uint32),
+                ("pad1", C.c_uint32), ("pad2", C.c_uint32),
+                ("minCorner", C.c_float * 3), ("padA", C.c_float),
+                ("maxCorner", C.c_float * 3), ("padB", C.c_float),
+                ("unitMass", C.c_float), ("unitDensity", C.c_float), ("unitLen", C.c_float),
+                ("unitTime", C.c_float), ("cameraScale", C.c_float),
+                ("gravitationalConstant", C.c_float), ("bodyLengthConstant", C.c_float),
+                ("uintDebug", C.c_uint32 * 100), ("floatDebug", C.c_float * 100), ("tail_pad", C.c_uint32)]
+
+
+class State(C.Structure):
+    _fields_ = [("n_init", C.c_uint32), ("bodies", C.c_void_p * 2), ("index", C.c_void_p * 2),
+                ("morton", C.c_void_p), ("leaf", C.c_void_p), ("internal", C.c_void_p),
+                ("tasks", C.c_void_p), ("task_cap", C.c_uint32), ("sim", Sim), ("cur", C.c_int),
+                ("fill_cap", C.c_int), ("stack_cap", C.c_int), ("threads", C.c_int),
+                ("last", WalkStats), ("last_height", C.c_uint32)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+
+
+def _declare(L: C.CDLL) -> None:
+    L.bho_units.argtypes = [C.c_float] * 4 + [C.POINTER(C.c_float)] * 2
+    L.bho_dead_partition.argtypes = [P, P, P, U32, P]
+    L.bho_dead_partition.restype = U32
+    L.bho_aabb.argtypes = [P, P, U32, P]
+    L.bho_expand_bits21.argtypes = [U32]
+    L.bho_expand_bits21.restype = C.c_uint64
+    L.bho_morton3d.argtypes = [U32, U32, U32]
+    L.bho_morton3d.restype = C.c_uint64
+    L.bho_morton_encode.argtypes = [P, P, U32, P, P]
+    L.bho_sort.argtypes = [P, P, U32]
+    L.bho_safe_lcp.argtypes = [P, U32, C.c_int, C.c_int]
+    L.bho_safe_lcp.restype = C.c_int
+    L.bho_build.argtypes = [P, U32, U32, P, P]
+    L.bho_fill.argtypes = [P, P, U32, U32, C.c_float, C.c_float, C.c_int, C.c_int, P, P]
+    L.bho_fill.restype = U32
+    L.bho_star_color.argtypes = [C.c_float, C.c_float, P]
+    L.bho_force.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int,
+                            P, P, P, U32, C.POINTER(WalkStats), C.c_int]
+    L.bho_force.restype = C.c_int
+    L.bho_merge.argtypes = [P, P, U32]
+    L.bho_merge.restype = U32
+    L.bho_merge_pair.argtypes = [P, P, P]
+    L.bho_state_init.argtypes = [C.POINTER(State), U32, P, P, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.bho_state_init.restype = C.c_int
+    L.bho_state_free.argtypes = [C.POINTER(State)]
+    L.bho_step.argtypes = [C.POINTER(State), C.POINTER(Params)]
+    L.bho_step.restype = C.c_int
+    L.bho_direct_sum_f64.argtypes = [P, U32, P, U32, C.c_double, C.c_double, P, C.c_int]
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def params(theta=0.5, dt=1e-3, softening=0.01, elasticity=0.5, restitution=0.0, merging=0, dynamic=1,
+           wrap_around=0) -> Params:
+    return Params(theta, dt, softening, elasticity, restitution, merging, dynamic, wrap_around)
+
+
+def node_fields(nodes: np.ndarray) -> dict:
+    """Split (n, 20) uint32 std430 nodes into named fields."""
+    f = nodes.view(np.float32)
+    return {"com": f[:, 0:3], "mass": f[:, 3], "aabb": f[:, 4:10], "childA": nodes[:, 10],
+            "childB": nodes[:, 11], "nodeDepth": nodes[:, 12], "bodiesContained": nodes[:, 13],
+            "readyChildren": nodes[:, 14], "parentId": nodes[:, 15], "avgColor": f[:, 16:20]}
+
+
+def units(mass, density, length, time):
+    g = C.c_float()
+    b = C.c_float()
+    lib().bho_units(mass, density, length, time, C.byref(g), C.byref(b))
+    return np.float32(g.value), np.float32(b.value)
+
+
+def dead_partition(src: np.ndarray, dst: np.ndarray | None, index: np.ndarray, num: int):
+    out = index.copy()
+    died = lib().bho_dead_partition(_p(src), _p(dst), _p(index), num, _p(out))
+    return out, int(died)
+
+
+def aabb(bodies: np.ndarray, index: np.ndarray, num: int, prev_bounds) -> np.ndarray:
+    b = np.ascontiguousarray(prev_bounds, dtype=np.float32).copy()
+    lib().bho_aabb(_p(bodies), _p(index), num, _p(b))
+    return b
+
+
+def morton_encode(bodies: np.ndarray, index: np.ndarray, num: int, bounds) -> np.ndarray:
+    codes = np.zeros(num, dtype=np.uint64)
+    b = np.ascontiguousarray(bounds, dtype=np.float32)
+    lib().bho_morton_encode(_p(bodies), _p(index), num, _p(b), _p(codes))
+    return codes
+
+
+def sort(keys: np.ndarray, index: np.ndarray):
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    i = np.ascontiguousarray(index, dtype=np.uint32).copy()
+    lib().bho_sort(_p(k), _p(i), k.shape[0])
+    return k, i
+
+
+def build(keys: np.ndarray, num: int, n_init: int, leaf=None, internal=None):
+    leaf = np.zeros((n_init, 20), np.uint32) if leaf is None else leaf
+    internal = np.zeros((max(n_init - 1, 1), 20), np.uint32) if internal is None else internal
+    lib().bho_build(_p(np.ascontiguousarray(keys, dtype=np.uint64)), num, n_init, _p(leaf), _p(internal))
+    return leaf, internal
+
+
+def fill(bodies, index, num, n_init, leaf, internal, unit_mass=1.0, unit_density=1.0, max_passes=64,
+         with_colors=True) -> int:
+    return int(lib().bho_fill(_p(bodies), _p(index), num, n_init, unit_mass, unit_density, max_passes,
+                              int(with_colors), _p(leaf), _p(internal)))
+
+
+def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, dst=None, stack_cap=64,
+          want_accel=True, task_cap=None, threads=0):
+    accel = np.zeros((n_init, 3), np.float32) if want_accel else None
+    inter = np.zeros(n_init, np.uint32) if want_accel else None
+    task_cap = n_init if task_cap is None else task_cap
+    tasks = np.zeros((max(task_cap, 1), 2), np.uint32)
+    st = WalkStats()
+    b = np.ascontiguousarray(bounds, dtype=np.float32)
+    rc = lib().bho_force(_p(src), _p(dst), _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm),
+                         float(G), float(blc), _p(b), stack_cap, _p(accel), _p(inter), _p(tasks), task_cap,
+                         C.byref(st), threads)
+    return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
+
+
+def merge(dst: np.ndarray, tasks: np.ndarray) -> int:
+    t = np.ascontiguousarray(tasks, dtype=np.uint32).copy()
+    return int(lib().bho_merge(_p(dst), _p(t), t.shape[0]))
+
+
+def direct_sum_f64(bodies: np.ndarray, sample: np.ndarray, G=1.0, softening=0.01, threads=0) -> np.ndarray:
+    s = np.ascontiguousarray(sample, dtype=np.uint32)
+    out = np.zeros((s.shape[0], 3), np.float64)
+    lib().bho_direct_sum_f64(_p(bodies), bodies.shape[0], _p(s), s.shape[0], G, softening, _p(out), threads)
+    return out
+
+
+class Simulation:
+    """Host-buffer restatement of ``BarnesHut.step`` (simulation/BarnesHut.java:99-164)."""
+
+    def __init__(self, bodies: np.ndarray, bounds, unit_mass, unit_density, unit_len=1.0, unit_time=1.0,
+                 G=None, body_len_const=None, fill_cap=64, stack_cap=64, threads=0):
+        self.st = State()
+        b = np.ascontiguousarray(bodies, dtype=np.float32)
+        bd = np.ascontiguousarray(bounds, dtype=np.float32)
+        rc = lib().bho_state_init(C.byref(self.st), b.shape[0], _p(b), _p(bd), unit_mass, unit_density,
+                                  unit_len, unit_time)
+        if rc != 0:
+            raise ValueError("bho_state_init failed")
+        if G is not None:
+            self.st.sim.gravitationalConstant = float(G)
+        if body_len_const is not None:
+            self.st.sim.bodyLengthConstant = float(body_len_const)
+        self.st.fill_cap = fill_cap
+        self.st.stack_cap = stack_cap
+        self.st.threads = threads
+        self.n = b.shape[0]
+
+    def step(self, prm: Params) -> int:
+        return lib().bho_step(C.byref(self.st), C.byref(prm))
+
+    def _arr(self, ptr, dtype, count):
+        buf = (C.c_char * (np.dtype(dtype).itemsize * count)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype, count=count).copy()
+
+    def bodies(self) -> np.ndarray:
+        return self._arr(self.st.bodies[self.st.cur], np.float32, self.n * 8).reshape(self.n, 8)
+
+    def index(self) -> np.ndarray:
+        return self._arr(self.st.index[0], np.uint32, self.n)
+
+    def morton(self) -> np.ndarray:
+        return self._arr(self.st.morton, np.uint64, self.n)
+
+    def nodes(self):
+        return (self._arr(self.st.leaf, np.uint32, self.n * 20).reshape(self.n, 20),
+                self._arr(self.st.internal, np.uint32, max(self.n - 1, 1) * 20).reshape(max(self.n - 1, 1), 20))
+
+    @property
+    def num_bodies(self) -> int:
+        return int(self.st.sim.numBodies)
+
+    @property
+    def bounds(self) -> np.ndarray:
+        return np.array(list(self.st.sim.minCorner) + list(self.st.sim.maxCorner), np.float32)
+
+    def __del__(self):
+        try:
+            lib().bho_state_free(C.byref(self.st))
+        except Exception:
+            pass
