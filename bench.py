@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Body-steps/s of the MI355X Barnes-Hut step on a 16M-body Plummer sphere.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--theta T]
+
+Workload: Plummer sphere, N = 16,777,216 bodies, equal masses 1/N, G = 1,
+scale radius 1, softening 0.01, theta 0.5 (the reference's half-longest-side
+opening criterion), dt 1e-3, merging off, bounds recomputed every step; the
+plain method (no learned correction). Bodies are generated on the host by
+gravity_amd.initial_conditions.plummer and uploaded once; the timed region
+is K calls of bh_step on resident device buffers, bracketed by stream
+synchronisation.
+
+Printed on one JSON line: value = alive bodies x K / wall time of the K
+steps; the roofline entry for the dominant kernel (the walk, VALU-bound:
+20 FLOP per body-node interaction / its mean duration from HIP events on the
+library's stream); p50/p99 force error of the walk against an FP64 direct
+sum for 1,024 sampled bodies; and the CPU baseline: the oracle's C port of
+the same step timed on this host over a bounded sample (full-size O(N)
+stages, the walk for a contiguous slice of sorted bodies, extrapolated to N).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-steps/sec @16M bodies at equal p99 force err; plain vs learned-corrected BH"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_INTERACTION = 20
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def force_error(bh, n: int, softening: float, n_sample: int = 1024, seed: int = 7) -> dict:
+    """p50/p99 of |a - a_fp64| / |a_fp64| over sampled bodies."""
+    from gravity_amd import barnes_hut as B
+    rng = np.random.default_rng(seed)
+    bodies = bh.bodies(B.BODIES_IN)
+    alive = np.nonzero(bodies[:, 3] != 0)[0]
+    sample = np.sort(rng.choice(alive, size=min(n_sample, alive.size), replace=False)).astype(np.uint32)
+    acc = bh.accelerations()[sample].astype(np.float64)
+    ref = bh.direct_sum_f64(sample, softening)
+    err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return {"p50": float(np.percentile(err, 50)), "p99": float(np.percentile(err, 99)),
+            "max": float(err.max()), "sample": int(sample.size), "judge": "fp64 direct sum (HIP kernel)"}
+
+
+def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float, softening: float,
+                 threads: int, walk_sample: int) -> dict:
+    """The oracle's C port of one step on this host: the O(N) stages at full
+    size, the walk for `walk_sample` consecutive sorted bodies, scaled to N."""
+    from oracle import bh_oracle as O
+    from gravity_amd.initial_conditions import g_one_units
+    n = bodies.shape[0]
+    u = g_one_units()
+    G, blc = O.units(u.mass, u.density, u.len, u.time)
+    prm = O.params(theta=theta, dt=dt, softening=softening)
+    src = bodies.copy()
+    dst = np.zeros_like(src)
+    idx = np.arange(n, dtype=np.uint32)
+    t0 = time.perf_counter()
+    out, died = O.dead_partition(src, dst, idx, n)
+    num = n - died
+    b = O.aabb(src, out, num, bounds)
+    codes = O.morton_encode(src, out, num, b)
+    keys, sidx = O.sort(codes, out[:num])
+    out[:num] = sidx
+    leaf, internal = O.build(keys, num, n)
+    O.fill(src, out, num, n, leaf, internal, u.mass, u.density, max_passes=0, with_colors=False)
+    t_stages = time.perf_counter() - t0
+    g0 = max(0, num // 2 - walk_sample // 2)
+    g1 = min(num, g0 + walk_sample)
+    t1 = time.perf_counter()
+    r = O.force_range(src, out, num, n, leaf, internal, prm, G, blc, b, g0, g1, dst=dst, stack_cap=256,
+                      threads=threads)
+    t_walk = time.perf_counter() - t1
+    t_step = t_stages + t_walk * (num / (g1 - g0))
+    return {"value": num / t_step, "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": (f"one step of the C oracle (oracle/bh_oracle.c) at N={n}: dead partition, bounds, Morton, "
+                       f"sort, build and fill at full size ({t_stages:.1f} s), walk + integrate for {g1 - g0} "
+                       f"consecutive sorted bodies from the middle of the order ({t_walk:.1f} s, "
+                       f"{r['stats'].interactions / (g1 - g0):.0f} interactions/body), walk time scaled by "
+                       f"N/sample"),
+            "seconds_measured": t_stages + t_walk}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=16_777_216)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-walk-sample", type=int, default=131072)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a multi-GPU split of one simulation is design-only, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+
+    n, theta, dt, soft = args.n, args.theta, 1e-3, 0.01
+    log(f"generating Plummer N={n}")
+    bodies = ic.plummer(n, args.seed)
+    bounds = ic.tight_bounds(bodies)
+    prm = B.make_params(theta=theta, dt=dt, softening=soft)
+    log("creating context and uploading")
+    bh = B.BarnesHut(bodies, bounds, prm)
+    bh.init()
+    for k in range(args.warmup):
+        bh.step()
+        bh.sync()
+        log(f"warmup step {k + 1}/{args.warmup}")
+    timed = B.Params.from_buffer_copy(prm)
+    timed.flags = B.FLAG_TIMING
+    bh.stage_times()                      # clear the event ring
+    bh.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bh.step(timed)
+    bh.sync()
+    wall = time.perf_counter() - t0
+    stage = bh.stage_times()
+    c = bh.counters()
+    log(f"timed {args.steps} steps in {wall:.3f} s; counters {c}")
+    num = int(c["num_bodies"])
+    value = num * args.steps / wall
+    walk_ms = stage["force"] / args.steps
+    achieved = c["interactions"] * FLOP_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plummer_{n}_plain_theta{theta}", "n_bodies": n, "theta": theta, "dt": dt,
+                   "softening": soft, "merging": "off", "bounds": "dynamic", "method": "plain Barnes-Hut",
+                   "mac": "half longest AABB side x inverse softened distance < theta"},
+        "roofline": {"bound": "valu", "kernel": "k_walk", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms},
+        "stage_ms": {k: v / args.steps for k, v in stage.items()},
+        "counters": {"opened": int(c["opened"]), "interactions_per_body": c["interactions"] / max(num, 1),
+                     "tree_height": int(c["tree_height"]), "max_stack": int(c["max_stack"]),
+                     "error_flags": int(c["error_flags"])},
+    }
+    if not args.no_error:
+        log("force error against the FP64 direct sum")
+        result["force_error"] = force_error(bh, n, soft)
+    bh.close()
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle port)")
+        result["cpu_baseline"] = cpu_baseline(bodies, bounds, theta, dt, soft, args.cpu_threads,
+                                              args.cpu_walk_sample)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

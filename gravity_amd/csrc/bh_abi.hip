@@ -1,0 +1,528 @@
+// bh_abi.hip — host side of the C ABI declared in include/bh_abi.h.
+//
+// A context owns the device memory behind the reference's 16 bindings
+// (gpu/GPU.java:199-341) plus the tree and scratch arrays of this
+// implementation, and one HIP stream. Swaps exchange pointers, as the
+// reference exchanges GL buffer names (gpu/GPU.java:1060-1086).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/bh_abi.h"
+#include "bh_device.hpp"
+#include "bh_kernels.hpp"
+
+using namespace bh;
+
+struct bh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+
+    // reference bindings
+    float4* bodies[2] = {nullptr, nullptr};   // 3 BodiesIn = bodies[body_in], 4 BodiesOut
+    uint64_t* morton[2] = {nullptr, nullptr}; // 5 MortonIn = morton[mi_in], 6 MortonOut
+    uint32_t* index[2] = {nullptr, nullptr};  // 7 IndexIn = index[mi_in], 8 IndexOut
+    SimValues* sim = nullptr;                 // 2
+    uint2* tasks = nullptr;                   // 14 (pairs)
+    int body_in = 0, mi_in = 0;
+
+    TreeArrays tree{};
+    DevCounters* ctr = nullptr;
+    uint32_t* part_scratch = nullptr;         // per-tile alive counts / offsets
+    float* bounds_partials = nullptr;
+    uint32_t bounds_partial_cap = 0;
+    uint32_t* tile_hist = nullptr;
+    uint32_t tiles_cap = 0;
+    uint32_t* digit_total = nullptr;
+    unsigned long long* reserve = nullptr;
+    uint32_t* task_done = nullptr;
+    NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1
+    bool colors = false;
+
+    // per-stage HIP events of up to kTimedSteps steps, read without
+    // synchronising inside the stepping loop
+    static constexpr int kTimedSteps = 512;
+    hipEvent_t* ev = nullptr;            // [kTimedSteps][BH_STAGE_COUNT + 1]
+    int timed = 0;                       // steps recorded since the last read
+    bool timing = false;
+    char err[512] = {0};
+};
+
+namespace {
+
+int fail(bh_context* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHECK(c, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail((c), BH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+int dalloc(bh_context* c, T** p, size_t count) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) return fail(c, BH_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return BH_OK;
+}
+
+int check_launch(bh_context* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, BH_ERR_HIP, "%s launch: %s", what, hipGetErrorString(e));
+    return BH_OK;
+}
+
+void derive_units(const bh_params* p, float* G, float* blc) {
+    // common/common.glsl:346-352, evaluated in FP32 as the shader does
+    const float GC = 6.67430e-11f;
+    *G = GC * powf(p->unit_len, -3.0f) * powf(p->unit_time, 2.0f) * powf(p->unit_mass, 1.0f);
+    *blc = 0.6203504909f * powf(p->unit_mass / p->unit_density, 1.0f / 3.0f) / p->unit_len;
+    if (p->gravitational_constant != 0.0f) *G = p->gravitational_constant;
+    if (p->body_length_constant != 0.0f) *blc = p->body_length_constant;
+}
+
+void stage_begin(bh_context* c, int k) {
+    if (c->timing && c->timed < bh_context::kTimedSteps)
+        (void)hipEventRecord(c->ev[c->timed * (BH_STAGE_COUNT + 1) + k], c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bh_last_error(const bh_context* c) { return c ? c->err : "null context"; }
+
+int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* units, bh_context** out) {
+    if (!out) return BH_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (n < 2 || !bounds6) return BH_ERR_INVALID_ARGUMENT;
+    // 32-bit positions and node ids: n + (n - 1) must fit below 0xFFFFFFFF
+    if (n > (1u << 30)) return BH_ERR_TOO_LARGE;
+    bh_context* c = new (std::nothrow) bh_context();
+    if (!c) return BH_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    c->n = n;
+    int rc;
+#define TRY(x)                     \
+    do {                           \
+        rc = (x);                  \
+        if (rc != BH_OK) goto bad; \
+    } while (0)
+    {
+        hipError_t e = hipSetDevice(device);
+        if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e)); goto bad; }
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); goto bad; }
+        for (int k = 0; k < 2; k++) {
+            TRY(dalloc(c, &c->bodies[k], 2 * (size_t)n));
+            TRY(dalloc(c, &c->morton[k], (size_t)n));
+            TRY(dalloc(c, &c->index[k], (size_t)n));
+        }
+        TRY(dalloc(c, &c->sim, 1));
+        TRY(dalloc(c, &c->tasks, (size_t)n));
+        TRY(dalloc(c, &c->ctr, 1));
+        TreeArrays& T = c->tree;
+        TRY(dalloc(c, &T.leaf_pos, (size_t)n));
+        TRY(dalloc(c, &T.leaf_parent, (size_t)n));
+        TRY(dalloc(c, &T.leaf_color, (size_t)n));
+        TRY(dalloc(c, &T.nodes, (size_t)n - 1));
+        TRY(dalloc(c, &T.node_aabb, 2 * ((size_t)n - 1)));
+        TRY(dalloc(c, &T.node_parent, (size_t)n - 1));
+        TRY(dalloc(c, &T.node_range, (size_t)n - 1));
+        TRY(dalloc(c, &T.node_color, (size_t)n - 1));
+        TRY(dalloc(c, &T.meet, (size_t)n - 1));
+        TRY(dalloc(c, &T.split_child, 2 * ((size_t)n - 1)));
+        TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
+        c->bounds_partial_cap = 4096;
+        TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
+        c->tiles_cap = (n + kSortTile - 1) / kSortTile;
+        TRY(dalloc(c, &c->tile_hist, 256 * (size_t)c->tiles_cap));
+        TRY(dalloc(c, &c->digit_total, 256));
+        TRY(dalloc(c, &c->reserve, (size_t)n));
+        TRY(dalloc(c, &c->task_done, (size_t)n));
+        TRY(dalloc(c, &c->staging_nodes, (size_t)n));
+        c->ev = new (std::nothrow) hipEvent_t[bh_context::kTimedSteps * (BH_STAGE_COUNT + 1)]();
+        if (!c->ev) { rc = fail(c, BH_ERR_OUT_OF_MEMORY, "events"); goto bad; }
+        for (int k = 0; k < bh_context::kTimedSteps * (BH_STAGE_COUNT + 1); k++) {
+            e = hipEventCreate(&c->ev[k]);
+            if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e)); goto bad; }
+        }
+        // zero everything the reference zero-initialises, then pack sim
+        // (gpu/GPU.java:1096-1143)
+        for (int k = 0; k < 2; k++) {
+            (void)hipMemsetAsync(c->bodies[k], 0, 32 * (size_t)n, c->stream);
+            (void)hipMemsetAsync(c->morton[k], 0, 8 * (size_t)n, c->stream);
+            (void)hipMemsetAsync(c->index[k], 0, 4 * (size_t)n, c->stream);
+        }
+        (void)hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), c->stream);
+        (void)hipMemsetAsync(T.meet, 0xFF, 4 * ((size_t)n - 1), c->stream);
+        (void)hipMemsetAsync(c->reserve, 0xFF, 8 * (size_t)n, c->stream);
+        (void)hipMemsetAsync(T.leaf_color, 0, 16 * (size_t)n, c->stream);
+        (void)hipMemsetAsync(T.node_color, 0, 16 * ((size_t)n - 1), c->stream);
+        SimValues h;
+        memset(&h, 0, sizeof h);
+        h.numBodies = n;
+        h.initialNumBodies = n;
+        memcpy(h.minCorner, bounds6, 12);
+        memcpy(h.maxCorner, bounds6 + 3, 12);
+        if (units) {
+            h.unitMass = units->unit_mass;
+            h.unitDensity = units->unit_density;
+            h.unitLen = units->unit_len;
+            h.unitTime = units->unit_time;
+            if (units->unit_mass != 0.0f) derive_units(units, &h.gravitationalConstant, &h.bodyLengthConstant);
+        }
+        e = hipMemcpyAsync(c->sim, &h, sizeof h, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "upload sim: %s", hipGetErrorString(e)); goto bad; }
+        e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "create sync: %s", hipGetErrorString(e)); goto bad; }
+    }
+#undef TRY
+    *out = c;
+    return BH_OK;
+bad:
+    fprintf(stderr, "bh_create: %s\n", c->err);
+    bh_destroy(c);
+    return rc;
+}
+
+void bh_destroy(bh_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
+                    c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.nodes,
+                    c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
+                    c->tree.split_child, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
+                    c->reserve, c->task_done, c->staging_nodes};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ev) {
+        for (int k = 0; k < bh_context::kTimedSteps * (BH_STAGE_COUNT + 1); k++)
+            if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+        delete[] c->ev;
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+size_t bh_binding_size(const bh_context* c, int b) {
+    if (!c) return 0;
+    const size_t n = c->n;
+    switch (b) {
+        case BH_BINDING_LEAF_NODES: return n * 80;
+        case BH_BINDING_INTERNAL_NODES: return (n - 1) * 80;
+        case BH_BINDING_SIMULATION_VALUES: return sizeof(SimValues);
+        case BH_BINDING_BODIES_IN:
+        case BH_BINDING_BODIES_OUT: return n * 32;
+        case BH_BINDING_MORTON_IN:
+        case BH_BINDING_MORTON_OUT: return n * 8;
+        case BH_BINDING_INDEX_IN:
+        case BH_BINDING_INDEX_OUT: return n * 4;
+        case BH_BINDING_MERGE_TASKS: return 8 + n * 8;
+        default: return 0;
+    }
+}
+
+static void* binding_ptr(bh_context* c, int b) {
+    switch (b) {
+        case BH_BINDING_SIMULATION_VALUES: return c->sim;
+        case BH_BINDING_BODIES_IN: return c->bodies[c->body_in];
+        case BH_BINDING_BODIES_OUT: return c->bodies[1 - c->body_in];
+        case BH_BINDING_MORTON_IN: return c->morton[c->mi_in];
+        case BH_BINDING_MORTON_OUT: return c->morton[1 - c->mi_in];
+        case BH_BINDING_INDEX_IN: return c->index[c->mi_in];
+        case BH_BINDING_INDEX_OUT: return c->index[1 - c->mi_in];
+        default: return nullptr;
+    }
+}
+
+static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
+    size_t sz = bh_binding_size(c, b);
+    if (sz == 0) return fail(c, BH_ERR_UNSUPPORTED, "binding %d is reference scratch with no counterpart here", b);
+    if (off > sz || bytes > sz - off) return fail(c, BH_ERR_INVALID_ARGUMENT, "binding %d: range %zu+%zu > %zu", b, off, bytes, sz);
+    return BH_OK;
+}
+
+int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) {
+    if (!c || (!host && bytes)) return BH_ERR_INVALID_ARGUMENT;
+    int rc = check_range(c, b, off, bytes);
+    if (rc) return rc;
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
+        if (off != 0 || bytes != bh_binding_size(c, b))
+            return fail(c, BH_ERR_INVALID_ARGUMENT, "node bindings are uploaded whole");
+        HIPCHECK(c, hipMemcpyAsync(c->staging_nodes, host, bytes, hipMemcpyHostToDevice, c->stream));
+        int which = b == BH_BINDING_LEAF_NODES ? 0 : 1;
+        launch_std430_to_nodes(c->stream, c->n, c->sim, c->tree, c->staging_nodes, c->staging_nodes, which);
+        return check_launch(c, "std430_to_nodes");
+    }
+    if (b == BH_BINDING_MERGE_TASKS) {
+        // layout of bh_merge.comp / common.glsl:117: head, tail, uvec2 tasks[]
+        const uint8_t* h = (const uint8_t*)host;
+        if (off != 0 || bytes < 8) return fail(c, BH_ERR_INVALID_ARGUMENT, "merge tasks upload starts at 0 with head/tail");
+        uint32_t tail;
+        memcpy(&tail, h + 4, 4);
+        size_t pairs = (bytes - 8) / 8;
+        if (tail > pairs) tail = (uint32_t)pairs;
+        HIPCHECK(c, hipMemcpyAsync(c->tasks, h + 8, (size_t)tail * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(&c->ctr->nTasks, &tail, 4, hipMemcpyHostToDevice, c->stream));
+        return hipStreamSynchronize(c->stream) == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "sync");
+    }
+    HIPCHECK(c, hipMemcpyAsync((uint8_t*)binding_ptr(c, b) + off, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return BH_OK;
+}
+
+int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {
+    if (!c || (!host && bytes)) return BH_ERR_INVALID_ARGUMENT;
+    int rc = check_range(c, b, off, bytes);
+    if (rc) return rc;
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
+        launch_nodes_to_std430(c->stream, c->n, c->sim, c->tree, c->colors, b == BH_BINDING_LEAF_NODES ? 0 : 1,
+                               c->staging_nodes);
+        rc = check_launch(c, "nodes_to_std430");
+        if (rc) return rc;
+        HIPCHECK(c, hipMemcpyAsync(host, (uint8_t*)c->staging_nodes + off, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        return BH_OK;
+    }
+    if (b == BH_BINDING_MERGE_TASKS) {
+        uint32_t nt = 0;
+        HIPCHECK(c, hipMemcpyAsync(&nt, &c->ctr->nTasks, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (nt > c->n) nt = c->n;
+        uint8_t* tmp = new (std::nothrow) uint8_t[8 + (size_t)nt * 8];
+        if (!tmp) return BH_ERR_OUT_OF_MEMORY;
+        uint32_t ht[2] = {0, nt};
+        memcpy(tmp, ht, 8);
+        hipError_t e = hipMemcpy(tmp + 8, c->tasks, (size_t)nt * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            size_t avail = 8 + (size_t)nt * 8;
+            memset(host, 0, bytes);
+            if (off < avail) memcpy(host, tmp + off, bytes < avail - off ? bytes : avail - off);
+        }
+        delete[] tmp;
+        return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "download tasks: %s", hipGetErrorString(e));
+    }
+    HIPCHECK(c, hipMemcpyAsync(host, (uint8_t*)binding_ptr(c, b) + off, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return BH_OK;
+}
+
+int bh_swap_body_buffers(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->body_in = 1 - c->body_in;
+    return BH_OK;
+}
+
+int bh_swap_morton_and_index_buffers(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->mi_in = 1 - c->mi_in;
+    return BH_OK;
+}
+
+int bh_init(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    launch_init(c->stream, c->index[c->mi_in], c->n);
+    return check_launch(c, "init");
+}
+
+int bh_reset_values(bh_context* c, const bh_params* p) {
+    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+    float G = 0.0f, blc = 0.0f;
+    int set = p->unit_mass != 0.0f;
+    if (set) derive_units(p, &G, &blc);
+    launch_reset(c->stream, c->sim, c->ctr, set, p->unit_mass, p->unit_density, p->unit_len, p->unit_time, G, blc);
+    return check_launch(c, "reset");
+}
+
+int bh_partition_dead_bodies(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    launch_dead_partition(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->bodies[1 - c->body_in],
+                          c->index[c->mi_in], c->index[1 - c->mi_in], c->part_scratch);
+    int rc = check_launch(c, "dead partition");
+    if (rc) return rc;
+    return bh_swap_morton_and_index_buffers(c);   // BarnesHut.java:117
+}
+
+int bh_update_bounds(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    launch_bounds(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->bounds_partials,
+                  c->bounds_partial_cap);
+    return check_launch(c, "bounds");
+}
+
+int bh_generate_morton_codes(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    launch_encode(c->stream, c->n, c->sim, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in]);
+    return check_launch(c, "encode");
+}
+
+int bh_radix_sort(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    for (uint32_t pass = 0; pass < kSortPasses; pass++) {
+        launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], c->index[c->mi_in], c->morton[1 - c->mi_in],
+                         c->index[1 - c->mi_in], pass * kSortBits, c->tile_hist, c->tiles_cap, c->digit_total);
+        int rc = check_launch(c, "sort pass");
+        if (rc) return rc;
+        c->mi_in = 1 - c->mi_in;
+    }
+    return BH_OK;
+}
+
+int bh_build_tree(bh_context* c, const bh_params* p) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->colors = p && (p->flags & BH_FLAG_COLORS);
+    launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
+                c->tree, c->colors ? 1 : 0);
+    return check_launch(c, "tree");
+}
+
+int bh_compute_force(bh_context* c, const bh_params* p) {
+    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+    if (p->merging_collision_or_neither == BH_COLLISION)
+        return fail(c, BH_ERR_UNSUPPORTED, "COLLISION mode (mergingCollisionOrNeither=2) is not implemented");
+    if (p->merging_collision_or_neither > BH_COLLISION || p->static_or_dynamic > 1)
+        return fail(c, BH_ERR_INVALID_ARGUMENT, "bad selector");
+    WalkArgs a;
+    a.sim = c->sim;
+    a.ctr = c->ctr;
+    a.src = c->bodies[c->body_in];
+    a.dst = c->bodies[1 - c->body_in];
+    a.index_in = c->index[c->mi_in];
+    a.leaf_pos = c->tree.leaf_pos;
+    a.nodes = c->tree.nodes;
+    a.tasks = c->tasks;
+    a.task_cap = c->n;
+    a.theta = p->theta;
+    a.dt = p->dt;
+    a.softening = p->softening;
+    a.merging = p->merging_collision_or_neither;
+    a.dynamic = p->static_or_dynamic;
+    a.wrap = p->wrap_around;
+    launch_walk(c->stream, c->n, a);
+    return check_launch(c, "walk");
+}
+
+int bh_merge_bodies(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    launch_merge(c->stream, c->bodies[1 - c->body_in], c->tasks, c->n, c->ctr, c->reserve, c->task_done);
+    return check_launch(c, "merge");
+}
+
+int bh_step(bh_context* c, const bh_params* p) {
+    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+    if (p->merging_collision_or_neither == BH_COLLISION)
+        return fail(c, BH_ERR_UNSUPPORTED, "COLLISION mode (mergingCollisionOrNeither=2) is not implemented");
+    HIPCHECK(c, hipSetDevice(c->device));
+    c->timing = (p->flags & BH_FLAG_TIMING) != 0;
+    int rc;
+    stage_begin(c, 0);
+    if ((rc = bh_reset_values(c, p))) return rc;
+    stage_begin(c, 1);
+    if ((rc = bh_partition_dead_bodies(c))) return rc;
+    stage_begin(c, 2);
+    if (p->static_or_dynamic == BH_DYNAMIC && (rc = bh_update_bounds(c))) return rc;
+    stage_begin(c, 3);
+    if ((rc = bh_generate_morton_codes(c))) return rc;
+    stage_begin(c, 4);
+    if ((rc = bh_radix_sort(c))) return rc;
+    stage_begin(c, 5);
+    if ((rc = bh_build_tree(c, p))) return rc;
+    stage_begin(c, 6);
+    if ((rc = bh_compute_force(c, p))) return rc;
+    stage_begin(c, 7);
+    if (p->merging_collision_or_neither == BH_MERGING && (rc = bh_merge_bodies(c))) return rc;
+    stage_begin(c, 8);
+    if (c->timing && c->timed < bh_context::kTimedSteps) c->timed++;
+    return bh_swap_body_buffers(c);
+}
+
+int bh_sync(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return BH_OK;
+}
+
+int bh_get_counters(bh_context* c, bh_counters* out) {
+    if (!c || !out) return BH_ERR_INVALID_ARGUMENT;
+    SimValues h;
+    DevCounters d;
+    HIPCHECK(c, hipMemcpyAsync(&h, c->sim, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(&d, c->ctr, sizeof d, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof *out);
+    out->num_bodies = h.numBodies;
+    out->initial_num_bodies = h.initialNumBodies;
+    out->just_died = d.died;
+    out->out_of_bounds = h.outOfBounds;
+    out->opened = d.opened;
+    out->interactions = d.interactions;
+    out->merge_tasks = d.nTasks;
+    out->merges_applied = d.mergesApplied;
+    out->tree_height = d.treeHeight;
+    out->max_stack = d.maxStack;
+    out->error_flags = d.errorFlags | (d.nTasks > c->n ? 2u : 0u);
+    return BH_OK;
+}
+
+int bh_get_stage_times(bh_context* c, float* ms, int n) {
+    if (!c || !ms || n < 0) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; k++) ms[k] = 0.0f;
+    for (int st = 0; st < c->timed; st++) {
+        for (int k = 0; k < n && k < BH_STAGE_COUNT; k++) {
+            float t = 0.0f;
+            hipEvent_t* e = c->ev + st * (BH_STAGE_COUNT + 1);
+            if (hipEventElapsedTime(&t, e[k], e[k + 1]) == hipSuccess) ms[k] += t;
+        }
+    }
+    if (n > BH_STAGE_COUNT) ms[BH_STAGE_COUNT] = (float)c->timed;
+    c->timed = 0;
+    return BH_OK;
+}
+
+int bh_direct_sum_f64(bh_context* c, const uint32_t* sample, uint32_t n_sample, double softening, double* out) {
+    if (!c || (!sample && n_sample) || (!out && n_sample)) return BH_ERR_INVALID_ARGUMENT;
+    if (n_sample == 0) return BH_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    for (uint32_t k = 0; k < n_sample; k++)
+        if (sample[k] >= c->n) return fail(c, BH_ERR_INVALID_ARGUMENT, "sample id %u >= n", sample[k]);
+    uint32_t* d_sample = nullptr;
+    double* d_out = nullptr;
+    SimValues h;
+    HIPCHECK(c, hipMemcpyAsync(&h, c->sim, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMalloc((void**)&d_sample, 4 * (size_t)n_sample));
+    hipError_t e = hipMalloc((void**)&d_out, 24 * (size_t)n_sample);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sample, sample, 4 * (size_t)n_sample, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_direct_sum_f64(c->stream, c->bodies[c->body_in], c->n, d_sample, n_sample,
+                              (double)h.gravitationalConstant, softening, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, 24 * (size_t)n_sample, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_sample);
+    if (d_out) (void)hipFree(d_out);
+    return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "direct sum: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -1,0 +1,800 @@
+// bh_kernels.hip — the Barnes-Hut step as hand-written HIP kernels for gfx950.
+//
+// Stages (reference program in brackets; shaders/compute/...):
+//   k_init                 [COMPUTE_INIT, init/bh_init.comp:10-28]
+//   k_reset                [COMPUTE_UPDATE flag=1, update/bh_update.comp:10-37]
+//   k_dead_count/scan/scatter
+//                          [COMPUTE_DEAD_*, dead/bh_dead.comp:13-112 + UPDATE flag=0]
+//   k_bounds               [COMPUTE_MORTON_AABB_REPOPULATE + COLLAPSE, morton/bh_morton.comp:50-231]
+//   k_encode               [COMPUTE_MORTON_ENCODE, morton/bh_morton.comp:6-34,238-262]
+//   k_sort_hist/scan/scatter
+//                          [COMPUTE_RADIX_*, radix/bh_radix.comp:12-166]
+//   k_tree                 [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
+//   (walk: bh_walk.hip)    [COMPUTE_FORCE_COMPUTE]
+//   k_merge                [COMPUTE_MERGE_BODIES, merge/bh_merge.comp:5-71]
+//   k_node_to_std430 / k_std430_to_node: the 80-byte Node layout at the boundary.
+#include "bh_device.hpp"
+#include "bh_kernels.hpp"
+
+namespace bh {
+
+// ------------------------------------------------------------------ init/reset
+__global__ void k_init(uint32_t* __restrict__ index_in, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) index_in[i] = i;
+}
+
+__global__ void k_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr, int set_units, float mass,
+                        float density, float len, float time, float G, float blc) {
+    if (threadIdx.x != 0) return;
+    ctr->opened = 0;
+    ctr->interactions = 0;
+    ctr->nTasks = 0;
+    ctr->mergesApplied = 0;
+    ctr->maxStack = 0;
+    ctr->errorFlags = 0;
+    if (set_units) {
+        sim->unitMass = mass;
+        sim->unitDensity = density;
+        sim->unitLen = len;
+        sim->unitTime = time;
+        sim->gravitationalConstant = G;
+        sim->bodyLengthConstant = blc;
+    }
+}
+
+// ------------------------------------------------------------------ dead partition
+// Tile of 1024 sorted positions per block of 256 threads, four rows of 256.
+__global__ __launch_bounds__(256) void k_dead_count(const SimValues* __restrict__ sim, float4* __restrict__ src,
+                                                    float4* __restrict__ dst, const uint32_t* __restrict__ index_in,
+                                                    uint32_t* __restrict__ block_alive) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t num = sim->numBodies;
+    const uint32_t base = blockIdx.x * kPartTile;
+    if (base >= num) return;
+    uint32_t alive = 0;
+    for (int r = 0; r < kPartTile / 256; r++) {
+        uint32_t e = base + r * 256 + threadIdx.x;
+        if (e < num) {
+            uint32_t b = index_in[e];
+            float m = src[2 * b].w;
+            if (m == 0.0f) {
+                // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
+                src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+                src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+                dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                alive++;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) alive += __shfl_xor(alive, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = alive;
+    __syncthreads();
+    if (threadIdx.x == 0) block_alive[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// One block of 1024 threads: exclusive scan of the per-tile alive counts;
+// then numBodies -= justDied, justDied = 0, uintDebug[0] = 0 (bh_update.comp:31-34).
+__global__ __launch_bounds__(1024) void k_dead_scan(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                    uint32_t* __restrict__ block_alive) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_total;
+    const uint32_t num = sim->numBodies;
+    const uint32_t nb = (num + kPartTile - 1) / kPartTile;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t t = threadIdx.x;
+    uint32_t start = t * per, end = min(start + per, nb);
+    uint32_t sum = 0;
+    for (uint32_t k = start; k < end; k++) sum += block_alive[k];
+    // block exclusive scan of sum
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off, 64);
+        if (lane_id() >= (uint32_t)off) incl += v;
+    }
+    if (lane_id() == 63) s_w[t >> 6] = incl;
+    __syncthreads();
+    if (t < 16) {
+        uint32_t v = s_w[t];
+        for (int off = 1; off < 16; off <<= 1) {
+            uint32_t u = __shfl_up(v, off, 16);
+            if (t >= (uint32_t)off) v += u;
+        }
+        s_w[t] = v;  // inclusive over waves
+        if (t == 15) s_total = v;
+    }
+    __syncthreads();
+    uint32_t run = incl - sum + ((t >> 6) ? s_w[(t >> 6) - 1] : 0);
+    for (uint32_t k = start; k < end; k++) {
+        uint32_t v = block_alive[k];
+        block_alive[k] = run;
+        run += v;
+    }
+    if (t == 0) {
+        uint32_t alive = s_total;
+        ctr->numBefore = num;
+        ctr->died = num - alive;
+        sim->numBodies = alive;
+        sim->justDied = 0;
+        sim->uintDebug[0] = 0;
+    }
+}
+
+// Stable scatter: alive first in order, dead after them in order (bh_dead.comp:141-181).
+__global__ __launch_bounds__(256) void k_dead_scatter(const SimValues* __restrict__ sim,
+                                                      const DevCounters* __restrict__ ctr,
+                                                      const float4* __restrict__ src,
+                                                      const uint32_t* __restrict__ index_in,
+                                                      uint32_t* __restrict__ index_out,
+                                                      const uint32_t* __restrict__ block_alive_off) {
+    __shared__ uint32_t s_wa[4], s_wd[4];
+    const uint32_t num = ctr->numBefore;
+    const uint32_t alive_total = sim->numBodies;
+    const uint32_t base = blockIdx.x * kPartTile;
+    if (base >= num) return;
+    const uint32_t aoff = block_alive_off[blockIdx.x];
+    const uint32_t doff = base - aoff;  // dead bodies in earlier tiles (all earlier tiles are full)
+    const uint32_t w = threadIdx.x >> 6;
+    uint32_t run_a = 0, run_d = 0;
+    for (int r = 0; r < kPartTile / 256; r++) {
+        uint32_t e = base + r * 256 + threadIdx.x;
+        bool in = e < num;
+        uint32_t b = in ? index_in[e] : 0;
+        bool alive = in && src[2 * b].w != 0.0f;
+        bool dead = in && !alive;
+        uint64_t ma = ballot(alive), md = ballot(dead);
+        uint32_t ra = __popcll(ma & lanemask_lt()), rd = __popcll(md & lanemask_lt());
+        if (lane_id() == 0) {
+            s_wa[w] = __popcll(ma);
+            s_wd[w] = __popcll(md);
+        }
+        __syncthreads();
+        uint32_t pa = 0, pd = 0, ta = 0, td = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            if (k < w) { pa += s_wa[k]; pd += s_wd[k]; }
+            ta += s_wa[k];
+            td += s_wd[k];
+        }
+        if (alive) index_out[aoff + run_a + pa + ra] = b;
+        if (dead) index_out[alive_total + doff + run_d + pd + rd] = b;
+        run_a += ta;
+        run_d += td;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ AABB
+__device__ __forceinline__ void window_axis(float pmin, float pmax, float& lo, float& hi) {
+    // mortonInlierWindow, bh_morton.comp:50-56
+    float center = fmul(0.5f, fadd(pmin, pmax));
+    float half = gmax(fmul(fmul(0.5f, fsub(pmax, pmin)), 1.05f), 1e-9f);
+    lo = fsub(center, half);
+    hi = fadd(center, half);
+}
+
+__global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                const float4* __restrict__ src,
+                                                const uint32_t* __restrict__ index_in,
+                                                float* __restrict__ partials) {
+    __shared__ float s_r[6][4];
+    __shared__ bool s_last;
+    const uint32_t num = sim->numBodies;
+    const float p0 = sim->minCorner[0], p1 = sim->minCorner[1], p2 = sim->minCorner[2];
+    const float q0 = sim->maxCorner[0], q1 = sim->maxCorner[1], q2 = sim->maxCorner[2];
+    const bool prev_empty = p0 > q0;
+    float lo0, hi0, lo1, hi1, lo2, hi2;
+    window_axis(p0, q0, lo0, hi0);
+    window_axis(p1, q1, lo1, hi1);
+    window_axis(p2, q2, lo2, hi2);
+    float r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < num; g += gridDim.x * blockDim.x) {
+        float4 pm = src[2 * index_in[g]];
+        if (pm.w == 0.0f) continue;                      // isEmpty
+        if (!finite3(pm.x, pm.y, pm.z)) continue;        // mortonContributorAABB :61-63
+        if (!prev_empty && (pm.x < lo0 || pm.y < lo1 || pm.z < lo2 || pm.x > hi0 || pm.y > hi1 || pm.z > hi2))
+            continue;
+        r[0] = gmin(r[0], pm.x); r[1] = gmin(r[1], pm.y); r[2] = gmin(r[2], pm.z);
+        r[3] = gmax(r[3], pm.x); r[4] = gmax(r[4], pm.y); r[5] = gmax(r[5], pm.z);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int k = 0; k < 3; k++) r[k] = gmin(r[k], __shfl_xor(r[k], off, 64));
+        for (int k = 3; k < 6; k++) r[k] = gmax(r[k], __shfl_xor(r[k], off, 64));
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; k++) s_r[k][w] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b[6];
+        for (int k = 0; k < 3; k++) b[k] = gmin(gmin(s_r[k][0], s_r[k][1]), gmin(s_r[k][2], s_r[k][3]));
+        for (int k = 3; k < 6; k++) b[k] = gmax(gmax(s_r[k][0], s_r[k][1]), gmax(s_r[k][2], s_r[k][3]));
+        for (int k = 0; k < 6; k++) partials[6 * blockIdx.x + k] = b[k];
+        __threadfence();
+        uint32_t ticket = atomicAdd(&ctr->aabbTicket, 1u);
+        s_last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    for (uint32_t k = 0; k < gridDim.x; k++) {
+        for (int c = 0; c < 3; c++) res[c] = gmin(res[c], __builtin_nontemporal_load(&partials[6 * k + c]));
+        for (int c = 3; c < 6; c++) res[c] = gmax(res[c], __builtin_nontemporal_load(&partials[6 * k + c]));
+    }
+    ctr->aabbTicket = 0;
+    float prev[6] = {p0, p1, p2, q0, q1, q2};
+    if (res[0] > res[3]) {                               // no finite inliers (:221-226)
+        if (prev_empty)
+            for (int c = 0; c < 3; c++) { sim->minCorner[c] = res[c]; sim->maxCorner[c] = res[3 + c]; }
+        return;
+    }
+    if (prev_empty) {
+        for (int c = 0; c < 3; c++) { sim->minCorner[c] = res[c]; sim->maxCorner[c] = res[3 + c]; }
+        return;
+    }
+    for (int c = 0; c < 3; c++) {                        // clampAABBExtentGrowth :77-88
+        float prevExt = gmax(fsub(prev[3 + c], prev[c]), 1e-9f);
+        float maxExt = fmul(prevExt, 1.1f);
+        float ctr_c = fmul(0.5f, fadd(res[c], res[3 + c]));
+        float h = fmul(0.5f, fsub(res[3 + c], res[c]));
+        float mh = fmul(0.5f, maxExt);
+        h = gmin(h, mh);
+        sim->minCorner[c] = fsub(ctr_c, h);
+        sim->maxCorner[c] = fadd(ctr_c, h);
+    }
+}
+
+// ------------------------------------------------------------------ Morton
+__device__ __forceinline__ uint64_t morton_of(float4 pm, const float* bmin, const float* bmax, bool empty) {
+    const float MAXV = 2097151.0f;
+    float p[3] = {pm.x, pm.y, pm.z};
+    if (!finite3(pm.x, pm.y, pm.z) || empty) {
+        for (int k = 0; k < 3; k++) p[k] = fmul(0.5f, fadd(bmin[k], bmax[k]));
+    } else {
+        for (int k = 0; k < 3; k++) p[k] = gmin(gmax(p[k], bmin[k]), bmax[k]);
+    }
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        float ext = gmax(fsub(bmax[k], bmin[k]), 1e-9f);
+        float pn = fdiv(fsub(p[k], bmin[k]), ext);
+        float f = gmin(gmax(floorf(fmul(pn, MAXV)), 0.0f), MAXV);
+        q[k] = (uint32_t)floorf(f);
+    }
+    return (expand_bits21(q[0]) << 2) | (expand_bits21(q[1]) << 1) | expand_bits21(q[2]);
+}
+
+__global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ sim, const float4* __restrict__ src,
+                                                const uint32_t* __restrict__ index_in,
+                                                uint64_t* __restrict__ morton_in) {
+    const uint32_t num = sim->numBodies;
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num) return;
+    float bmin[3] = {sim->minCorner[0], sim->minCorner[1], sim->minCorner[2]};
+    float bmax[3] = {sim->maxCorner[0], sim->maxCorner[1], sim->maxCorner[2]};
+    morton_in[g] = morton_of(src[2 * index_in[g]], bmin, bmax, bmin[0] > bmax[0]);
+}
+
+// ------------------------------------------------------------------ radix sort
+// Stable LSD sort of (key, index) pairs over positions [0, numBodies), eight
+// passes of eight bits. Per pass: tile histograms, per-digit scan over tiles,
+// stable scatter with in-wave ballot ranking.
+__global__ __launch_bounds__(256) void k_sort_hist(const SimValues* __restrict__ sim,
+                                                   const uint64_t* __restrict__ keys, uint32_t shift,
+                                                   uint32_t* __restrict__ tile_hist, uint32_t tiles_cap) {
+    __shared__ uint32_t s_h[256];
+    const uint32_t num = sim->numBodies;
+    const uint32_t base = blockIdx.x * kSortTile;
+    if (base >= num) return;
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < kSortTile / 256; r++) {
+        uint32_t e = base + r * 256 + threadIdx.x;
+        if (e < num) atomicAdd(&s_h[(uint32_t)(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// one block per digit: exclusive scan of that digit's tile counts
+__global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__ sim, uint32_t* __restrict__ tile_hist,
+                                                   uint32_t tiles_cap, uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t num = sim->numBodies;
+    const uint32_t T = (num + kSortTile - 1) / kSortTile;
+    uint32_t* row = tile_hist + (size_t)blockIdx.x * tiles_cap;
+    const uint32_t per = (T + 255) / 256;
+    const uint32_t t = threadIdx.x;
+    uint32_t start = t * per, end = min(start + per, T);
+    uint32_t sum = 0;
+    for (uint32_t k = start; k < end; k++) sum += row[k];
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off, 64);
+        if (lane_id() >= (uint32_t)off) incl += v;
+    }
+    if (lane_id() == 63) s_w[t >> 6] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < (t >> 6); k++) before += s_w[k];
+    uint32_t run = before + incl - sum;
+    for (uint32_t k = start; k < end; k++) {
+        uint32_t v = row[k];
+        row[k] = run;
+        run += v;
+    }
+    if (t == 255) digit_total[blockIdx.x] = before + incl;
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restrict__ sim,
+                                                      const uint64_t* __restrict__ keys_in,
+                                                      const uint32_t* __restrict__ idx_in,
+                                                      uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
+                                                      uint32_t shift, const uint32_t* __restrict__ tile_hist,
+                                                      uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wc[4][256];
+    __shared__ uint32_t s_w[4];
+    const uint32_t num = sim->numBodies;
+    const uint32_t base = blockIdx.x * kSortTile;
+    if (base >= num) return;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    // digit bases: exclusive scan of the 256 digit totals
+    uint32_t tot = digit_total[t];
+    uint32_t incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += v;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < w; k++) before += s_w[k];
+    s_base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
+    s_run[t] = 0;
+    const uint64_t lt = lanemask_lt();
+    for (int r = 0; r < kSortTile / 256; r++) {
+        for (int k = 0; k < 4; k++) s_wc[w][lane * 4 + k] = 0;
+        uint32_t e = base + r * 256 + t;
+        bool valid = e < num;
+        uint64_t key = valid ? keys_in[e] : 0;
+        uint32_t idx = valid ? idx_in[e] : 0;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t bb = ballot(valid && ((d >> bit) & 1u));
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = s_run[d];
+            for (uint32_t k = 0; k < w; k++) off += s_wc[k][d];
+            uint32_t dest = s_base[d] + off + rank;
+            keys_out[dest] = key;
+            idx_out[dest] = idx;
+        }
+        __syncthreads();
+        s_run[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ tree (build + fill)
+// star colour, common/common.glsl:235-276
+__device__ float4 star_color(float mass, float density) {
+    float temp = fdiv(fmul(fmul(5.95589e-19f, powf(mass, 0.875f - 1.0f / 6.0f)), powf(density, 1.0f / 6.0f)), 100.0f);
+    float r, g, b;
+    if (temp <= 66.0f) r = 255.0f;
+    else r = fmul(329.698727446f, powf(fsub(temp, 60.0f), -0.1332047592f));
+    if (temp <= 66.0f) g = fsub(fmul(99.4708025861f, logf(gmax(temp, 1e-3f))), 161.1195681661f);
+    else g = fmul(288.1221695283f, powf(temp, -0.0755148492f));
+    if (temp >= 66.0f) b = 255.0f;
+    else b = fsub(fmul(138.5177312231f, logf(gmax(fsub(temp, 10.0f), 1e-3f))), 305.0447927307f);
+    return make_float4(fdiv(gmin(gmax(r, 0.f), 255.f), 255.f), fdiv(gmin(gmax(g, 0.f), 255.f), 255.f),
+                       fdiv(gmin(gmax(b, 0.f), 255.f), 255.f), 0.0f);
+}
+
+__device__ __forceinline__ int delta_at(const uint64_t* __restrict__ keys, int num, int k) {
+    return safe_lcp(keys, num, k, k + 1);
+}
+
+// Bottom-up build of the Karras-2012 radix tree (Apetrei 2014 agglomeration:
+// one thread per leaf climbs while it is the second child to arrive), fused
+// with the centre-of-mass/AABB/height fill of bh_tree.comp:183-259. The node
+// numbering is Karras's: an internal node covering [lo, hi] is numbered hi
+// when it is its parent's left child and lo when it is the right child or the
+// root, which is exactly the node index treeBuildBinaryRadixTreeKernel assigns.
+__global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                              const float4* __restrict__ src,
+                                              const uint32_t* __restrict__ index_in,
+                                              const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
+    const uint32_t n = sim->initialNumBodies;
+    const int num = (int)sim->numBodies;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint32_t b = index_in[i];
+    float4 pm = src[2 * b];
+    T.leaf_pos[i] = pm;
+    if (colors) {
+        float4 vd = src[2 * b + 1];
+        T.leaf_color[i] = (pm.w > 0.0f && vd.w > 0.0f)
+                              ? star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity))
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (num == 1) {
+        // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
+        WalkNode r;
+        r.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
+        r.half = -1e38f;
+        r.childA = 0;
+        r.childB = kEmpty;
+        r.height = 0;
+        T.nodes[0] = r;
+        T.node_aabb[0] = make_float4(1e38f, 1e38f, 1e38f, 0.f);
+        T.node_aabb[1] = make_float4(-1e38f, -1e38f, -1e38f, 0.f);
+        T.node_parent[0] = kEmpty;
+        T.node_range[0] = make_uint2(0, 0);
+        T.leaf_parent[0] = n;
+        ctr->treeHeight = 0;
+        return;
+    }
+    int lo = i, hi = i;
+    uint32_t id = (uint32_t)i;
+    for (;;) {
+        if (lo == 0 && hi == num - 1) break;
+        const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
+        const int split = left ? hi : lo - 1;
+        T.split_child[2 * split + (left ? 0 : 1)] = id;
+        __threadfence();
+        const uint32_t other = atomicExch(&T.meet[split], (uint32_t)(left ? lo : hi));
+        if (other == kEmpty) return;                      // first to arrive: the sibling continues
+        __threadfence();
+        T.meet[split] = kEmpty;                           // only this thread touches the slot now
+        const int plo = left ? lo : (int)other;
+        const int phi = left ? (int)other : hi;
+        uint32_t pid;
+        if (plo == 0 && phi == num - 1) {
+            pid = 0;
+        } else {
+            const bool pl = (plo == 0) || (phi != num - 1 && delta_at(keys, num, phi) > delta_at(keys, num, plo - 1));
+            pid = (uint32_t)(pl ? phi : plo);
+        }
+        const uint32_t cA = __builtin_nontemporal_load(&T.split_child[2 * split]);
+        const uint32_t cB = __builtin_nontemporal_load(&T.split_child[2 * split + 1]);
+        // children: leaves are points, internal nodes carry their fill
+        float4 ca, cb, amin, amax, bmin, bmax, cola, colb;
+        uint32_t ha, hb;
+        if (cA < n) {
+            ca = T.leaf_pos[cA];
+            amin = amax = ca;
+            ha = 0;
+            cola = colors ? T.leaf_color[cA] : make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            const WalkNode& w = T.nodes[cA - n];
+            ca = w.comMass;
+            ha = w.height;
+            amin = T.node_aabb[2 * (cA - n)];
+            amax = T.node_aabb[2 * (cA - n) + 1];
+            cola = colors ? T.node_color[cA - n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (cB < n) {
+            cb = T.leaf_pos[cB];
+            bmin = bmax = cb;
+            hb = 0;
+            colb = colors ? T.leaf_color[cB] : make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            const WalkNode& w = T.nodes[cB - n];
+            cb = w.comMass;
+            hb = w.height;
+            bmin = T.node_aabb[2 * (cB - n)];
+            bmax = T.node_aabb[2 * (cB - n) + 1];
+            colb = colors ? T.node_color[cB - n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const float M = fadd(ca.w, cb.w);
+        float4 com, col;
+        if (M > 0.0f) {
+            com.x = fdiv(fadd(fmul(ca.w, ca.x), fmul(cb.w, cb.x)), M);
+            com.y = fdiv(fadd(fmul(ca.w, ca.y), fmul(cb.w, cb.y)), M);
+            com.z = fdiv(fadd(fmul(ca.w, ca.z), fmul(cb.w, cb.z)), M);
+            col.x = fdiv(fadd(fmul(ca.w, cola.x), fmul(cb.w, colb.x)), M);
+            col.y = fdiv(fadd(fmul(ca.w, cola.y), fmul(cb.w, colb.y)), M);
+            col.z = fdiv(fadd(fmul(ca.w, cola.z), fmul(cb.w, colb.z)), M);
+        } else {
+            com.x = fmul(fadd(ca.x, cb.x), 0.5f);
+            com.y = fmul(fadd(ca.y, cb.y), 0.5f);
+            com.z = fmul(fadd(ca.z, cb.z), 0.5f);
+            col.x = fmul(fadd(cola.x, colb.x), 0.5f);
+            col.y = fmul(fadd(cola.y, colb.y), 0.5f);
+            col.z = fmul(fadd(cola.z, colb.z), 0.5f);
+        }
+        com.w = M;
+        col.w = 0.0f;
+        float4 nmin = make_float4(gmin(amin.x, bmin.x), gmin(amin.y, bmin.y), gmin(amin.z, bmin.z), 0.f);
+        float4 nmax = make_float4(gmax(amax.x, bmax.x), gmax(amax.y, bmax.y), gmax(amax.z, bmax.z), 0.f);
+        WalkNode nd;
+        nd.comMass = com;
+        float ext = gmax(fsub(nmax.x, nmin.x), gmax(fsub(nmax.y, nmin.y), fsub(nmax.z, nmin.z)));
+        nd.half = fdiv(ext, 2.0f);
+        nd.childA = cA;
+        nd.childB = cB;
+        nd.height = 1u + (ha > hb ? ha : hb);
+        T.nodes[pid] = nd;
+        T.node_aabb[2 * pid] = nmin;
+        T.node_aabb[2 * pid + 1] = nmax;
+        if (colors) T.node_color[pid] = col;
+        T.node_range[pid] = make_uint2((uint32_t)plo, (uint32_t)(phi - plo + 1));
+        if (cA < n) T.leaf_parent[cA] = n + pid; else T.node_parent[cA - n] = n + pid;
+        if (cB < n) T.leaf_parent[cB] = n + pid; else T.node_parent[cB - n] = n + pid;
+        if (pid == 0) {
+            T.node_parent[0] = kEmpty;
+            ctr->treeHeight = nd.height;
+        }
+        lo = plo;
+        hi = phi;
+        id = n + pid;
+    }
+}
+
+// ------------------------------------------------------------------ merge
+// bh_merge.comp:5-23
+__device__ __forceinline__ void merge_pair(float4 p1, float4 v1, float4 p2, float4 v2, float4& po, float4& vo) {
+    float m = fadd(p1.w, p2.w);
+    if (m == 0.0f) { po = vo = make_float4(0.f, 0.f, 0.f, 0.f); return; }
+    if (p1.w == 0.0f) { po = p2; vo = v2; return; }
+    if (p2.w == 0.0f) { po = p1; vo = v1; return; }
+    po.x = fdiv(fadd(fmul(p1.x, p1.w), fmul(p2.x, p2.w)), m);
+    po.y = fdiv(fadd(fmul(p1.y, p1.w), fmul(p2.y, p2.w)), m);
+    po.z = fdiv(fadd(fmul(p1.z, p1.w), fmul(p2.z, p2.w)), m);
+    po.w = m;
+    vo.x = fdiv(fadd(fmul(v1.x, p1.w), fmul(v2.x, p2.w)), m);
+    vo.y = fdiv(fadd(fmul(v1.y, p1.w), fmul(v2.y, p2.w)), m);
+    vo.z = fdiv(fadd(fmul(v1.z, p1.w), fmul(v2.z, p2.w)), m);
+    vo.w = fdiv(fadd(p1.w, p2.w), fadd(fdiv(p1.w, v1.w), fdiv(p2.w, v2.w)));
+}
+
+// Deterministic resolution in ascending (a, b) order (deterministic
+// reservations: a task commits once it holds the smallest pending key on both
+// of its bodies, so every earlier task touching them has already committed).
+// One block; the reservation words are all ~0 between calls.
+__global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const uint2* __restrict__ tasks,
+                                                uint32_t cap, DevCounters* __restrict__ ctr,
+                                                unsigned long long* __restrict__ reserve,
+                                                uint32_t* __restrict__ done) {
+    __shared__ int s_pending;
+    __shared__ uint32_t s_applied;
+    const uint32_t nt = min(ctr->nTasks, cap);
+    if (threadIdx.x == 0) s_applied = 0;
+    for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) done[t] = 0;
+    __syncthreads();
+    uint32_t applied = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_pending = 0;
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            if (done[t]) continue;
+            uint2 ab = tasks[t];
+            if (ab.x == ab.y) { done[t] = 1; continue; }   // bh_merge.comp:40
+            unsigned long long key = ((unsigned long long)ab.x << 32) | ab.y;
+            atomicMin(&reserve[ab.x], key);
+            atomicMin(&reserve[ab.y], key);
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            if (done[t]) continue;
+            uint2 ab = tasks[t];
+            unsigned long long key = ((unsigned long long)ab.x << 32) | ab.y;
+            unsigned long long ra = __hip_atomic_load(&reserve[ab.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long rb = __hip_atomic_load(&reserve[ab.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ra == key && rb == key) {
+                float4 pa = dst[2 * ab.x], va = dst[2 * ab.x + 1];
+                float4 pb = dst[2 * ab.y], vb = dst[2 * ab.y + 1];
+                if (pa.w != 0.0f && pb.w != 0.0f) {
+                    float4 po, vo;
+                    merge_pair(pa, va, pb, vb, po, vo);
+                    dst[2 * ab.x] = po;
+                    dst[2 * ab.x + 1] = vo;
+                    dst[2 * ab.y] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    dst[2 * ab.y + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    applied++;
+                }
+                done[t] = 1;
+            } else {
+                s_pending = 1;
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            uint2 ab = tasks[t];
+            reserve[ab.x] = ~0ull;
+            reserve[ab.y] = ~0ull;
+        }
+        __threadfence();
+        __syncthreads();
+        if (!s_pending) break;
+        __syncthreads();
+    }
+    atomicAdd(&s_applied, applied);
+    __syncthreads();
+    if (threadIdx.x == 0) ctr->mergesApplied = s_applied;
+}
+
+// ------------------------------------------------------------------ boundary layouts
+__global__ void k_nodes_to_std430(const SimValues* __restrict__ sim, TreeArrays T, int colors, int which,
+                                  NodeStd430* __restrict__ out) {
+    const uint32_t n = sim->initialNumBodies, num = sim->numBodies;
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    NodeStd430 o = {};
+    if (which == 0) {
+        if (k >= n) return;
+        if (k < num) {
+            float4 p = T.leaf_pos[k];
+            o.comMass[0] = p.x; o.comMass[1] = p.y; o.comMass[2] = p.z; o.comMass[3] = p.w;
+            o.aabb[0] = o.aabb[3] = p.x; o.aabb[1] = o.aabb[4] = p.y; o.aabb[2] = o.aabb[5] = p.z;
+            o.childA = o.childB = kEmpty;
+            o.nodeDepth = 0;
+            o.readyChildren = kEmpty;
+            o.parentId = T.leaf_parent[k];
+            if (colors) { float4 c = T.leaf_color[k]; o.avgColor[0] = c.x; o.avgColor[1] = c.y; o.avgColor[2] = c.z; }
+        }
+        out[k] = o;
+    } else {
+        if (k + 1 >= n) return;
+        if (k + 1 < num || (num == 1 && k == 0)) {
+            WalkNode w = T.nodes[k];
+            float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
+            o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = w.comMass.w;
+            o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
+            o.childA = w.childA;
+            o.childB = w.childB;
+            o.nodeDepth = w.height;
+            o.bodiesContained = T.node_range[k].y;
+            o.readyChildren = kEmpty;
+            o.parentId = T.node_parent[k];
+            if (colors) { float4 c = T.node_color[k]; o.avgColor[0] = c.x; o.avgColor[1] = c.y; o.avgColor[2] = c.z; }
+        }
+        out[k] = o;
+    }
+}
+
+__global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays T,
+                                  const NodeStd430* __restrict__ leaf_in, const NodeStd430* __restrict__ int_in,
+                                  int which) {
+    const uint32_t n = sim->initialNumBodies;
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (which == 0 && k < n) {
+        const NodeStd430& o = leaf_in[k];
+        T.leaf_pos[k] = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], o.comMass[3]);
+        T.leaf_parent[k] = o.parentId;
+        T.leaf_color[k] = make_float4(o.avgColor[0], o.avgColor[1], o.avgColor[2], o.avgColor[3]);
+    }
+    if (which == 1 && k + 1 < n) {
+        const NodeStd430& o = int_in[k];
+        WalkNode w;
+        w.comMass = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], o.comMass[3]);
+        float ext = gmax(fsub(o.aabb[3], o.aabb[0]), gmax(fsub(o.aabb[4], o.aabb[1]), fsub(o.aabb[5], o.aabb[2])));
+        w.half = fdiv(ext, 2.0f);
+        w.childA = o.childA;
+        w.childB = o.childB;
+        w.height = o.nodeDepth;
+        T.nodes[k] = w;
+        T.node_aabb[2 * k] = make_float4(o.aabb[0], o.aabb[1], o.aabb[2], 0.f);
+        T.node_aabb[2 * k + 1] = make_float4(o.aabb[3], o.aabb[4], o.aabb[5], 0.f);
+        T.node_parent[k] = o.parentId;
+        T.node_range[k] = make_uint2(0, o.bodiesContained);
+        T.node_color[k] = make_float4(o.avgColor[0], o.avgColor[1], o.avgColor[2], o.avgColor[3]);
+    }
+}
+
+// fills a u32 array (graph-safe replacement for a memset)
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ launchers
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void launch_init(hipStream_t s, uint32_t* index_in, uint32_t n) {
+    k_init<<<cdiv(n, 256), 256, 0, s>>>(index_in, n);
+}
+void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units, float mass, float density,
+                  float len, float time, float G, float blc) {
+    k_reset<<<1, 64, 0, s>>>(sim, ctr, set_units, mass, density, len, time, G, blc);
+}
+void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
+                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch) {
+    uint32_t nb = cdiv(n, kPartTile);
+    k_dead_count<<<nb, 256, 0, s>>>(sim, src, dst, index_in, block_scratch);
+    k_dead_scan<<<1, 1024, 0, s>>>(sim, ctr, block_scratch);
+    k_dead_scatter<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, index_out, block_scratch);
+}
+void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, const float4* src,
+                   const uint32_t* index_in, float* partials, uint32_t partial_cap) {
+    uint32_t nb = cdiv(n, 256 * 8);
+    if (nb > partial_cap) nb = partial_cap;
+    if (nb < 1) nb = 1;
+    k_bounds<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, partials);
+}
+void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const float4* src, const uint32_t* index_in,
+                   uint64_t* morton_in) {
+    k_encode<<<cdiv(n, 256), 256, 0, s>>>(sim, src, index_in, morton_in);
+}
+void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
+                      const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
+                      uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total) {
+    uint32_t nt = cdiv(n, kSortTile);
+    k_sort_hist<<<nt, 256, 0, s>>>(sim, keys_in, shift, tile_hist, tiles_cap);
+    k_sort_scan<<<256, 256, 0, s>>>(sim, tile_hist, tiles_cap, digit_total);
+    k_sort_scatter<<<nt, 256, 0, s>>>(sim, keys_in, idx_in, keys_out, idx_out, shift, tile_hist, tiles_cap,
+                                      digit_total);
+}
+void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
+                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors) {
+    k_tree<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T, colors);
+}
+void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
+                  unsigned long long* reserve, uint32_t* done) {
+    k_merge<<<1, 1024, 0, s>>>(dst, tasks, cap, ctr, reserve, done);
+}
+void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
+                            int which, NodeStd430* out) {
+    k_nodes_to_std430<<<cdiv(n, 256), 256, 0, s>>>(sim, T, colors, which, out);
+}
+void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
+                            const NodeStd430* leaf_in, const NodeStd430* int_in, int which) {
+    k_std430_to_nodes<<<cdiv(n, 256), 256, 0, s>>>(sim, T, leaf_in, int_in, which);
+}
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
+    if (n) k_fill_u32<<<cdiv(n, 256), 256, 0, s>>>(p, v, n);
+}
+
+}  // namespace bh
+
+namespace bh {
+
+// ------------------------------------------------------------------ FP64 judge
+// a_i = sum_{j != i, m_j != 0} G m_j r_ij (|r_ij|^2 + softening)^-3/2 in FP64 for
+// sampled bodies i; one block per sampled body, fixed-order reduction.
+__global__ __launch_bounds__(256) void k_direct_sum_f64(const float4* __restrict__ bodies, uint32_t n,
+                                                        const uint32_t* __restrict__ sample, double G,
+                                                        double soft, double* __restrict__ out) {
+    __shared__ double s_acc[3][4];
+    const uint32_t i = sample[blockIdx.x];
+    const float4 pi = bodies[2 * i];
+    const double xi = pi.x, yi = pi.y, zi = pi.z;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const float4 pj = bodies[2 * j];
+        if (j == i || pj.w == 0.0f) continue;
+        const double rx = (double)pj.x - xi, ry = (double)pj.y - yi, rz = (double)pj.z - zi;
+        const double d2 = rx * rx + ry * ry + rz * rz + soft;
+        const double inv = 1.0 / sqrt(d2);
+        const double f = G * (double)pj.w * inv * inv * inv;
+        ax += f * rx;
+        ay += f * ry;
+        az += f * rz;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off, 64);
+        ay += __shfl_xor(ay, off, 64);
+        az += __shfl_xor(az, off, 64);
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) { s_acc[0][w] = ax; s_acc[1][w] = ay; s_acc[2][w] = az; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) out[3 * blockIdx.x + k] = (s_acc[k][0] + s_acc[k][1]) + (s_acc[k][2] + s_acc[k][3]);
+    }
+}
+
+void launch_direct_sum_f64(hipStream_t s, const float4* bodies, uint32_t n, const uint32_t* sample,
+                           uint32_t n_sample, double G, double soft, double* out) {
+    if (n_sample) k_direct_sum_f64<<<n_sample, 256, 0, s>>>(bodies, n, sample, G, soft, out);
+}
+
+}  // namespace bh

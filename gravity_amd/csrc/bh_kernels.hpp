@@ -1,0 +1,73 @@
+// bh_kernels.hpp — launcher declarations and the tree arrays shared by the
+// kernels and the host side of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bh_device.hpp"
+
+namespace bh {
+
+constexpr uint32_t kPartTile = 1024;   // dead partition tile (sorted positions per block)
+constexpr uint32_t kSortTile = 2048;   // radix sort tile (keys per block)
+constexpr uint32_t kSortBits = 8;
+constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
+constexpr int kWalkWaves = 4;          // waves per walk block
+constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
+
+// Tree in HBM. Leaves are sorted positions 0..n-1, internal nodes 0..n-2
+// (global id n + k). The walk reads `nodes` (32 B) and `leaf_pos` (16 B);
+// the rest serves the fill and the 80-byte Node layout at the boundary.
+struct TreeArrays {
+    float4* leaf_pos;        // [n] posMass in sorted order
+    uint32_t* leaf_parent;   // [n]
+    float4* leaf_color;      // [n]
+    WalkNode* nodes;         // [n-1]
+    float4* node_aabb;       // [2(n-1)] min, max
+    uint32_t* node_parent;   // [n-1]
+    uint2* node_range;       // [n-1] (first sorted position, bodies contained)
+    float4* node_color;      // [n-1]
+    uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
+    uint32_t* split_child;   // [2(n-1)] child ids parked at their parent's split
+};
+
+struct WalkArgs {
+    const SimValues* sim;
+    DevCounters* ctr;
+    const float4* src;
+    float4* dst;
+    const uint32_t* index_in;
+    const float4* leaf_pos;
+    const WalkNode* nodes;
+    uint2* tasks;
+    uint32_t task_cap;
+    float theta, dt, softening;
+    uint32_t merging, dynamic, wrap;
+};
+
+void launch_init(hipStream_t s, uint32_t* index_in, uint32_t n);
+void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units, float mass, float density,
+                  float len, float time, float G, float blc);
+void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
+                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch);
+void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, const float4* src,
+                   const uint32_t* index_in, float* partials, uint32_t partial_cap);
+void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const float4* src, const uint32_t* index_in,
+                   uint64_t* morton_in);
+void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
+                      const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
+                      uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total);
+void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
+                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors);
+void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
+void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
+                  unsigned long long* reserve, uint32_t* done);
+void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
+                            int which, NodeStd430* out);
+void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
+                            const NodeStd430* leaf_in, const NodeStd430* int_in, int which);
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
+void launch_direct_sum_f64(hipStream_t s, const float4* bodies, uint32_t n, const uint32_t* sample,
+                           uint32_t n_sample, double G, double soft, double* out);
+
+}  // namespace bh

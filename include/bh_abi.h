@@ -1,0 +1,170 @@
+/*
+ * bh_abi.h — C ABI of the MI355X Barnes-Hut step (libgravity_amd.so).
+ *
+ * This is the drop-in boundary for the reference's compute-shader pipeline.
+ * The Java host of the reference drives that pipeline through
+ * ComputeProgram.run() (gpu/ComputeProgram.java:51-55, gpu/GLSLProgram.java:72-92)
+ * on 16 SSBO bindings (shaders/common/common.glsl:80-120) from
+ * BarnesHut.step() (simulation/BarnesHut.java:99-164). Every entry point below
+ * replaces one of those calls; buffers keep the reference's binding ids and
+ * std430 layouts so the Java mirrors Body (gpu/Body.java:22-25), Node
+ * (gpu/Node.java:31-41) and SimulationValues (gpu/GPU.java:1096-1143) stay valid.
+ *
+ * Conventions: plain pointers and sizes, no C++ or framework types; every
+ * function returns BH_OK (0) or a negative BH_ERR_* code and never lets an
+ * exception or abort cross the boundary; bh_last_error() gives the text.
+ * A context is single-threaded and stream-ordered: calls enqueue work on the
+ * context's HIP stream and return without waiting, except bh_download,
+ * bh_sync, bh_get_counters and bh_get_stage_times, which wait for the stream.
+ */
+#ifndef BH_ABI_H
+#define BH_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints GL errors and continues,
+ *      simulation/GPUSimulation.java:300-305; here every failure is a status) */
+#define BH_OK 0
+#define BH_ERR_INVALID_ARGUMENT (-1)
+#define BH_ERR_OUT_OF_MEMORY (-2)
+#define BH_ERR_HIP (-3)
+#define BH_ERR_UNSUPPORTED (-4)
+#define BH_ERR_TOO_LARGE (-5)        /* replaces gpu/GPU.java:201-209 and the int overflow of GPU.java:241,246 */
+#define BH_ERR_STACK_OVERFLOW (-6)   /* a walk needed more stack than BH_WALK_STACK_DEPTH */
+#define BH_ERR_TASK_OVERFLOW (-7)    /* more merge tasks than the queue holds (N pairs, gpu/GPU.java:323-330) */
+
+/* ---- binding ids: shaders/common/common.glsl:80-120 */
+#define BH_BINDING_LEAF_NODES 0
+#define BH_BINDING_INTERNAL_NODES 1
+#define BH_BINDING_SIMULATION_VALUES 2
+#define BH_BINDING_BODIES_IN 3
+#define BH_BINDING_BODIES_OUT 4
+#define BH_BINDING_MORTON_IN 5
+#define BH_BINDING_MORTON_OUT 6
+#define BH_BINDING_INDEX_IN 7
+#define BH_BINDING_INDEX_OUT 8
+#define BH_BINDING_WORK_QUEUE_IN 9
+#define BH_BINDING_WORK_QUEUE_OUT 10
+#define BH_BINDING_RADIX_WG_HIST 11
+#define BH_BINDING_RADIX_WG_SCANNED 12
+#define BH_BINDING_RADIX_BUCKET_TOTALS 13
+#define BH_BINDING_MERGE_TASKS 14
+#define BH_BINDING_MERGE_BODY_LOCKS 15
+
+/* mergingCollisionOrNeither selector (common/common.glsl:136-138,
+ * settings/defaultProperties.json:64-70) and staticOrDynamic (:71-77) */
+#define BH_NEITHER 0u
+#define BH_MERGING 1u
+#define BH_COLLISION 2u
+#define BH_STATIC 0u
+#define BH_DYNAMIC 1u
+
+/* bh_params.flags */
+#define BH_FLAG_COLORS 1u      /* compute leaf star colours and node avgColor (render interop) */
+#define BH_FLAG_TIMING 2u      /* record per-stage HIP-event timings */
+
+#define BH_WALK_STACK_DEPTH 192
+
+/* The uniforms of the step (gpu/GPU.java:402-498) as one explicit struct. */
+typedef struct {
+    float theta;            /* opening angle of the half-longest-side MAC (bh_force.comp:6-20) */
+    float dt;
+    float softening;        /* added to |r|^2, i.e. epsilon^2 (bh_force.comp:27-32) */
+    float elasticity;
+    float restitution;      /* never uploaded to the reference's FORCE program: 0 there */
+    uint32_t merging_collision_or_neither;
+    uint32_t static_or_dynamic;
+    uint32_t wrap_around;
+    /* unit uniforms of COMPUTE_UPDATE (bh_update.comp:10-37); unit_mass == 0
+     * keeps the current units, as in the reference */
+    float unit_mass, unit_density, unit_len, unit_time;
+    /* optional exact values of the derived units; 0 derives them from the
+     * unit uniforms with the formulas of common/common.glsl:346-352 */
+    float gravitational_constant;
+    float body_length_constant;
+    uint32_t flags;
+} bh_params;
+
+/* Integer counters the reference keeps in SimulationValues, plus walk totals. */
+typedef struct {
+    uint32_t num_bodies;          /* sim.numBodies */
+    uint32_t initial_num_bodies;  /* sim.initialNumBodies */
+    uint32_t just_died;           /* bodies removed by the last dead partition */
+    uint32_t out_of_bounds;       /* sim.outOfBounds (accumulates) */
+    uint64_t opened;              /* internal nodes opened by the last walk (sim.uintDebug[0], unwrapped) */
+    uint64_t interactions;        /* accepted internal nodes + non-self leaves in the last walk */
+    uint32_t merge_tasks;         /* tasks emitted by the last walk */
+    uint32_t merges_applied;      /* merges applied by the last merge resolution */
+    uint32_t tree_height;         /* height of the last tree (the reference fills at most 64 levels) */
+    uint32_t max_stack;           /* deepest walk stack of the last walk */
+    uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow */
+    uint32_t reserved;
+} bh_counters;
+
+typedef struct bh_context bh_context;
+
+/* Context create/destroy: device memory for n_initial bodies, a HIP stream,
+ * sim.numBodies = sim.initialNumBodies = n_initial, sim.bounds = bounds
+ * (min xyz, max xyz), sim.units from the unit uniforms.
+ * Replaces GPU.initGPU (gpu/GPU.java:172-188) + BarnesHut.init (BarnesHut.java:170-173). */
+int bh_create(int device, uint32_t n_initial, const float *bounds6, const bh_params *units,
+              bh_context **out);
+void bh_destroy(bh_context *ctx);
+const char *bh_last_error(const bh_context *ctx);
+
+/* Buffers keyed by binding id (SSBO.bind, gpu/SSBO.java:204-218; upload
+ * gpu/GPU.java:1166-1196; readback gpu/SSBO.java:274-284). Node bindings are
+ * converted to and from the 80-byte std430 Node on the device. Bindings 9-13
+ * and 15 are the reference's scratch, with no counterpart here:
+ * BH_ERR_UNSUPPORTED. */
+size_t bh_binding_size(const bh_context *ctx, int binding);
+int bh_upload(bh_context *ctx, int binding, const void *host, size_t offset, size_t bytes);
+int bh_download(bh_context *ctx, int binding, void *host, size_t offset, size_t bytes);
+
+/* Buffer-name swaps: GPU.swapBodyBuffers / swapMortonAndIndexBuffers (gpu/GPU.java:1060-1086) */
+int bh_swap_body_buffers(bh_context *ctx);
+int bh_swap_morton_and_index_buffers(bh_context *ctx);
+
+/* BarnesHut.init(): COMPUTE_INIT, indexIn[i] = i (bh_init.comp:10-28) */
+int bh_init(bh_context *ctx);
+
+/* One entry per stage of BarnesHut.step (simulation/BarnesHut.java:99-164) */
+int bh_reset_values(bh_context *ctx, const bh_params *p);          /* COMPUTE_UPDATE flag=1, :193-212 */
+int bh_partition_dead_bodies(bh_context *ctx);                     /* DEAD_COUNT/SCAN/SCATTER + swap + UPDATE flag=0, :117,:217-288 */
+int bh_update_bounds(bh_context *ctx);                             /* AABB_REPOPULATE + COLLAPSE, :293-329 */
+int bh_generate_morton_codes(bh_context *ctx);                     /* MORTON_ENCODE, :334-353 */
+int bh_radix_sort(bh_context *ctx);                                /* 16 x {HIST,BUCKET_SCAN,GLOBAL_SCAN,SCATTER}, :358-448 */
+int bh_build_tree(bh_context *ctx, const bh_params *p);            /* TREE_BUILD + INIT_LEAVES + 64 x PROPAGATE, :453-535 */
+int bh_compute_force(bh_context *ctx, const bh_params *p);         /* FORCE_COMPUTE, :540-558 */
+int bh_merge_bodies(bh_context *ctx);                              /* MERGE_BODIES, :563-579 */
+
+/* The whole step: all stages above in order, then the body-buffer swap. */
+int bh_step(bh_context *ctx, const bh_params *p);
+
+/* glFinish equivalent; counters and per-stage times of the last step */
+int bh_sync(bh_context *ctx);
+int bh_get_counters(bh_context *ctx, bh_counters *out);
+#define BH_STAGE_COUNT 8
+/* Per-stage times summed over the steps run with BH_FLAG_TIMING since the
+ * last call (at most 512 steps), in milliseconds, order: reset, dead, bounds,
+ * morton, sort, tree, force, merge; ms[8], if n > 8, receives the number of
+ * steps summed. Waits for the stream. */
+int bh_get_stage_times(bh_context *ctx, float *ms, int n);
+
+/* FP64 direct summation over the bodies of BodiesIn for n_sample sampled
+ * body ids: accel_out[3k..3k+2] = sum_{j != i, m_j != 0} G m_j r_ij
+ * (|r_ij|^2 + softening)^-3/2 with i = sample[k], in double precision, G and
+ * softening taken from sim and from `softening`. This is the judge of the
+ * walk's force error (no reference counterpart; the reference publishes no
+ * error figure). Host arrays in and out; waits for the stream. */
+int bh_direct_sum_f64(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
+                      double *accel_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

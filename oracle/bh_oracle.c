@@ -509,7 +509,17 @@ int bho_force(const float *src, float *dst, const uint32_t *index, uint32_t num,
               float G, float blc, const float *bounds, int stack_cap,
               float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
               bho_walk_stats *st, int threads) {
+    return bho_force_range(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
+                           interactions, tasks, task_cap, st, threads, 0, num);
+}
+
+int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                    const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                    float G, float blc, const float *bounds, int stack_cap,
+                    float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
+                    bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
     memset(st, 0, sizeof *st);
+    if (g_end > num) g_end = num;
     if (stack_cap < 2) stack_cap = 2;
     uint32_t nt = 0;
     if (num <= 1) {
@@ -535,7 +545,7 @@ int bho_force(const float *src, float *dst, const uint32_t *index, uint32_t num,
     {
         uint32_t *stack = malloc((size_t)stack_cap * 4);
 #pragma omp for schedule(dynamic, 256)
-        for (uint32_t g = 0; g < num; g++)
+        for (uint32_t g = g_begin; g < g_end; g++)
             walk_one(g, src, dst, index, n_init, leaf, internal, p, G, blc, bounds, stack, stack_cap,
                      accel, interactions, tasks, task_cap, &nt, st);
         free(stack);

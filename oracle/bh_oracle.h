@@ -106,6 +106,13 @@ int bho_force(const float *src, float *dst, const uint32_t *index, uint32_t num,
               float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
               bho_walk_stats *stats, int threads);
 
+/* the same walk for the sorted positions [g_begin, g_end) only */
+int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                    const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                    float G, float body_len_const, const float *bounds, int stack_cap,
+                    float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
+                    bho_walk_stats *stats, int threads, uint32_t g_begin, uint32_t g_end);
+
 /* merge resolution: shaders/compute/merge/bh_merge.comp:5-71, resolved in
  * ascending (a, b) task order. Returns the number of merges applied. */
 uint32_t bho_merge(float *dst, uint32_t *tasks, uint32_t n_tasks);

@@ -94,6 +94,8 @@ def _declare(L: C.CDLL) -> None:
     L.bho_force.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int,
                             P, P, P, U32, C.POINTER(WalkStats), C.c_int]
     L.bho_force.restype = C.c_int
+    L.bho_force_range.argtypes = L.bho_force.argtypes + [U32, U32]
+    L.bho_force_range.restype = C.c_int
     L.bho_merge.argtypes = [P, P, U32]
     L.bho_merge.restype = U32
     L.bho_merge_pair.argtypes = [P, P, P]
@@ -183,6 +185,17 @@ def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, 
                          float(G), float(blc), _p(b), stack_cap, _p(accel), _p(inter), _p(tasks), task_cap,
                          C.byref(st), threads)
     return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
+
+
+def force_range(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, g_begin, g_end,
+                dst=None, stack_cap=64, threads=0):
+    """Walk only the sorted positions [g_begin, g_end); returns the walk statistics."""
+    st = WalkStats()
+    b = np.ascontiguousarray(bounds, dtype=np.float32)
+    rc = lib().bho_force_range(_p(src), _p(dst), _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm),
+                               float(G), float(blc), _p(b), stack_cap, None, None, None, 0, C.byref(st), threads,
+                               g_begin, g_end)
+    return {"rc": rc, "stats": st}
 
 
 def merge(dst: np.ndarray, tasks: np.ndarray) -> int:
