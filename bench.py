@@ -17,7 +17,7 @@ steps; the roofline entry for the dominant kernel (the walk, VALU-bound:
 library's stream); p50/p99 force error of the walk against an FP64 direct
 sum for 1,024 sampled bodies; and the CPU baseline: the oracle's C port of
 the same step timed on this host over a bounded sample (full-size O(N)
-stages, the walk for a contiguous slice of sorted bodies, extrapolated to N).
+stages, the walk for 8 evenly spaced runs of sorted bodies, scaled to N).
 """
 from __future__ import annotations
 
@@ -79,19 +79,25 @@ def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float
     leaf, internal = O.build(keys, num, n)
     O.fill(src, out, num, n, leaf, internal, u.mass, u.density, max_passes=0, with_colors=False)
     t_stages = time.perf_counter() - t0
-    g0 = max(0, num // 2 - walk_sample // 2)
-    g1 = min(num, g0 + walk_sample)
+    chunks = 8
+    per = max(1, walk_sample // chunks)
+    walked = 0
+    inter = 0
     t1 = time.perf_counter()
-    r = O.force_range(src, out, num, n, leaf, internal, prm, G, blc, b, g0, g1, dst=dst, stack_cap=256,
-                      threads=threads)
+    for k in range(chunks):
+        g0 = (num * (2 * k + 1)) // (2 * chunks) - per // 2
+        g0 = max(0, min(g0, num - per))
+        r = O.force_range(src, out, num, n, leaf, internal, prm, G, blc, b, g0, g0 + per, dst=dst, stack_cap=256,
+                          threads=threads)
+        walked += per
+        inter += r["stats"].interactions
     t_walk = time.perf_counter() - t1
-    t_step = t_stages + t_walk * (num / (g1 - g0))
+    t_step = t_stages + t_walk * (num / walked)
     return {"value": num / t_step, "unit": "body-steps/s", "cores": threads, "kind": "port",
             "sample": (f"one step of the C oracle (oracle/bh_oracle.c) at N={n}: dead partition, bounds, Morton, "
-                       f"sort, build and fill at full size ({t_stages:.1f} s), walk + integrate for {g1 - g0} "
-                       f"consecutive sorted bodies from the middle of the order ({t_walk:.1f} s, "
-                       f"{r['stats'].interactions / (g1 - g0):.0f} interactions/body), walk time scaled by "
-                       f"N/sample"),
+                       f"sort, build and fill at full size ({t_stages:.1f} s), walk + integrate for {walked} "
+                       f"sorted bodies in {chunks} evenly spaced runs of {per} ({t_walk:.1f} s, "
+                       f"{inter / walked:.0f} interactions/body), walk time scaled by N/{walked}"),
             "seconds_measured": t_stages + t_walk}
 
 
@@ -106,7 +112,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-walk-sample", type=int, default=131072)
+    ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -85,6 +85,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.bh_get_counters.argtypes = [ctx, C.POINTER(Counters)]
     L.bh_get_stage_times.argtypes = [ctx, C.POINTER(C.c_float), C.c_int]
     L.bh_direct_sum_f64.argtypes = [ctx, P, C.c_uint32, C.c_double, P]
+    L.bh_selftest_math.argtypes = [ctx, P]
     _lib = L
     return L
 
@@ -195,6 +196,11 @@ class BarnesHut:
         self._check(self.L.bh_direct_sum_f64(self.ctx, s.ctypes.data_as(C.c_void_p), s.shape[0], softening,
                                              out.ctypes.data_as(C.c_void_p)), "direct_sum_f64")
         return out
+
+    def selftest_math(self) -> list:
+        out = np.zeros(3, np.uint32)
+        self._check(self.L.bh_selftest_math(self.ctx, out.ctypes.data_as(C.c_void_p)), "selftest_math")
+        return out.tolist()
 
     def accelerations(self) -> np.ndarray:
         """Walk accelerations of every body for the current BodiesIn, by the

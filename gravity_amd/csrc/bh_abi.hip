@@ -145,7 +145,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_range, (size_t)n - 1));
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
-        TRY(dalloc(c, &T.split_child, 2 * ((size_t)n - 1)));
+        TRY(dalloc(c, &T.xchg, 16 * ((size_t)n - 1)));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
@@ -207,7 +207,7 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.nodes,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.split_child, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
+                    c->tree.xchg, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -497,6 +497,22 @@ int bh_get_stage_times(bh_context* c, float* ms, int n) {
     if (n > BH_STAGE_COUNT) ms[BH_STAGE_COUNT] = (float)c->timed;
     c->timed = 0;
     return BH_OK;
+}
+
+int bh_selftest_math(bh_context* c, uint32_t* mismatches3) {
+    if (!c || !mismatches3) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    uint32_t* d = nullptr;
+    HIPCHECK(c, hipMalloc((void**)&d, 12));
+    hipError_t e = hipMemsetAsync(d, 0, 12, c->stream);
+    if (e == hipSuccess) {
+        launch_selftest_math(c->stream, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches3, d, 12, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "selftest: %s", hipGetErrorString(e));
 }
 
 int bh_direct_sum_f64(bh_context* c, const uint32_t* sample, uint32_t n_sample, double softening, double* out) {

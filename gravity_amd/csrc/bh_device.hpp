@@ -70,7 +70,37 @@ __device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b)
 __device__ __forceinline__ float fsub(float a, float b) { return __fsub_rn(a, b); }
 __device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
 __device__ __forceinline__ float fdiv(float a, float b) { return __fdiv_rn(a, b); }
-__device__ __forceinline__ float fsqrt(float a) { return __fsqrt_rn(a); }
+// __builtin_sqrtf is lowered to the correctly rounded sequence (hardware
+// v_sqrt_f32 plus a one-ulp residual correction); __fsqrt_rn is not on gfx950.
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
+
+// 1 / sqrt(d2) rounded exactly as RN(1 / RN(sqrt(d2))), the reference's
+// inversesqrt, for normal finite d2 (the walk's d2 >= softening > 0).
+// Hardware v_sqrt_f32 and v_rcp_f32 are within one ulp; one residual step
+// each (the same correction the compiler emits for a correctly rounded sqrt,
+// without its denormal and special-value handling) makes them correctly
+// rounded. bh_selftest_math checks both against the general sequences over
+// every float of the domain.
+__device__ __forceinline__ float sqrt_cr_normal(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    uint32_t b = __float_as_uint(s);
+    float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+    float rd = __builtin_fmaf(-sd, s, x);
+    float ru = __builtin_fmaf(-su, s, x);
+    s = (rd <= 0.0f) ? sd : s;
+    s = (ru > 0.0f) ? su : s;
+    return s;
+}
+__device__ __forceinline__ float rcp_cr_normal(float y) {
+    float r = __builtin_amdgcn_rcpf(y);
+    float e = __builtin_fmaf(-y, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
+__device__ __forceinline__ float inv_sqrt_exact(float d2) {
+    // fast sequence on [2^-100, 2^100], the general IEEE one elsewhere
+    if (d2 >= 7.88860905e-31f && d2 <= 1.26765060e30f) return rcp_cr_normal(sqrt_cr_normal(d2));
+    return fdiv(1.0f, fsqrt(d2));
+}
 __device__ __forceinline__ float gmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float gmax(float a, float b) { return fmaxf(a, b); }
 

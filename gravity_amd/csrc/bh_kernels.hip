@@ -210,18 +210,21 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
         float b[6];
         for (int k = 0; k < 3; k++) b[k] = gmin(gmin(s_r[k][0], s_r[k][1]), gmin(s_r[k][2], s_r[k][3]));
         for (int k = 3; k < 6; k++) b[k] = gmax(gmax(s_r[k][0], s_r[k][1]), gmax(s_r[k][2], s_r[k][3]));
-        for (int k = 0; k < 6; k++) partials[6 * blockIdx.x + k] = b[k];
-        __threadfence();
-        uint32_t ticket = atomicAdd(&ctr->aabbTicket, 1u);
+        for (int k = 0; k < 6; k++)
+            __hip_atomic_store(&partials[6 * blockIdx.x + k], b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        uint32_t ticket = __hip_atomic_fetch_add(&ctr->aabbTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = ticket == gridDim.x - 1;
     }
     __syncthreads();
     if (!s_last || threadIdx.x != 0) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
     for (uint32_t k = 0; k < gridDim.x; k++) {
-        for (int c = 0; c < 3; c++) res[c] = gmin(res[c], __builtin_nontemporal_load(&partials[6 * k + c]));
-        for (int c = 3; c < 6; c++) res[c] = gmax(res[c], __builtin_nontemporal_load(&partials[6 * k + c]));
+        for (int c = 0; c < 3; c++)
+            res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int c = 3; c < 6; c++)
+            res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
     ctr->aabbTicket = 0;
     float prev[6] = {p0, p1, p2, q0, q1, q2};
@@ -402,6 +405,66 @@ __device__ __forceinline__ int delta_at(const uint64_t* __restrict__ keys, int n
     return safe_lcp(keys, num, k, k + 1);
 }
 
+// Cross-XCD exchange of a child's summary between the two threads that meet
+// at a parent. MI355X L2 caches are per XCD and not coherent with each other;
+// agent-scope relaxed atomic stores and loads are (they bypass the non-
+// coherent L2 state), so the record goes through them and no agent-scope fence
+// (an L2 write-back plus invalidate on this chip) is needed. Ordering between
+// the record stores and the arrival exchange is enforced by waiting for the
+// stores to complete (release fence at workgroup scope) before the exchange.
+__device__ __forceinline__ void xstore(unsigned long long* p, float a, float b) {
+    unsigned long long v = ((unsigned long long)__float_as_uint(b) << 32) | __float_as_uint(a);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xload(const unsigned long long* p, float& a, float& b) {
+    unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a = __uint_as_float((uint32_t)v);
+    b = __uint_as_float((uint32_t)(v >> 32));
+}
+
+// Wait until this wave's stores are acknowledged, so that the exchange that
+// publishes them cannot overtake them (a workgroup-scope release fence emits
+// no wait on this target; an agent-scope one would also write back the L2).
+__device__ __forceinline__ void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+struct NodeState {
+    float4 com;          // centre of mass, mass
+    float4 mn, mx;       // AABB
+    float4 col;          // mass-weighted colour
+    uint32_t height, id; // height, global node id
+};
+
+__device__ __forceinline__ void put_state(unsigned long long* x, const NodeState& s, int colors) {
+    xstore(x + 0, s.com.x, s.com.y);
+    xstore(x + 1, s.com.z, s.com.w);
+    xstore(x + 2, s.mn.x, s.mn.y);
+    xstore(x + 3, s.mn.z, s.mx.x);
+    xstore(x + 4, s.mx.y, s.mx.z);
+    xstore(x + 5, __uint_as_float(s.height), __uint_as_float(s.id));
+    if (colors) {
+        xstore(x + 6, s.col.x, s.col.y);
+        xstore(x + 7, s.col.z, 0.0f);
+    }
+}
+
+__device__ __forceinline__ void get_state(const unsigned long long* x, NodeState& s, int colors) {
+    float h, id, pad;
+    xload(x + 0, s.com.x, s.com.y);
+    xload(x + 1, s.com.z, s.com.w);
+    xload(x + 2, s.mn.x, s.mn.y);
+    xload(x + 3, s.mn.z, s.mx.x);
+    xload(x + 4, s.mx.y, s.mx.z);
+    xload(x + 5, h, id);
+    s.height = __float_as_uint(h);
+    s.id = __float_as_uint(id);
+    if (colors) {
+        xload(x + 6, s.col.x, s.col.y);
+        xload(x + 7, s.col.z, pad);
+    } else {
+        s.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
 // Bottom-up build of the Karras-2012 radix tree (Apetrei 2014 agglomeration:
 // one thread per leaf climbs while it is the second child to arrive), fused
 // with the centre-of-mass/AABB/height fill of bh_tree.comp:183-259. The node
@@ -417,13 +480,18 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
     const uint32_t b = index_in[i];
-    float4 pm = src[2 * b];
+    const float4 pm = src[2 * b];
+    NodeState me;
+    me.com = pm;
+    me.mn = me.mx = make_float4(pm.x, pm.y, pm.z, 0.f);
+    me.height = 0;
+    me.id = (uint32_t)i;
+    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
     T.leaf_pos[i] = pm;
     if (colors) {
         float4 vd = src[2 * b + 1];
-        T.leaf_color[i] = (pm.w > 0.0f && vd.w > 0.0f)
-                              ? star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity))
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
+        T.leaf_color[i] = me.col;
     }
     if (num == 1) {
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
@@ -443,16 +511,20 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
         return;
     }
     int lo = i, hi = i;
-    uint32_t id = (uint32_t)i;
     for (;;) {
         if (lo == 0 && hi == num - 1) break;
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
-        T.split_child[2 * split + (left ? 0 : 1)] = id;
-        __threadfence();
-        const uint32_t other = atomicExch(&T.meet[split], (uint32_t)(left ? lo : hi));
+        // each side of a split has its own record; the sibling's is the other one
+        unsigned long long* x = T.xchg + 16 * (size_t)split;
+        put_state(x + (left ? 0 : 8), me, colors);
+        stores_done();
+        const uint32_t other = __hip_atomic_exchange(&T.meet[split], (uint32_t)(left ? lo : hi), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (other == kEmpty) return;                      // first to arrive: the sibling continues
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        NodeState sib;
+        get_state(x + (left ? 8 : 0), sib, colors);
         T.meet[split] = kEmpty;                           // only this thread touches the slot now
         const int plo = left ? lo : (int)other;
         const int phi = left ? (int)other : hi;
@@ -463,79 +535,53 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
             const bool pl = (plo == 0) || (phi != num - 1 && delta_at(keys, num, phi) > delta_at(keys, num, plo - 1));
             pid = (uint32_t)(pl ? phi : plo);
         }
-        const uint32_t cA = __builtin_nontemporal_load(&T.split_child[2 * split]);
-        const uint32_t cB = __builtin_nontemporal_load(&T.split_child[2 * split + 1]);
-        // children: leaves are points, internal nodes carry their fill
-        float4 ca, cb, amin, amax, bmin, bmax, cola, colb;
-        uint32_t ha, hb;
-        if (cA < n) {
-            ca = T.leaf_pos[cA];
-            amin = amax = ca;
-            ha = 0;
-            cola = colors ? T.leaf_color[cA] : make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-            const WalkNode& w = T.nodes[cA - n];
-            ca = w.comMass;
-            ha = w.height;
-            amin = T.node_aabb[2 * (cA - n)];
-            amax = T.node_aabb[2 * (cA - n) + 1];
-            cola = colors ? T.node_color[cA - n] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        if (cB < n) {
-            cb = T.leaf_pos[cB];
-            bmin = bmax = cb;
-            hb = 0;
-            colb = colors ? T.leaf_color[cB] : make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-            const WalkNode& w = T.nodes[cB - n];
-            cb = w.comMass;
-            hb = w.height;
-            bmin = T.node_aabb[2 * (cB - n)];
-            bmax = T.node_aabb[2 * (cB - n) + 1];
-            colb = colors ? T.node_color[cB - n] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        const float M = fadd(ca.w, cb.w);
-        float4 com, col;
+        const NodeState& A = left ? me : sib;             // childA is the left child
+        const NodeState& B = left ? sib : me;
+        NodeState p;
+        const float M = fadd(A.com.w, B.com.w);
         if (M > 0.0f) {
-            com.x = fdiv(fadd(fmul(ca.w, ca.x), fmul(cb.w, cb.x)), M);
-            com.y = fdiv(fadd(fmul(ca.w, ca.y), fmul(cb.w, cb.y)), M);
-            com.z = fdiv(fadd(fmul(ca.w, ca.z), fmul(cb.w, cb.z)), M);
-            col.x = fdiv(fadd(fmul(ca.w, cola.x), fmul(cb.w, colb.x)), M);
-            col.y = fdiv(fadd(fmul(ca.w, cola.y), fmul(cb.w, colb.y)), M);
-            col.z = fdiv(fadd(fmul(ca.w, cola.z), fmul(cb.w, colb.z)), M);
+            p.com.x = fdiv(fadd(fmul(A.com.w, A.com.x), fmul(B.com.w, B.com.x)), M);
+            p.com.y = fdiv(fadd(fmul(A.com.w, A.com.y), fmul(B.com.w, B.com.y)), M);
+            p.com.z = fdiv(fadd(fmul(A.com.w, A.com.z), fmul(B.com.w, B.com.z)), M);
+            p.col.x = fdiv(fadd(fmul(A.com.w, A.col.x), fmul(B.com.w, B.col.x)), M);
+            p.col.y = fdiv(fadd(fmul(A.com.w, A.col.y), fmul(B.com.w, B.col.y)), M);
+            p.col.z = fdiv(fadd(fmul(A.com.w, A.col.z), fmul(B.com.w, B.col.z)), M);
         } else {
-            com.x = fmul(fadd(ca.x, cb.x), 0.5f);
-            com.y = fmul(fadd(ca.y, cb.y), 0.5f);
-            com.z = fmul(fadd(ca.z, cb.z), 0.5f);
-            col.x = fmul(fadd(cola.x, colb.x), 0.5f);
-            col.y = fmul(fadd(cola.y, colb.y), 0.5f);
-            col.z = fmul(fadd(cola.z, colb.z), 0.5f);
+            p.com.x = fmul(fadd(A.com.x, B.com.x), 0.5f);
+            p.com.y = fmul(fadd(A.com.y, B.com.y), 0.5f);
+            p.com.z = fmul(fadd(A.com.z, B.com.z), 0.5f);
+            p.col.x = fmul(fadd(A.col.x, B.col.x), 0.5f);
+            p.col.y = fmul(fadd(A.col.y, B.col.y), 0.5f);
+            p.col.z = fmul(fadd(A.col.z, B.col.z), 0.5f);
         }
-        com.w = M;
-        col.w = 0.0f;
-        float4 nmin = make_float4(gmin(amin.x, bmin.x), gmin(amin.y, bmin.y), gmin(amin.z, bmin.z), 0.f);
-        float4 nmax = make_float4(gmax(amax.x, bmax.x), gmax(amax.y, bmax.y), gmax(amax.z, bmax.z), 0.f);
+        p.com.w = M;
+        p.col.w = 0.0f;
+        p.mn = make_float4(gmin(A.mn.x, B.mn.x), gmin(A.mn.y, B.mn.y), gmin(A.mn.z, B.mn.z), 0.f);
+        p.mx = make_float4(gmax(A.mx.x, B.mx.x), gmax(A.mx.y, B.mx.y), gmax(A.mx.z, B.mx.z), 0.f);
+        p.height = 1u + (A.height > B.height ? A.height : B.height);
+        p.id = n + pid;
         WalkNode nd;
-        nd.comMass = com;
-        float ext = gmax(fsub(nmax.x, nmin.x), gmax(fsub(nmax.y, nmin.y), fsub(nmax.z, nmin.z)));
+        nd.comMass = p.com;
+        const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
         nd.half = fdiv(ext, 2.0f);
-        nd.childA = cA;
-        nd.childB = cB;
-        nd.height = 1u + (ha > hb ? ha : hb);
+        nd.childA = A.id;
+        nd.childB = B.id;
+        nd.height = p.height;
+        // these arrays are read only by later kernels
         T.nodes[pid] = nd;
-        T.node_aabb[2 * pid] = nmin;
-        T.node_aabb[2 * pid + 1] = nmax;
-        if (colors) T.node_color[pid] = col;
+        T.node_aabb[2 * pid] = p.mn;
+        T.node_aabb[2 * pid + 1] = p.mx;
+        if (colors) T.node_color[pid] = p.col;
         T.node_range[pid] = make_uint2((uint32_t)plo, (uint32_t)(phi - plo + 1));
-        if (cA < n) T.leaf_parent[cA] = n + pid; else T.node_parent[cA - n] = n + pid;
-        if (cB < n) T.leaf_parent[cB] = n + pid; else T.node_parent[cB - n] = n + pid;
+        if (A.id < n) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - n] = p.id;
+        if (B.id < n) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - n] = p.id;
         if (pid == 0) {
             T.node_parent[0] = kEmpty;
-            ctr->treeHeight = nd.height;
+            ctr->treeHeight = p.height;
         }
+        me = p;
         lo = plo;
         hi = phi;
-        id = n + pid;
     }
 }
 
@@ -581,7 +627,6 @@ __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const 
             atomicMin(&reserve[ab.x], key);
             atomicMin(&reserve[ab.y], key);
         }
-        __threadfence();
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
             if (done[t]) continue;
@@ -606,14 +651,12 @@ __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const 
                 s_pending = 1;
             }
         }
-        __threadfence();
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
             uint2 ab = tasks[t];
-            reserve[ab.x] = ~0ull;
-            reserve[ab.y] = ~0ull;
+            __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __threadfence();
         __syncthreads();
         if (!s_pending) break;
         __syncthreads();
@@ -796,5 +839,32 @@ void launch_direct_sum_f64(hipStream_t s, const float4* bodies, uint32_t n, cons
                            uint32_t n_sample, double G, double soft, double* out) {
     if (n_sample) k_direct_sum_f64<<<n_sample, 256, 0, s>>>(bodies, n, sample, G, soft, out);
 }
+
+}  // namespace bh
+
+namespace bh {
+
+// Self-test of the walk's fast exact inverse square root against the general
+// IEEE sequences, over every float bit pattern of its fast domain.
+__global__ void k_selftest_math(uint32_t* __restrict__ bad) {
+    const uint32_t lo = 0x0D800000u;   // 2^-100
+    const uint32_t hi = 0x71800000u;   // 2^100
+    const uint32_t rlo = 0x26800000u;  // 2^-50
+    const uint32_t rhi = 0x58800000u;  // 2^50
+    uint32_t b0 = 0, b1 = 0, b2 = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t u = lo + blockIdx.x * blockDim.x + threadIdx.x; u <= hi; u += stride) {
+        const float x = __uint_as_float(u);
+        const float s = sqrt_cr_normal(x);
+        if (__float_as_uint(s) != __float_as_uint(fsqrt(x))) b0++;
+        if (__float_as_uint(inv_sqrt_exact(x)) != __float_as_uint(fdiv(1.0f, fsqrt(x)))) b2++;
+        if (u >= rlo && u <= rhi && __float_as_uint(rcp_cr_normal(x)) != __float_as_uint(fdiv(1.0f, x))) b1++;
+    }
+    if (b0) atomicAdd(&bad[0], b0);
+    if (b1) atomicAdd(&bad[1], b1);
+    if (b2) atomicAdd(&bad[2], b2);
+}
+
+void launch_selftest_math(hipStream_t s, uint32_t* bad) { k_selftest_math<<<4096, 256, 0, s>>>(bad); }
 
 }  // namespace bh

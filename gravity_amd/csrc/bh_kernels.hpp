@@ -28,7 +28,7 @@ struct TreeArrays {
     uint2* node_range;       // [n-1] (first sorted position, bodies contained)
     float4* node_color;      // [n-1]
     uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
-    uint32_t* split_child;   // [2(n-1)] child ids parked at their parent's split
+    unsigned long long* xchg; // [16(n-1)] summaries of the two children meeting at each split
 };
 
 struct WalkArgs {
@@ -67,6 +67,7 @@ void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, con
 void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
                             const NodeStd430* leaf_in, const NodeStd430* int_in, int which);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
+void launch_selftest_math(hipStream_t s, uint32_t* bad);
 void launch_direct_sum_f64(hipStream_t s, const float4* bodies, uint32_t n, const uint32_t* sample,
                            uint32_t n_sample, double G, double soft, double* out);
 

@@ -7,7 +7,8 @@
 // lane in the mask evaluates its own acceptance test, and the children are
 // pushed with the mask of the lanes that opened it. Restricted to one lane,
 // this visits exactly the nodes of that body's own per-thread walk in exactly
-// the reference's order (push childA, push childB, pop childB first), so the
+// the reference's order (push childA, push childB, pop childB first: here
+// childA is pushed and childB visited directly), so the
 // accepted set, the opened-node count and the FP32 sums are the reference's.
 #include "bh_device.hpp"
 #include "bh_kernels.hpp"
@@ -54,23 +55,22 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
     uint32_t maxsp = 1, err = 0;
 
     if (num > 1) {
-        s_node[w][0] = n;
-        s_mask[w][0] = ballot(active);
-        int sp = 1;
-        while (sp > 0) {
-            sp--;
-            const uint32_t node = uniform(s_node[w][sp]);
-            const unsigned long long mraw = s_mask[w][sp];
-            const unsigned long long mask =
-                ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
+        // current node in scalar registers; the stack holds the pending
+        // childA entries (after opening a node its childB, the top of the
+        // reference's stack, is visited next without a round trip through LDS)
+        uint32_t node = n;
+        unsigned long long mask = ballot(active);
+        int sp = 0;
+        for (;;) {
             const bool mine = (mask >> lane) & 1ull;
+            bool descend = false;
             if (node >= n) {
                 const WalkNode nd = nodes[node - n];
                 const float rx = fsub(nd.comMass.x, pm.x);
                 const float ry = fsub(nd.comMass.y, pm.y);
                 const float rz = fsub(nd.comMass.z, pm.z);
                 const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = fdiv(1.0f, fsqrt(d2));
+                const float inv = inv_sqrt_exact(d2);
                 const bool accept = fmul(nd.half, inv) < theta;
                 const unsigned long long open = ballot(mine && !accept);
                 if (mine && accept) {
@@ -81,21 +81,21 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                 }
                 inter += (unsigned long long)__popcll(mask & ~open);
                 if (open) {
-                    if (sp + 2 > kStackDepth) {
+                    if (sp + 1 > kStackDepth) {
                         err |= 1u;
                         break;
                     }
                     if (lane == 0) {
                         s_node[w][sp] = nd.childA;
                         s_mask[w][sp] = open;
-                        s_node[w][sp + 1] = nd.childB;
-                        s_mask[w][sp + 1] = open;
                     }
-                    sp += 2;
+                    sp += 1;
                     opened += (unsigned long long)__popcll(open);
-                    if ((uint32_t)sp > maxsp) maxsp = (uint32_t)sp;
+                    if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
+                    node = uniform(nd.childB);
+                    mask = open;
+                    descend = true;
                 }
-                __builtin_amdgcn_wave_barrier();
             } else {
                 const float4 lp = leaf_pos[node];
                 const bool use = mine && node != g;          // bh_force.comp:77-79 (self skip)
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                 const float ry = fsub(lp.y, pm.y);
                 const float rz = fsub(lp.z, pm.z);
                 const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = fdiv(1.0f, fsqrt(d2));
+                const float inv = inv_sqrt_exact(d2);
                 if (use) {
                     const float gm = fmul(G, lp.w);
                     ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
@@ -127,6 +127,13 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                         else err |= 2u;
                     }
                 }
+            }
+            if (!descend) {
+                if (sp == 0) break;
+                sp--;
+                node = uniform(s_node[w][sp]);
+                const unsigned long long mraw = s_mask[w][sp];
+                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
             }
         }
     }

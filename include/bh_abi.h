@@ -164,6 +164,12 @@ int bh_get_stage_times(bh_context *ctx, float *ms, int n);
 int bh_direct_sum_f64(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
                       double *accel_out);
 
+/* Self-test of the walk's arithmetic: counts, over every float of the walk's
+ * fast domain, the results of its correctly rounded sqrt [0], reciprocal [1]
+ * and inverse square root [2] that differ from the general IEEE sequences.
+ * All three must be 0. (No reference counterpart.) */
+int bh_selftest_math(bh_context *ctx, uint32_t *mismatches3);
+
 #ifdef __cplusplus
 }
 #endif

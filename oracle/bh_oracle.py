@@ -105,6 +105,10 @@ def _declare(L: C.CDLL) -> None:
     L.bho_step.argtypes = [C.POINTER(State), C.POINTER(Params)]
     L.bho_step.restype = C.c_int
     L.bho_direct_sum_f64.argtypes = [P, U32, P, U32, C.c_double, C.c_double, P, C.c_int]
+    L.mlp_num_params.restype = C.c_int
+    L.mlp_forward_f64.argtypes = [P, P, C.c_int, P]
+    L.mlp_loss_grad_f64.argtypes = [P, P, P, C.c_int, P]
+    L.mlp_loss_grad_f64.restype = C.c_double
 
 
 def _p(a: np.ndarray | None):
@@ -208,6 +212,29 @@ def direct_sum_f64(bodies: np.ndarray, sample: np.ndarray, G=1.0, softening=0.01
     out = np.zeros((s.shape[0], 3), np.float64)
     lib().bho_direct_sum_f64(_p(bodies), bodies.shape[0], _p(s), s.shape[0], G, softening, _p(out), threads)
     return out
+
+
+def mlp_num_params() -> int:
+    return int(lib().mlp_num_params())
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """FP64 forward of the 12-64-64-3 SiLU network (oracle/mlp_oracle.c)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((xx.shape[0], 3), np.float64)
+    lib().mlp_forward_f64(_p(p), _p(xx), xx.shape[0], _p(y))
+    return y
+
+
+def mlp_loss_grad(params: np.ndarray, x: np.ndarray, t: np.ndarray):
+    """FP64 mean squared error and its gradient."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(p.shape[0], np.float64)
+    loss = lib().mlp_loss_grad_f64(_p(p), _p(xx), _p(tt), xx.shape[0], _p(g))
+    return float(loss), g
 
 
 class Simulation:
