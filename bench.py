@@ -56,6 +56,95 @@ def force_error(bh, n: int, softening: float, n_sample: int = 1024, seed: int = 
             "max": float(err.max()), "sample": int(sample.size), "judge": "fp64 direct sum (HIP kernel)"}
 
 
+def variant_time(bh, prm, steps: int = 3) -> float:
+    """Mean ms per step of `steps` steps with parameters prm (HIP-event stage sum)."""
+    from gravity_amd import barnes_hut as B
+    p = B.Params.from_buffer_copy(prm)
+    p.flags |= B.FLAG_TIMING
+    bh.step(p)                          # warm the variant
+    bh.sync()
+    bh.stage_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bh.step(p)
+    bh.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, seed: int) -> dict:
+    """Train the far-field network on pairs from two 1M-body snapshots
+    (Plummer, two-disk galaxy) with FP64 labels, then compare on the bench
+    workload: plain walks at several angles vs the corrected walk at theta_wide,
+    each with its p99 force error against the FP64 direct sum."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    from gravity_amd import learned as Lr
+    t0 = time.perf_counter()
+    feats, targs = [], []
+    for k, snap in enumerate((ic.plummer(1 << 20, seed + 11), ic.two_galaxy(1 << 19, seed + 12))):
+        sb = B.BarnesHut(snap, ic.tight_bounds(snap), B.make_params(theta=theta_wide, softening=soft))
+        sb.init()
+        sb.build_current_tree()
+        f, t = Lr.collect_training_pairs(sb, theta_wide, theta_plain, 3, soft, max_node_bodies=1 << 18,
+                                         cap=1 << 21)
+        feats.append(f)
+        targs.append(t)
+        sb.close()
+        log(f"learned: snapshot {k}: {f.shape[0]} pairs")
+    x = np.concatenate(feats)
+    t = np.concatenate(targs)
+    t_data = time.perf_counter() - t0
+    mlp = Lr.Mlp(params=Lr.init_params(seed))
+    loss0, _ = mlp.loss_grad(x[:65536], t[:65536])
+    t1 = time.perf_counter()
+    loss = mlp.train(x, t, epochs=12, batch=4096, lr=2e-3, seed=seed)
+    t_train = time.perf_counter() - t1
+    loss1, _ = mlp.loss_grad(x[:65536], t[:65536])
+    log(f"learned: trained on {x.shape[0]} pairs, loss {loss0:.4g} -> {loss1:.4g}")
+    Lr.attach(bh, mlp, theta_plain)
+    rows = []
+    for th in (theta_plain, 0.4, 0.3, 0.25, 0.2):
+        prm = B.make_params(theta=th, softening=soft)
+        ms = variant_time(bh, prm)
+        bh.params = prm
+        err = force_error(bh, n, soft, n_sample=1024)
+        rows.append({"method": "plain", "theta": th, "ms_per_step": ms, "body_steps_per_s": n * 1e3 / ms,
+                     "p50": err["p50"], "p99": err["p99"]})
+        log(f"learned leg: plain theta {th}: {ms:.1f} ms, p99 {err['p99']:.3g}")
+    for th in (theta_wide, 0.7):
+        prm = B.make_params(theta=th, softening=soft)
+        ms_unc = variant_time(bh, prm)
+        bh.params = prm
+        err_unc = force_error(bh, n, soft, n_sample=1024)
+        rows.append({"method": "plain", "theta": th, "ms_per_step": ms_unc, "body_steps_per_s": n * 1e3 / ms_unc,
+                     "p50": err_unc["p50"], "p99": err_unc["p99"]})
+        lp = B.make_params(theta=th, softening=soft, flags=B.FLAG_LEARNED)
+        ms = variant_time(bh, lp)
+        bh.params = lp
+        err = force_error(bh, n, soft, n_sample=1024)
+        rows.append({"method": "learned", "theta": th, "theta_plain": theta_plain, "ms_per_step": ms,
+                     "body_steps_per_s": n * 1e3 / ms, "p50": err["p50"], "p99": err["p99"],
+                     "network_evals_per_body": bh.counters()["corrected"] / n})
+        log(f"learned leg: theta {th}: uncorrected {ms_unc:.1f} ms p99 {err_unc['p99']:.3g}; "
+            f"corrected {ms:.1f} ms p99 {err['p99']:.3g}")
+    bh.params = B.make_params(theta=theta_plain, softening=soft)
+    # equal-p99 comparison: the fastest plain angle whose p99 is at most the corrected walk's
+    best = {}
+    for r in rows:
+        if r["method"] != "learned":
+            continue
+        ok = [q for q in rows if q["method"] == "plain" and q["p99"] <= r["p99"]]
+        if ok:
+            q = max(ok, key=lambda q: q["body_steps_per_s"])
+            best[str(r["theta"])] = {"plain_theta_at_equal_p99": q["theta"],
+                                     "speedup_vs_plain_at_equal_p99": r["body_steps_per_s"] / q["body_steps_per_s"]}
+    mlp_ref = mlp
+    Lr.attach(bh, None, theta_plain)
+    mlp_ref.close()
+    return {"train_pairs": int(x.shape[0]), "data_s": t_data, "train_s": t_train, "loss_before": loss0,
+            "loss_after": loss1, "variants": rows, "equal_p99": best}
+
+
 def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float, softening: float,
                  threads: int, walk_sample: int) -> dict:
     """The oracle's C port of one step on this host: the O(N) stages at full
@@ -111,6 +200,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
     args = ap.parse_args()
@@ -178,6 +268,9 @@ def main() -> int:
     if not args.no_error:
         log("force error against the FP64 direct sum")
         result["force_error"] = force_error(bh, n, soft)
+    if not args.no_learned:
+        log("learned-correction leg")
+        result["learned"] = learned_leg(bh, n, soft, theta, 0.9, args.seed)
     bh.close()
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle port)")

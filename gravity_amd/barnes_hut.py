@@ -27,7 +27,7 @@ MERGE_TASKS = 14
 
 NEITHER, MERGING, COLLISION = 0, 1, 2
 STATIC, DYNAMIC = 0, 1
-FLAG_COLORS, FLAG_TIMING = 1, 2
+FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED = 1, 2, 4
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
 
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "unsupported", -5: "too large",
@@ -49,7 +49,8 @@ class Counters(C.Structure):
     _fields_ = [("num_bodies", C.c_uint32), ("initial_num_bodies", C.c_uint32), ("just_died", C.c_uint32),
                 ("out_of_bounds", C.c_uint32), ("opened", C.c_uint64), ("interactions", C.c_uint64),
                 ("merge_tasks", C.c_uint32), ("merges_applied", C.c_uint32), ("tree_height", C.c_uint32),
-                ("max_stack", C.c_uint32), ("error_flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("max_stack", C.c_uint32), ("error_flags", C.c_uint32), ("reserved", C.c_uint32),
+                ("corrected", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -202,7 +203,18 @@ class BarnesHut:
         self._check(self.L.bh_selftest_math(self.ctx, out.ctypes.data_as(C.c_void_p)), "selftest_math")
         return out.tolist()
 
-    def accelerations(self) -> np.ndarray:
+    def build_current_tree(self, params: Params | None = None) -> None:
+        """Run the stages up to the tree on the current BodiesIn (no walk)."""
+        p = params or self.params
+        self.reset_values(p)
+        self.partition_dead_bodies()
+        if p.static_or_dynamic == DYNAMIC:
+            self.update_bounds()
+        self.generate_morton_codes()
+        self.radix_sort()
+        self.build_tree(p)
+
+    def accelerations(self, params: Params | None = None) -> np.ndarray:
         """Walk accelerations of every body for the current BodiesIn, by the
         reference's own extraction: velocities set to 0 and dt = 1 make the
         integrated velocity equal the acceleration. BodiesIn is restored;
@@ -210,9 +222,9 @@ class BarnesHut:
         saved = self.bodies(BODIES_IN)
         z = saved.copy()
         z[:, 4:7] = 0.0
-        p1 = Params.from_buffer_copy(self.params)
+        p1 = Params.from_buffer_copy(params or self.params)
         p1.dt = 1.0
-        p1.flags = 0
+        p1.flags &= FLAG_LEARNED
         p1.merging_collision_or_neither = NEITHER
         p1.static_or_dynamic = DYNAMIC
         self.upload(BODIES_IN, z)

@@ -15,8 +15,26 @@
 #include "../../include/bh_abi.h"
 #include "bh_device.hpp"
 #include "bh_kernels.hpp"
+#include "bh_mlp.hpp"
 
 using namespace bh;
+
+struct bh_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* params = nullptr;   // FP32 master [kMlpParams]
+    float* grad = nullptr;
+    float* m1 = nullptr;
+    float* m2 = nullptr;
+    __bf16* w = nullptr;       // BF16 MFMA weights [kBfTotal]
+    float* bias = nullptr;     // [144]
+    int adam_t = 0;
+    // training workspace for `cap` rows
+    uint32_t cap = 0;
+    float *x = nullptr, *t = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr, *y = nullptr,
+          *dy = nullptr, *partial = nullptr;
+    char err[512] = {0};
+};
 
 struct bh_context {
     int device = 0;
@@ -43,6 +61,8 @@ struct bh_context {
     uint32_t* task_done = nullptr;
     NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1
     bool colors = false;
+    const bh_mlp* correction = nullptr;       // attached network of the corrected walk
+    float theta_plain = 0.5f;
 
     // per-stage HIP events of up to kTimedSteps steps, read without
     // synchronising inside the stepping loop
@@ -93,6 +113,72 @@ void derive_units(const bh_params* p, float* G, float* blc) {
     *blc = 0.6203504909f * powf(p->unit_mass / p->unit_density, 1.0f / 3.0f) / p->unit_len;
     if (p->gravitational_constant != 0.0f) *G = p->gravitational_constant;
     if (p->body_length_constant != 0.0f) *blc = p->body_length_constant;
+}
+
+int mfail(bh_mlp* m, int code, const char* fmt, ...) {
+    if (m) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(m->err, sizeof m->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define MHIP(m, expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return mfail((m), BH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+int mlp_workspace(bh_mlp* m, uint32_t rows) {
+    if (rows <= m->cap) return BH_OK;
+    float** bufs[] = {&m->x, &m->t, &m->z1, &m->h1, &m->z2, &m->h2, &m->y, &m->dy, &m->partial};
+    for (float** b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    size_t r = rows;
+    size_t sizes[] = {r * kMlpIn, r * kMlpOut, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden,
+                      r * kMlpOut, r * kMlpOut, (r * kMlpOut + 255) / 256 + 1};
+    for (int k = 0; k < 9; k++) MHIP(m, hipMalloc((void**)bufs[k], sizes[k] * sizeof(float)));
+    m->cap = rows;
+    return BH_OK;
+}
+
+// forward over rows already in m->x; leaves Z1,H1,Z2,H2,Y in the workspace
+void mlp_forward_fp32(bh_mlp* m, uint32_t rows) {
+    hipStream_t s = m->stream;
+    const int B = (int)rows;
+    launch_gemm(s, m->x, kMlpIn, 1, m->params + kOffW1, 1, kMlpIn, m->z1, B, kMlpHidden, kMlpIn, 0);
+    launch_bias_act(s, m->z1, m->h1, m->params + kOffB1, B, kMlpHidden);
+    launch_gemm(s, m->h1, kMlpHidden, 1, m->params + kOffW2, 1, kMlpHidden, m->z2, B, kMlpHidden, kMlpHidden, 0);
+    launch_bias_act(s, m->z2, m->h2, m->params + kOffB2, B, kMlpHidden);
+    launch_gemm(s, m->h2, kMlpHidden, 1, m->params + kOffW3, 1, kMlpHidden, m->y, B, kMlpOut, kMlpHidden, 0);
+    launch_bias_act(s, m->y, nullptr, m->params + kOffB3, B, kMlpOut);
+}
+
+// backward for targets in m->t: gradient of the mean squared error into m->grad
+uint32_t mlp_backward_fp32(bh_mlp* m, uint32_t rows) {
+    hipStream_t s = m->stream;
+    const int B = (int)rows;
+    float* g = m->grad;
+    uint32_t nparts = 0;
+    launch_loss_grad(s, m->y, m->t, m->dy, B, m->partial, &nparts);
+    // layer 3
+    launch_gemm(s, m->dy, 1, kMlpOut, m->h2, kMlpHidden, 1, g + kOffW3, kMlpOut, kMlpHidden, B, 0);
+    launch_colsum(s, m->dy, g + kOffB3, B, kMlpOut);
+    launch_gemm(s, m->dy, kMlpOut, 1, m->params + kOffW3, kMlpHidden, 1, m->h2, B, kMlpHidden, kMlpOut, 0);
+    launch_act_grad(s, m->z2, m->h2, (size_t)B * kMlpHidden);          // h2 now holds dZ2
+    // layer 2
+    launch_gemm(s, m->h2, 1, kMlpHidden, m->h1, kMlpHidden, 1, g + kOffW2, kMlpHidden, kMlpHidden, B, 0);
+    launch_colsum(s, m->h2, g + kOffB2, B, kMlpHidden);
+    launch_gemm(s, m->h2, kMlpHidden, 1, m->params + kOffW2, kMlpHidden, 1, m->h1, B, kMlpHidden, kMlpHidden, 0);
+    launch_act_grad(s, m->z1, m->h1, (size_t)B * kMlpHidden);          // h1 now holds dZ1
+    // layer 1
+    launch_gemm(s, m->h1, 1, kMlpHidden, m->x, kMlpIn, 1, g + kOffW1, kMlpHidden, kMlpIn, B, 0);
+    launch_colsum(s, m->h1, g + kOffB1, B, kMlpHidden);
+    return nparts;
 }
 
 void stage_begin(bh_context* c, int k) {
@@ -417,6 +503,14 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
     a.merging = p->merging_collision_or_neither;
     a.dynamic = p->static_or_dynamic;
     a.wrap = p->wrap_around;
+    if (p->flags & BH_FLAG_LEARNED) {
+        if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
+        if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)
+            return fail(c, BH_ERR_UNSUPPORTED, "the corrected walk supports dynamic bounds without merging");
+        launch_walk_learned(c->stream, c->n, a, c->tree.node_aabb, c->tree.node_range, c->correction->w,
+                            c->correction->bias, c->theta_plain);
+        return check_launch(c, "walk (corrected)");
+    }
     launch_walk(c->stream, c->n, a);
     return check_launch(c, "walk");
 }
@@ -480,6 +574,7 @@ int bh_get_counters(bh_context* c, bh_counters* out) {
     out->tree_height = d.treeHeight;
     out->max_stack = d.maxStack;
     out->error_flags = d.errorFlags | (d.nTasks > c->n ? 2u : 0u);
+    out->corrected = d.corrected;
     return BH_OK;
 }
 
@@ -496,6 +591,207 @@ int bh_get_stage_times(bh_context* c, float* ms, int n) {
     }
     if (n > BH_STAGE_COUNT) ms[BH_STAGE_COUNT] = (float)c->timed;
     c->timed = 0;
+    return BH_OK;
+}
+
+// ------------------------------------------------------------------ learned correction
+int bh_mlp_create(int device, bh_mlp** out) {
+    if (!out) return BH_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    bh_mlp* m = new (std::nothrow) bh_mlp();
+    if (!m) return BH_ERR_OUT_OF_MEMORY;
+    m->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete m;
+        return BH_ERR_HIP;
+    }
+    float** bufs[] = {&m->params, &m->grad, &m->m1, &m->m2};
+    for (float** b : bufs)
+        if (hipMalloc((void**)b, kMlpParams * sizeof(float)) != hipSuccess) { bh_mlp_destroy(m); return BH_ERR_OUT_OF_MEMORY; }
+    if (hipMalloc((void**)&m->w, kBfTotal * sizeof(__bf16)) != hipSuccess ||
+        hipMalloc((void**)&m->bias, 144 * sizeof(float)) != hipSuccess) {
+        bh_mlp_destroy(m);
+        return BH_ERR_OUT_OF_MEMORY;
+    }
+    for (float** b : bufs) (void)hipMemsetAsync(*b, 0, kMlpParams * sizeof(float), m->stream);
+    launch_pack_weights(m->stream, m->params, m->w, m->bias);
+    if (hipStreamSynchronize(m->stream) != hipSuccess) { bh_mlp_destroy(m); return BH_ERR_HIP; }
+    *out = m;
+    return BH_OK;
+}
+
+void bh_mlp_destroy(bh_mlp* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void* ptrs[] = {m->params, m->grad, m->m1, m->m2, m->w, m->bias, m->x, m->t, m->z1, m->h1, m->z2, m->h2,
+                    m->y, m->dy, m->partial};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char* bh_mlp_last_error(const bh_mlp* m) { return m ? m->err : "null network"; }
+
+uint32_t bh_mlp_num_params(void) { return (uint32_t)kMlpParams; }
+
+int bh_mlp_set_params(bh_mlp* m, const float* params, uint32_t n) {
+    if (!m || !params || n != (uint32_t)kMlpParams) return BH_ERR_INVALID_ARGUMENT;
+    MHIP(m, hipSetDevice(m->device));
+    MHIP(m, hipMemcpyAsync(m->params, params, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MHIP(m, hipMemsetAsync(m->m1, 0, n * sizeof(float), m->stream));
+    MHIP(m, hipMemsetAsync(m->m2, 0, n * sizeof(float), m->stream));
+    m->adam_t = 0;
+    launch_pack_weights(m->stream, m->params, m->w, m->bias);
+    MHIP(m, hipStreamSynchronize(m->stream));
+    return BH_OK;
+}
+
+int bh_mlp_get_params(bh_mlp* m, float* params, uint32_t n) {
+    if (!m || !params || n != (uint32_t)kMlpParams) return BH_ERR_INVALID_ARGUMENT;
+    MHIP(m, hipMemcpyAsync(params, m->params, n * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MHIP(m, hipStreamSynchronize(m->stream));
+    return BH_OK;
+}
+
+int bh_mlp_forward(bh_mlp* m, const float* x, uint32_t rows, float* y, int use_mfma) {
+    if (!m || (!x && rows) || (!y && rows)) return BH_ERR_INVALID_ARGUMENT;
+    if (rows == 0) return BH_OK;
+    MHIP(m, hipSetDevice(m->device));
+    int rc = mlp_workspace(m, rows);
+    if (rc) return rc;
+    MHIP(m, hipMemcpyAsync(m->x, x, (size_t)rows * kMlpIn * 4, hipMemcpyHostToDevice, m->stream));
+    if (use_mfma) launch_mlp_forward_mfma(m->stream, m->x, rows, m->w, m->bias, m->y);
+    else mlp_forward_fp32(m, rows);
+    MHIP(m, hipGetLastError());
+    MHIP(m, hipMemcpyAsync(y, m->y, (size_t)rows * kMlpOut * 4, hipMemcpyDeviceToHost, m->stream));
+    MHIP(m, hipStreamSynchronize(m->stream));
+    return BH_OK;
+}
+
+static int mlp_loss_value(bh_mlp* m, uint32_t nparts, uint32_t rows, float* loss) {
+    float* part = new (std::nothrow) float[nparts];
+    if (!part) return BH_ERR_OUT_OF_MEMORY;
+    hipError_t e = hipMemcpyAsync(part, m->partial, nparts * 4, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    double acc = 0.0;
+    for (uint32_t k = 0; k < nparts; k++) acc += part[k];
+    delete[] part;
+    *loss = (float)(acc / rows);
+    return e == hipSuccess ? BH_OK : mfail(m, BH_ERR_HIP, "loss readback: %s", hipGetErrorString(e));
+}
+
+int bh_mlp_loss_grad(bh_mlp* m, const float* x, const float* t, uint32_t rows, float* grad, float* loss) {
+    if (!m || !x || !t || !grad || !loss || rows == 0) return BH_ERR_INVALID_ARGUMENT;
+    MHIP(m, hipSetDevice(m->device));
+    int rc = mlp_workspace(m, rows);
+    if (rc) return rc;
+    MHIP(m, hipMemcpyAsync(m->x, x, (size_t)rows * kMlpIn * 4, hipMemcpyHostToDevice, m->stream));
+    MHIP(m, hipMemcpyAsync(m->t, t, (size_t)rows * kMlpOut * 4, hipMemcpyHostToDevice, m->stream));
+    mlp_forward_fp32(m, rows);
+    uint32_t nparts = mlp_backward_fp32(m, rows);
+    MHIP(m, hipGetLastError());
+    MHIP(m, hipMemcpyAsync(grad, m->grad, kMlpParams * 4, hipMemcpyDeviceToHost, m->stream));
+    return mlp_loss_value(m, nparts, rows, loss);
+}
+
+int bh_mlp_train(bh_mlp* m, const float* x, const float* t, uint32_t rows, uint32_t epochs, uint32_t batch,
+                 float lr, uint64_t seed, float* final_loss) {
+    if (!m || !x || !t || rows == 0 || batch == 0 || !final_loss) return BH_ERR_INVALID_ARGUMENT;
+    MHIP(m, hipSetDevice(m->device));
+    if (batch > rows) batch = rows;
+    int rc = mlp_workspace(m, batch);
+    if (rc) return rc;
+    float *dx = nullptr, *dt = nullptr;
+    uint32_t* dperm = nullptr;
+    MHIP(m, hipMalloc((void**)&dx, (size_t)rows * kMlpIn * 4));
+    MHIP(m, hipMalloc((void**)&dt, (size_t)rows * kMlpOut * 4));
+    MHIP(m, hipMalloc((void**)&dperm, (size_t)rows * 4));
+    uint32_t* perm = new (std::nothrow) uint32_t[rows];
+    hipError_t e = perm ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)rows * kMlpIn * 4, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, (size_t)rows * kMlpOut * 4, hipMemcpyHostToDevice, m->stream);
+    uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+    uint32_t nparts = 0, last = batch;
+    for (uint32_t ep = 0; ep < epochs && e == hipSuccess; ep++) {
+        for (uint32_t i = 0; i < rows; i++) perm[i] = i;
+        for (uint32_t i = rows - 1; i > 0; i--) {   // Fisher-Yates with a 64-bit LCG
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            uint32_t j = (uint32_t)((st >> 33) % (i + 1));
+            uint32_t tmp = perm[i]; perm[i] = perm[j]; perm[j] = tmp;
+        }
+        e = hipMemcpyAsync(dperm, perm, (size_t)rows * 4, hipMemcpyHostToDevice, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);   // perm is reused next epoch
+        for (uint32_t off = 0; off + batch <= rows && e == hipSuccess; off += batch) {
+            launch_gather_rows(m->stream, dx, dperm, off, batch, kMlpIn, m->x);
+            launch_gather_rows(m->stream, dt, dperm, off, batch, kMlpOut, m->t);
+            mlp_forward_fp32(m, batch);
+            nparts = mlp_backward_fp32(m, batch);
+            m->adam_t++;
+            launch_adam(m->stream, m->params, m->grad, m->m1, m->m2, kMlpParams, lr, m->adam_t);
+            last = batch;
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) launch_pack_weights(m->stream, m->params, m->w, m->bias);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) rc = mlp_loss_value(m, nparts, last, final_loss);
+    else rc = mfail(m, BH_ERR_HIP, "train: %s", hipGetErrorString(e));
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    (void)hipFree(dt);
+    (void)hipFree(dperm);
+    delete[] perm;
+    return rc;
+}
+
+int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uint32_t stride, float softening,
+                              uint32_t max_node_bodies, uint32_t cap, float* features, float* targets,
+                              uint32_t* pairs_out, uint32_t* n_out) {
+    if (!c || !features || !targets || !n_out || stride == 0 || cap == 0) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    uint2* pairs = nullptr;
+    uint32_t* count = nullptr;
+    float *f = nullptr, *t = nullptr;
+    SimValues h;
+    HIPCHECK(c, hipMemcpyAsync(&h, c->sim, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMalloc((void**)&pairs, (size_t)cap * 8));
+    HIPCHECK(c, hipMalloc((void**)&count, 4));
+    hipError_t e = hipMemsetAsync(count, 0, 4, c->stream);
+    if (e == hipSuccess) {
+        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, theta, theta_plain, softening,
+                             stride, pairs, cap, count);
+        e = hipGetLastError();
+    }
+    uint32_t np = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&np, count, 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (np > cap) np = cap;
+    if (e == hipSuccess) e = hipMalloc((void**)&f, (size_t)(np ? np : 1) * 12 * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&t, (size_t)(np ? np : 1) * 3 * 4);
+    if (e == hipSuccess) {
+        launch_pair_labels(c->stream, pairs, np, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb,
+                           c->tree.node_range, (double)h.gravitationalConstant, softening, max_node_bodies, f, t);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(features, f, (size_t)np * 12 * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(targets, t, (size_t)np * 3 * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && pairs_out)
+        e = hipMemcpyAsync(pairs_out, pairs, (size_t)np * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(pairs);
+    (void)hipFree(count);
+    if (f) (void)hipFree(f);
+    if (t) (void)hipFree(t);
+    *n_out = np;
+    return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "collect pairs: %s", hipGetErrorString(e));
+}
+
+int bh_attach_correction(bh_context* c, const bh_mlp* m, float theta_plain) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->correction = m;
+    c->theta_plain = theta_plain;
     return BH_OK;
 }
 

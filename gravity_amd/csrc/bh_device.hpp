@@ -62,7 +62,9 @@ struct DevCounters {
     uint32_t died;             // bodies removed by the last dead partition
     uint32_t aabbTicket;       // last-block-done ticket of the bounds reduction
     uint32_t mergeRound;
-    uint32_t pad[7];
+    uint32_t pad0;
+    unsigned long long corrected;   // network evaluations used by the corrected walk
+    uint32_t pad[4];
 };
 
 // ---- exact FP32 operations (no contraction, IEEE rounding)

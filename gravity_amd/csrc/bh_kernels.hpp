@@ -60,6 +60,26 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
+void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
+                         const uint2* node_range, const __bf16* w, const float* bias, float theta_plain);
+void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
+                          const WalkNode* nodes, float theta, float theta_plain, float soft, uint32_t stride,
+                          uint2* pairs, uint32_t cap, uint32_t* count);
+void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
+                        const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
+                        float soft, uint32_t max_bodies, float* feat, float* targ);
+void launch_gather_rows(hipStream_t s, const float* src, const uint32_t* perm, uint32_t offset, uint32_t rows,
+                        int cols, float* dst);
+void launch_gemm(hipStream_t s, const float* A, int sam, int sak, const float* B, int sbk, int sbn, float* C, int M,
+                 int N, int K, int accumulate);
+void launch_bias_act(hipStream_t s, float* Z, float* H, const float* bias, int M, int N);
+void launch_loss_grad(hipStream_t s, const float* Y, const float* T, float* dY, int M, float* partial,
+                      uint32_t* nblocks);
+void launch_act_grad(hipStream_t s, const float* Z, float* dH, size_t n);
+void launch_colsum(hipStream_t s, const float* dZ, float* db, int M, int N);
+void launch_adam(hipStream_t s, float* p, const float* g, float* m1, float* m2, int n, float lr, int t);
+void launch_pack_weights(hipStream_t s, const float* p, __bf16* w, float* bias);
+void launch_mlp_forward_mfma(hipStream_t s, const float* x, uint32_t n, const __bf16* w, const float* bias, float* y);
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done);
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,

@@ -1,0 +1,307 @@
+// bh_learned.hip — the learned far-field correction (no reference counterpart).
+//
+//  * k_collect_pairs: for sampled bodies, walk the tree with the wide opening
+//    angle and record every accepted internal node (the pairs the corrected
+//    walk will evaluate the network on).
+//  * k_pair_labels: FP64 label of each pair: the exact field of the node's
+//    bodies on the target minus the node's monopole, in units of G M / d^2.
+//  * training: FP32 GEMM / activation / Adam kernels (forward and backward of
+//    the 12-64-64-3 SiLU network), checked against the FP64 oracle.
+//  * k_mlp_forward_mfma: BF16 MFMA inference of 64 samples per wave, the same
+//    device routine the corrected walk uses.
+#include "bh_device.hpp"
+#include "bh_kernels.hpp"
+#include "bh_mlp.hpp"
+
+namespace bh {
+
+// ------------------------------------------------------------------ pairs
+// Records the pairs the corrected walk evaluates the network on: accepted at
+// the wide angle theta but opened at theta_plain (half * inv >= theta_plain).
+__global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restrict__ sim,
+                                                       const float4* __restrict__ leaf_pos,
+                                                       const WalkNode* __restrict__ nodes, float theta,
+                                                       float theta_plain, float soft, uint32_t stride,
+                                                       uint2* __restrict__ pairs, uint32_t cap,
+                                                       uint32_t* __restrict__ count) {
+    const uint32_t n = sim->initialNumBodies, num = sim->numBodies;
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) * stride;
+    if (g >= num || num < 2) return;
+    const float4 pm = leaf_pos[g];
+    uint32_t stack[kStackDepth];
+    int sp = 0;
+    stack[sp++] = n;
+    while (sp > 0) {
+        const uint32_t node = stack[--sp];
+        if (node < n) continue;
+        const WalkNode nd = nodes[node - n];
+        const float rx = nd.comMass.x - pm.x, ry = nd.comMass.y - pm.y, rz = nd.comMass.z - pm.z;
+        const float inv = inv_sqrt_exact(fadd(dot3(rx, ry, rz), soft));
+        const float ratio = fmul(nd.half, inv);
+        if (ratio < theta) {
+            if (ratio >= theta_plain) {
+                uint32_t slot = atomicAdd(count, 1u);
+                if (slot < cap) pairs[slot] = make_uint2(g, node - n);
+            }
+        } else if (sp + 2 <= kStackDepth) {
+            stack[sp++] = nd.childA;
+            stack[sp++] = nd.childB;
+        }
+    }
+}
+
+// One block per pair: exact field of the node's bodies (sorted positions
+// [lo, lo + count), the target itself excluded) in FP64, minus the monopole.
+__global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ pairs, uint32_t npairs,
+                                                     const float4* __restrict__ leaf_pos,
+                                                     const WalkNode* __restrict__ nodes,
+                                                     const float4* __restrict__ node_aabb,
+                                                     const uint2* __restrict__ node_range, double G, float soft,
+                                                     uint32_t max_bodies, float* __restrict__ feat,
+                                                     float* __restrict__ targ) {
+    __shared__ double s_a[3][4];
+    const uint32_t p = blockIdx.x;
+    if (p >= npairs) return;
+    const uint32_t g = pairs[p].x, k = pairs[p].y;
+    const float4 x = leaf_pos[g];
+    const WalkNode nd = nodes[k];
+    const uint2 rg = node_range[k];
+    if (rg.y > max_bodies) {   // too large to label exactly: marked, dropped by the host
+        if (threadIdx.x == 0) targ[3 * p] = targ[3 * p + 1] = targ[3 * p + 2] = __builtin_nanf("");
+        return;
+    }
+    double ax = 0, ay = 0, az = 0;
+    for (uint32_t j = rg.x + threadIdx.x; j < rg.x + rg.y; j += blockDim.x) {
+        if (j == g) continue;
+        const float4 b = leaf_pos[j];
+        const double rx = (double)b.x - x.x, ry = (double)b.y - x.y, rz = (double)b.z - x.z;
+        const double d2 = rx * rx + ry * ry + rz * rz + (double)soft;
+        const double inv = 1.0 / sqrt(d2);
+        const double f = G * (double)b.w * inv * inv * inv;
+        ax += f * rx;
+        ay += f * ry;
+        az += f * rz;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off, 64);
+        ay += __shfl_xor(ay, off, 64);
+        az += __shfl_xor(az, off, 64);
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) { s_a[0][w] = ax; s_a[1][w] = ay; s_a[2][w] = az; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    ax = (s_a[0][0] + s_a[0][1]) + (s_a[0][2] + s_a[0][3]);
+    ay = (s_a[1][0] + s_a[1][1]) + (s_a[1][2] + s_a[1][3]);
+    az = (s_a[2][0] + s_a[2][1]) + (s_a[2][2] + s_a[2][3]);
+    const double rx = (double)nd.comMass.x - x.x, ry = (double)nd.comMass.y - x.y, rz = (double)nd.comMass.z - x.z;
+    const double ds2 = rx * rx + ry * ry + rz * rz + (double)soft;
+    const double ds = sqrt(ds2);
+    const double scale = ds2 / (G * (double)nd.comMass.w);   // 1 / (G M / d^2)
+    targ[3 * p + 0] = (float)(ax * scale - rx / ds);
+    targ[3 * p + 1] = (float)(ay * scale - ry / ds);
+    targ[3 * p + 2] = (float)(az * scale - rz / ds);
+    const float frx = nd.comMass.x - x.x, fry = nd.comMass.y - x.y, frz = nd.comMass.z - x.z;
+    const float inv = inv_sqrt_exact(fadd(dot3(frx, fry, frz), soft));
+    mlp_features(frx, fry, frz, inv, soft, nd.comMass.w, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1], nd.comMass,
+                 feat + 12 * (size_t)p);
+}
+
+// ------------------------------------------------------------------ FP32 training kernels
+// C[M][N] = sum_k A(m,k) B(k,n) (+ C if accumulate); A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn] (strides express the transposes). 16x16 tiles.
+__global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, int sam, int sak,
+                                              const float* __restrict__ B, int sbk, int sbn,
+                                              float* __restrict__ C, int M, int N, int K, int accumulate) {
+    __shared__ float sA[16][17], sB[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m = blockIdx.y * 16 + ty, nn = blockIdx.x * 16 + tx;
+    float acc = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        int ka = k0 + tx, kb = k0 + ty;
+        sA[ty][tx] = (m < M && ka < K) ? A[(size_t)m * sam + (size_t)ka * sak] : 0.0f;
+        int nb = blockIdx.x * 16 + tx;
+        sB[ty][tx] = (kb < K && nb < N) ? B[(size_t)kb * sbk + (size_t)nb * sbn] : 0.0f;
+        __syncthreads();
+        for (int kk = 0; kk < 16; kk++) acc += sA[ty][kk] * sB[kk][tx];
+        __syncthreads();
+    }
+    if (m < M && nn < N) {
+        float* c = C + (size_t)m * N + nn;
+        *c = accumulate ? *c + acc : acc;
+    }
+}
+
+// Z = Z + bias (row broadcast); H = SiLU(Z) (if H)
+__global__ void k_bias_act(float* __restrict__ Z, float* __restrict__ H, const float* __restrict__ bias, int M,
+                           int N) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    float z = Z[i] + bias[i % N];
+    Z[i] = z;
+    if (H) H[i] = z / (1.0f + expf(-z));
+}
+
+// dY = 2 (Y - T) / batch; per-block partial sums of the squared error
+__global__ void k_loss_grad(const float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
+                            int M, float* __restrict__ loss_partial) {
+    __shared__ float s[256];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float e2 = 0.0f;
+    if (i < (size_t)M * kMlpOut) {
+        float e = Y[i] - T[i];
+        dY[i] = 2.0f * e / (float)M;
+        e2 = e * e;
+    }
+    s[threadIdx.x] = e2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_partial[blockIdx.x] = s[0];
+}
+
+// dZ = dH * silu'(Z)
+__global__ void k_act_grad(const float* __restrict__ Z, float* __restrict__ dH, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float z = Z[i];
+    float sg = 1.0f / (1.0f + expf(-z));
+    dH[i] = dH[i] * sg * (1.0f + z * (1.0f - sg));
+}
+
+// db[n] = sum over rows of dZ[m][n]
+__global__ void k_colsum(const float* __restrict__ dZ, float* __restrict__ db, int M, int N) {
+    int nn = blockIdx.x;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) s += dZ[(size_t)m * N + nn];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ float w[4];
+    if (lane_id() == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) db[nn] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float a = b1 * m1[i] + (1.0f - b1) * gi;
+    float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// FP32 master parameters -> BF16 MFMA weights (padded, zero elsewhere) + FP32 bias block
+__global__ void k_pack_weights(const float* __restrict__ p, __bf16* __restrict__ w, float* __restrict__ bias) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kBfTotal) {
+        float v = 0.0f;
+        if (i < kBfW2) {
+            int o = i / kMlpInPad, k = i % kMlpInPad;
+            if (k < kMlpIn) v = p[kOffW1 + o * kMlpIn + k];
+        } else if (i < kBfW3) {
+            int j = i - kBfW2;
+            v = p[kOffW2 + j];
+        } else {
+            int j = i - kBfW3;
+            int o = j / kMlpHidden, k = j % kMlpHidden;
+            if (o < kMlpOut) v = p[kOffW3 + o * kMlpHidden + k];
+        }
+        w[i] = (__bf16)v;
+    }
+    if (i < 144) {
+        float v = 0.0f;
+        if (i < 64) v = p[kOffB1 + i];
+        else if (i < 128) v = p[kOffB2 + i - 64];
+        else if (i - 128 < kMlpOut) v = p[kOffB3 + i - 128];
+        bias[i] = v;
+    }
+}
+
+// ------------------------------------------------------------------ MFMA inference
+// 64 samples per wave, 4 waves per block; x is [n][12] FP32, y is [n][3].
+__global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restrict__ x, uint32_t n,
+                                                          const __bf16* __restrict__ wg,
+                                                          const float* __restrict__ bg, float* __restrict__ y) {
+    __shared__ __bf16 s_w[kBfTotal];
+    __shared__ float s_b[144];
+    __shared__ __bf16 s_x[4][64 * kMlpInPad];
+    __shared__ __bf16 s_h[4][64 * kMlpHidden];
+    __shared__ float s_y[4][64 * 4];
+    for (int i = threadIdx.x; i < kBfTotal; i += blockDim.x) s_w[i] = wg[i];
+    for (int i = threadIdx.x; i < 144; i += blockDim.x) s_b[i] = bg[i];
+    const int w = threadIdx.x >> 6, lane = (int)lane_id();
+    const uint32_t row = (blockIdx.x * 4 + w) * 64 + lane;
+    for (int k = 0; k < kMlpInPad; k++)
+        s_x[w][lane * kMlpInPad + k] = (__bf16)((row < n && k < kMlpIn) ? x[(size_t)row * kMlpIn + k] : 0.0f);
+    __syncthreads();
+    mlp_wave_forward(s_x[w], s_h[w], s_w, s_b, s_y[w]);
+    if (row < n)
+        for (int j = 0; j < kMlpOut; j++) y[(size_t)row * 3 + j] = s_y[w][lane * 4 + j];
+}
+
+// ------------------------------------------------------------------ launchers
+static inline uint32_t cdiv2(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
+                          const WalkNode* nodes, float theta, float theta_plain, float soft, uint32_t stride,
+                          uint2* pairs, uint32_t cap, uint32_t* count) {
+    uint32_t threads = (n + stride - 1) / stride;
+    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, theta, theta_plain, soft, stride, pairs,
+                                                        cap, count);
+}
+void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
+                        const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
+                        float soft, uint32_t max_bodies, float* feat, float* targ) {
+    if (npairs) k_pair_labels<<<npairs, 256, 0, s>>>(pairs, npairs, leaf_pos, nodes, node_aabb, node_range, G, soft,
+                                                     max_bodies, feat, targ);
+}
+// rows of x (n x cols) gathered through perm[offset .. offset + rows)
+__global__ void k_gather_rows(const float* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t offset,
+                              uint32_t rows, int cols, float* __restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    uint32_t r = (uint32_t)(i / cols);
+    int c = (int)(i % cols);
+    dst[i] = src[(size_t)perm[offset + r] * cols + c];
+}
+void launch_gather_rows(hipStream_t s, const float* src, const uint32_t* perm, uint32_t offset, uint32_t rows,
+                        int cols, float* dst) {
+    if (rows) k_gather_rows<<<cdiv2((size_t)rows * cols, 256), 256, 0, s>>>(src, perm, offset, rows, cols, dst);
+}
+void launch_gemm(hipStream_t s, const float* A, int sam, int sak, const float* B, int sbk, int sbn, float* C, int M,
+                 int N, int K, int accumulate) {
+    dim3 grid(cdiv2(N, 16), cdiv2(M, 16));
+    k_gemm<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, C, M, N, K, accumulate);
+}
+void launch_bias_act(hipStream_t s, float* Z, float* H, const float* bias, int M, int N) {
+    k_bias_act<<<cdiv2((size_t)M * N, 256), 256, 0, s>>>(Z, H, bias, M, N);
+}
+void launch_loss_grad(hipStream_t s, const float* Y, const float* T, float* dY, int M, float* partial,
+                      uint32_t* nblocks) {
+    *nblocks = cdiv2((size_t)M * kMlpOut, 256);
+    k_loss_grad<<<*nblocks, 256, 0, s>>>(Y, T, dY, M, partial);
+}
+void launch_act_grad(hipStream_t s, const float* Z, float* dH, size_t n) {
+    k_act_grad<<<cdiv2(n, 256), 256, 0, s>>>(Z, dH, n);
+}
+void launch_colsum(hipStream_t s, const float* dZ, float* db, int M, int N) {
+    k_colsum<<<N, 256, 0, s>>>(dZ, db, M, N);
+}
+void launch_adam(hipStream_t s, float* p, const float* g, float* m1, float* m2, int n, float lr, int t) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    k_adam<<<cdiv2(n, 256), 256, 0, s>>>(p, g, m1, m2, n, lr, b1, b2, 1e-8f, c1, c2);
+}
+void launch_pack_weights(hipStream_t s, const float* p, __bf16* w, float* bias) {
+    k_pack_weights<<<cdiv2(kBfTotal, 256), 256, 0, s>>>(p, w, bias);
+}
+void launch_mlp_forward_mfma(hipStream_t s, const float* x, uint32_t n, const __bf16* w, const float* bias, float* y) {
+    if (n) k_mlp_forward_mfma<<<cdiv2(n, 256), 256, 0, s>>>(x, n, w, bias, y);
+}
+
+}  // namespace bh

@@ -193,3 +193,175 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
 }
 
 }  // namespace bh
+
+// ---------------------------------------------------------------------------
+// The corrected walk (no reference counterpart): the same masked-stack walk
+// with a wide opening angle `theta`; an internal node accepted at `theta` but
+// one the plain angle `theta_plain` would have opened gets, besides its
+// monopole, the network's residual: a += G M / d^2 * MLP(features). The
+// wave stages the 64 lanes' features in LDS and evaluates them together with
+// BF16 MFMA (FP32 accumulation).
+#include "bh_mlp.hpp"
+
+namespace bh {
+
+__global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                      const float4* __restrict__ src, float4* __restrict__ dst,
+                                                      const uint32_t* __restrict__ index_in,
+                                                      const float4* __restrict__ leaf_pos,
+                                                      const WalkNode* __restrict__ nodes,
+                                                      const float4* __restrict__ node_aabb,
+                                                      const uint2* __restrict__ node_range,
+                                                      const __bf16* __restrict__ mlp_w,
+                                                      const float* __restrict__ mlp_b, float theta_, float theta_plain,
+                                                      float dt_, float softening) {
+    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
+    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+    __shared__ __bf16 s_w[kBfTotal];
+    __shared__ float s_b[144];
+    __shared__ __bf16 s_x[kWalkWaves][64 * kMlpInPad];
+    __shared__ __bf16 s_h[kWalkWaves][64 * kMlpHidden];
+    __shared__ float s_y[kWalkWaves][64 * 4];
+
+    for (int i = threadIdx.x; i < kBfTotal; i += blockDim.x) s_w[i] = mlp_w[i];
+    for (int i = threadIdx.x; i < 144; i += blockDim.x) s_b[i] = mlp_b[i];
+    __syncthreads();
+
+    const uint32_t n = sim->initialNumBodies;
+    const uint32_t num = sim->numBodies;
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = (lb * kWalkWaves + w) * kWave;
+    if (base >= num) return;
+    const uint32_t g = base + lane;
+    const bool active = g < num;
+    const uint32_t bodyIdx = active ? index_in[g] : 0;
+    const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
+    const float G = sim->gravitationalConstant;
+    const float theta = theta_, soft = softening;
+
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    unsigned long long opened = 0, inter = 0, corrected = 0;
+    uint32_t maxsp = 1, err = 0;
+
+    if (num > 1) {
+        uint32_t node = n;
+        unsigned long long mask = ballot(active);
+        int sp = 0;
+        for (;;) {
+            const bool mine = (mask >> lane) & 1ull;
+            bool descend = false;
+            if (node >= n) {
+                const WalkNode nd = nodes[node - n];
+                const float rx = fsub(nd.comMass.x, pm.x);
+                const float ry = fsub(nd.comMass.y, pm.y);
+                const float rz = fsub(nd.comMass.z, pm.z);
+                const float d2 = fadd(dot3(rx, ry, rz), soft);
+                const float inv = inv_sqrt_exact(d2);
+                const float ratio = fmul(nd.half, inv);
+                const bool accept = ratio < theta;
+                const bool correct = accept && ratio >= theta_plain;
+                const unsigned long long open = ballot(mine && !accept);
+                const unsigned long long corr = ballot(mine && correct);
+                float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                if (corr) {
+                    const uint32_t k = node - n;
+                    const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
+                    const uint32_t cnt = node_range[k].y;
+                    float f[12];
+                    mlp_features(rx, ry, rz, inv, soft, nd.comMass.w, cnt, mn, mx, nd.comMass, f);
+                    for (int c = 0; c < kMlpInPad; c++) s_x[w][lane * kMlpInPad + c] = (__bf16)(c < kMlpIn ? f[c] : 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    mlp_wave_forward(s_x[w], s_h[w], s_w, s_b, s_y[w]);
+                    cx = s_y[w][lane * 4 + 0];
+                    cy = s_y[w][lane * 4 + 1];
+                    cz = s_y[w][lane * 4 + 2];
+                    corrected += (unsigned long long)__popcll(corr);
+                }
+                if (mine && accept) {
+                    const float gm = fmul(G, nd.comMass.w);
+                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+                    if (correct) {
+                        const float s2 = gm * inv * inv;
+                        ax += s2 * cx;
+                        ay += s2 * cy;
+                        az += s2 * cz;
+                    }
+                }
+                inter += (unsigned long long)__popcll(mask & ~open);
+                if (open) {
+                    if (sp + 1 > kStackDepth) {
+                        err |= 1u;
+                        break;
+                    }
+                    if (lane == 0) {
+                        s_node[w][sp] = nd.childA;
+                        s_mask[w][sp] = open;
+                    }
+                    sp += 1;
+                    opened += (unsigned long long)__popcll(open);
+                    if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
+                    node = uniform(nd.childB);
+                    mask = open;
+                    descend = true;
+                }
+            } else {
+                const float4 lp = leaf_pos[node];
+                const bool use = mine && node != g;
+                const float rx = fsub(lp.x, pm.x);
+                const float ry = fsub(lp.y, pm.y);
+                const float rz = fsub(lp.z, pm.z);
+                const float d2 = fadd(dot3(rx, ry, rz), soft);
+                const float inv = inv_sqrt_exact(d2);
+                if (use) {
+                    const float gm = fmul(G, lp.w);
+                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+                }
+                const unsigned long long selfbit =
+                    (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
+                inter += (unsigned long long)__popcll(mask & ~selfbit);
+            }
+            if (!descend) {
+                if (sp == 0) break;
+                sp--;
+                node = uniform(s_node[w][sp]);
+                const unsigned long long mraw = s_mask[w][sp];
+                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
+            }
+        }
+    }
+    if (active) {
+        const float dt = dt_;
+        const float nvx = fadd(vd.x, fmul(ax, dt));
+        const float nvy = fadd(vd.y, fmul(ay, dt));
+        const float nvz = fadd(vd.z, fmul(az, dt));
+        dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)), fadd(pm.z, fmul(nvz, dt)),
+                                       pm.w);
+        dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
+    }
+    const uint32_t err_w = (ballot(err & 1u) ? 1u : 0u);
+    if (lane == 0) {
+        atomicAdd(&ctr->opened, opened);
+        atomicAdd(&ctr->interactions, inter);
+        atomicAdd(&ctr->corrected, corrected);
+        atomicMax(&ctr->maxStack, maxsp);
+        if (err_w) atomicOr(&ctr->errorFlags, err_w);
+        atomicAdd(&sim->uintDebug[0], (uint32_t)opened);
+    }
+}
+
+void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
+                         const uint2* node_range, const __bf16* w, const float* bias, float theta_plain) {
+    const uint32_t per_block = kWalkWaves * kWave;
+    const uint32_t nb = (n + per_block - 1) / per_block;
+    k_walk_learned<<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in,
+                                                      a.leaf_pos, a.nodes, node_aabb, node_range, w, bias, a.theta,
+                                                      theta_plain, a.dt, a.softening);
+}
+
+}  // namespace bh

@@ -66,6 +66,8 @@ extern "C" {
 /* bh_params.flags */
 #define BH_FLAG_COLORS 1u      /* compute leaf star colours and node avgColor (render interop) */
 #define BH_FLAG_TIMING 2u      /* record per-stage HIP-event timings */
+#define BH_FLAG_LEARNED 4u     /* corrected walk: the attached network corrects nodes accepted at theta
+                                  that theta_plain would open (no reference counterpart) */
 
 #define BH_WALK_STACK_DEPTH 192
 
@@ -103,6 +105,7 @@ typedef struct {
     uint32_t max_stack;           /* deepest walk stack of the last walk */
     uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow */
     uint32_t reserved;
+    uint64_t corrected;           /* network evaluations of the last corrected walk */
 } bh_counters;
 
 typedef struct bh_context bh_context;
@@ -163,6 +166,40 @@ int bh_get_stage_times(bh_context *ctx, float *ms, int n);
  * error figure). Host arrays in and out; waits for the stream. */
 int bh_direct_sum_f64(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
                       double *accel_out);
+
+/* ---- learned far-field correction (no reference counterpart) ----
+ * A 12-64-64-3 SiLU network predicts, for a (target, node) pair accepted by
+ * the wide opening angle, the residual between the node's exact field and its
+ * monopole in units of G M / d^2. Parameters: FP32 master copy (5,187 values,
+ * order W1[64][12], b1, W2[64][64], b2, W3[3][64], b3); training uses FP32
+ * GEMM kernels and Adam; the corrected walk evaluates it with BF16 MFMA. */
+typedef struct bh_mlp bh_mlp;
+int bh_mlp_create(int device, bh_mlp **out);
+void bh_mlp_destroy(bh_mlp *m);
+const char *bh_mlp_last_error(const bh_mlp *m);
+uint32_t bh_mlp_num_params(void);
+int bh_mlp_set_params(bh_mlp *m, const float *params, uint32_t n);
+int bh_mlp_get_params(bh_mlp *m, float *params, uint32_t n);
+/* y[rows][3] = network(x[rows][12]); use_mfma = 0: FP32 kernels, 1: BF16 MFMA (the walk's routine) */
+int bh_mlp_forward(bh_mlp *m, const float *x, uint32_t rows, float *y, int use_mfma);
+/* mean over rows of the squared error summed over the 3 outputs, and its gradient (FP32) */
+int bh_mlp_loss_grad(bh_mlp *m, const float *x, const float *t, uint32_t rows, float *grad, float *loss);
+/* Adam (beta 0.9/0.999) over shuffled minibatches; loss of the last minibatch */
+int bh_mlp_train(bh_mlp *m, const float *x, const float *t, uint32_t rows, uint32_t epochs, uint32_t batch,
+                 float lr, uint64_t seed, float *final_loss);
+/* On the context's current tree (after bh_build_tree): every stride-th sorted
+ * body walks with angle theta; pairs with a node accepted at theta and opened
+ * at theta_plain get FP32 features and FP64-labelled targets (nodes holding
+ * more than max_node_bodies bodies get NaN targets). Returns n_out <= cap.
+ * Features: extents / d (3), (COM - box centre) / d (3), r / d (3),
+ * ln(mass) / 16, ln(bodies in node) / 16, softening / d^2, with
+ * d^2 = |r|^2 + softening and r = COM - target. */
+int bh_collect_training_pairs(bh_context *ctx, float theta, float theta_plain, uint32_t stride, float softening,
+                              uint32_t max_node_bodies, uint32_t cap, float *features, float *targets,
+                              uint32_t *pairs_out /* optional: (sorted body, internal node) */, uint32_t *n_out);
+/* Attach (or with m = NULL detach) the network used under BH_FLAG_LEARNED;
+ * it must outlive the steps that use it. */
+int bh_attach_correction(bh_context *ctx, const bh_mlp *m, float theta_plain);
 
 /* Self-test of the walk's arithmetic: counts, over every float of the walk's
  * fast domain, the results of its correctly rounded sqrt [0], reciprocal [1]
