@@ -72,10 +72,11 @@ def variant_time(bh, prm, steps: int = 3) -> float:
 
 
 def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, seed: int) -> dict:
-    """Train the far-field network on pairs from two 1M-body snapshots
+    """Train the far-field network on 2M pairs from two 1M-body snapshots
     (Plummer, two-disk galaxy) with FP64 labels, then compare on the bench
-    workload: plain walks at several angles vs the corrected walk at theta_wide,
-    each with its p99 force error against the FP64 direct sum."""
+    workload: plain walks over a sweep of angles vs the corrected walk at
+    theta_wide and 0.7 (and the same box-aware walk with no network), each
+    with its p50/p99 force error against the FP64 direct sum."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
     from gravity_amd import learned as Lr
@@ -85,8 +86,8 @@ def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, 
         sb = B.BarnesHut(snap, ic.tight_bounds(snap), B.make_params(theta=theta_wide, softening=soft))
         sb.init()
         sb.build_current_tree()
-        f, t = Lr.collect_training_pairs(sb, theta_wide, theta_plain, 3, soft, max_node_bodies=1 << 18,
-                                         cap=1 << 21)
+        f, t = Lr.collect_training_pairs(sb, theta_wide, theta_plain, 3, soft, max_node_bodies=1 << 20,
+                                         cap=1 << 20)
         feats.append(f)
         targs.append(t)
         sb.close()
@@ -97,37 +98,35 @@ def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, 
     mlp = Lr.Mlp(params=Lr.init_params(seed))
     loss0, _ = mlp.loss_grad(x[:65536], t[:65536])
     t1 = time.perf_counter()
-    loss = mlp.train(x, t, epochs=12, batch=4096, lr=2e-3, seed=seed)
+    mlp.train(x, t, epochs=20, batch=4096, lr=2e-3, seed=seed)
     t_train = time.perf_counter() - t1
     loss1, _ = mlp.loss_grad(x[:65536], t[:65536])
-    log(f"learned: trained on {x.shape[0]} pairs, loss {loss0:.4g} -> {loss1:.4g}")
-    Lr.attach(bh, mlp, theta_plain)
+    log(f"learned: trained on {x.shape[0]} pairs in {t_train:.1f} s, loss {loss0:.4g} -> {loss1:.4g}")
+    zero = Lr.Mlp(params=np.zeros(Lr.num_params(), np.float32))
     rows = []
-    for th in (theta_plain, 0.4, 0.3, 0.25, 0.2):
-        prm = B.make_params(theta=th, softening=soft)
+
+    def run(method, th, net):
+        flags = 0 if net is None else B.FLAG_LEARNED
+        Lr.attach(bh, net, theta_plain)
+        prm = B.make_params(theta=th, softening=soft, flags=flags)
         ms = variant_time(bh, prm)
         bh.params = prm
         err = force_error(bh, n, soft, n_sample=1024)
-        rows.append({"method": "plain", "theta": th, "ms_per_step": ms, "body_steps_per_s": n * 1e3 / ms,
-                     "p50": err["p50"], "p99": err["p99"]})
-        log(f"learned leg: plain theta {th}: {ms:.1f} ms, p99 {err['p99']:.3g}")
+        row = {"method": method, "theta": th, "ms_per_step": ms, "body_steps_per_s": n * 1e3 / ms,
+               "p50": err["p50"], "p99": err["p99"]}
+        if net is not None:
+            row["theta_plain"] = theta_plain
+            row["network_evals_per_body"] = bh.counters()["corrected"] / n
+        rows.append(row)
+        log(f"learned leg: {method} theta {th}: {ms:.1f} ms, p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+
+    for th in (0.3, 0.4, theta_plain, 0.6, 0.7, 0.8, theta_wide):
+        run("plain", th, None)
     for th in (theta_wide, 0.7):
-        prm = B.make_params(theta=th, softening=soft)
-        ms_unc = variant_time(bh, prm)
-        bh.params = prm
-        err_unc = force_error(bh, n, soft, n_sample=1024)
-        rows.append({"method": "plain", "theta": th, "ms_per_step": ms_unc, "body_steps_per_s": n * 1e3 / ms_unc,
-                     "p50": err_unc["p50"], "p99": err_unc["p99"]})
-        lp = B.make_params(theta=th, softening=soft, flags=B.FLAG_LEARNED)
-        ms = variant_time(bh, lp)
-        bh.params = lp
-        err = force_error(bh, n, soft, n_sample=1024)
-        rows.append({"method": "learned", "theta": th, "theta_plain": theta_plain, "ms_per_step": ms,
-                     "body_steps_per_s": n * 1e3 / ms, "p50": err["p50"], "p99": err["p99"],
-                     "network_evals_per_body": bh.counters()["corrected"] / n})
-        log(f"learned leg: theta {th}: uncorrected {ms_unc:.1f} ms p99 {err_unc['p99']:.3g}; "
-            f"corrected {ms:.1f} ms p99 {err['p99']:.3g}")
+        run("box walk, no network", th, zero)
+        run("learned", th, mlp)
     bh.params = B.make_params(theta=theta_plain, softening=soft)
+    Lr.attach(bh, None, theta_plain)
     # equal-p99 comparison: the fastest plain angle whose p99 is at most the corrected walk's
     best = {}
     for r in rows:
@@ -138,9 +137,8 @@ def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, 
             q = max(ok, key=lambda q: q["body_steps_per_s"])
             best[str(r["theta"])] = {"plain_theta_at_equal_p99": q["theta"],
                                      "speedup_vs_plain_at_equal_p99": r["body_steps_per_s"] / q["body_steps_per_s"]}
-    mlp_ref = mlp
-    Lr.attach(bh, None, theta_plain)
-    mlp_ref.close()
+    mlp.close()
+    zero.close()
     return {"train_pairs": int(x.shape[0]), "data_s": t_data, "train_s": t_train, "loss_before": loss0,
             "loss_after": loss1, "variants": rows, "equal_p99": best}
 

@@ -27,7 +27,7 @@ struct bh_mlp {
     float* m1 = nullptr;
     float* m2 = nullptr;
     __bf16* w = nullptr;       // BF16 MFMA weights [kBfTotal]
-    float* bias = nullptr;     // [144]
+    float* bias = nullptr;     // [kBiasTotal]
     int adam_t = 0;
     // training workspace for `cap` rows
     uint32_t cap = 0;
@@ -231,7 +231,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_range, (size_t)n - 1));
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
-        TRY(dalloc(c, &T.xchg, 16 * ((size_t)n - 1)));
+        TRY(dalloc(c, &T.xchg, 8 * ((size_t)n - 1)));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
@@ -609,7 +609,7 @@ int bh_mlp_create(int device, bh_mlp** out) {
     for (float** b : bufs)
         if (hipMalloc((void**)b, kMlpParams * sizeof(float)) != hipSuccess) { bh_mlp_destroy(m); return BH_ERR_OUT_OF_MEMORY; }
     if (hipMalloc((void**)&m->w, kBfTotal * sizeof(__bf16)) != hipSuccess ||
-        hipMalloc((void**)&m->bias, 144 * sizeof(float)) != hipSuccess) {
+        hipMalloc((void**)&m->bias, kBiasTotal * sizeof(float)) != hipSuccess) {
         bh_mlp_destroy(m);
         return BH_ERR_OUT_OF_MEMORY;
     }
@@ -760,14 +760,14 @@ int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uin
     HIPCHECK(c, hipMalloc((void**)&count, 4));
     hipError_t e = hipMemsetAsync(count, 0, 4, c->stream);
     if (e == hipSuccess) {
-        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, theta, theta_plain, softening,
-                             stride, pairs, cap, count);
+        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb, theta,
+                             theta_plain, softening, stride, pairs, cap, count);
         e = hipGetLastError();
     }
-    uint32_t np = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&np, count, 4, hipMemcpyDeviceToHost, c->stream);
+    uint32_t found = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&found, count, 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (np > cap) np = cap;
+    const uint32_t np = found > cap ? cap : found;
     if (e == hipSuccess) e = hipMalloc((void**)&f, (size_t)(np ? np : 1) * 12 * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&t, (size_t)(np ? np : 1) * 3 * 4);
     if (e == hipSuccess) {
@@ -784,7 +784,7 @@ int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uin
     (void)hipFree(count);
     if (f) (void)hipFree(f);
     if (t) (void)hipFree(t);
-    *n_out = np;
+    *n_out = found;
     return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "collect pairs: %s", hipGetErrorString(e));
 }
 
@@ -814,6 +814,8 @@ int bh_selftest_math(bh_context* c, uint32_t* mismatches3) {
 int bh_direct_sum_f64(bh_context* c, const uint32_t* sample, uint32_t n_sample, double softening, double* out) {
     if (!c || (!sample && n_sample) || (!out && n_sample)) return BH_ERR_INVALID_ARGUMENT;
     if (n_sample == 0) return BH_OK;
+    // one 256-thread block per sample: the grid must stay below 2^32 work-items
+    if (n_sample > (1u << 22)) return fail(c, BH_ERR_INVALID_ARGUMENT, "n_sample %u > 2^22", n_sample);
     HIPCHECK(c, hipSetDevice(c->device));
     for (uint32_t k = 0; k < n_sample; k++)
         if (sample[k] >= c->n) return fail(c, BH_ERR_INVALID_ARGUMENT, "sample id %u >= n", sample[k]);

@@ -33,6 +33,7 @@ __global__ void k_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ c
     ctr->mergesApplied = 0;
     ctr->maxStack = 0;
     ctr->errorFlags = 0;
+    ctr->corrected = 0;
     if (set_units) {
         sim->unitMass = mass;
         sim->unitDensity = density;
@@ -407,19 +408,24 @@ __device__ __forceinline__ int delta_at(const uint64_t* __restrict__ keys, int n
 
 // Cross-XCD exchange of a child's summary between the two threads that meet
 // at a parent. MI355X L2 caches are per XCD and not coherent with each other;
-// agent-scope relaxed atomic stores and loads are (they bypass the non-
-// coherent L2 state), so the record goes through them and no agent-scope fence
-// (an L2 write-back plus invalidate on this chip) is needed. Ordering between
-// the record stores and the arrival exchange is enforced by waiting for the
-// stores to complete (release fence at workgroup scope) before the exchange.
-__device__ __forceinline__ void xstore(unsigned long long* p, float a, float b) {
-    unsigned long long v = ((unsigned long long)__float_as_uint(b) << 32) | __float_as_uint(a);
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// stores and loads with the sc1 coherence bit (what agent-scope atomic
+// accesses compile to) are, so the record goes through them and no agent-scope
+// fence (an L2 write-back plus invalidate on this chip) is needed. The record
+// stores are waited for before the arrival exchange that publishes them.
+// 16-byte store with the coherence bit the compiler uses for agent-scope
+// atomic stores (sc1): the record is visible to every XCD once the store has
+// completed. A store has no result register, so inline assembly is safe here.
+__device__ __forceinline__ void store_coherent(float4* p, float4 v) {
+    typedef float f32x4_t __attribute__((ext_vector_type(4)));
+    f32x4_t w = {v.x, v.y, v.z, v.w};
+    asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(w) : "memory");
 }
-__device__ __forceinline__ void xload(const unsigned long long* p, float& a, float& b) {
-    unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a = __uint_as_float((uint32_t)v);
-    b = __uint_as_float((uint32_t)(v >> 32));
+__device__ __forceinline__ float4 load_coherent(const float4* p) {
+    const unsigned long long* q = (const unsigned long long*)p;
+    unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
+                       __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
 }
 
 // Wait until this wave's stores are acknowledged, so that the exchange that
@@ -434,35 +440,22 @@ struct NodeState {
     uint32_t height, id; // height, global node id
 };
 
-__device__ __forceinline__ void put_state(unsigned long long* x, const NodeState& s, int colors) {
-    xstore(x + 0, s.com.x, s.com.y);
-    xstore(x + 1, s.com.z, s.com.w);
-    xstore(x + 2, s.mn.x, s.mn.y);
-    xstore(x + 3, s.mn.z, s.mx.x);
-    xstore(x + 4, s.mx.y, s.mx.z);
-    xstore(x + 5, __uint_as_float(s.height), __uint_as_float(s.id));
-    if (colors) {
-        xstore(x + 6, s.col.x, s.col.y);
-        xstore(x + 7, s.col.z, 0.0f);
-    }
+// record of one side of a split: 3 (4 with colours) 16-byte words
+__device__ __forceinline__ void put_state(float4* x, const NodeState& s, int colors) {
+    store_coherent(x + 0, s.com);
+    store_coherent(x + 1, make_float4(s.mn.x, s.mn.y, s.mn.z, s.mx.x));
+    store_coherent(x + 2, make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id)));
+    if (colors) store_coherent(x + 3, s.col);
 }
 
-__device__ __forceinline__ void get_state(const unsigned long long* x, NodeState& s, int colors) {
-    float h, id, pad;
-    xload(x + 0, s.com.x, s.com.y);
-    xload(x + 1, s.com.z, s.com.w);
-    xload(x + 2, s.mn.x, s.mn.y);
-    xload(x + 3, s.mn.z, s.mx.x);
-    xload(x + 4, s.mx.y, s.mx.z);
-    xload(x + 5, h, id);
-    s.height = __float_as_uint(h);
-    s.id = __float_as_uint(id);
-    if (colors) {
-        xload(x + 6, s.col.x, s.col.y);
-        xload(x + 7, s.col.z, pad);
-    } else {
-        s.col = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+__device__ __forceinline__ void get_state(const float4* x, NodeState& s, int colors) {
+    s.com = load_coherent(x + 0);
+    const float4 a = load_coherent(x + 1), b = load_coherent(x + 2);
+    s.mn = make_float4(a.x, a.y, a.z, 0.f);
+    s.mx = make_float4(a.w, b.x, b.y, 0.f);
+    s.height = __float_as_uint(b.z);
+    s.id = __float_as_uint(b.w);
+    s.col = colors ? load_coherent(x + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Bottom-up build of the Karras-2012 radix tree (Apetrei 2014 agglomeration:
@@ -516,15 +509,15 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
-        unsigned long long* x = T.xchg + 16 * (size_t)split;
-        put_state(x + (left ? 0 : 8), me, colors);
+        float4* x = T.xchg + 8 * (size_t)split;
+        put_state(x + (left ? 0 : 4), me, colors);
         stores_done();
         const uint32_t other = __hip_atomic_exchange(&T.meet[split], (uint32_t)(left ? lo : hi), __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
         if (other == kEmpty) return;                      // first to arrive: the sibling continues
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         NodeState sib;
-        get_state(x + (left ? 8 : 0), sib, colors);
+        get_state(x + (left ? 4 : 0), sib, colors);
         T.meet[split] = kEmpty;                           // only this thread touches the slot now
         const int plo = left ? lo : (int)other;
         const int phi = left ? (int)other : hi;

@@ -28,7 +28,7 @@ struct TreeArrays {
     uint2* node_range;       // [n-1] (first sorted position, bodies contained)
     float4* node_color;      // [n-1]
     uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
-    unsigned long long* xchg; // [16(n-1)] summaries of the two children meeting at each split
+    float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
 };
 
 struct WalkArgs {
@@ -63,8 +63,8 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
                          const uint2* node_range, const __bf16* w, const float* bias, float theta_plain);
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, float theta, float theta_plain, float soft, uint32_t stride,
-                          uint2* pairs, uint32_t cap, uint32_t* count);
+                          const WalkNode* nodes, const float4* node_aabb, float theta, float theta_plain, float soft,
+                          uint32_t stride, uint2* pairs, uint32_t cap, uint32_t* count);
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
                         float soft, uint32_t max_bodies, float* feat, float* targ);

@@ -17,10 +17,12 @@ namespace bh {
 
 // ------------------------------------------------------------------ pairs
 // Records the pairs the corrected walk evaluates the network on: accepted at
-// the wide angle theta but opened at theta_plain (half * inv >= theta_plain).
+// the wide angle theta with the target outside the node's box, but opened at
+// theta_plain (half * inv >= theta_plain).
 __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restrict__ sim,
                                                        const float4* __restrict__ leaf_pos,
-                                                       const WalkNode* __restrict__ nodes, float theta,
+                                                       const WalkNode* __restrict__ nodes,
+                                                       const float4* __restrict__ node_aabb, float theta,
                                                        float theta_plain, float soft, uint32_t stride,
                                                        uint2* __restrict__ pairs, uint32_t cap,
                                                        uint32_t* __restrict__ count) {
@@ -38,7 +40,10 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
         const float rx = nd.comMass.x - pm.x, ry = nd.comMass.y - pm.y, rz = nd.comMass.z - pm.z;
         const float inv = inv_sqrt_exact(fadd(dot3(rx, ry, rz), soft));
         const float ratio = fmul(nd.half, inv);
-        if (ratio < theta) {
+        const float4 mn = node_aabb[2 * (node - n)], mx = node_aabb[2 * (node - n) + 1];
+        const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y && pm.z >= mn.z &&
+                            pm.z <= mx.z;
+        if (ratio < theta && !inside) {
             if (ratio >= theta_plain) {
                 uint32_t slot = atomicAdd(count, 1u);
                 if (slot < cap) pairs[slot] = make_uint2(g, node - n);
@@ -50,8 +55,9 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
     }
 }
 
-// One block per pair: exact field of the node's bodies (sorted positions
-// [lo, lo + count), the target itself excluded) in FP64, minus the monopole.
+// One wave per pair (grid-stride over the pairs): exact field of the node's
+// bodies (sorted positions [lo, lo + count), the target itself excluded) in
+// FP64, minus the monopole, in units of G M / d^2.
 __global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ pairs, uint32_t npairs,
                                                      const float4* __restrict__ leaf_pos,
                                                      const WalkNode* __restrict__ nodes,
@@ -59,52 +65,47 @@ __global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ p
                                                      const uint2* __restrict__ node_range, double G, float soft,
                                                      uint32_t max_bodies, float* __restrict__ feat,
                                                      float* __restrict__ targ) {
-    __shared__ double s_a[3][4];
-    const uint32_t p = blockIdx.x;
-    if (p >= npairs) return;
-    const uint32_t g = pairs[p].x, k = pairs[p].y;
-    const float4 x = leaf_pos[g];
-    const WalkNode nd = nodes[k];
-    const uint2 rg = node_range[k];
-    if (rg.y > max_bodies) {   // too large to label exactly: marked, dropped by the host
-        if (threadIdx.x == 0) targ[3 * p] = targ[3 * p + 1] = targ[3 * p + 2] = __builtin_nanf("");
-        return;
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < npairs; p += waves) {
+        const uint32_t g = pairs[p].x, k = pairs[p].y;
+        const float4 x = leaf_pos[g];
+        const WalkNode nd = nodes[k];
+        const uint2 rg = node_range[k];
+        if (rg.y > max_bodies) {   // too large to label exactly: marked, dropped by the host
+            if (lane == 0) targ[3 * p] = targ[3 * p + 1] = targ[3 * p + 2] = __builtin_nanf("");
+            continue;
+        }
+        double ax = 0, ay = 0, az = 0;
+        for (uint32_t j = rg.x + lane; j < rg.x + rg.y; j += kWave) {
+            if (j == g) continue;
+            const float4 b = leaf_pos[j];
+            const double rx = (double)b.x - x.x, ry = (double)b.y - x.y, rz = (double)b.z - x.z;
+            const double d2 = rx * rx + ry * ry + rz * rz + (double)soft;
+            const double inv = 1.0 / sqrt(d2);
+            const double f = G * (double)b.w * inv * inv * inv;
+            ax += f * rx;
+            ay += f * ry;
+            az += f * rz;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            ax += __shfl_xor(ax, off, 64);
+            ay += __shfl_xor(ay, off, 64);
+            az += __shfl_xor(az, off, 64);
+        }
+        if (lane != 0) continue;
+        const double rx = (double)nd.comMass.x - x.x, ry = (double)nd.comMass.y - x.y, rz = (double)nd.comMass.z - x.z;
+        const double ds2 = rx * rx + ry * ry + rz * rz + (double)soft;
+        const double ds = sqrt(ds2);
+        const double scale = ds2 / (G * (double)nd.comMass.w);   // 1 / (G M / d^2)
+        targ[3 * p + 0] = (float)(ax * scale - rx / ds);
+        targ[3 * p + 1] = (float)(ay * scale - ry / ds);
+        targ[3 * p + 2] = (float)(az * scale - rz / ds);
+        const float frx = nd.comMass.x - x.x, fry = nd.comMass.y - x.y, frz = nd.comMass.z - x.z;
+        const float inv = inv_sqrt_exact(fadd(dot3(frx, fry, frz), soft));
+        mlp_features(frx, fry, frz, inv, soft, nd.comMass.w, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1],
+                     nd.comMass, feat + 12 * (size_t)p);
     }
-    double ax = 0, ay = 0, az = 0;
-    for (uint32_t j = rg.x + threadIdx.x; j < rg.x + rg.y; j += blockDim.x) {
-        if (j == g) continue;
-        const float4 b = leaf_pos[j];
-        const double rx = (double)b.x - x.x, ry = (double)b.y - x.y, rz = (double)b.z - x.z;
-        const double d2 = rx * rx + ry * ry + rz * rz + (double)soft;
-        const double inv = 1.0 / sqrt(d2);
-        const double f = G * (double)b.w * inv * inv * inv;
-        ax += f * rx;
-        ay += f * ry;
-        az += f * rz;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        ax += __shfl_xor(ax, off, 64);
-        ay += __shfl_xor(ay, off, 64);
-        az += __shfl_xor(az, off, 64);
-    }
-    const uint32_t w = threadIdx.x >> 6;
-    if (lane_id() == 0) { s_a[0][w] = ax; s_a[1][w] = ay; s_a[2][w] = az; }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    ax = (s_a[0][0] + s_a[0][1]) + (s_a[0][2] + s_a[0][3]);
-    ay = (s_a[1][0] + s_a[1][1]) + (s_a[1][2] + s_a[1][3]);
-    az = (s_a[2][0] + s_a[2][1]) + (s_a[2][2] + s_a[2][3]);
-    const double rx = (double)nd.comMass.x - x.x, ry = (double)nd.comMass.y - x.y, rz = (double)nd.comMass.z - x.z;
-    const double ds2 = rx * rx + ry * ry + rz * rz + (double)soft;
-    const double ds = sqrt(ds2);
-    const double scale = ds2 / (G * (double)nd.comMass.w);   // 1 / (G M / d^2)
-    targ[3 * p + 0] = (float)(ax * scale - rx / ds);
-    targ[3 * p + 1] = (float)(ay * scale - ry / ds);
-    targ[3 * p + 2] = (float)(az * scale - rz / ds);
-    const float frx = nd.comMass.x - x.x, fry = nd.comMass.y - x.y, frz = nd.comMass.z - x.z;
-    const float inv = inv_sqrt_exact(fadd(dot3(frx, fry, frz), soft));
-    mlp_features(frx, fry, frz, inv, soft, nd.comMass.w, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1], nd.comMass,
-                 feat + 12 * (size_t)p);
 }
 
 // ------------------------------------------------------------------ FP32 training kernels
@@ -196,24 +197,27 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // FP32 master parameters -> BF16 MFMA weights (padded, zero elsewhere) + FP32 bias block
+// FP32 master parameters -> BF16 weight fragments (layout in bh_mlp.hpp) and
+// the padded FP32 biases.
 __global__ void k_pack_weights(const float* __restrict__ p, __bf16* __restrict__ w, float* __restrict__ bias) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < kBfTotal) {
+        const int f = i >> 9, lane = (i >> 3) & 63, j = i & 7;
+        const int r16 = lane & 15, grp = lane >> 4;
         float v = 0.0f;
-        if (i < kBfW2) {
-            int o = i / kMlpInPad, k = i % kMlpInPad;
-            if (k < kMlpIn) v = p[kOffW1 + o * kMlpIn + k];
-        } else if (i < kBfW3) {
-            int j = i - kBfW2;
-            v = p[kOffW2 + j];
+        if (f < kFragW2) {
+            const int k = 8 * grp + j;
+            if (k < kMlpIn) v = p[kOffW1 + (16 * f + r16) * kMlpIn + k];
+        } else if (f < kFragW3) {
+            const int mo = (f - kFragW2) >> 1, sl = (f - kFragW2) & 1;
+            v = p[kOffW2 + (16 * mo + r16) * kMlpHidden + mlp_hid(sl, grp, j)];
         } else {
-            int j = i - kBfW3;
-            int o = j / kMlpHidden, k = j % kMlpHidden;
-            if (o < kMlpOut) v = p[kOffW3 + o * kMlpHidden + k];
+            const int sl = f - kFragW3;
+            if (r16 < kMlpOut) v = p[kOffW3 + r16 * kMlpHidden + mlp_hid(sl, grp, j)];
         }
         w[i] = (__bf16)v;
     }
-    if (i < 144) {
+    if (i < kBiasTotal) {
         float v = 0.0f;
         if (i < 64) v = p[kOffB1 + i];
         else if (i < 128) v = p[kOffB2 + i - 64];
@@ -227,19 +231,20 @@ __global__ void k_pack_weights(const float* __restrict__ p, __bf16* __restrict__
 __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restrict__ x, uint32_t n,
                                                           const __bf16* __restrict__ wg,
                                                           const float* __restrict__ bg, float* __restrict__ y) {
-    __shared__ __bf16 s_w[kBfTotal];
-    __shared__ float s_b[144];
-    __shared__ __bf16 s_x[4][64 * kMlpInPad];
-    __shared__ __bf16 s_h[4][64 * kMlpHidden];
-    __shared__ float s_y[4][64 * 4];
-    for (int i = threadIdx.x; i < kBfTotal; i += blockDim.x) s_w[i] = wg[i];
-    for (int i = threadIdx.x; i < 144; i += blockDim.x) s_b[i] = bg[i];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
+    __shared__ __attribute__((aligned(16))) float s_y[4][64 * 4];
     const int w = threadIdx.x >> 6, lane = (int)lane_id();
-    const uint32_t row = (blockIdx.x * 4 + w) * 64 + lane;
-    for (int k = 0; k < kMlpInPad; k++)
-        s_x[w][lane * kMlpInPad + k] = (__bf16)((row < n && k < kMlpIn) ? x[(size_t)row * kMlpIn + k] : 0.0f);
-    __syncthreads();
-    mlp_wave_forward(s_x[w], s_h[w], s_w, s_b, s_y[w]);
+    MlpRegs R;
+    mlp_load(R, wg, bg);
+    const uint32_t base = (blockIdx.x * 4 + w) * 64;
+    if (base >= n) return;
+    const uint32_t row = base + lane;
+    float f[kMlpIn];
+    for (int k = 0; k < kMlpIn; k++) f[k] = row < n ? x[(size_t)row * kMlpIn + k] : 0.0f;
+    mlp_stage_features(s_x[w], f);
+    const uint32_t live = n - base;   // rows of this wave in range
+    const uint32_t blocks = live >= 64 ? 0xFu : (1u << ((live + 15) / 16)) - 1u;
+    mlp_wave_forward(R, s_x[w], s_y[w], blocks);
     if (row < n)
         for (int j = 0; j < kMlpOut; j++) y[(size_t)row * 3 + j] = s_y[w][lane * 4 + j];
 }
@@ -248,16 +253,18 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restric
 static inline uint32_t cdiv2(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, float theta, float theta_plain, float soft, uint32_t stride,
-                          uint2* pairs, uint32_t cap, uint32_t* count) {
+                          const WalkNode* nodes, const float4* node_aabb, float theta, float theta_plain, float soft,
+                          uint32_t stride, uint2* pairs, uint32_t cap, uint32_t* count) {
     uint32_t threads = (n + stride - 1) / stride;
-    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, theta, theta_plain, soft, stride, pairs,
-                                                        cap, count);
+    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, node_aabb, theta, theta_plain, soft,
+                                                        stride, pairs, cap, count);
 }
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
                         float soft, uint32_t max_bodies, float* feat, float* targ) {
-    if (npairs) k_pair_labels<<<npairs, 256, 0, s>>>(pairs, npairs, leaf_pos, nodes, node_aabb, node_range, G, soft,
+    // 4 waves per block, at most 2^16 blocks: the waves stride over the pairs
+    const uint32_t blocks = npairs < (1u << 18) ? cdiv2(npairs, 4) : (1u << 16);
+    if (npairs) k_pair_labels<<<blocks, 256, 0, s>>>(pairs, npairs, leaf_pos, nodes, node_aabb, node_range, G, soft,
                                                      max_bodies, feat, targ);
 }
 // rows of x (n x cols) gathered through perm[offset .. offset + rows)

@@ -196,11 +196,13 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
 
 // ---------------------------------------------------------------------------
 // The corrected walk (no reference counterpart): the same masked-stack walk
-// with a wide opening angle `theta`; an internal node accepted at `theta` but
-// one the plain angle `theta_plain` would have opened gets, besides its
-// monopole, the network's residual: a += G M / d^2 * MLP(features). The
-// wave stages the 64 lanes' features in LDS and evaluates them together with
-// BF16 MFMA (FP32 accumulation).
+// with a wide opening angle `theta`, where a node is accepted only if the
+// target also lies outside the node's box (a node that contains its target
+// has a residual the network cannot know from the node summary); an internal
+// node accepted at `theta` that the plain angle `theta_plain` would have
+// opened gets, besides its monopole, the network's residual:
+// a += G M / d^2 * MLP(features). The wave stages the 64 lanes' features in
+// LDS and evaluates them together with BF16 MFMA (FP32 accumulation).
 #include "bh_mlp.hpp"
 
 namespace bh {
@@ -217,15 +219,11 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                                                       float dt_, float softening) {
     __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
     __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
-    __shared__ __bf16 s_w[kBfTotal];
-    __shared__ float s_b[144];
-    __shared__ __bf16 s_x[kWalkWaves][64 * kMlpInPad];
-    __shared__ __bf16 s_h[kWalkWaves][64 * kMlpHidden];
-    __shared__ float s_y[kWalkWaves][64 * 4];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kWalkWaves][64 * kMlpXStride];
+    __shared__ __attribute__((aligned(16))) float s_y[kWalkWaves][64 * 4];
 
-    for (int i = threadIdx.x; i < kBfTotal; i += blockDim.x) s_w[i] = mlp_w[i];
-    for (int i = threadIdx.x; i < 144; i += blockDim.x) s_b[i] = mlp_b[i];
-    __syncthreads();
+    MlpRegs R;
+    mlp_load(R, mlp_w, mlp_b);
 
     const uint32_t n = sim->initialNumBodies;
     const uint32_t num = sim->numBodies;
@@ -260,20 +258,24 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                 const float d2 = fadd(dot3(rx, ry, rz), soft);
                 const float inv = inv_sqrt_exact(d2);
                 const float ratio = fmul(nd.half, inv);
-                const bool accept = ratio < theta;
+                const uint32_t k = node - n;
+                const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
+                const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y && pm.z >= mn.z &&
+                                    pm.z <= mx.z;
+                const bool accept = ratio < theta && !inside;
                 const bool correct = accept && ratio >= theta_plain;
                 const unsigned long long open = ballot(mine && !accept);
                 const unsigned long long corr = ballot(mine && correct);
                 float cx = 0.0f, cy = 0.0f, cz = 0.0f;
                 if (corr) {
-                    const uint32_t k = node - n;
-                    const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
                     const uint32_t cnt = node_range[k].y;
-                    float f[12];
+                    float f[kMlpIn];
                     mlp_features(rx, ry, rz, inv, soft, nd.comMass.w, cnt, mn, mx, nd.comMass, f);
-                    for (int c = 0; c < kMlpInPad; c++) s_x[w][lane * kMlpInPad + c] = (__bf16)(c < kMlpIn ? f[c] : 0.0f);
-                    __builtin_amdgcn_wave_barrier();
-                    mlp_wave_forward(s_x[w], s_h[w], s_w, s_b, s_y[w]);
+                    mlp_stage_features(s_x[w], f);
+                    // 16-sample blocks holding at least one lane to correct
+                    uint32_t blocks = 0;
+                    for (int b = 0; b < 4; b++) blocks |= ((corr >> (16 * b)) & 0xFFFFull) ? (1u << b) : 0u;
+                    mlp_wave_forward(R, s_x[w], s_y[w], blocks);
                     cx = s_y[w][lane * 4 + 0];
                     cy = s_y[w][lane * 4 + 1];
                     cz = s_y[w][lane * 4 + 2];

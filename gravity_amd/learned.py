@@ -129,16 +129,24 @@ class Mlp:
 def collect_training_pairs(bh: B.BarnesHut, theta: float, theta_plain: float, stride: int, softening: float,
                            max_node_bodies: int = 1 << 20, cap: int = 1 << 22, with_pairs: bool = False):
     """Features and FP64-labelled targets of the pairs on bh's current tree
-    (and the (sorted body, internal node) pairs). Pairs whose node is too large
-    to label are dropped."""
+    (and the (sorted body, internal node) pairs). When the walkers at
+    `stride` find more than `cap` pairs, the stride is raised until they fit,
+    so the sample always spans the whole sorted order. Pairs whose node is
+    too large to label are dropped."""
     L = _lib()
     f = np.zeros((cap, N_FEATURES), np.float32)
     t = np.zeros((cap, N_OUT), np.float32)
     pr = np.zeros((cap, 2), np.uint32)
     n = C.c_uint32()
-    rc = L.bh_collect_training_pairs(bh.ctx, theta, theta_plain, stride, softening, max_node_bodies, cap, _ptr(f),
-                                     _ptr(t), _ptr(pr), C.byref(n))
-    bh._check(rc, "collect_training_pairs")
+    for _ in range(8):
+        rc = L.bh_collect_training_pairs(bh.ctx, theta, theta_plain, stride, softening, max_node_bodies, cap,
+                                         _ptr(f), _ptr(t), _ptr(pr), C.byref(n))
+        bh._check(rc, "collect_training_pairs")
+        if n.value <= cap:
+            break
+        stride = int(np.ceil(stride * n.value / cap * 1.05))
+    else:
+        raise RuntimeError("collect_training_pairs: pairs do not fit in cap")
     f, t, pr = f[: n.value], t[: n.value], pr[: n.value]
     keep = np.isfinite(t).all(axis=1) & np.isfinite(f).all(axis=1)
     if with_pairs:

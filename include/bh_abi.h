@@ -66,8 +66,9 @@ extern "C" {
 /* bh_params.flags */
 #define BH_FLAG_COLORS 1u      /* compute leaf star colours and node avgColor (render interop) */
 #define BH_FLAG_TIMING 2u      /* record per-stage HIP-event timings */
-#define BH_FLAG_LEARNED 4u     /* corrected walk: the attached network corrects nodes accepted at theta
-                                  that theta_plain would open (no reference counterpart) */
+#define BH_FLAG_LEARNED 4u     /* corrected walk: nodes are accepted at theta only with the target outside
+                                  their box; the attached network corrects those that theta_plain would
+                                  open (no reference counterpart) */
 
 #define BH_WALK_STACK_DEPTH 192
 
@@ -188,11 +189,14 @@ int bh_mlp_loss_grad(bh_mlp *m, const float *x, const float *t, uint32_t rows, f
 int bh_mlp_train(bh_mlp *m, const float *x, const float *t, uint32_t rows, uint32_t epochs, uint32_t batch,
                  float lr, uint64_t seed, float *final_loss);
 /* On the context's current tree (after bh_build_tree): every stride-th sorted
- * body walks with angle theta; pairs with a node accepted at theta and opened
- * at theta_plain get FP32 features and FP64-labelled targets (nodes holding
- * more than max_node_bodies bodies get NaN targets). Returns n_out <= cap.
+ * body walks with angle theta; pairs with a node accepted at theta (target
+ * outside the node's box) and opened at theta_plain get FP32 features and
+ * FP64-labelled targets (nodes holding more than max_node_bodies bodies get
+ * NaN targets). n_out = pairs found; when it exceeds cap only the first cap
+ * (in no particular order) are written, and a caller that wants an unbiased
+ * sample raises stride and calls again.
  * Features: extents / d (3), (COM - box centre) / d (3), r / d (3),
- * ln(mass) / 16, ln(bodies in node) / 16, softening / d^2, with
+ * ln(mass) / 16, 1 / sqrt(bodies in node), softening / d^2, with
  * d^2 = |r|^2 + softening and r = COM - target. */
 int bh_collect_training_pairs(bh_context *ctx, float theta, float theta_plain, uint32_t stride, float softening,
                               uint32_t max_node_bodies, uint32_t cap, float *features, float *targets,

@@ -401,7 +401,7 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
                      const bho_node *leaf, const bho_node *internal, const bho_params *p,
                      float G, float blc, const float *bounds, uint32_t *stack, int stack_cap,
                      float *accel_out, uint32_t *inter_out, uint32_t *tasks, uint32_t task_cap,
-                     uint32_t *n_tasks, bho_walk_stats *st) {
+                     uint32_t *n_tasks, bho_walk_stats *st, int box_mac) {
     uint32_t bodyIdx = index[g];
     float body[8];
     memcpy(body, src + (size_t)bodyIdx * 8, 32);
@@ -419,7 +419,9 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
             tests++;
             float ex = nd->aabb[3] - nd->aabb[0], ey = nd->aabb[4] - nd->aabb[1], ez = nd->aabb[5] - nd->aabb[2];
             float longest = gmax(ex, gmax(ey, ez)) / 2.0f;
-            if (longest * inv < p->theta) {
+            int inside = box_mac && body[0] >= nd->aabb[0] && body[0] <= nd->aabb[3] && body[1] >= nd->aabb[1] &&
+                         body[1] <= nd->aabb[4] && body[2] >= nd->aabb[2] && body[2] <= nd->aabb[5];
+            if (longest * inv < p->theta && !inside) {
                 float gm = G * nd->comMass[3];
                 for (int k = 0; k < 3; k++) acc[k] = acc[k] + gm * r[k] * inv * inv * inv;
                 inter++;
@@ -513,11 +515,11 @@ int bho_force(const float *src, float *dst, const uint32_t *index, uint32_t num,
                            interactions, tasks, task_cap, st, threads, 0, num);
 }
 
-int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
-                    const bho_node *leaf, const bho_node *internal, const bho_params *p,
-                    float G, float blc, const float *bounds, int stack_cap,
-                    float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
-                    bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
+static int force_impl(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                      const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                      float G, float blc, const float *bounds, int stack_cap,
+                      float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
+                      bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end, int box_mac) {
     memset(st, 0, sizeof *st);
     if (g_end > num) g_end = num;
     if (stack_cap < 2) stack_cap = 2;
@@ -547,13 +549,33 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
 #pragma omp for schedule(dynamic, 256)
         for (uint32_t g = g_begin; g < g_end; g++)
             walk_one(g, src, dst, index, n_init, leaf, internal, p, G, blc, bounds, stack, stack_cap,
-                     accel, interactions, tasks, task_cap, &nt, st);
+                     accel, interactions, tasks, task_cap, &nt, st, box_mac);
         free(stack);
     }
     st->n_tasks = nt < task_cap ? nt : task_cap;
     st->task_overflow = nt > task_cap ? nt - task_cap : 0;
     if (tasks && st->n_tasks > 1) qsort(tasks, st->n_tasks, 8, cmp_pair);
     return st->max_stack == 0xFFFFFFFFu ? -1 : 0;
+}
+
+int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                    const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                    float G, float blc, const float *bounds, int stack_cap,
+                    float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
+                    bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
+    return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
+                      interactions, tasks, task_cap, st, threads, g_begin, g_end, 0);
+}
+
+/* The walk with the box-aware acceptance test of the learned-correction
+ * method (no reference counterpart): an internal node is accepted only if
+ * half its longest side * inv < theta AND the target lies outside its AABB. */
+int bho_force_boxmac(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                     const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                     float G, float blc, const float *bounds, int stack_cap,
+                     float *accel, uint32_t *interactions, bho_walk_stats *st, int threads) {
+    return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
+                      interactions, NULL, 0, st, threads, 0, num, 1);
 }
 
 /* ------------------------------------------------------------ merge */

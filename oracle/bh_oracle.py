@@ -96,6 +96,9 @@ def _declare(L: C.CDLL) -> None:
     L.bho_force.restype = C.c_int
     L.bho_force_range.argtypes = L.bho_force.argtypes + [U32, U32]
     L.bho_force_range.restype = C.c_int
+    L.bho_force_boxmac.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int,
+                                   P, P, C.POINTER(WalkStats), C.c_int]
+    L.bho_force_boxmac.restype = C.c_int
     L.bho_merge.argtypes = [P, P, U32]
     L.bho_merge.restype = U32
     L.bho_merge_pair.argtypes = [P, P, P]
@@ -189,6 +192,18 @@ def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, 
                          float(G), float(blc), _p(b), stack_cap, _p(accel), _p(inter), _p(tasks), task_cap,
                          C.byref(st), threads)
     return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
+
+
+def force_boxmac(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, stack_cap=256, threads=0):
+    """Accelerations of the walk whose acceptance also requires the target to
+    lie outside the node's box (the learned-correction method's walk)."""
+    accel = np.zeros((n_init, 3), np.float32)
+    inter = np.zeros(n_init, np.uint32)
+    st = WalkStats()
+    b = np.ascontiguousarray(bounds, dtype=np.float32)
+    rc = lib().bho_force_boxmac(_p(src), None, _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm),
+                                float(G), float(blc), _p(b), stack_cap, _p(accel), _p(inter), C.byref(st), threads)
+    return {"rc": rc, "accel": accel, "interactions": inter, "stats": st}
 
 
 def force_range(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, g_begin, g_end,

@@ -110,17 +110,20 @@ def test_pair_labels_match_cpu_fp64():
     bh.close()
 
 
-def test_corrected_walk_with_zero_network_equals_plain_walk():
+def test_corrected_walk_with_zero_network_equals_box_aware_oracle_walk():
+    """With a zero network the corrected walk is the box-aware walk: bit for
+    bit the oracle's walk that opens every node containing its target."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
     from gravity_amd import learned as Lr
     print("[gpu] zero-network corrected walk", flush=True)
     n = 1 << 16
+    soft = 0.01
     bodies = ic.plummer(n, 5)
-    prm = B.make_params(theta=0.9, softening=0.01)
-    bh = B.BarnesHut(bodies, ic.tight_bounds(bodies), prm)
+    bounds = ic.tight_bounds(bodies)
+    prm = B.make_params(theta=0.9, softening=soft, G=1.0)
+    bh = B.BarnesHut(bodies, bounds, prm)
     bh.init()
-    plain = bh.accelerations()
     m = Lr.Mlp(params=np.zeros(Lr.num_params(), np.float32))
     Lr.attach(bh, m, 0.5)
     lp = B.Params.from_buffer_copy(prm)
@@ -128,6 +131,57 @@ def test_corrected_walk_with_zero_network_equals_plain_walk():
     corr = bh.accelerations(lp)
     c = bh.counters()
     assert c["corrected"] > 0
-    np.testing.assert_array_equal(np.abs(corr).view(np.uint32), np.abs(plain).view(np.uint32))
+    idx = bh.index()
+    leaf = bh.nodes(B.LEAF_NODES)
+    internal = bh.nodes(B.INTERNAL_NODES)
+    r = O.force_boxmac(bodies, idx, n, n, leaf, internal, O.params(theta=0.9, dt=1.0, softening=soft), 1.0, 1.0,
+                       bh.bounds(), threads=16)
+    assert r["rc"] == 0
+    assert c["interactions"] == int(r["interactions"].astype(np.int64).sum())
+    np.testing.assert_array_equal(corr[idx].view(np.uint32), r["accel"].view(np.uint32))
     bh.close()
     m.close()
+
+
+def test_trained_network_reduces_force_error():
+    """What the trained network gains: on a Plummer sphere not seen in
+    training, the corrected walk at theta 0.9 has a lower p50 and p99 force
+    error against the FP64 direct sum than the same walk without it."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    from gravity_amd import learned as Lr
+    print("[gpu] trained network gain", flush=True)
+    soft = 0.01
+    xs, ts = [], []
+    for seed in (21, 22):
+        b = ic.plummer(1 << 17, seed)
+        bh = B.BarnesHut(b, ic.tight_bounds(b), B.make_params(theta=0.9, softening=soft, G=1.0))
+        bh.init()
+        bh.build_current_tree()
+        f, t = Lr.collect_training_pairs(bh, 0.9, 0.5, 3, soft)
+        xs.append(f)
+        ts.append(t)
+        bh.close()
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    m = Lr.Mlp(params=Lr.init_params(0))
+    m.train(x, t, epochs=6, batch=4096, lr=2e-3, seed=0)
+    n = 1 << 17
+    b = ic.plummer(n, 23)
+    bh = B.BarnesHut(b, ic.tight_bounds(b), B.make_params(theta=0.9, softening=soft, G=1.0))
+    bh.init()
+    sample = np.random.default_rng(0).choice(n, 512, replace=False).astype(np.uint32)
+    exact = bh.direct_sum_f64(sample, soft)
+    zero = Lr.Mlp(params=np.zeros(Lr.num_params(), np.float32))
+    errs = {}
+    for name, net in (("without", zero), ("with", m)):
+        Lr.attach(bh, net, 0.5)
+        lp = B.make_params(theta=0.9, softening=soft, G=1.0, flags=B.FLAG_LEARNED)
+        a = bh.accelerations(lp)[sample].astype(np.float64)
+        e = np.linalg.norm(a - exact, axis=1) / np.linalg.norm(exact, axis=1)
+        errs[name] = (np.percentile(e, 50), np.percentile(e, 99))
+    print("[gpu] force error p50/p99 without", errs["without"], "with", errs["with"], flush=True)
+    assert errs["with"][0] < 0.7 * errs["without"][0]
+    assert errs["with"][1] < errs["without"][1]
+    bh.close()
+    m.close()
+    zero.close()
