@@ -181,6 +181,17 @@ uint32_t mlp_backward_fp32(bh_mlp* m, uint32_t rows) {
     return nparts;
 }
 
+// stage events are created on the first timed step
+int ensure_events(bh_context* c) {
+    if (c->ev) return BH_OK;
+    const int total = bh_context::kTimedSteps * (BH_STAGE_COUNT + 1);
+    c->ev = new (std::nothrow) hipEvent_t[total]();
+    if (!c->ev) return fail(c, BH_ERR_OUT_OF_MEMORY, "events");
+    for (int k = 0; k < total; k++)
+        HIPCHECK(c, hipEventCreate(&c->ev[k]));
+    return BH_OK;
+}
+
 void stage_begin(bh_context* c, int k) {
     if (c->timing && c->timed < bh_context::kTimedSteps)
         (void)hipEventRecord(c->ev[c->timed * (BH_STAGE_COUNT + 1) + k], c->stream);
@@ -241,12 +252,6 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->reserve, (size_t)n));
         TRY(dalloc(c, &c->task_done, (size_t)n));
         TRY(dalloc(c, &c->staging_nodes, (size_t)n));
-        c->ev = new (std::nothrow) hipEvent_t[bh_context::kTimedSteps * (BH_STAGE_COUNT + 1)]();
-        if (!c->ev) { rc = fail(c, BH_ERR_OUT_OF_MEMORY, "events"); goto bad; }
-        for (int k = 0; k < bh_context::kTimedSteps * (BH_STAGE_COUNT + 1); k++) {
-            e = hipEventCreate(&c->ev[k]);
-            if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e)); goto bad; }
-        }
         // zero everything the reference zero-initialises, then pack sim
         // (gpu/GPU.java:1096-1143)
         for (int k = 0; k < 2; k++) {
@@ -528,6 +533,7 @@ int bh_step(bh_context* c, const bh_params* p) {
     HIPCHECK(c, hipSetDevice(c->device));
     c->timing = (p->flags & BH_FLAG_TIMING) != 0;
     int rc;
+    if (c->timing && (rc = ensure_events(c))) return rc;
     stage_begin(c, 0);
     if ((rc = bh_reset_values(c, p))) return rc;
     stage_begin(c, 1);

@@ -175,30 +175,32 @@ __device__ __forceinline__ void window_axis(float pmin, float pmax, float& lo, f
     hi = fadd(center, half);
 }
 
-__global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                                const float4* __restrict__ src,
-                                                const uint32_t* __restrict__ index_in,
-                                                float* __restrict__ partials) {
-    __shared__ float s_r[6][4];
-    __shared__ bool s_last;
-    const uint32_t num = sim->numBodies;
-    const float p0 = sim->minCorner[0], p1 = sim->minCorner[1], p2 = sim->minCorner[2];
-    const float q0 = sim->maxCorner[0], q1 = sim->maxCorner[1], q2 = sim->maxCorner[2];
-    const bool prev_empty = p0 > q0;
-    float lo0, hi0, lo1, hi1, lo2, hi2;
-    window_axis(p0, q0, lo0, hi0);
-    window_axis(p1, q1, lo1, hi1);
-    window_axis(p2, q2, lo2, hi2);
-    float r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < num; g += gridDim.x * blockDim.x) {
-        float4 pm = src[2 * index_in[g]];
-        if (pm.w == 0.0f) continue;                      // isEmpty
-        if (!finite3(pm.x, pm.y, pm.z)) continue;        // mortonContributorAABB :61-63
-        if (!prev_empty && (pm.x < lo0 || pm.y < lo1 || pm.z < lo2 || pm.x > hi0 || pm.y > hi1 || pm.z > hi2))
-            continue;
-        r[0] = gmin(r[0], pm.x); r[1] = gmin(r[1], pm.y); r[2] = gmin(r[2], pm.z);
-        r[3] = gmax(r[3], pm.x); r[4] = gmax(r[4], pm.y); r[5] = gmax(r[5], pm.z);
-    }
+// Inlier window of the previous bounds (bh_morton.comp:50-63): lo/hi per axis.
+struct BoundsWindow {
+    float lo[3], hi[3];
+    bool prev_empty;
+};
+
+__device__ __forceinline__ BoundsWindow bounds_window(const float* pmin, const float* pmax) {
+    BoundsWindow W;
+    W.prev_empty = pmin[0] > pmax[0];
+    for (int c = 0; c < 3; c++) window_axis(pmin[c], pmax[c], W.lo[c], W.hi[c]);
+    return W;
+}
+
+// fold one body into the running min (r[0..2]) / max (r[3..5])
+__device__ __forceinline__ void bounds_accum(const BoundsWindow& W, float4 pm, float* r) {
+    if (pm.w == 0.0f) return;                          // isEmpty
+    if (!finite3(pm.x, pm.y, pm.z)) return;            // mortonContributorAABB :61-63
+    if (!W.prev_empty && (pm.x < W.lo[0] || pm.y < W.lo[1] || pm.z < W.lo[2] || pm.x > W.hi[0] ||
+                          pm.y > W.hi[1] || pm.z > W.hi[2]))
+        return;
+    r[0] = gmin(r[0], pm.x); r[1] = gmin(r[1], pm.y); r[2] = gmin(r[2], pm.z);
+    r[3] = gmax(r[3], pm.x); r[4] = gmax(r[4], pm.y); r[5] = gmax(r[5], pm.z);
+}
+
+// min/max over the 256 threads of a block; the result is valid in thread 0
+__device__ __forceinline__ void bounds_block_reduce(float* r, float (*s_r)[4]) {
     for (int off = 32; off > 0; off >>= 1) {
         for (int k = 0; k < 3; k++) r[k] = gmin(r[k], __shfl_xor(r[k], off, 64));
         for (int k = 3; k < 6; k++) r[k] = gmax(r[k], __shfl_xor(r[k], off, 64));
@@ -208,11 +210,53 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
         for (int k = 0; k < 6; k++) s_r[k][w] = r[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-        float b[6];
-        for (int k = 0; k < 3; k++) b[k] = gmin(gmin(s_r[k][0], s_r[k][1]), gmin(s_r[k][2], s_r[k][3]));
-        for (int k = 3; k < 6; k++) b[k] = gmax(gmax(s_r[k][0], s_r[k][1]), gmax(s_r[k][2], s_r[k][3]));
+        for (int k = 0; k < 3; k++) r[k] = gmin(gmin(s_r[k][0], s_r[k][1]), gmin(s_r[k][2], s_r[k][3]));
+        for (int k = 3; k < 6; k++) r[k] = gmax(gmax(s_r[k][0], s_r[k][1]), gmax(s_r[k][2], s_r[k][3]));
+    }
+}
+
+// New bounds from the reduced inliers res and the previous bounds
+// (bh_morton.comp:77-88, 205-231).
+__device__ __forceinline__ void bounds_collapse(const float* prev, const float* res, float* mn, float* mx) {
+    const bool prev_empty = prev[0] > prev[3];
+    if (res[0] > res[3]) {                               // no finite inliers (:221-226)
+        if (prev_empty)
+            for (int c = 0; c < 3; c++) { mn[c] = res[c]; mx[c] = res[3 + c]; }
+        return;
+    }
+    if (prev_empty) {
+        for (int c = 0; c < 3; c++) { mn[c] = res[c]; mx[c] = res[3 + c]; }
+        return;
+    }
+    for (int c = 0; c < 3; c++) {                        // clampAABBExtentGrowth :77-88
+        float prevExt = gmax(fsub(prev[3 + c], prev[c]), 1e-9f);
+        float maxExt = fmul(prevExt, 1.1f);
+        float ctr_c = fmul(0.5f, fadd(res[c], res[3 + c]));
+        float h = fmul(0.5f, fsub(res[3 + c], res[c]));
+        float mh = fmul(0.5f, maxExt);
+        h = gmin(h, mh);
+        mn[c] = fsub(ctr_c, h);
+        mx[c] = fadd(ctr_c, h);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                const float4* __restrict__ src,
+                                                const uint32_t* __restrict__ index_in,
+                                                float* __restrict__ partials) {
+    __shared__ float s_r[6][4];
+    __shared__ bool s_last;
+    const uint32_t num = sim->numBodies;
+    const float prev[6] = {sim->minCorner[0], sim->minCorner[1], sim->minCorner[2],
+                           sim->maxCorner[0], sim->maxCorner[1], sim->maxCorner[2]};
+    const BoundsWindow W = bounds_window(prev, prev + 3);
+    float r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < num; g += gridDim.x * blockDim.x)
+        bounds_accum(W, src[2 * index_in[g]], r);
+    bounds_block_reduce(r, s_r);
+    if (threadIdx.x == 0) {
         for (int k = 0; k < 6; k++)
-            __hip_atomic_store(&partials[6 * blockIdx.x + k], b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&partials[6 * blockIdx.x + k], r[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         uint32_t ticket = __hip_atomic_fetch_add(&ctr->aabbTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = ticket == gridDim.x - 1;
@@ -228,26 +272,7 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
             res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
     ctr->aabbTicket = 0;
-    float prev[6] = {p0, p1, p2, q0, q1, q2};
-    if (res[0] > res[3]) {                               // no finite inliers (:221-226)
-        if (prev_empty)
-            for (int c = 0; c < 3; c++) { sim->minCorner[c] = res[c]; sim->maxCorner[c] = res[3 + c]; }
-        return;
-    }
-    if (prev_empty) {
-        for (int c = 0; c < 3; c++) { sim->minCorner[c] = res[c]; sim->maxCorner[c] = res[3 + c]; }
-        return;
-    }
-    for (int c = 0; c < 3; c++) {                        // clampAABBExtentGrowth :77-88
-        float prevExt = gmax(fsub(prev[3 + c], prev[c]), 1e-9f);
-        float maxExt = fmul(prevExt, 1.1f);
-        float ctr_c = fmul(0.5f, fadd(res[c], res[3 + c]));
-        float h = fmul(0.5f, fsub(res[3 + c], res[c]));
-        float mh = fmul(0.5f, maxExt);
-        h = gmin(h, mh);
-        sim->minCorner[c] = fsub(ctr_c, h);
-        sim->maxCorner[c] = fadd(ctr_c, h);
-    }
+    bounds_collapse(prev, res, sim->minCorner, sim->maxCorner);
 }
 
 // ------------------------------------------------------------------ Morton
@@ -464,61 +489,36 @@ __device__ __forceinline__ void get_state(const float4* x, NodeState& s, int col
 // numbering is Karras's: an internal node covering [lo, hi] is numbered hi
 // when it is its parent's left child and lo when it is the right child or the
 // root, which is exactly the node index treeBuildBinaryRadixTreeKernel assigns.
-__global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                              const float4* __restrict__ src,
-                                              const uint32_t* __restrict__ index_in,
-                                              const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
-    const uint32_t n = sim->initialNumBodies;
-    const int num = (int)sim->numBodies;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num) return;
-    const uint32_t b = index_in[i];
-    const float4 pm = src[2 * b];
-    NodeState me;
-    me.com = pm;
-    me.mn = me.mx = make_float4(pm.x, pm.y, pm.z, 0.f);
-    me.height = 0;
-    me.id = (uint32_t)i;
-    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
-    T.leaf_pos[i] = pm;
-    if (colors) {
-        float4 vd = src[2 * b + 1];
-        if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
-        T.leaf_color[i] = me.col;
-    }
-    if (num == 1) {
-        // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
-        WalkNode r;
-        r.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
-        r.half = -1e38f;
-        r.childA = 0;
-        r.childB = kEmpty;
-        r.height = 0;
-        T.nodes[0] = r;
-        T.node_aabb[0] = make_float4(1e38f, 1e38f, 1e38f, 0.f);
-        T.node_aabb[1] = make_float4(-1e38f, -1e38f, -1e38f, 0.f);
-        T.node_parent[0] = kEmpty;
-        T.node_range[0] = make_uint2(0, 0);
-        T.leaf_parent[0] = n;
-        ctr->treeHeight = 0;
-        return;
-    }
+//
+// tree_climb builds the tree of one segment of the sorted order: positions
+// [off, off + num) (local positions 0..num-1, keys given from the segment's
+// first key), internal nodes stored at ibase + local index, leaf ids
+// off + local, internal ids n_ids + ibase + local. A single simulation is the
+// segment off = 0, ibase = 0, n_ids = n; an ensemble has one per simulation.
+struct TreeSeg {
+    uint32_t off, num, ibase, n_ids;
+};
+
+__device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me, const uint64_t* __restrict__ keys,
+                                           const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
+                                           int colors) {
+    const int num = (int)S.num;
     int lo = i, hi = i;
     for (;;) {
         if (lo == 0 && hi == num - 1) break;
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
-        float4* x = T.xchg + 8 * (size_t)split;
+        float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
         put_state(x + (left ? 0 : 4), me, colors);
         stores_done();
-        const uint32_t other = __hip_atomic_exchange(&T.meet[split], (uint32_t)(left ? lo : hi), __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t other = __hip_atomic_exchange(&T.meet[S.ibase + split], (uint32_t)(left ? lo : hi),
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (other == kEmpty) return;                      // first to arrive: the sibling continues
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         NodeState sib;
         get_state(x + (left ? 4 : 0), sib, colors);
-        T.meet[split] = kEmpty;                           // only this thread touches the slot now
+        T.meet[S.ibase + split] = kEmpty;                 // only this thread touches the slot now
         const int plo = left ? lo : (int)other;
         const int phi = left ? (int)other : hi;
         uint32_t pid;
@@ -552,7 +552,8 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
         p.mn = make_float4(gmin(A.mn.x, B.mn.x), gmin(A.mn.y, B.mn.y), gmin(A.mn.z, B.mn.z), 0.f);
         p.mx = make_float4(gmax(A.mx.x, B.mx.x), gmax(A.mx.y, B.mx.y), gmax(A.mx.z, B.mx.z), 0.f);
         p.height = 1u + (A.height > B.height ? A.height : B.height);
-        p.id = n + pid;
+        const uint32_t k = S.ibase + pid;                 // storage index of the parent
+        p.id = S.n_ids + k;
         WalkNode nd;
         nd.comMass = p.com;
         const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
@@ -561,21 +562,70 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
         nd.childB = B.id;
         nd.height = p.height;
         // these arrays are read only by later kernels
-        T.nodes[pid] = nd;
-        T.node_aabb[2 * pid] = p.mn;
-        T.node_aabb[2 * pid + 1] = p.mx;
-        if (colors) T.node_color[pid] = p.col;
-        T.node_range[pid] = make_uint2((uint32_t)plo, (uint32_t)(phi - plo + 1));
-        if (A.id < n) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - n] = p.id;
-        if (B.id < n) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - n] = p.id;
+        T.nodes[k] = nd;
+        T.node_aabb[2 * k] = p.mn;
+        T.node_aabb[2 * k + 1] = p.mx;
+        if (colors) T.node_color[k] = p.col;
+        T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
+        if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
+        if (B.id < S.n_ids) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - S.n_ids] = p.id;
         if (pid == 0) {
-            T.node_parent[0] = kEmpty;
-            ctr->treeHeight = p.height;
+            T.node_parent[k] = kEmpty;
+            if (height_max) atomicMax(height_out, p.height);
+            else *height_out = p.height;
         }
         me = p;
         lo = plo;
         hi = phi;
     }
+}
+
+__device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
+    NodeState me;
+    me.com = pm;
+    me.mn = me.mx = make_float4(pm.x, pm.y, pm.z, 0.f);
+    me.height = 0;
+    me.id = id;
+    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    return me;
+}
+
+__global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                              const float4* __restrict__ src,
+                                              const uint32_t* __restrict__ index_in,
+                                              const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
+    const uint32_t n = sim->initialNumBodies;
+    const int num = (int)sim->numBodies;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint32_t b = index_in[i];
+    const float4 pm = src[2 * b];
+    NodeState me = leaf_state(pm, (uint32_t)i);
+    T.leaf_pos[i] = pm;
+    if (colors) {
+        float4 vd = src[2 * b + 1];
+        if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
+        T.leaf_color[i] = me.col;
+    }
+    if (num == 1) {
+        // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
+        WalkNode r;
+        r.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
+        r.half = -1e38f;
+        r.childA = 0;
+        r.childB = kEmpty;
+        r.height = 0;
+        T.nodes[0] = r;
+        T.node_aabb[0] = make_float4(1e38f, 1e38f, 1e38f, 0.f);
+        T.node_aabb[1] = make_float4(-1e38f, -1e38f, -1e38f, 0.f);
+        T.node_parent[0] = kEmpty;
+        T.node_range[0] = make_uint2(0, 0);
+        T.leaf_parent[0] = n;
+        ctr->treeHeight = 0;
+        return;
+    }
+    const TreeSeg S = {0u, (uint32_t)num, 0u, n};
+    tree_climb(i, S, me, keys, T, &ctr->treeHeight, false, colors);
 }
 
 // ------------------------------------------------------------------ merge
@@ -859,5 +909,252 @@ __global__ void k_selftest_math(uint32_t* __restrict__ bad) {
 }
 
 void launch_selftest_math(hipStream_t s, uint32_t* bad) { k_selftest_math<<<4096, 256, 0, s>>>(bad); }
+
+}  // namespace bh
+
+// ====================================================================== ensembles
+// Independent simulations concatenated in one set of arrays (no reference
+// counterpart: the reference runs one simulation). Every stage runs over all
+// simulations in one launch; simulation s touches only its own segment of
+// positions, body ids and internal nodes, so each simulation's bounds, codes,
+// permutation, tree and accelerations are those of a run of it alone.
+namespace bh {
+
+__global__ void k_ens_reset(DevCounters* __restrict__ ctr, uint32_t* __restrict__ queue) {
+    if (threadIdx.x != 0) return;
+    ctr->opened = 0;
+    ctr->interactions = 0;
+    ctr->nTasks = 0;
+    ctr->mergesApplied = 0;
+    ctr->maxStack = 0;
+    ctr->errorFlags = 0;
+    ctr->corrected = 0;
+    ctr->treeHeight = 0;
+    queue[0] = 0;
+}
+
+// One block per tile; the last tile of a simulation to finish (ticket)
+// reduces that simulation's tile partials and updates its bounds.
+__global__ __launch_bounds__(256) void k_ens_bounds(EnsSim* __restrict__ sims, const EnsTile* __restrict__ tiles,
+                                                    const float4* __restrict__ src, float* __restrict__ partials) {
+    __shared__ float s_r[6][4];
+    __shared__ bool s_last;
+    const EnsTile tl = tiles[blockIdx.x];
+    EnsSim* S = sims + tl.sim;
+    const float prev[6] = {S->bmin[0], S->bmin[1], S->bmin[2], S->bmax[0], S->bmax[1], S->bmax[2]};
+    const BoundsWindow W = bounds_window(prev, prev + 3);
+    float r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    // the bodies of a simulation are its ids [off, off + num): read them in id order
+    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x) bounds_accum(W, src[2 * (tl.start + e)], r);
+    bounds_block_reduce(r, s_r);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++)
+            __hip_atomic_store(&partials[6 * blockIdx.x + k], r[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        uint32_t ticket = __hip_atomic_fetch_add(&S->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = ticket == S->ntiles - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    for (uint32_t k = S->tile0; k < S->tile0 + S->ntiles; k++) {
+        for (int c = 0; c < 3; c++)
+            res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int c = 3; c < 6; c++)
+            res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    S->ticket = 0;
+    bounds_collapse(prev, res, S->bmin, S->bmax);
+}
+
+__global__ __launch_bounds__(256) void k_ens_encode(const EnsSim* __restrict__ sims,
+                                                    const EnsTile* __restrict__ tiles,
+                                                    const float4* __restrict__ src,
+                                                    const uint32_t* __restrict__ index_in,
+                                                    uint64_t* __restrict__ keys) {
+    const EnsTile tl = tiles[blockIdx.x];
+    const EnsSim* S = sims + tl.sim;
+    const float bmin[3] = {S->bmin[0], S->bmin[1], S->bmin[2]};
+    const float bmax[3] = {S->bmax[0], S->bmax[1], S->bmax[2]};
+    const bool empty = bmin[0] > bmax[0];
+    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x) {
+        const uint32_t g = tl.start + e;
+        keys[g] = morton_of(src[2 * index_in[g]], bmin, bmax, empty);
+    }
+}
+
+// Segmented LSD radix sort: digit histograms per tile, then per simulation an
+// exclusive scan over (digit, tile) that starts at the simulation's first
+// position, then a stable scatter with in-wave ballot ranking. A simulation's
+// keys never leave its segment, so the result is the stable sort by
+// (simulation, Morton code) of the whole ensemble.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_ens_sort_hist(const EnsTile* __restrict__ tiles,
+                                                       const uint64_t* __restrict__ keys, uint32_t shift,
+                                                       uint32_t* __restrict__ hist) {
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t s_h[256];
+    const EnsTile tl = tiles[blockIdx.x];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x)
+        atomicAdd(&s_h[(uint32_t)(keys[tl.start + e] >> shift) & (R - 1u)], 1u);
+    __syncthreads();
+    if (threadIdx.x < R) hist[(size_t)blockIdx.x * 256 + threadIdx.x] = s_h[threadIdx.x];
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_ens_sort_scan(const EnsSim* __restrict__ sims, uint32_t* __restrict__ hist) {
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t s_w[4];
+    const EnsSim S = sims[blockIdx.x];
+    uint32_t* h = hist + (size_t)S.tile0 * 256;
+    const uint32_t d = threadIdx.x, lane = lane_id();
+    uint32_t total = 0;
+    if (d < R)
+        for (uint32_t k = 0; k < S.ntiles; k++) {
+            const uint32_t v = h[(size_t)k * 256 + d];
+            h[(size_t)k * 256 + d] = total;
+            total += v;
+        }
+    uint32_t incl = total;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += v;
+    }
+    if (lane == 63) s_w[d >> 6] = incl;
+    __syncthreads();
+    uint32_t before = S.off;
+    for (uint32_t k = 0; k < (d >> 6); k++) before += s_w[k];
+    const uint32_t base = before + incl - total;
+    if (d < R)
+        for (uint32_t k = 0; k < S.ntiles; k++) h[(size_t)k * 256 + d] += base;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restrict__ tiles,
+                                                          const uint64_t* __restrict__ keys_in,
+                                                          const uint32_t* __restrict__ idx_in,
+                                                          uint64_t* __restrict__ keys_out,
+                                                          uint32_t* __restrict__ idx_out, uint32_t shift,
+                                                          const uint32_t* __restrict__ hist) {
+    constexpr uint32_t R = 1u << BITS;
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wc[4][256];
+    const EnsTile tl = tiles[blockIdx.x];
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    s_base[t] = t < R ? hist[(size_t)blockIdx.x * 256 + t] : 0u;
+    s_run[t] = 0;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t r = 0; r < kEnsTile; r += 256) {
+        for (int k = 0; k < 4; k++) s_wc[w][lane * 4 + k] = 0;
+        const uint32_t e = r + t;
+        const bool valid = e < tl.count;
+        const uint64_t key = valid ? keys_in[tl.start + e] : 0;
+        const uint32_t idx = valid ? idx_in[tl.start + e] : 0;
+        const uint32_t d = (uint32_t)(key >> shift) & (R - 1u);
+        uint64_t peers = ballot(valid);
+        for (int bit = 0; bit < BITS; bit++) {
+            const uint64_t bb = ballot(valid && ((d >> bit) & 1u));
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = s_run[d];
+            for (uint32_t k = 0; k < w; k++) off += s_wc[k][d];
+            const uint32_t dest = s_base[d] + off + rank;
+            keys_out[dest] = key;
+            idx_out[dest] = idx;
+        }
+        __syncthreads();
+        s_run[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
+        __syncthreads();
+        if (r + 256 >= tl.count) break;
+    }
+}
+
+// Tree of every simulation in one launch: thread = global sorted position.
+__global__ __launch_bounds__(256) void k_ens_tree(const EnsSim* __restrict__ sims, uint32_t nsims, uint32_t N,
+                                                  DevCounters* __restrict__ ctr, const float4* __restrict__ src,
+                                                  const uint32_t* __restrict__ index_in,
+                                                  const uint64_t* __restrict__ keys, TreeArrays T) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t lo = 0, hi = nsims - 1;   // last simulation whose first position is <= i
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (sims[mid].off <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const EnsSim& S = sims[lo];
+    const float4 pm = src[2 * index_in[i]];
+    T.leaf_pos[i] = pm;
+    const TreeSeg seg = {S.off, S.num, S.ibase, N};
+    tree_climb((int)(i - S.off), seg, leaf_state(pm, i), keys + S.off, T, &ctr->treeHeight, true, 0);
+}
+
+__global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_internal) return;
+    NodeStd430 o = {};
+    const WalkNode w = T.nodes[k];
+    const float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
+    o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = w.comMass.w;
+    o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
+    o.childA = w.childA;
+    o.childB = w.childB;
+    o.nodeDepth = w.height;
+    o.bodiesContained = T.node_range[k].y;
+    o.readyChildren = kEmpty;
+    o.parentId = T.node_parent[k];
+    out[k] = o;
+}
+
+void launch_ens_reset(hipStream_t s, DevCounters* ctr, uint32_t* queue) { k_ens_reset<<<1, 64, 0, s>>>(ctr, queue); }
+
+void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
+                       float* partials) {
+    k_ens_bounds<<<ntiles, 256, 0, s>>>(sims, tiles, src, partials);
+}
+
+void launch_ens_encode(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
+                       const uint32_t* index_in, uint64_t* keys) {
+    k_ens_encode<<<ntiles, 256, 0, s>>>(sims, tiles, src, index_in, keys);
+}
+
+template <int BITS>
+static void ens_sort_pass(hipStream_t s, const EnsSim* sims, uint32_t nsims, const EnsTile* tiles, uint32_t ntiles,
+                          const uint64_t* keys_in, const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out,
+                          uint32_t shift, uint32_t* hist) {
+    k_ens_sort_hist<BITS><<<ntiles, 256, 0, s>>>(tiles, keys_in, shift, hist);
+    k_ens_sort_scan<BITS><<<nsims, 256, 0, s>>>(sims, hist);
+    k_ens_sort_scatter<BITS><<<ntiles, 256, 0, s>>>(tiles, keys_in, idx_in, keys_out, idx_out, shift, hist);
+}
+
+int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t nsims, const EnsTile* tiles,
+                         uint32_t ntiles, const uint64_t* keys_in, const uint32_t* idx_in, uint64_t* keys_out,
+                         uint32_t* idx_out, uint32_t shift, uint32_t* hist) {
+    switch (bits) {
+        case 5: ens_sort_pass<5>(s, sims, nsims, tiles, ntiles, keys_in, idx_in, keys_out, idx_out, shift, hist); break;
+        case 6: ens_sort_pass<6>(s, sims, nsims, tiles, ntiles, keys_in, idx_in, keys_out, idx_out, shift, hist); break;
+        case 7: ens_sort_pass<7>(s, sims, nsims, tiles, ntiles, keys_in, idx_in, keys_out, idx_out, shift, hist); break;
+        case 8: ens_sort_pass<8>(s, sims, nsims, tiles, ntiles, keys_in, idx_in, keys_out, idx_out, shift, hist); break;
+        default: return 1;
+    }
+    return 0;
+}
+
+void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t N, DevCounters* ctr,
+                     const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T) {
+    k_ens_tree<<<cdiv(N, 256), 256, 0, s>>>(sims, nsims, N, ctr, src, index_in, keys, T);
+}
+
+void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out) {
+    if (n_internal) k_ens_nodes_std430<<<cdiv(n_internal, 256), 256, 0, s>>>(T, n_internal, out);
+}
 
 }  // namespace bh

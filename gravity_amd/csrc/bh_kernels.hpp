@@ -45,6 +45,43 @@ struct WalkArgs {
     uint32_t merging, dynamic, wrap;
 };
 
+// ---- ensembles: independent simulations concatenated in one set of arrays.
+// Simulation s owns sorted positions and body ids [off, off + num) and
+// internal-node storage [ibase, ibase + num - 1) with ibase = off - s; node
+// ids are global (leaves < N, internal N + storage index).
+struct EnsSim {
+    uint32_t off, num, ibase, tile0;   // first position, bodies, first internal node, first sort tile
+    uint32_t ntiles, ticket, height, pad0;
+    float bmin[4], bmax[4];            // bounds (xyz; w unused)
+};
+static_assert(sizeof(EnsSim) == 64, "EnsSim layout");
+struct EnsTile {                       // sort tile: positions [start, start + count) of simulation sim
+    uint32_t sim, start, count, pad;
+};
+struct EnsTask {                       // walk task: up to 64 positions of one simulation
+    uint32_t sim, start, count, pad;
+};
+constexpr uint32_t kEnsTile = 2048;    // positions per sort/bounds/encode tile
+
+void launch_ens_reset(hipStream_t s, DevCounters* ctr, uint32_t* queue);
+void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
+                       float* partials);
+void launch_ens_encode(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
+                       const uint32_t* index_in, uint64_t* keys);
+// one LSD pass of `bits`-bit digits over every simulation's segment
+int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t nsims, const EnsTile* tiles,
+                         uint32_t ntiles, const uint64_t* keys_in, const uint32_t* idx_in, uint64_t* keys_out,
+                         uint32_t* idx_out, uint32_t shift, uint32_t* hist);
+void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t N, DevCounters* ctr,
+                     const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T);
+// persistent walk over the task queue; returns nonzero for an unsupported shape
+int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
+                    uint32_t ntasks, uint32_t* queue, DevCounters* ctr, const float4* src, float4* dst,
+                    const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
+                    float theta, float dt, float soft);
+int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu);
+void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out);
+
 void launch_init(hipStream_t s, uint32_t* index_in, uint32_t n);
 void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units, float mass, float density,
                   float len, float time, float G, float blc);

@@ -20,6 +20,111 @@ __device__ __forceinline__ float body_radius(float m, float rho, float blc) {
     return fmul(blc, powf(fdiv(m, rho), 1.0f / 3.0f));
 }
 
+// Accumulated results of one wave's walk.
+struct WalkAcc {
+    float ax, ay, az;
+    unsigned long long opened, inter;
+    uint32_t maxsp, err;
+};
+
+// The masked-stack walk of one wave over the tree rooted at `root`: lanes in
+// `mask` walk for the bodies at sorted positions base + lane (position g,
+// position-mass pm); leaf ids are below n_ids, internal node id k >= n_ids is
+// stored at nodes[k - n_ids]. s_node/s_mask are this wave's LDS stack of
+// kDepth entries. With merging set, leaf visits also emit merge tasks
+// (bh_force.comp:85-112).
+template <int kDepth>
+__device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsigned long long* __restrict__ s_mask,
+                                           uint32_t root, uint32_t n_ids, uint32_t base, uint32_t g,
+                                           unsigned long long mask, float4 pm, const float4* __restrict__ leaf_pos,
+                                           const WalkNode* __restrict__ nodes, float G, float theta, float soft,
+                                           bool merging, float Ri, uint32_t bodyIdx, float blc,
+                                           const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
+                                           DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
+                                           uint32_t task_cap, WalkAcc& o) {
+    const uint32_t lane = lane_id();
+    // current node in scalar registers; the stack holds the pending childA
+    // entries (after opening a node its childB, the top of the reference's
+    // stack, is visited next without a round trip through LDS)
+    uint32_t node = root;
+    int sp = 0;
+    for (;;) {
+        const bool mine = (mask >> lane) & 1ull;
+        bool descend = false;
+        if (node >= n_ids) {
+            const WalkNode nd = nodes[node - n_ids];
+            const float rx = fsub(nd.comMass.x, pm.x);
+            const float ry = fsub(nd.comMass.y, pm.y);
+            const float rz = fsub(nd.comMass.z, pm.z);
+            const float d2 = fadd(dot3(rx, ry, rz), soft);
+            const float inv = inv_sqrt_exact(d2);
+            const bool accept = fmul(nd.half, inv) < theta;
+            const unsigned long long open = ballot(mine && !accept);
+            if (mine && accept) {
+                const float gm = fmul(G, nd.comMass.w);
+                o.ax = fadd(o.ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                o.ay = fadd(o.ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                o.az = fadd(o.az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+            }
+            o.inter += (unsigned long long)__popcll(mask & ~open);
+            if (open) {
+                if (sp + 1 > kDepth) {
+                    o.err |= 1u;
+                    break;
+                }
+                if (lane == 0) {
+                    s_node[sp] = nd.childA;
+                    s_mask[sp] = open;
+                }
+                sp += 1;
+                o.opened += (unsigned long long)__popcll(open);
+                if ((uint32_t)sp + 1 > o.maxsp) o.maxsp = (uint32_t)sp + 1;
+                node = uniform(nd.childB);
+                mask = open;
+                descend = true;
+            }
+        } else {
+            const float4 lp = leaf_pos[node];
+            const bool use = mine && node != g;          // bh_force.comp:77-79 (self skip)
+            const float rx = fsub(lp.x, pm.x);
+            const float ry = fsub(lp.y, pm.y);
+            const float rz = fsub(lp.z, pm.z);
+            const float d2 = fadd(dot3(rx, ry, rz), soft);
+            const float inv = inv_sqrt_exact(d2);
+            if (use) {
+                const float gm = fmul(G, lp.w);
+                o.ax = fadd(o.ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                o.ay = fadd(o.ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                o.az = fadd(o.az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+            }
+            const unsigned long long selfbit =
+                (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
+            o.inter += (unsigned long long)__popcll(mask & ~selfbit);
+            if (merging && (mask & ~selfbit)) {
+                // bh_force.comp:85-112: overlap test against the leaf's body,
+                // merge task from the lower-index body
+                const uint32_t other = uniform(index_in[node]);
+                const float rho = src[2 * other + 1].w;
+                const float Rj = body_radius(lp.w, rho, blc);
+                const float dist = fsqrt(dot3(rx, ry, rz));
+                const float pen = fsub(fadd(Ri, Rj), dist);
+                if (use && pen > 0.0f && bodyIdx < other) {
+                    uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
+                    if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
+                    else o.err |= 2u;
+                }
+            }
+        }
+        if (!descend) {
+            if (sp == 0) break;
+            sp--;
+            node = uniform(s_node[sp]);
+            const unsigned long long mraw = s_mask[sp];
+            mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
+        }
+    }
+}
+
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
@@ -46,97 +151,16 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
     const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float G = sim->gravitationalConstant;
     const float blc = sim->bodyLengthConstant;
-    const float theta = theta_, soft = softening;
     const bool merging = merging_mode == 1u;
     const float Ri = (merging && active) ? body_radius(pm.w, vd.w, blc) : 0.0f;
 
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    unsigned long long opened = 0, inter = 0;
-    uint32_t maxsp = 1, err = 0;
-
-    if (num > 1) {
-        // current node in scalar registers; the stack holds the pending
-        // childA entries (after opening a node its childB, the top of the
-        // reference's stack, is visited next without a round trip through LDS)
-        uint32_t node = n;
-        unsigned long long mask = ballot(active);
-        int sp = 0;
-        for (;;) {
-            const bool mine = (mask >> lane) & 1ull;
-            bool descend = false;
-            if (node >= n) {
-                const WalkNode nd = nodes[node - n];
-                const float rx = fsub(nd.comMass.x, pm.x);
-                const float ry = fsub(nd.comMass.y, pm.y);
-                const float rz = fsub(nd.comMass.z, pm.z);
-                const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = inv_sqrt_exact(d2);
-                const bool accept = fmul(nd.half, inv) < theta;
-                const unsigned long long open = ballot(mine && !accept);
-                if (mine && accept) {
-                    const float gm = fmul(G, nd.comMass.w);
-                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                }
-                inter += (unsigned long long)__popcll(mask & ~open);
-                if (open) {
-                    if (sp + 1 > kStackDepth) {
-                        err |= 1u;
-                        break;
-                    }
-                    if (lane == 0) {
-                        s_node[w][sp] = nd.childA;
-                        s_mask[w][sp] = open;
-                    }
-                    sp += 1;
-                    opened += (unsigned long long)__popcll(open);
-                    if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
-                    node = uniform(nd.childB);
-                    mask = open;
-                    descend = true;
-                }
-            } else {
-                const float4 lp = leaf_pos[node];
-                const bool use = mine && node != g;          // bh_force.comp:77-79 (self skip)
-                const float rx = fsub(lp.x, pm.x);
-                const float ry = fsub(lp.y, pm.y);
-                const float rz = fsub(lp.z, pm.z);
-                const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = inv_sqrt_exact(d2);
-                if (use) {
-                    const float gm = fmul(G, lp.w);
-                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                }
-                const unsigned long long selfbit =
-                    (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
-                inter += (unsigned long long)__popcll(mask & ~selfbit);
-                if (merging && (mask & ~selfbit)) {
-                    // bh_force.comp:85-112: overlap test against the leaf's body,
-                    // merge task from the lower-index body
-                    const uint32_t other = uniform(index_in[node]);
-                    const float rho = src[2 * other + 1].w;
-                    const float Rj = body_radius(lp.w, rho, blc);
-                    const float dist = fsqrt(dot3(rx, ry, rz));
-                    const float pen = fsub(fadd(Ri, Rj), dist);
-                    if (use && pen > 0.0f && bodyIdx < other) {
-                        uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
-                        if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                        else err |= 2u;
-                    }
-                }
-            }
-            if (!descend) {
-                if (sp == 0) break;
-                sp--;
-                node = uniform(s_node[w][sp]);
-                const unsigned long long mraw = s_mask[w][sp];
-                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
-            }
-        }
-    }
+    WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
+    if (num > 1)
+        walk_chunk<kStackDepth>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, leaf_pos, nodes, G, theta_,
+                                softening, merging, Ri, bodyIdx, blc, index_in, src, ctr, tasks, task_cap, o);
+    const float ax = o.ax, ay = o.ay, az = o.az;
+    const unsigned long long opened = o.opened, inter = o.inter;
+    const uint32_t maxsp = o.maxsp, err = o.err;
 
     // kick-drift into the output buffer (bh_force.comp:118-140)
     uint32_t oob = 0;
@@ -190,6 +214,90 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     k_walk<<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in,
                                               a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,
                                               a.softening, a.merging, a.dynamic, a.wrap);
+}
+
+// Ensemble walk (no reference counterpart): a persistent grid whose waves
+// pull 64-body tasks of any simulation from one global queue until it is
+// drained; each task walks its simulation's tree (root = that simulation's
+// first internal node) with the same masked-stack walk and integrates.
+template <int kWaves, int kDepth>
+__global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restrict__ sims,
+                                                          const EnsTask* __restrict__ tasks, uint32_t ntasks,
+                                                          uint32_t* __restrict__ queue, DevCounters* __restrict__ ctr,
+                                                          const float4* __restrict__ src, float4* __restrict__ dst,
+                                                          const uint32_t* __restrict__ index_in,
+                                                          const float4* __restrict__ leaf_pos,
+                                                          const WalkNode* __restrict__ nodes, uint32_t N, float G,
+                                                          float theta, float dt, float soft) {
+    __shared__ uint32_t s_node[kWaves][kDepth];
+    __shared__ unsigned long long s_mask[kWaves][kDepth];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    unsigned long long opened = 0, inter = 0;
+    uint32_t maxsp = 1, err = 0;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1u);
+        t = uniform(__shfl(t, 0, 64));
+        if (t >= ntasks) break;   // every wave ends here once the queue is drained
+        const EnsTask tk = tasks[t];
+        const uint32_t root = N + sims[tk.sim].ibase;
+        const uint32_t base = tk.start, g = base + lane;
+        const bool active = lane < tk.count;
+        const uint32_t bodyIdx = active ? index_in[g] : 0;
+        const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
+        WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
+        walk_chunk<kDepth>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, leaf_pos, nodes, G, theta, soft,
+                           false, 0.0f, bodyIdx, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+        if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
+            const float nvx = fadd(vd.x, fmul(o.ax, dt));
+            const float nvy = fadd(vd.y, fmul(o.ay, dt));
+            const float nvz = fadd(vd.z, fmul(o.az, dt));
+            dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)),
+                                           fadd(pm.z, fmul(nvz, dt)), pm.w);
+            dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
+        }
+        opened += o.opened;
+        inter += o.inter;
+        maxsp = o.maxsp > maxsp ? o.maxsp : maxsp;
+        err |= o.err;
+    }
+    if (lane == 0) {
+        atomicAdd(&ctr->opened, opened);
+        atomicAdd(&ctr->interactions, inter);
+        atomicMax(&ctr->maxStack, maxsp);
+        if (err) atomicOr(&ctr->errorFlags, err);
+    }
+}
+
+#define BH_ENS_WALK_SHAPES(X) \
+    X(1, 64) X(1, 128) X(1, 192) X(2, 64) X(2, 128) X(2, 192) X(4, 64) X(4, 128) X(4, 192) X(8, 64) X(8, 128) X(8, 192)
+
+int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
+                    uint32_t ntasks, uint32_t* queue, DevCounters* ctr, const float4* src, float4* dst,
+                    const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
+                    float theta, float dt, float soft) {
+#define BH_LAUNCH(W, D)                                                                                            \
+    if (waves == W && depth == D) {                                                                                \
+        k_ens_walk<W, D><<<blocks, W * 64, 0, s>>>(sims, tasks, ntasks, queue, ctr, src, dst, index_in, leaf_pos,  \
+                                                   nodes, N, G, theta, dt, soft);                                  \
+        return 0;                                                                                                  \
+    }
+    BH_ENS_WALK_SHAPES(BH_LAUNCH)
+#undef BH_LAUNCH
+    return 1;
+}
+
+int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu) {
+#define BH_OCC(W, D)                                                                                               \
+    if (waves == W && depth == D)                                                                                  \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_ens_walk<W, D>, W * 64, 0) ==         \
+                       hipSuccess                                                                                  \
+                   ? 0                                                                                             \
+                   : 2;
+    BH_ENS_WALK_SHAPES(BH_OCC)
+#undef BH_OCC
+    return 1;
 }
 
 }  // namespace bh
