@@ -143,6 +143,64 @@ def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, 
             "loss_after": loss1, "variants": rows, "equal_p99": best}
 
 
+def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, seed: int) -> dict:
+    """Batched vs sequential: n_sims independent Plummer spheres (N per
+    simulation log-uniform in [4096, 524288], virial ratio in [0.3, 0.7]) for
+    `steps` steps, once as one ensemble (autotuned, hipGraph replay) and once
+    one simulation after another through the single-simulation path. Both are
+    aggregate body-steps/s over the timed steps with inputs resident."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import ensemble as E
+    from gravity_amd import initial_conditions as ic
+    sims = ic.ensemble(n_sims, seed)
+    total = int(sum(s.shape[0] for s in sims))
+    prm = B.make_params(theta=theta, dt=dt, softening=soft)
+    log(f"ensemble: {n_sims} simulations, {total} bodies")
+    ens = E.Ensemble(sims, prm)
+    ens.step(1)
+    ens.sync()
+    t0 = time.perf_counter()
+    cfg, timings = ens.autotune()
+    t_tune = time.perf_counter() - t0
+    log(f"ensemble: autotune {t_tune:.1f} s chose {cfg.as_dict()}")
+    ens.step(1)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    t_batched = time.perf_counter() - t0
+    c = ens.counters()
+    ens.close()
+    log(f"ensemble: batched {steps} steps in {t_batched:.3f} s")
+    t_seq = 0.0
+    errs = []
+    for k, b in enumerate(sims):
+        bh = B.BarnesHut(b, ic.tight_bounds(b), prm)
+        bh.init()
+        bh.step()
+        bh.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bh.step()
+        bh.sync()
+        t_seq += time.perf_counter() - t0
+        if k < 4:
+            errs.append(force_error(bh, b.shape[0], soft, n_sample=256))
+        bh.close()
+    log(f"ensemble: sequential {steps} steps of each simulation in {t_seq:.3f} s")
+    batched = total * steps / t_batched
+    sequential = total * steps / t_seq
+    return {"n_sims": n_sims, "n_bodies": total, "steps": steps,
+            "sizes": {"min": int(min(s.shape[0] for s in sims)), "max": int(max(s.shape[0] for s in sims))},
+            "batched_body_steps_per_s": batched, "batched_ms_per_step": t_batched * 1e3 / steps,
+            "sequential_body_steps_per_s": sequential, "sequential_ms_per_step": t_seq * 1e3 / steps,
+            "speedup_batched_vs_sequential": batched / sequential,
+            "autotuned_config": cfg.as_dict(), "autotune_s": t_tune, "autotune_ms": timings,
+            "interactions_per_body": c["interactions"] / total, "tree_height_max": int(c["tree_height"]),
+            "error_flags": int(c["error_flags"]),
+            "force_error_first_4_sims": [{"p50": e["p50"], "p99": e["p99"]} for e in errs]}
+
+
 def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float, softening: float,
                  threads: int, walk_sample: int) -> dict:
     """The oracle's C port of one step on this host: the O(N) stages at full
@@ -199,6 +257,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-sims", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
     args = ap.parse_args()
@@ -270,6 +330,9 @@ def main() -> int:
         log("learned-correction leg")
         result["learned"] = learned_leg(bh, n, soft, theta, 0.9, args.seed)
     bh.close()
+    if not args.no_ensemble:
+        log("ensemble leg")
+        result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, theta, dt, args.seed)
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle port)")
         result["cpu_baseline"] = cpu_baseline(bodies, bounds, theta, dt, soft, args.cpu_threads,

@@ -36,7 +36,7 @@ struct bh_ensemble {
 
     TreeArrays tree{};
     DevCounters* ctr = nullptr;
-    uint32_t* queue = nullptr;
+    EnsQueue* queues = nullptr;
     float* partials = nullptr;
     uint32_t* hist = nullptr;
 
@@ -124,7 +124,7 @@ int enqueue_sort(bh_ensemble* e, uint32_t bits) {
 
 int enqueue_walk(bh_ensemble* e, int waves, int depth, uint32_t blocks) {
     const int b = e->body_in;
-    if (launch_ens_walk(e->stream, waves, depth, blocks, e->sims, e->tasks, e->ntasks, e->queue, e->ctr,
+    if (launch_ens_walk(e->stream, waves, depth, blocks, e->sims, e->tasks, e->queues, e->ctr,
                         e->bodies[b], e->bodies[1 - b], e->index[e->mi_in], e->tree.leaf_pos, e->tree.nodes, e->N,
                         e->G, e->theta, e->dt, e->soft))
         return efail(e, BH_ERR_INVALID_ARGUMENT, "walk shape %d x %d", waves, depth);
@@ -133,7 +133,7 @@ int enqueue_walk(bh_ensemble* e, int waves, int depth, uint32_t blocks) {
 
 int enqueue_step(bh_ensemble* e) {
     const int b = e->body_in;
-    launch_ens_reset(e->stream, e->ctr, e->queue);
+    launch_ens_reset(e->stream, e->ctr, e->queues);
     launch_ens_bounds(e->stream, e->sims, e->tiles, e->ntiles, e->bodies[b], e->partials);
     launch_ens_encode(e->stream, e->sims, e->tiles, e->ntiles, e->bodies[b], e->index[e->mi_in], e->keys[e->mi_in]);
     int rc = enqueue_sort(e, e->cfg.radix_bits);
@@ -207,7 +207,7 @@ void bh_ensemble_destroy(bh_ensemble* e) {
     drop_graphs(e);
     void* ptrs[] = {e->sims, e->tiles, e->tasks, e->bodies[0], e->bodies[1], e->keys[0], e->keys[1], e->index[0],
                     e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.nodes, e->tree.node_aabb,
-                    e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->ctr, e->queue,
+                    e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->ctr, e->queues,
                     e->partials, e->hist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -273,12 +273,29 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         e->off[s] = o;
         o += counts[s];
     }
-    // walk tasks, largest simulations (deepest walks) first
-    std::vector<uint32_t> order(S);
-    for (uint32_t s = 0; s < S; s++) order[s] = s;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return counts[a] > counts[b]; });
-    for (uint32_t s : order)
-        for (uint32_t t = 0; t < counts[s]; t += kWave) hk.push_back({s, hs[s].off + t, std::min(64u, counts[s] - t), 0u});
+    // walk tasks in simulation order, split into one contiguous share per XCD
+    // queue of about equal cost (64-body tasks cost ~ log2 of their
+    // simulation's size: the depth of its walks)
+    std::vector<double> cost_prefix(1, 0.0);
+    for (uint32_t s = 0; s < S; s++)
+        for (uint32_t t = 0; t < counts[s]; t += kWave) {
+            const uint32_t c = std::min(64u, counts[s] - t);
+            hk.push_back({s, hs[s].off + t, c, 0u});
+            cost_prefix.push_back(cost_prefix.back() + c * std::log2((double)counts[s] + 1.0));
+        }
+    std::vector<EnsQueue> hq(kEnsQueues);
+    {
+        uint32_t begin = 0;
+        for (int q = 0; q < kEnsQueues; q++) {
+            const double goal = cost_prefix.back() * (q + 1) / kEnsQueues;
+            uint32_t end = begin;
+            while (end < hk.size() && (q == kEnsQueues - 1 || cost_prefix[end + 1] <= goal)) end++;
+            memset(&hq[q], 0, sizeof(EnsQueue));
+            hq[q].begin = begin;
+            hq[q].count = end - begin;
+            begin = end;
+        }
+    }
     e->ntiles = (uint32_t)ht.size();
     e->ntasks = (uint32_t)hk.size();
     std::vector<uint32_t> iota(N);
@@ -314,7 +331,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &T.meet, ni));
         TRY(ealloc(e, &T.xchg, 8 * ni));
         TRY(ealloc(e, &e->ctr, 1));
-        TRY(ealloc(e, &e->queue, 1));
+        TRY(ealloc(e, &e->queues, kEnsQueues));
         TRY(ealloc(e, &e->partials, 6 * (size_t)e->ntiles));
         TRY(ealloc(e, &e->hist, 256 * (size_t)e->ntiles));
         TRY(walk_grid(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, &e->walk_blocks));
@@ -322,6 +339,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         if (hipMemcpyAsync(e->sims, hs.data(), S * sizeof(EnsSim), H2D, e->stream) != hipSuccess ||
             hipMemcpyAsync(e->tiles, ht.data(), ht.size() * sizeof(EnsTile), H2D, e->stream) != hipSuccess ||
             hipMemcpyAsync(e->tasks, hk.data(), hk.size() * sizeof(EnsTask), H2D, e->stream) != hipSuccess ||
+            hipMemcpyAsync(e->queues, hq.data(), kEnsQueues * sizeof(EnsQueue), H2D, e->stream) != hipSuccess ||
             hipMemcpyAsync(e->bodies[0], bodies, 32 * (size_t)N, H2D, e->stream) != hipSuccess ||
             hipMemcpyAsync(e->index[0], iota.data(), 4 * (size_t)N, H2D, e->stream) != hipSuccess ||
             hipMemsetAsync(e->bodies[1], 0, 32 * (size_t)N, e->stream) != hipSuccess ||
@@ -462,7 +480,7 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
         e->mi_in = m0;
         EHIP(e, hipMemcpyAsync(e->sims, saved_sims, e->S * sizeof(EnsSim), hipMemcpyDeviceToDevice, e->stream));
         EHIP(e, hipMemcpyAsync(e->index[m0], saved_index, 4 * (size_t)e->N, hipMemcpyDeviceToDevice, e->stream));
-        launch_ens_reset(e->stream, e->ctr, e->queue);
+        launch_ens_reset(e->stream, e->ctr, e->queues);
         launch_ens_bounds(e->stream, e->sims, e->tiles, e->ntiles, e->bodies[b0], e->partials);
         launch_ens_encode(e->stream, e->sims, e->tiles, e->ntiles, e->bodies[b0], e->index[m0], e->keys[m0]);
         return BH_OK;
@@ -520,7 +538,7 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
                 if ((rc = walk_grid(e, waves_c[wi], depth_c[di], &blocks))) break;
                 float t_min = 1e30f;
                 for (int rep = 0; rep < 3 && !rc; rep++) {
-                    launch_ens_reset(e->stream, e->ctr, e->queue);
+                    launch_ens_reset(e->stream, e->ctr, e->queues);
                     (void)hipEventRecord(ev[0], e->stream);
                     if ((rc = enqueue_walk(e, waves_c[wi], depth_c[di], blocks))) break;
                     (void)hipEventRecord(ev[1], e->stream);

@@ -920,7 +920,8 @@ void launch_selftest_math(hipStream_t s, uint32_t* bad) { k_selftest_math<<<4096
 // permutation, tree and accelerations are those of a run of it alone.
 namespace bh {
 
-__global__ void k_ens_reset(DevCounters* __restrict__ ctr, uint32_t* __restrict__ queue) {
+__global__ void k_ens_reset(DevCounters* __restrict__ ctr, EnsQueue* __restrict__ queues) {
+    if (threadIdx.x < kEnsQueues) queues[threadIdx.x].head = 0;
     if (threadIdx.x != 0) return;
     ctr->opened = 0;
     ctr->interactions = 0;
@@ -930,7 +931,6 @@ __global__ void k_ens_reset(DevCounters* __restrict__ ctr, uint32_t* __restrict_
     ctr->errorFlags = 0;
     ctr->corrected = 0;
     ctr->treeHeight = 0;
-    queue[0] = 0;
 }
 
 // One block per tile; the last tile of a simulation to finish (ticket)
@@ -1114,7 +1114,9 @@ __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430
     out[k] = o;
 }
 
-void launch_ens_reset(hipStream_t s, DevCounters* ctr, uint32_t* queue) { k_ens_reset<<<1, 64, 0, s>>>(ctr, queue); }
+void launch_ens_reset(hipStream_t s, DevCounters* ctr, EnsQueue* queues) {
+    k_ens_reset<<<1, 64, 0, s>>>(ctr, queues);
+}
 
 void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
                        float* partials) {

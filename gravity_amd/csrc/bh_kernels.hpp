@@ -61,9 +61,16 @@ struct EnsTile {                       // sort tile: positions [start, start + c
 struct EnsTask {                       // walk task: up to 64 positions of one simulation
     uint32_t sim, start, count, pad;
 };
+// Walk work queues, one per XCD: tasks [begin, begin + count) of the task
+// list, handed out through head. One 128-byte line each.
+constexpr int kEnsQueues = 8;
+struct EnsQueue {
+    uint32_t begin, count, head, pad[29];
+};
+static_assert(sizeof(EnsQueue) == 128, "EnsQueue layout");
 constexpr uint32_t kEnsTile = 2048;    // positions per sort/bounds/encode tile
 
-void launch_ens_reset(hipStream_t s, DevCounters* ctr, uint32_t* queue);
+void launch_ens_reset(hipStream_t s, DevCounters* ctr, EnsQueue* queues);
 void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
                        float* partials);
 void launch_ens_encode(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
@@ -76,7 +83,7 @@ void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t
                      const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T);
 // persistent walk over the task queue; returns nonzero for an unsupported shape
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
-                    uint32_t ntasks, uint32_t* queue, DevCounters* ctr, const float4* src, float4* dst,
+                    EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
                     float theta, float dt, float soft);
 int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu);

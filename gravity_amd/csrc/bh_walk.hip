@@ -217,13 +217,16 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
 }
 
 // Ensemble walk (no reference counterpart): a persistent grid whose waves
-// pull 64-body tasks of any simulation from one global queue until it is
-// drained; each task walks its simulation's tree (root = that simulation's
-// first internal node) with the same masked-stack walk and integrates.
+// pull 64-body tasks from work queues until every queue is drained; each
+// task walks its simulation's tree (root = that simulation's first internal
+// node) with the same masked-stack walk and integrates. There is one queue
+// per XCD holding a contiguous, equal-cost share of the task list (tasks in
+// simulation order), so an XCD's waves walk few trees and keep them in its own
+// L2; a wave whose queue is empty takes tasks from the next queues.
 template <int kWaves, int kDepth>
 __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restrict__ sims,
-                                                          const EnsTask* __restrict__ tasks, uint32_t ntasks,
-                                                          uint32_t* __restrict__ queue, DevCounters* __restrict__ ctr,
+                                                          const EnsTask* __restrict__ tasks,
+                                                          EnsQueue* __restrict__ queues, DevCounters* __restrict__ ctr,
                                                           const float4* __restrict__ src, float4* __restrict__ dst,
                                                           const uint32_t* __restrict__ index_in,
                                                           const float4* __restrict__ leaf_pos,
@@ -232,35 +235,41 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
     __shared__ uint32_t s_node[kWaves][kDepth];
     __shared__ unsigned long long s_mask[kWaves][kDepth];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    // workgroups are dealt to the 8 XCDs round robin: this block's XCD
+    const uint32_t home = blockIdx.x % kEnsQueues;
     unsigned long long opened = 0, inter = 0;
     uint32_t maxsp = 1, err = 0;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(queue, 1u);
-        t = uniform(__shfl(t, 0, 64));
-        if (t >= ntasks) break;   // every wave ends here once the queue is drained
-        const EnsTask tk = tasks[t];
-        const uint32_t root = N + sims[tk.sim].ibase;
-        const uint32_t base = tk.start, g = base + lane;
-        const bool active = lane < tk.count;
-        const uint32_t bodyIdx = active ? index_in[g] : 0;
-        const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
-        WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-        walk_chunk<kDepth>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, leaf_pos, nodes, G, theta, soft,
-                           false, 0.0f, bodyIdx, 0.0f, index_in, src, ctr, nullptr, 0u, o);
-        if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
-            const float nvx = fadd(vd.x, fmul(o.ax, dt));
-            const float nvy = fadd(vd.y, fmul(o.ay, dt));
-            const float nvz = fadd(vd.z, fmul(o.az, dt));
-            dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)),
-                                           fadd(pm.z, fmul(nvz, dt)), pm.w);
-            dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
+    for (uint32_t qi = 0; qi < kEnsQueues; qi++) {
+        EnsQueue* q = queues + (home + qi) % kEnsQueues;
+        const uint32_t qbegin = q->begin, qcount = q->count;
+        for (;;) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(&q->head, 1u);
+            t = uniform(__shfl(t, 0, 64));
+            if (t >= qcount) break;   // this queue is drained; every wave ends after the last queue
+            const EnsTask tk = tasks[qbegin + t];
+            const uint32_t root = N + sims[tk.sim].ibase;
+            const uint32_t base = tk.start, g = base + lane;
+            const bool active = lane < tk.count;
+            const uint32_t bodyIdx = active ? index_in[g] : 0;
+            const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
+            WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
+            walk_chunk<kDepth>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, leaf_pos, nodes, G, theta,
+                               soft, false, 0.0f, bodyIdx, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+            if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
+                const float nvx = fadd(vd.x, fmul(o.ax, dt));
+                const float nvy = fadd(vd.y, fmul(o.ay, dt));
+                const float nvz = fadd(vd.z, fmul(o.az, dt));
+                dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)),
+                                               fadd(pm.z, fmul(nvz, dt)), pm.w);
+                dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
+            }
+            opened += o.opened;
+            inter += o.inter;
+            maxsp = o.maxsp > maxsp ? o.maxsp : maxsp;
+            err |= o.err;
         }
-        opened += o.opened;
-        inter += o.inter;
-        maxsp = o.maxsp > maxsp ? o.maxsp : maxsp;
-        err |= o.err;
     }
     if (lane == 0) {
         atomicAdd(&ctr->opened, opened);
@@ -274,13 +283,13 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
     X(1, 64) X(1, 128) X(1, 192) X(2, 64) X(2, 128) X(2, 192) X(4, 64) X(4, 128) X(4, 192) X(8, 64) X(8, 128) X(8, 192)
 
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
-                    uint32_t ntasks, uint32_t* queue, DevCounters* ctr, const float4* src, float4* dst,
+                    EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
                     float theta, float dt, float soft) {
 #define BH_LAUNCH(W, D)                                                                                            \
     if (waves == W && depth == D) {                                                                                \
-        k_ens_walk<W, D><<<blocks, W * 64, 0, s>>>(sims, tasks, ntasks, queue, ctr, src, dst, index_in, leaf_pos,  \
-                                                   nodes, N, G, theta, dt, soft);                                  \
+        k_ens_walk<W, D><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos, nodes,  \
+                                                   N, G, theta, dt, soft);                                         \
         return 0;                                                                                                  \
     }
     BH_ENS_WALK_SHAPES(BH_LAUNCH)

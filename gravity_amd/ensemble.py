@@ -4,7 +4,7 @@ Wraps ``include/bh_ensemble.h`` (no reference counterpart: the reference runs
 one simulation per process). The simulations' bodies are concatenated in one
 set of device arrays; one step is one launch per stage over all of them (a
 segmented (simulation, Morton code) radix sort, one tree build over all
-segments, a persistent walk pulling 64-body tasks from a global queue), and
+segments, a persistent walk pulling 64-body tasks from per-XCD work queues), and
 can be replayed as a captured hipGraph. Each simulation evolves bit for bit
 as it would stepped alone through :class:`gravity_amd.barnes_hut.BarnesHut`.
 """

@@ -11,8 +11,10 @@
  * stage over all simulations with one launch per stage: per-simulation
  * bounds, Morton codes against each simulation's own bounds, a segmented
  * radix sort keyed on (simulation, Morton code), one bottom-up tree build over
- * all segments, and a persistent walk whose waves pull 64-body tasks of any
- * simulation from one global queue. Each simulation's bounds, codes,
+ * all segments, and a persistent walk whose waves pull 64-body tasks from
+ * global work queues (one per XCD, each holding an equal-cost contiguous share
+ * of the tasks; a wave whose queue is empty takes from the others). Each
+ * simulation's bounds, codes,
  * permutation, tree and integrated bodies are bit for bit those of the same
  * simulation stepped alone through include/bh_abi.h.
  *
