@@ -62,7 +62,7 @@ struct DevCounters {
     uint32_t died;             // bodies removed by the last dead partition
     uint32_t aabbTicket;       // last-block-done ticket of the bounds reduction
     uint32_t mergeRound;
-    uint32_t pad0;
+    uint32_t zeroMassIds;      // body ids with zero mass found by the dead-body census
     unsigned long long corrected;   // network evaluations used by the corrected walk
     uint32_t pad[4];
 };

@@ -45,30 +45,57 @@ __global__ void k_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ c
 }
 
 // ------------------------------------------------------------------ dead partition
-// Tile of 1024 sorted positions per block of 256 threads, four rows of 256.
-__global__ __launch_bounds__(256) void k_dead_count(const SimValues* __restrict__ sim, float4* __restrict__ src,
-                                                    float4* __restrict__ dst, const uint32_t* __restrict__ index_in,
+// The reference examines sorted positions [0, numBodies) through the index
+// and moves the bodies of zero mass behind the others, stably, setting them
+// EMPTY in both body buffers. Bodies at positions >= numBodies are always
+// EMPTY (a body leaves [0, numBodies) only through this partition), so the
+// census reads the bodies in id order (coalesced, no gather through the
+// index): it sets every zero-mass body EMPTY (idempotent for those already
+// dead) and counts them. If that count equals n - numBodies nobody died and
+// the partition is the identity: the scatter copies the index. Otherwise the
+// stable partition runs in index order.
+__global__ __launch_bounds__(256) void k_dead_census(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                     float4* __restrict__ src, float4* __restrict__ dst) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t n = sim->initialNumBodies;
+    uint32_t zero = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+        if (src[2 * b].w == 0.0f) {
+            // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
+            src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            zero++;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) zero += __shfl_xor(zero, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = zero;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (t) atomicAdd(&ctr->zeroMassIds, t);
+    }
+}
+
+__device__ __forceinline__ bool nobody_died(const SimValues* sim, const DevCounters* ctr) {
+    return ctr->zeroMassIds == sim->initialNumBodies - sim->numBodies;
+}
+
+// Tile of 1024 sorted positions per block of 256 threads: alive bodies per tile.
+__global__ __launch_bounds__(256) void k_dead_count(const SimValues* __restrict__ sim,
+                                                    const DevCounters* __restrict__ ctr,
+                                                    const float4* __restrict__ src,
+                                                    const uint32_t* __restrict__ index_in,
                                                     uint32_t* __restrict__ block_alive) {
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kPartTile;
-    if (base >= num) return;
+    if (base >= num || nobody_died(sim, ctr)) return;
     uint32_t alive = 0;
     for (int r = 0; r < kPartTile / 256; r++) {
         uint32_t e = base + r * 256 + threadIdx.x;
-        if (e < num) {
-            uint32_t b = index_in[e];
-            float m = src[2 * b].w;
-            if (m == 0.0f) {
-                // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
-                src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-                src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-                dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-                dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                alive++;
-            }
-        }
+        if (e < num && src[2 * index_in[e]].w != 0.0f) alive++;
     }
     for (int off = 32; off > 0; off >>= 1) alive += __shfl_xor(alive, off, 64);
     if (lane_id() == 0) s_w[threadIdx.x >> 6] = alive;
@@ -83,6 +110,17 @@ __global__ __launch_bounds__(1024) void k_dead_scan(SimValues* __restrict__ sim,
     __shared__ uint32_t s_w[16];
     __shared__ uint32_t s_total;
     const uint32_t num = sim->numBodies;
+    if (nobody_died(sim, ctr)) {
+        __syncthreads();   // every thread has read the census before it is cleared
+        if (threadIdx.x == 0) {
+            ctr->numBefore = num;
+            ctr->died = 0;
+            ctr->zeroMassIds = 0;
+            sim->justDied = 0;
+            sim->uintDebug[0] = 0;
+        }
+        return;
+    }
     const uint32_t nb = (num + kPartTile - 1) / kPartTile;
     const uint32_t per = (nb + 1023) / 1024;
     const uint32_t t = threadIdx.x;
@@ -117,6 +155,7 @@ __global__ __launch_bounds__(1024) void k_dead_scan(SimValues* __restrict__ sim,
         uint32_t alive = s_total;
         ctr->numBefore = num;
         ctr->died = num - alive;
+        ctr->zeroMassIds = 0;
         sim->numBodies = alive;
         sim->justDied = 0;
         sim->uintDebug[0] = 0;
@@ -124,6 +163,7 @@ __global__ __launch_bounds__(1024) void k_dead_scan(SimValues* __restrict__ sim,
 }
 
 // Stable scatter: alive first in order, dead after them in order (bh_dead.comp:141-181).
+// When nobody died it is a copy of positions [0, numBodies).
 __global__ __launch_bounds__(256) void k_dead_scatter(const SimValues* __restrict__ sim,
                                                       const DevCounters* __restrict__ ctr,
                                                       const float4* __restrict__ src,
@@ -135,6 +175,13 @@ __global__ __launch_bounds__(256) void k_dead_scatter(const SimValues* __restric
     const uint32_t alive_total = sim->numBodies;
     const uint32_t base = blockIdx.x * kPartTile;
     if (base >= num) return;
+    if (ctr->died == 0) {
+        for (int r = 0; r < kPartTile / 256; r++) {
+            const uint32_t e = base + r * 256 + threadIdx.x;
+            if (e < num) index_out[e] = index_in[e];
+        }
+        return;
+    }
     const uint32_t aoff = block_alive_off[blockIdx.x];
     const uint32_t doff = base - aoff;  // dead bodies in earlier tiles (all earlier tiles are full)
     const uint32_t w = threadIdx.x >> 6;
@@ -251,8 +298,14 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
                            sim->maxCorner[0], sim->maxCorner[1], sim->maxCorner[2]};
     const BoundsWindow W = bounds_window(prev, prev + 3);
     float r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < num; g += gridDim.x * blockDim.x)
-        bounds_accum(W, src[2 * index_in[g]], r);
+    // The alive bodies are those at positions [0, numBodies); every body at a
+    // later position is EMPTY, which bounds_accum skips, so reading all ids in
+    // order (coalesced) covers the same bodies as the gather through the index.
+    (void)index_in;
+    (void)num;
+    const uint32_t n = sim->initialNumBodies;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
+        bounds_accum(W, src[2 * b], r);
     bounds_block_reduce(r, s_r);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 6; k++)
@@ -262,15 +315,18 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
         s_last = ticket == gridDim.x - 1;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
+    if (!s_last) return;
+    // the last block reduces every block's partial (min/max: order-free)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
-    for (uint32_t k = 0; k < gridDim.x; k++) {
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
         for (int c = 0; c < 3; c++)
             res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         for (int c = 3; c < 6; c++)
             res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
+    bounds_block_reduce(res, s_r);
+    if (threadIdx.x != 0) return;
     ctr->aabbTicket = 0;
     bounds_collapse(prev, res, sim->minCorner, sim->maxCorner);
 }
@@ -356,15 +412,95 @@ __global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__
     if (t == 255) digit_total[blockIdx.x] = before + incl;
 }
 
+// Stable scatter of one tile of up to 2048 keys (8 rows of 256): each
+// element's rank among the tile's keys of its digit comes from in-wave ballot
+// matching, the tile is reordered by digit in LDS, and the ordered tile is
+// written out so that consecutive threads write consecutive positions of each
+// digit's run (coalesced, instead of eight scattered short runs per row).
+// s_base[d]: the global position of the tile's first key with digit d.
+struct ScatterLds {
+    uint64_t key[2048];
+    uint32_t idx[2048];
+    uint32_t base[256], start[256], run[256];
+    uint32_t wc[4][256];
+};
+
+template <int BITS>
+__device__ __forceinline__ void scatter_tile(ScatterLds& L, const uint64_t* __restrict__ keys_in,
+                                             const uint32_t* __restrict__ idx_in, uint32_t start, uint32_t count,
+                                             uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
+                                             uint32_t shift) {
+    constexpr uint32_t R = 1u << BITS;
+    constexpr int kRows = 2048 / 256;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    uint64_t key[kRows];
+    uint32_t idx[kRows];
+    L.start[t] = 0;
+    L.run[t] = 0;
+    __syncthreads();
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t e = r * 256 + t;
+        key[r] = e < count ? keys_in[start + e] : 0ull;
+        idx[r] = e < count ? idx_in[start + e] : 0u;
+        if (e < count) atomicAdd(&L.start[(uint32_t)(key[r] >> shift) & (R - 1u)], 1u);
+    }
+    __syncthreads();
+    {   // exclusive scan of the tile's digit counts (thread t = digit t)
+        const uint32_t c = L.start[t];
+        uint32_t incl = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += v;
+        }
+        if (lane == 63) L.wc[0][w] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; k++) before += L.wc[0][k];
+        __syncthreads();
+        L.start[t] = before + incl - c;
+    }
+    const uint64_t lt = lanemask_lt();
+    for (int r = 0; r < kRows; r++) {
+        if (r * 256u >= count) break;
+        // this wave's row of per-digit counts; only this wave writes it, and
+        // the other waves finished reading it before the last barrier
+        for (int k = 0; k < 4; k++) L.wc[w][lane * 4 + k] = 0;
+        const bool valid = r * 256 + t < count;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & (R - 1u);
+        uint64_t peers = ballot(valid);
+        for (int bit = 0; bit < BITS; bit++) {
+            const uint64_t bb = ballot(valid && ((d >> bit) & 1u));
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) L.wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = L.start[d] + L.run[d] + rank;
+            for (uint32_t k = 0; k < w; k++) pos += L.wc[k][d];
+            L.key[pos] = key[r];
+            L.idx[pos] = idx[r];
+        }
+        __syncthreads();
+        L.run[t] += L.wc[0][t] + L.wc[1][t] + L.wc[2][t] + L.wc[3][t];
+        __syncthreads();
+    }
+    for (uint32_t j = t; j < count; j += 256) {
+        const uint64_t k = L.key[j];
+        const uint32_t d = (uint32_t)(k >> shift) & (R - 1u);
+        const uint32_t dest = L.base[d] + (j - L.start[d]);
+        keys_out[dest] = k;
+        idx_out[dest] = L.idx[j];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restrict__ sim,
                                                       const uint64_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ idx_in,
                                                       uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                                       uint32_t shift, const uint32_t* __restrict__ tile_hist,
                                                       uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_run[256];
-    __shared__ uint32_t s_wc[4][256];
+    __shared__ ScatterLds L;
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kSortTile;
@@ -381,35 +517,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restric
     __syncthreads();
     uint32_t before = 0;
     for (uint32_t k = 0; k < w; k++) before += s_w[k];
-    s_base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
-    s_run[t] = 0;
-    const uint64_t lt = lanemask_lt();
-    for (int r = 0; r < kSortTile / 256; r++) {
-        for (int k = 0; k < 4; k++) s_wc[w][lane * 4 + k] = 0;
-        uint32_t e = base + r * 256 + t;
-        bool valid = e < num;
-        uint64_t key = valid ? keys_in[e] : 0;
-        uint32_t idx = valid ? idx_in[e] : 0;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
-        uint64_t peers = ballot(valid);
-        for (int bit = 0; bit < 8; bit++) {
-            uint64_t bb = ballot(valid && ((d >> bit) & 1u));
-            peers &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
-        uint32_t rank = __popcll(peers & lt);
-        if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = s_run[d];
-            for (uint32_t k = 0; k < w; k++) off += s_wc[k][d];
-            uint32_t dest = s_base[d] + off + rank;
-            keys_out[dest] = key;
-            idx_out[dest] = idx;
-        }
-        __syncthreads();
-        s_run[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
-        __syncthreads();
-    }
+    L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
+    scatter_tile<8>(L, keys_in, idx_in, base, min(kSortTile, num - base), keys_out, idx_out, shift);
 }
 
 // ------------------------------------------------------------------ tree (build + fill)
@@ -499,9 +608,41 @@ struct TreeSeg {
     uint32_t off, num, ibase, n_ids;
 };
 
+// The summary of leaf (sorted position) j as its own thread starts it.
+__device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
+    NodeState me;
+    me.com = pm;
+    me.mn = me.mx = make_float4(pm.x, pm.y, pm.z, 0.f);
+    me.height = 0;
+    me.id = id;
+    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    return me;
+}
+
+struct LeafSource {
+    const float4* src;          // bodies (written by earlier kernels: coherent)
+    const uint32_t* index_in;   // sorted position -> body id
+    float unit_mass, unit_density;
+};
+
+__device__ __forceinline__ NodeState leaf_at(const LeafSource& L, uint32_t j, int colors) {
+    const uint32_t b = L.index_in[j];
+    const float4 pm = L.src[2 * b];
+    NodeState me = leaf_state(pm, j);
+    if (colors) {
+        const float4 vd = L.src[2 * b + 1];
+        if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, L.unit_mass), fmul(vd.w, L.unit_density));
+    }
+    return me;
+}
+
+// A child that is a leaf publishes nothing: the thread that completes the
+// parent rebuilds the leaf's summary from the body arrays, which earlier
+// kernels wrote (so every XCD sees them). Only internal children pass their
+// summary through the coherent exchange record.
 __device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me, const uint64_t* __restrict__ keys,
-                                           const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
-                                           int colors) {
+                                           const LeafSource& L, const TreeArrays& T,
+                                           uint32_t* __restrict__ height_out, bool height_max, int colors) {
     const int num = (int)S.num;
     int lo = i, hi = i;
     for (;;) {
@@ -510,14 +651,20 @@ __device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
         float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
-        put_state(x + (left ? 0 : 4), me, colors);
-        stores_done();
+        if (lo != hi) {
+            put_state(x + (left ? 0 : 4), me, colors);
+            stores_done();
+        }
         const uint32_t other = __hip_atomic_exchange(&T.meet[S.ibase + split], (uint32_t)(left ? lo : hi),
                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (other == kEmpty) return;                      // first to arrive: the sibling continues
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // the sibling covers [split + 1, other] (I am the left child) or
+        // [other, split]: a single position is a leaf
+        const bool sib_leaf = left ? other == (uint32_t)split + 1u : other == (uint32_t)split;
         NodeState sib;
-        get_state(x + (left ? 4 : 0), sib, colors);
+        if (sib_leaf) sib = leaf_at(L, S.off + other, colors);
+        else get_state(x + (left ? 4 : 0), sib, colors);
         T.meet[S.ibase + split] = kEmpty;                 // only this thread touches the slot now
         const int plo = left ? lo : (int)other;
         const int phi = left ? (int)other : hi;
@@ -580,16 +727,6 @@ __device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me
     }
 }
 
-__device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
-    NodeState me;
-    me.com = pm;
-    me.mn = me.mx = make_float4(pm.x, pm.y, pm.z, 0.f);
-    me.height = 0;
-    me.id = id;
-    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
-    return me;
-}
-
 __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src,
                                               const uint32_t* __restrict__ index_in,
@@ -625,7 +762,8 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
         return;
     }
     const TreeSeg S = {0u, (uint32_t)num, 0u, n};
-    tree_climb(i, S, me, keys, T, &ctr->treeHeight, false, colors);
+    const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
+    tree_climb(i, S, me, keys, L, T, &ctr->treeHeight, false, colors);
 }
 
 // ------------------------------------------------------------------ merge
@@ -795,7 +933,9 @@ void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units
 void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
                            const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch) {
     uint32_t nb = cdiv(n, kPartTile);
-    k_dead_count<<<nb, 256, 0, s>>>(sim, src, dst, index_in, block_scratch);
+    uint32_t nc = cdiv(n, 256 * 8);
+    k_dead_census<<<nc < 1 ? 1 : nc, 256, 0, s>>>(sim, ctr, src, dst);
+    k_dead_count<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, block_scratch);
     k_dead_scan<<<1, 1024, 0, s>>>(sim, ctr, block_scratch);
     k_dead_scatter<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, index_out, block_scratch);
 }
@@ -955,15 +1095,17 @@ __global__ __launch_bounds__(256) void k_ens_bounds(EnsSim* __restrict__ sims, c
         s_last = ticket == S->ntiles - 1;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
+    if (!s_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
-    for (uint32_t k = S->tile0; k < S->tile0 + S->ntiles; k++) {
+    for (uint32_t k = S->tile0 + threadIdx.x; k < S->tile0 + S->ntiles; k += blockDim.x) {
         for (int c = 0; c < 3; c++)
             res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         for (int c = 3; c < 6; c++)
             res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
+    bounds_block_reduce(res, s_r);
+    if (threadIdx.x != 0) return;
     S->ticket = 0;
     bounds_collapse(prev, res, S->bmin, S->bmax);
 }
@@ -1040,41 +1182,11 @@ __global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restr
                                                           uint32_t* __restrict__ idx_out, uint32_t shift,
                                                           const uint32_t* __restrict__ hist) {
     constexpr uint32_t R = 1u << BITS;
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_run[256];
-    __shared__ uint32_t s_wc[4][256];
+    __shared__ ScatterLds L;
     const EnsTile tl = tiles[blockIdx.x];
-    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
-    s_base[t] = t < R ? hist[(size_t)blockIdx.x * 256 + t] : 0u;
-    s_run[t] = 0;
-    const uint64_t lt = lanemask_lt();
-    for (uint32_t r = 0; r < kEnsTile; r += 256) {
-        for (int k = 0; k < 4; k++) s_wc[w][lane * 4 + k] = 0;
-        const uint32_t e = r + t;
-        const bool valid = e < tl.count;
-        const uint64_t key = valid ? keys_in[tl.start + e] : 0;
-        const uint32_t idx = valid ? idx_in[tl.start + e] : 0;
-        const uint32_t d = (uint32_t)(key >> shift) & (R - 1u);
-        uint64_t peers = ballot(valid);
-        for (int bit = 0; bit < BITS; bit++) {
-            const uint64_t bb = ballot(valid && ((d >> bit) & 1u));
-            peers &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
-        const uint32_t rank = __popcll(peers & lt);
-        if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = s_run[d];
-            for (uint32_t k = 0; k < w; k++) off += s_wc[k][d];
-            const uint32_t dest = s_base[d] + off + rank;
-            keys_out[dest] = key;
-            idx_out[dest] = idx;
-        }
-        __syncthreads();
-        s_run[t] += s_wc[0][t] + s_wc[1][t] + s_wc[2][t] + s_wc[3][t];
-        __syncthreads();
-        if (r + 256 >= tl.count) break;
-    }
+    const uint32_t t = threadIdx.x;
+    L.base[t] = t < R ? hist[(size_t)blockIdx.x * 256 + t] : 0u;
+    scatter_tile<BITS>(L, keys_in, idx_in, tl.start, tl.count, keys_out, idx_out, shift);
 }
 
 // Tree of every simulation in one launch: thread = global sorted position.
@@ -1094,7 +1206,8 @@ __global__ __launch_bounds__(256) void k_ens_tree(const EnsSim* __restrict__ sim
     const float4 pm = src[2 * index_in[i]];
     T.leaf_pos[i] = pm;
     const TreeSeg seg = {S.off, S.num, S.ibase, N};
-    tree_climb((int)(i - S.off), seg, leaf_state(pm, i), keys + S.off, T, &ctr->treeHeight, true, 0);
+    const LeafSource L = {src, index_in, 0.0f, 0.0f};
+    tree_climb((int)(i - S.off), seg, leaf_state(pm, i), keys + S.off, L, T, &ctr->treeHeight, true, 0);
 }
 
 __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430* __restrict__ out) {

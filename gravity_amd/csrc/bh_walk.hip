@@ -43,6 +43,11 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
                                            DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
                                            uint32_t task_cap, WalkAcc& o) {
     const uint32_t lane = lane_id();
+    // the sums and counters live in locals for the loop (a struct carried
+    // around the loop costs register copies on every iteration)
+    float ax = o.ax, ay = o.ay, az = o.az;
+    unsigned long long opened = o.opened, inter = o.inter;
+    uint32_t maxsp = o.maxsp, err = o.err;
     // current node in scalar registers; the stack holds the pending childA
     // entries (after opening a node its childB, the top of the reference's
     // stack, is visited next without a round trip through LDS)
@@ -62,14 +67,14 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             const unsigned long long open = ballot(mine && !accept);
             if (mine && accept) {
                 const float gm = fmul(G, nd.comMass.w);
-                o.ax = fadd(o.ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                o.ay = fadd(o.ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                o.az = fadd(o.az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+                ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
             }
-            o.inter += (unsigned long long)__popcll(mask & ~open);
+            inter += (unsigned long long)__popcll(mask & ~open);
             if (open) {
                 if (sp + 1 > kDepth) {
-                    o.err |= 1u;
+                    err |= 1u;
                     break;
                 }
                 if (lane == 0) {
@@ -77,8 +82,8 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
                     s_mask[sp] = open;
                 }
                 sp += 1;
-                o.opened += (unsigned long long)__popcll(open);
-                if ((uint32_t)sp + 1 > o.maxsp) o.maxsp = (uint32_t)sp + 1;
+                opened += (unsigned long long)__popcll(open);
+                if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
                 node = uniform(nd.childB);
                 mask = open;
                 descend = true;
@@ -93,13 +98,13 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             const float inv = inv_sqrt_exact(d2);
             if (use) {
                 const float gm = fmul(G, lp.w);
-                o.ax = fadd(o.ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                o.ay = fadd(o.ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                o.az = fadd(o.az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+                ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
             }
             const unsigned long long selfbit =
                 (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
-            o.inter += (unsigned long long)__popcll(mask & ~selfbit);
+            inter += (unsigned long long)__popcll(mask & ~selfbit);
             if (merging && (mask & ~selfbit)) {
                 // bh_force.comp:85-112: overlap test against the leaf's body,
                 // merge task from the lower-index body
@@ -111,7 +116,7 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
                 if (use && pen > 0.0f && bodyIdx < other) {
                     uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
                     if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                    else o.err |= 2u;
+                    else err |= 2u;
                 }
             }
         }
@@ -123,6 +128,13 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
         }
     }
+    o.ax = ax;
+    o.ay = ay;
+    o.az = az;
+    o.opened = opened;
+    o.inter = inter;
+    o.maxsp = maxsp;
+    o.err = err;
 }
 
 // Pointers are separate __restrict__ parameters so that the compiler can
