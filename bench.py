@@ -251,7 +251,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=16_777_216)
+    ap.add_argument("--workload", choices=("plummer", "cube_core_merging", "two_galaxy"), default="plummer",
+                    help="plummer: the headline 16M Plummer sphere (default); cube_core_merging: uniform cube "
+                         "plus Plummer core, log-normal masses, merging on, dt 5e-4 (default N 4,194,304); "
+                         "two_galaxy: two disk+bulge galaxies 10 apart on a parabolic orbit (default N 33,554,432)")
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -270,11 +274,32 @@ def main() -> int:
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
 
-    n, theta, dt, soft = args.n, args.theta, 1e-3, 0.01
-    log(f"generating Plummer N={n}")
-    bodies = ic.plummer(n, args.seed)
+    theta, soft = args.theta, 0.01
+    if args.workload == "plummer":
+        n, dt = args.n or 16_777_216, 1e-3
+        log(f"generating Plummer N={n}")
+        bodies = ic.plummer(n, args.seed)
+        prm = B.make_params(theta=theta, dt=dt, softening=soft)
+        workload = {"workload": f"plummer_{n}_plain_theta{theta}", "merging": "off"}
+    elif args.workload == "cube_core_merging":
+        # a softened distance (softening = eps^2) accepts every node smaller
+        # than ~eps before its leaves are reached, and merges are only found
+        # at leaves: this workload uses eps^2 = 1e-6 so that overlaps merge
+        n, dt, soft = args.n or 4_194_304, 5e-4, 1e-6
+        log(f"generating cube + Plummer core N={n}")
+        bodies = ic.cube_plus_core(n, args.seed)
+        prm = B.make_params(theta=theta, dt=dt, softening=soft, merging=B.MERGING)
+        workload = {"workload": f"cube_plus_core_{n}_merging_theta{theta}", "merging": "on",
+                    "masses": "log-normal(0, 0.5), total 1", "density": 64.0}
+    else:
+        n, dt = args.n or 33_554_432, 1e-3
+        log(f"generating two galaxies N={n}")
+        bodies = ic.two_galaxy(n // 2, args.seed)
+        n = bodies.shape[0]
+        prm = B.make_params(theta=theta, dt=dt, softening=soft)
+        workload = {"workload": f"two_galaxy_{n}_plain_theta{theta}", "merging": "off"}
     bounds = ic.tight_bounds(bodies)
-    prm = B.make_params(theta=theta, dt=dt, softening=soft)
+    headline = args.workload == "plummer"
     log("creating context and uploading")
     bh = B.BarnesHut(bodies, bounds, prm)
     bh.init()
@@ -311,9 +336,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plummer_{n}_plain_theta{theta}", "n_bodies": n, "theta": theta, "dt": dt,
-                   "softening": soft, "merging": "off", "bounds": "dynamic", "method": "plain Barnes-Hut",
-                   "mac": "half longest AABB side x inverse softened distance < theta"},
+        "config": dict(workload, n_bodies=n, theta=theta, dt=dt, softening=soft, bounds="dynamic",
+                       method="plain Barnes-Hut", mac="half longest AABB side x inverse softened distance < theta"),
         "roofline": {"bound": "valu", "kernel": "k_walk", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "flop_per_interaction": FLOP_PER_INTERACTION,
@@ -321,19 +345,20 @@ def main() -> int:
         "stage_ms": {k: v / args.steps for k, v in stage.items()},
         "counters": {"opened": int(c["opened"]), "interactions_per_body": c["interactions"] / max(num, 1),
                      "tree_height": int(c["tree_height"]), "max_stack": int(c["max_stack"]),
-                     "error_flags": int(c["error_flags"])},
+                     "error_flags": int(c["error_flags"]), "alive_bodies": num,
+                     "merges_last_step": int(c["merges_applied"])},
     }
     if not args.no_error:
         log("force error against the FP64 direct sum")
         result["force_error"] = force_error(bh, n, soft)
-    if not args.no_learned:
+    if not args.no_learned and headline:
         log("learned-correction leg")
         result["learned"] = learned_leg(bh, n, soft, theta, 0.9, args.seed)
     bh.close()
-    if not args.no_ensemble:
+    if not args.no_ensemble and headline:
         log("ensemble leg")
         result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, theta, dt, args.seed)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and headline:
         log("CPU baseline (oracle port)")
         result["cpu_baseline"] = cpu_baseline(bodies, bounds, theta, dt, soft, args.cpu_threads,
                                               args.cpu_walk_sample)
