@@ -548,11 +548,14 @@ __device__ __forceinline__ int delta_at(const uint64_t* __restrict__ keys, int n
 // stores are waited for before the arrival exchange that publishes them.
 // 16-byte store with the coherence bit the compiler uses for agent-scope
 // atomic stores (sc1): the record is visible to every XCD once the store has
-// completed. A store has no result register, so inline assembly is safe here.
+// completed. The hardware reads the data VGPRs of a store wider than 8 bytes
+// after it issues, and a VALU that rewrites them in the next cycles corrupts
+// the stored data; the compiler pads its own stores for this but cannot see
+// into inline assembly, so the padding is part of the asm block.
 __device__ __forceinline__ void store_coherent(float4* p, float4 v) {
     typedef float f32x4_t __attribute__((ext_vector_type(4)));
     f32x4_t w = {v.x, v.y, v.z, v.w};
-    asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(w) : "memory");
+    asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 2" ::"v"(p), "v"(w) : "memory");
 }
 __device__ __forceinline__ float4 load_coherent(const float4* p) {
     const unsigned long long* q = (const unsigned long long*)p;
@@ -636,10 +639,15 @@ __device__ __forceinline__ NodeState leaf_at(const LeafSource& L, uint32_t j, in
     return me;
 }
 
-// A child that is a leaf publishes nothing: the thread that completes the
-// parent rebuilds the leaf's summary from the body arrays, which earlier
-// kernels wrote (so every XCD sees them). Only internal children pass their
-// summary through the coherent exchange record.
+// Whether a child is a leaf is known from the keys alone: the sibling of a
+// node ending (starting) at a split is the single leaf next to that split iff
+// that leaf's own parent lies towards the split. So no exchange is needed at a
+// parent with a leaf child: the internal child's thread (or, for two leaves,
+// the left leaf's thread) builds the parent at once, rebuilding the leaf's
+// summary from the body arrays, which earlier kernels wrote (so every XCD
+// sees them), and a leaf whose sibling is internal simply stops. Only parents
+// of two internal children use the arrival exchange and the coherent
+// records; that is about a third of the nodes of a radix tree.
 __device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me, const uint64_t* __restrict__ keys,
                                            const LeafSource& L, const TreeArrays& T,
                                            uint32_t* __restrict__ height_out, bool height_max, int colors) {
@@ -650,22 +658,29 @@ __device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
-        float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
-        if (lo != hi) {
+        // the sibling covers [split + 1, ...] (I am the left child) or
+        // [..., split]; it is the leaf split + 1 (split) iff that leaf's parent
+        // is towards the split
+        const bool sib_leaf = left ? (split + 1 == num - 1 || delta_at(keys, num, split + 1) <= delta_at(keys, num, split))
+                                   : (split == 0 || delta_at(keys, num, split) > delta_at(keys, num, split - 1));
+        uint32_t other;
+        NodeState sib;
+        if (sib_leaf) {
+            if (lo == hi && !left) return;                // two leaves: the left one builds the parent
+            other = (uint32_t)(left ? split + 1 : split);
+            sib = leaf_at(L, S.off + other, colors);
+        } else {
+            if (lo == hi) return;                         // the internal sibling's thread builds the parent
+            float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
             put_state(x + (left ? 0 : 4), me, colors);
             stores_done();
+            other = __hip_atomic_exchange(&T.meet[S.ibase + split], (uint32_t)(left ? lo : hi), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+            if (other == kEmpty) return;                  // first to arrive: the sibling continues
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            get_state(x + (left ? 4 : 0), sib, colors);
+            T.meet[S.ibase + split] = kEmpty;             // only this thread touches the slot now
         }
-        const uint32_t other = __hip_atomic_exchange(&T.meet[S.ibase + split], (uint32_t)(left ? lo : hi),
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (other == kEmpty) return;                      // first to arrive: the sibling continues
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        // the sibling covers [split + 1, other] (I am the left child) or
-        // [other, split]: a single position is a leaf
-        const bool sib_leaf = left ? other == (uint32_t)split + 1u : other == (uint32_t)split;
-        NodeState sib;
-        if (sib_leaf) sib = leaf_at(L, S.off + other, colors);
-        else get_state(x + (left ? 4 : 0), sib, colors);
-        T.meet[S.ibase + split] = kEmpty;                 // only this thread touches the slot now
         const int plo = left ? lo : (int)other;
         const int phi = left ? (int)other : hi;
         uint32_t pid;
