@@ -412,79 +412,86 @@ __global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__
     if (t == 255) digit_total[blockIdx.x] = before + incl;
 }
 
-// Stable scatter of one tile of up to 2048 keys (8 rows of 256): each
-// element's rank among the tile's keys of its digit comes from in-wave ballot
-// matching, the tile is reordered by digit in LDS, and the ordered tile is
-// written out so that consecutive threads write consecutive positions of each
-// digit's run (coalesced, instead of eight scattered short runs per row).
-// s_base[d]: the global position of the tile's first key with digit d.
+// Stable scatter of one tile of up to TILE keys. Wave w owns the tile's
+// elements [w TILE/4, (w + 1) TILE/4) and ranks them by digit on its own (in-wave
+// ballot matching, a per-wave running count per digit in LDS, no block
+// barrier); the block then turns the four waves' digit counts into per-wave
+// offsets and the tile's digit starts, reorders the tile by digit in LDS and
+// writes it out so that consecutive threads write consecutive positions of
+// each digit's run. The order within a digit is the element order: stable.
+// L.base[d]: the global position of the tile's first key with digit d.
+template <int TILE>
 struct ScatterLds {
-    uint64_t key[2048];
-    uint32_t idx[2048];
-    uint32_t base[256], start[256], run[256];
-    uint32_t wc[4][256];
+    uint64_t key[TILE];
+    uint32_t idx[TILE];
+    uint32_t base[256], start[256];
+    uint32_t cnt[4][256];   // per-wave digit counts, then per-wave offsets
+    uint32_t w_tot[4];
 };
 
-template <int BITS>
-__device__ __forceinline__ void scatter_tile(ScatterLds& L, const uint64_t* __restrict__ keys_in,
+template <int BITS, int TILE>
+__device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t* __restrict__ keys_in,
                                              const uint32_t* __restrict__ idx_in, uint32_t start, uint32_t count,
                                              uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                              uint32_t shift) {
     constexpr uint32_t R = 1u << BITS;
-    constexpr int kRows = 2048 / 256;
+    constexpr int kPer = TILE / 4;         // elements per wave
+    constexpr int kRows = kPer / 64;       // rows of 64 per wave
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
-    uint64_t key[kRows];
-    uint32_t idx[kRows];
-    L.start[t] = 0;
-    L.run[t] = 0;
-    __syncthreads();
-    for (int r = 0; r < kRows; r++) {
-        const uint32_t e = r * 256 + t;
-        key[r] = e < count ? keys_in[start + e] : 0ull;
-        idx[r] = e < count ? idx_in[start + e] : 0u;
-        if (e < count) atomicAdd(&L.start[(uint32_t)(key[r] >> shift) & (R - 1u)], 1u);
-    }
-    __syncthreads();
-    {   // exclusive scan of the tile's digit counts (thread t = digit t)
-        const uint32_t c = L.start[t];
-        uint32_t incl = c;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(incl, off, 64);
-            if (lane >= (uint32_t)off) incl += v;
-        }
-        if (lane == 63) L.wc[0][w] = incl;
-        __syncthreads();
-        uint32_t before = 0;
-        for (uint32_t k = 0; k < w; k++) before += L.wc[0][k];
-        __syncthreads();
-        L.start[t] = before + incl - c;
-    }
     const uint64_t lt = lanemask_lt();
+    for (int k = 0; k < 4; k++) L.cnt[w][lane * 4 + k] = 0;   // this wave's counts
+    uint64_t key[kRows];
+    uint32_t idx[kRows], rank[kRows];
     for (int r = 0; r < kRows; r++) {
-        if (r * 256u >= count) break;
-        // this wave's row of per-digit counts; only this wave writes it, and
-        // the other waves finished reading it before the last barrier
-        for (int k = 0; k < 4; k++) L.wc[w][lane * 4 + k] = 0;
-        const bool valid = r * 256 + t < count;
+        const uint32_t e = w * kPer + r * 64 + lane;
+        const bool valid = e < count;
+        key[r] = valid ? keys_in[start + e] : 0ull;
+        idx[r] = valid ? idx_in[start + e] : 0u;
+    }
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t e = w * kPer + r * 64 + lane;
+        const bool valid = e < count;
         const uint32_t d = (uint32_t)(key[r] >> shift) & (R - 1u);
         uint64_t peers = ballot(valid);
         for (int bit = 0; bit < BITS; bit++) {
             const uint64_t bb = ballot(valid && ((d >> bit) & 1u));
             peers &= ((d >> bit) & 1u) ? bb : ~bb;
         }
-        const uint32_t rank = __popcll(peers & lt);
-        if (valid && rank == 0) L.wc[w][d] = __popcll(peers);
+        const uint32_t before = valid ? L.cnt[w][d] : 0u;   // same digit: same value in every peer
+        const uint32_t in_row = __popcll(peers & lt);
+        rank[r] = before + in_row;
+        if (valid && in_row == 0) L.cnt[w][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+    {   // thread t = digit t: per-wave offsets and the tile's digit starts
+        const uint32_t c0 = L.cnt[0][t], c1 = L.cnt[1][t], c2 = L.cnt[2][t], c3 = L.cnt[3][t];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        uint32_t incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += v;
+        }
+        if (lane == 63) L.w_tot[w] = incl;
         __syncthreads();
-        if (valid) {
-            uint32_t pos = L.start[d] + L.run[d] + rank;
-            for (uint32_t k = 0; k < w; k++) pos += L.wc[k][d];
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; k++) before += L.w_tot[k];
+        L.start[t] = before + incl - tot;
+        L.cnt[0][t] = 0;
+        L.cnt[1][t] = c0;
+        L.cnt[2][t] = c0 + c1;
+        L.cnt[3][t] = c0 + c1 + c2;
+    }
+    __syncthreads();
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t e = w * kPer + r * 64 + lane;
+        if (e < count) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & (R - 1u);
+            const uint32_t pos = L.start[d] + L.cnt[w][d] + rank[r];
             L.key[pos] = key[r];
             L.idx[pos] = idx[r];
         }
-        __syncthreads();
-        L.run[t] += L.wc[0][t] + L.wc[1][t] + L.wc[2][t] + L.wc[3][t];
-        __syncthreads();
     }
+    __syncthreads();
     for (uint32_t j = t; j < count; j += 256) {
         const uint64_t k = L.key[j];
         const uint32_t d = (uint32_t)(k >> shift) & (R - 1u);
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restric
                                                       uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                                       uint32_t shift, const uint32_t* __restrict__ tile_hist,
                                                       uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
-    __shared__ ScatterLds L;
+    __shared__ ScatterLds<kSortTile> L;
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kSortTile;
@@ -518,7 +525,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restric
     uint32_t before = 0;
     for (uint32_t k = 0; k < w; k++) before += s_w[k];
     L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
-    scatter_tile<8>(L, keys_in, idx_in, base, min(kSortTile, num - base), keys_out, idx_out, shift);
+    scatter_tile<8, kSortTile>(L, keys_in, idx_in, base, min(kSortTile, num - base), keys_out, idx_out, shift);
 }
 
 // ------------------------------------------------------------------ tree (build + fill)
@@ -1197,11 +1204,11 @@ __global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restr
                                                           uint32_t* __restrict__ idx_out, uint32_t shift,
                                                           const uint32_t* __restrict__ hist) {
     constexpr uint32_t R = 1u << BITS;
-    __shared__ ScatterLds L;
+    __shared__ ScatterLds<kEnsTile> L;
     const EnsTile tl = tiles[blockIdx.x];
     const uint32_t t = threadIdx.x;
     L.base[t] = t < R ? hist[(size_t)blockIdx.x * 256 + t] : 0u;
-    scatter_tile<BITS>(L, keys_in, idx_in, tl.start, tl.count, keys_out, idx_out, shift);
+    scatter_tile<BITS, kEnsTile>(L, keys_in, idx_in, tl.start, tl.count, keys_out, idx_out, shift);
 }
 
 // Tree of every simulation in one launch: thread = global sorted position.
