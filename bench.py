@@ -38,6 +38,21 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, calibration in the file), or None."""
+    path = os.path.join(ROOT, "profiles", "k_walk_plummer16M_step_pmc.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -339,7 +354,9 @@ def main() -> int:
         "config": dict(workload, n_bodies=n, theta=theta, dt=dt, softening=soft, bounds="dynamic",
                        method="plain Barnes-Hut", mac="half longest AABB side x inverse softened distance < theta"),
         "roofline": {"bound": "valu", "kernel": "k_walk", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic("k_walk") if headline and n == 16_777_216 else None,
+                     "traffic_source": "profiles/k_walk_plummer16M_step_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms},
         "stage_ms": {k: v / args.steps for k, v in stage.items()},
