@@ -53,6 +53,7 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     // stack, is visited next without a round trip through LDS)
     uint32_t node = root;
     int sp = 0;
+    const float theta_hi = theta * 1.0000153f;   // theta (1 + 2^-16), rounded up
     for (;;) {
         const bool mine = (mask >> lane) & 1ull;
         bool descend = false;
@@ -62,14 +63,25 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             const float ry = fsub(nd.comMass.y, pm.y);
             const float rz = fsub(nd.comMass.z, pm.z);
             const float d2 = fadd(dot3(rx, ry, rz), soft);
-            const float inv = inv_sqrt_exact(d2);
-            const bool accept = fmul(nd.half, inv) < theta;
-            const unsigned long long open = ballot(mine && !accept);
-            if (mine && accept) {
-                const float gm = fmul(G, nd.comMass.w);
-                ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+            // The hardware 1/sqrt (within 2 ulp) decides a node that every
+            // lane of the mask opens by a wide margin without the correctly
+            // rounded inverse square root: half * rsq >= theta (1 + 2^-16)
+            // implies the exact ratio is >= theta. Only the decision is taken
+            // from it; every value that is accumulated uses the exact one.
+            const bool sure_open = fmul(nd.half, __builtin_amdgcn_rsqf(d2)) >= theta_hi;
+            unsigned long long open;
+            if (ballot(mine && !sure_open) == 0ull) {
+                open = mask;
+            } else {
+                const float inv = inv_sqrt_exact(d2);
+                const bool accept = fmul(nd.half, inv) < theta;
+                open = ballot(mine && !accept);
+                if (mine && accept) {
+                    const float gm = fmul(G, nd.comMass.w);
+                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+                }
             }
             inter += (unsigned long long)__popcll(mask & ~open);
             if (open) {
