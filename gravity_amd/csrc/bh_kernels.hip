@@ -807,66 +807,108 @@ __device__ __forceinline__ void merge_pair(float4 p1, float4 v1, float4 p2, floa
 
 // Deterministic resolution in ascending (a, b) order (deterministic
 // reservations: a task commits once it holds the smallest pending key on both
-// of its bodies, so every earlier task touching them has already committed).
-// One block; the reservation words are all ~0 between calls.
+// of its bodies, so every earlier task touching them has already committed;
+// the result equals processing the tasks one by one in ascending order).
+// The reservation words are all ~0 between calls.
+//
+// A round is reserve -> commit -> release. The first kMergeGridRounds rounds
+// run over the whole grid, one launch per phase (most tasks commit in the
+// first rounds); k_merge, one block, then finishes the remaining chains with
+// block barriers between its phases.
+constexpr int kMergeGridRounds = 4;
+constexpr uint32_t kMergeGridBlocks = 1024;
+
+__device__ __forceinline__ unsigned long long task_key(uint2 ab) {
+    return ((unsigned long long)ab.x << 32) | ab.y;
+}
+
+// commit one task if it holds both reservations; returns whether it is done
+__device__ __forceinline__ bool merge_commit(float4* __restrict__ dst, uint2 ab,
+                                             const unsigned long long* __restrict__ reserve, uint32_t& applied) {
+    const unsigned long long key = task_key(ab);
+    const unsigned long long ra = __hip_atomic_load(&reserve[ab.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long rb = __hip_atomic_load(&reserve[ab.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ra != key || rb != key) return false;
+    const float4 pa = dst[2 * ab.x], va = dst[2 * ab.x + 1];
+    const float4 pb = dst[2 * ab.y], vb = dst[2 * ab.y + 1];
+    if (pa.w != 0.0f && pb.w != 0.0f) {
+        float4 po, vo;
+        merge_pair(pa, va, pb, vb, po, vo);
+        dst[2 * ab.x] = po;
+        dst[2 * ab.x + 1] = vo;
+        dst[2 * ab.y] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[2 * ab.y + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        applied++;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_merge_round(float4* __restrict__ dst, const uint2* __restrict__ tasks,
+                                                     uint32_t cap, DevCounters* __restrict__ ctr,
+                                                     unsigned long long* __restrict__ reserve,
+                                                     uint32_t* __restrict__ done, int phase) {
+    const uint32_t nt = min(ctr->nTasks, cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t applied = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+        const uint2 ab = tasks[t];
+        if (phase == 0) {          // start: nothing done yet; a self-pair is done at once (bh_merge.comp:40)
+            done[t] = ab.x == ab.y ? 1u : 0u;
+        } else if (phase == 1) {   // reserve
+            if (done[t]) continue;
+            atomicMin(&reserve[ab.x], task_key(ab));
+            atomicMin(&reserve[ab.y], task_key(ab));
+        } else if (phase == 2) {   // commit
+            if (done[t]) continue;
+            if (merge_commit(dst, ab, reserve, applied)) done[t] = 2u;   // 2: done in this round, release pending
+        } else {                   // release every reservation this round made
+            if (done[t] == 1u) continue;
+            __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (done[t] == 2u) done[t] = 1u;
+        }
+    }
+    if (phase == 0 && blockIdx.x == 0 && threadIdx.x == 0) ctr->mergesApplied = 0;
+    for (int off = 32; off > 0; off >>= 1) applied += __shfl_xor(applied, off, 64);
+    if (applied && lane_id() == 0) atomicAdd(&ctr->mergesApplied, applied);
+}
+
+// The remaining chains, one block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const uint2* __restrict__ tasks,
                                                 uint32_t cap, DevCounters* __restrict__ ctr,
                                                 unsigned long long* __restrict__ reserve,
                                                 uint32_t* __restrict__ done) {
     __shared__ int s_pending;
-    __shared__ uint32_t s_applied;
     const uint32_t nt = min(ctr->nTasks, cap);
-    if (threadIdx.x == 0) s_applied = 0;
-    for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) done[t] = 0;
-    __syncthreads();
     uint32_t applied = 0;
     for (;;) {
         if (threadIdx.x == 0) s_pending = 0;
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            if (done[t]) continue;
-            uint2 ab = tasks[t];
-            if (ab.x == ab.y) { done[t] = 1; continue; }   // bh_merge.comp:40
-            unsigned long long key = ((unsigned long long)ab.x << 32) | ab.y;
-            atomicMin(&reserve[ab.x], key);
-            atomicMin(&reserve[ab.y], key);
-        }
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
             if (done[t]) continue;
-            uint2 ab = tasks[t];
-            unsigned long long key = ((unsigned long long)ab.x << 32) | ab.y;
-            unsigned long long ra = __hip_atomic_load(&reserve[ab.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned long long rb = __hip_atomic_load(&reserve[ab.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (ra == key && rb == key) {
-                float4 pa = dst[2 * ab.x], va = dst[2 * ab.x + 1];
-                float4 pb = dst[2 * ab.y], vb = dst[2 * ab.y + 1];
-                if (pa.w != 0.0f && pb.w != 0.0f) {
-                    float4 po, vo;
-                    merge_pair(pa, va, pb, vb, po, vo);
-                    dst[2 * ab.x] = po;
-                    dst[2 * ab.x + 1] = vo;
-                    dst[2 * ab.y] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    dst[2 * ab.y + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    applied++;
-                }
-                done[t] = 1;
-            } else {
-                s_pending = 1;
-            }
-        }
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            uint2 ab = tasks[t];
-            __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint2 ab = tasks[t];
+            atomicMin(&reserve[ab.x], task_key(ab));
+            atomicMin(&reserve[ab.y], task_key(ab));
+            s_pending = 1;
         }
         __syncthreads();
         if (!s_pending) break;
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            if (done[t]) continue;
+            if (merge_commit(dst, tasks[t], reserve, applied)) done[t] = 2u;
+        }
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+            if (done[t] == 1u) continue;
+            const uint2 ab = tasks[t];
+            __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (done[t] == 2u) done[t] = 1u;
+        }
         __syncthreads();
     }
-    atomicAdd(&s_applied, applied);
-    __syncthreads();
-    if (threadIdx.x == 0) ctr->mergesApplied = s_applied;
+    for (int off = 32; off > 0; off >>= 1) applied += __shfl_xor(applied, off, 64);
+    if (applied && lane_id() == 0) atomicAdd(&ctr->mergesApplied, applied);
 }
 
 // ------------------------------------------------------------------ boundary layouts
@@ -987,6 +1029,10 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {
+    k_merge_round<<<kMergeGridBlocks, 256, 0, s>>>(dst, tasks, cap, ctr, reserve, done, 0);
+    for (int r = 0; r < kMergeGridRounds; r++)
+        for (int phase = 1; phase <= 3; phase++)
+            k_merge_round<<<kMergeGridBlocks, 256, 0, s>>>(dst, tasks, cap, ctr, reserve, done, phase);
     k_merge<<<1, 1024, 0, s>>>(dst, tasks, cap, ctr, reserve, done);
 }
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
