@@ -488,8 +488,6 @@ int bh_build_tree(bh_context* c, const bh_params* p) {
 
 int bh_compute_force(bh_context* c, const bh_params* p) {
     if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
-    if (p->merging_collision_or_neither == BH_COLLISION)
-        return fail(c, BH_ERR_UNSUPPORTED, "COLLISION mode (mergingCollisionOrNeither=2) is not implemented");
     if (p->merging_collision_or_neither > BH_COLLISION || p->static_or_dynamic > 1)
         return fail(c, BH_ERR_INVALID_ARGUMENT, "bad selector");
     WalkArgs a;
@@ -508,6 +506,11 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
     a.merging = p->merging_collision_or_neither;
     a.dynamic = p->static_or_dynamic;
     a.wrap = p->wrap_around;
+    a.elasticity = p->elasticity;
+    // the reference never uploads restitution to the force program
+    // (gpu/GPU.java:775-783 vs :442-444), so the shader's correction term
+    // multiplies by 0; bh_params.restitution is kept for the interface
+    a.restitution = 0.0f;
     if (p->flags & BH_FLAG_LEARNED) {
         if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
         if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)
@@ -528,8 +531,6 @@ int bh_merge_bodies(bh_context* c) {
 
 int bh_step(bh_context* c, const bh_params* p) {
     if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
-    if (p->merging_collision_or_neither == BH_COLLISION)
-        return fail(c, BH_ERR_UNSUPPORTED, "COLLISION mode (mergingCollisionOrNeither=2) is not implemented");
     HIPCHECK(c, hipSetDevice(c->device));
     c->timing = (p->flags & BH_FLAG_TIMING) != 0;
     int rc;

@@ -43,6 +43,7 @@ struct WalkArgs {
     uint32_t task_cap;
     float theta, dt, softening;
     uint32_t merging, dynamic, wrap;
+    float elasticity, restitution;   // COLLISION mode
 };
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.

@@ -29,16 +29,19 @@ struct WalkAcc {
 
 // The masked-stack walk of one wave over the tree rooted at `root`: lanes in
 // `mask` walk for the bodies at sorted positions base + lane (position g,
-// position-mass pm); leaf ids are below n_ids, internal node id k >= n_ids is
-// stored at nodes[k - n_ids]. s_node/s_mask are this wave's LDS stack of
-// kDepth entries. With merging set, leaf visits also emit merge tasks
-// (bh_force.comp:85-112).
-template <int kDepth>
+// position-mass pm, velocity-density vd); leaf ids are below n_ids, internal
+// node id k >= n_ids is stored at nodes[k - n_ids]. s_node/s_mask are this
+// wave's LDS stack of kDepth entries. mode (mergingCollisionOrNeither):
+// 1 = leaf visits emit merge tasks, 2 = an overlapping leaf changes this
+// lane's body at once, velocity impulse and position correction, and the rest
+// of the walk and the integration use the changed body (bh_force.comp:85-112).
+template <int kDepth, uint32_t mode>
 __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsigned long long* __restrict__ s_mask,
                                            uint32_t root, uint32_t n_ids, uint32_t base, uint32_t g,
-                                           unsigned long long mask, float4 pm, const float4* __restrict__ leaf_pos,
-                                           const WalkNode* __restrict__ nodes, float G, float theta, float soft,
-                                           bool merging, float Ri, uint32_t bodyIdx, float blc,
+                                           unsigned long long mask, float4& pm, float4& vd,
+                                           const float4* __restrict__ leaf_pos, const WalkNode* __restrict__ nodes,
+                                           float G, float theta, float soft, float Ri,
+                                           uint32_t bodyIdx, float blc, float elasticity, float restitution,
                                            const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
                                            DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
                                            uint32_t task_cap, WalkAcc& o) {
@@ -117,18 +120,36 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             const unsigned long long selfbit =
                 (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
             inter += (unsigned long long)__popcll(mask & ~selfbit);
-            if (merging && (mask & ~selfbit)) {
-                // bh_force.comp:85-112: overlap test against the leaf's body,
-                // merge task from the lower-index body
+            if constexpr (mode != 0u) if (mask & ~selfbit) {
+                // bh_force.comp:85-112: overlap test against the leaf's body
                 const uint32_t other = uniform(index_in[node]);
-                const float rho = src[2 * other + 1].w;
-                const float Rj = body_radius(lp.w, rho, blc);
+                const float4 ov = src[2 * other + 1];
+                const float Rj = body_radius(lp.w, ov.w, blc);
                 const float dist = fsqrt(dot3(rx, ry, rz));
                 const float pen = fsub(fadd(Ri, Rj), dist);
-                if (use && pen > 0.0f && bodyIdx < other) {
-                    uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
-                    if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                    else err |= 2u;
+                if constexpr (mode == 1u) {
+                    // merge task from the lower-index body
+                    if (use && pen > 0.0f && bodyIdx < other) {
+                        uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
+                        if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
+                        else err |= 2u;
+                    }
+                } else if (use && pen > 0.0f) {
+                    // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
+                    const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
+                    const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
+                    const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
+                    if (vimp < 0.0f) {
+                        const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
+                        const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
+                        vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
+                        vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
+                        vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
+                    }
+                    const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
+                    pm.x = fsub(pm.x, fmul(cs, nx));
+                    pm.y = fsub(pm.y, fmul(cs, ny));
+                    pm.z = fsub(pm.z, fmul(cs, nz));
                 }
             }
         }
@@ -152,13 +173,14 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
+template <uint32_t kMode>
 __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
                                               const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
-                                              uint32_t merging_mode, uint32_t dynamic, uint32_t wrap) {
+                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution) {
     __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
     __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
 
@@ -171,17 +193,17 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
     const uint32_t g = base + lane;
     const bool active = g < num;
     const uint32_t bodyIdx = active ? index_in[g] : 0;
-    const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
+    float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float G = sim->gravitationalConstant;
     const float blc = sim->bodyLengthConstant;
-    const bool merging = merging_mode == 1u;
-    const float Ri = (merging && active) ? body_radius(pm.w, vd.w, blc) : 0.0f;
+    const float Ri = (kMode != 0u && active) ? body_radius(pm.w, vd.w, blc) : 0.0f;
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
     if (num > 1)
-        walk_chunk<kStackDepth>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, leaf_pos, nodes, G, theta_,
-                                softening, merging, Ri, bodyIdx, blc, index_in, src, ctr, tasks, task_cap, o);
+        walk_chunk<kStackDepth, kMode>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd, leaf_pos, nodes,
+                                       G, theta_, softening, Ri, bodyIdx, blc, elasticity, restitution, index_in, src,
+                                       ctr, tasks, task_cap, o);
     const float ax = o.ax, ay = o.ay, az = o.az;
     const unsigned long long opened = o.opened, inter = o.inter;
     const uint32_t maxsp = o.maxsp, err = o.err;
@@ -235,9 +257,14 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
-    k_walk<<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in,
-                                              a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,
-                                              a.softening, a.merging, a.dynamic, a.wrap);
+#define BH_WALK(M)                                                                                               \
+    k_walk<M><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+                                                a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
+                                                a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution)
+    if (a.merging == 1u) BH_WALK(1u);
+    else if (a.merging == 2u) BH_WALK(2u);
+    else BH_WALK(0u);
+#undef BH_WALK
 }
 
 // Ensemble walk (no reference counterpart): a persistent grid whose waves
@@ -276,11 +303,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             const uint32_t base = tk.start, g = base + lane;
             const bool active = lane < tk.count;
             const uint32_t bodyIdx = active ? index_in[g] : 0;
-            const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
+            float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-            walk_chunk<kDepth>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, leaf_pos, nodes, G, theta,
-                               soft, false, 0.0f, bodyIdx, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+            walk_chunk<kDepth, 0u>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos, nodes, G,
+                                   theta, soft, 0.0f, bodyIdx, 0.0f, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
                 const float nvx = fadd(vd.x, fmul(o.ax, dt));
                 const float nvy = fadd(vd.y, fmul(o.ay, dt));

@@ -117,6 +117,9 @@ def scenario_list():
                    merging=0, dynamic=1, wrap=0))
     sc.append(dict(name="merge_4096", bodies=ic.cube_plus_core(4096, 11, density=1e3), steps=3, theta=0.5,
                    dt=5e-4, soft=1e-6, merging=1, dynamic=1, wrap=0))
+    # COLLISION mode: impulses and (restitution 0) position corrections mid-walk
+    sc.append(dict(name="collision_4096", bodies=ic.cube_plus_core(4096, 13, density=1e3), steps=2, theta=0.5,
+                   dt=5e-4, soft=1e-6, merging=2, dynamic=1, wrap=0))
     # dead bodies, NaN/Inf and far outliers, N not a multiple of 256
     b = ic.plummer(1000, 21)
     rng = np.random.default_rng(5)

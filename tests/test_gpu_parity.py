@@ -161,8 +161,8 @@ def test_gpu_stage_parity(name, s):
     bh.close()
 
 
-@pytest.mark.parametrize("name", ["plummer_4096", "edge_dead_nan_1000", "edge_duplicates_512", "static_kill_1024",
-                                  "static_wrap_1024", "theta0_512"])
+@pytest.mark.parametrize("name", ["plummer_4096", "collision_4096", "edge_dead_nan_1000", "edge_duplicates_512",
+                                  "static_kill_1024", "static_wrap_1024", "theta0_512"])
 def test_gpu_free_running_steps(name):
     """bh_step from the initial bodies reproduces the reference trajectory bit for bit."""
     from gravity_amd import barnes_hut as B

@@ -129,7 +129,7 @@ def test_stage_parity(name, s):
     np.testing.assert_allclose(mine[:, 3].astype(np.float64).sum(), ref_m[:, 3].astype(np.float64).sum(), rtol=1e-6)
 
 
-@pytest.mark.parametrize("name", ["plummer_4096", "edge_dead_nan_1000", "edge_duplicates_512",
+@pytest.mark.parametrize("name", ["plummer_4096", "collision_4096", "edge_dead_nan_1000", "edge_duplicates_512",
                                   "static_kill_1024", "static_wrap_1024", "theta0_512"])
 def test_free_running_trajectory(name):
     """Without merges the whole multi-step trajectory is deterministic: the
