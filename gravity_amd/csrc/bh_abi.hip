@@ -60,6 +60,7 @@ struct bh_context {
     unsigned long long* reserve = nullptr;
     uint32_t* task_done = nullptr;
     NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1
+    float* leaf_radius = nullptr;             // body radius per sorted position (merging/collision walks)
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
     float theta_plain = 0.5f;
@@ -252,6 +253,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->reserve, (size_t)n));
         TRY(dalloc(c, &c->task_done, (size_t)n));
         TRY(dalloc(c, &c->staging_nodes, (size_t)n));
+        TRY(dalloc(c, &c->leaf_radius, (size_t)n));
         // zero everything the reference zero-initialises, then pack sim
         // (gpu/GPU.java:1096-1143)
         for (int k = 0; k < 2; k++) {
@@ -299,7 +301,7 @@ void bh_destroy(bh_context* c) {
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.nodes,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
-                    c->reserve, c->task_done, c->staging_nodes};
+                    c->reserve, c->task_done, c->staging_nodes, c->leaf_radius};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -511,6 +513,7 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
     // (gpu/GPU.java:775-783 vs :442-444), so the shader's correction term
     // multiplies by 0; bh_params.restitution is kept for the interface
     a.restitution = 0.0f;
+    a.leaf_radius = c->leaf_radius;
     if (p->flags & BH_FLAG_LEARNED) {
         if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
         if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)

@@ -44,6 +44,7 @@ struct WalkArgs {
     float theta, dt, softening;
     uint32_t merging, dynamic, wrap;
     float elasticity, restitution;   // COLLISION mode
+    float* leaf_radius;              // [n] scratch of the merging and collision modes
 };
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.

@@ -20,6 +20,20 @@ __device__ __forceinline__ float body_radius(float m, float rho, float blc) {
     return fmul(blc, powf(fdiv(m, rho), 1.0f / 3.0f));
 }
 
+// Radius of the body at every sorted position, for the overlap tests of the
+// merging and collision modes: the walk reads it per leaf instead of
+// evaluating pow for every (lane, leaf) visit. Same function, same inputs,
+// same bits as the per-visit evaluation.
+__global__ __launch_bounds__(256) void k_leaf_radius(const SimValues* __restrict__ sim,
+                                                     const float4* __restrict__ src,
+                                                     const uint32_t* __restrict__ index_in,
+                                                     const float4* __restrict__ leaf_pos,
+                                                     float* __restrict__ leaf_radius) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= sim->numBodies) return;
+    leaf_radius[g] = body_radius(leaf_pos[g].w, src[2 * index_in[g] + 1].w, sim->bodyLengthConstant);
+}
+
 // Accumulated results of one wave's walk.
 struct WalkAcc {
     float ax, ay, az;
@@ -41,7 +55,8 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
                                            unsigned long long mask, float4& pm, float4& vd,
                                            const float4* __restrict__ leaf_pos, const WalkNode* __restrict__ nodes,
                                            float G, float theta, float soft, float Ri,
-                                           uint32_t bodyIdx, float blc, float elasticity, float restitution,
+                                           const float* __restrict__ leaf_radius, uint32_t bodyIdx,
+                                           float elasticity, float restitution,
                                            const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
                                            DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
                                            uint32_t task_cap, WalkAcc& o) {
@@ -122,34 +137,37 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             inter += (unsigned long long)__popcll(mask & ~selfbit);
             if constexpr (mode != 0u) if (mask & ~selfbit) {
                 // bh_force.comp:85-112: overlap test against the leaf's body
-                const uint32_t other = uniform(index_in[node]);
-                const float4 ov = src[2 * other + 1];
-                const float Rj = body_radius(lp.w, ov.w, blc);
+                const float Rj = leaf_radius[node];
                 const float dist = fsqrt(dot3(rx, ry, rz));
                 const float pen = fsub(fadd(Ri, Rj), dist);
-                if constexpr (mode == 1u) {
-                    // merge task from the lower-index body
-                    if (use && pen > 0.0f && bodyIdx < other) {
-                        uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
-                        if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                        else err |= 2u;
+                const bool hit = use && pen > 0.0f;
+                if (ballot(hit)) {
+                    const uint32_t other = uniform(index_in[node]);
+                    if constexpr (mode == 1u) {
+                        // merge task from the lower-index body
+                        if (hit && bodyIdx < other) {
+                            uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
+                            if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
+                            else err |= 2u;
+                        }
+                    } else if (hit) {
+                        // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
+                        const float4 ov = src[2 * other + 1];
+                        const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
+                        const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
+                        const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
+                        if (vimp < 0.0f) {
+                            const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
+                            const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
+                            vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
+                            vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
+                            vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
+                        }
+                        const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
+                        pm.x = fsub(pm.x, fmul(cs, nx));
+                        pm.y = fsub(pm.y, fmul(cs, ny));
+                        pm.z = fsub(pm.z, fmul(cs, nz));
                     }
-                } else if (use && pen > 0.0f) {
-                    // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
-                    const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
-                    const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
-                    const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
-                    if (vimp < 0.0f) {
-                        const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
-                        const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
-                        vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
-                        vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
-                        vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
-                    }
-                    const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
-                    pm.x = fsub(pm.x, fmul(cs, nx));
-                    pm.y = fsub(pm.y, fmul(cs, ny));
-                    pm.z = fsub(pm.z, fmul(cs, nz));
                 }
             }
         }
@@ -180,7 +198,8 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                                               const float4* __restrict__ leaf_pos,
                                               const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
-                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution) {
+                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
+                                              const float* __restrict__ leaf_radius) {
     __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
     __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
 
@@ -196,14 +215,13 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
     float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
     const float G = sim->gravitationalConstant;
-    const float blc = sim->bodyLengthConstant;
-    const float Ri = (kMode != 0u && active) ? body_radius(pm.w, vd.w, blc) : 0.0f;
+    const float Ri = (kMode != 0u && active) ? leaf_radius[g] : 0.0f;
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
     if (num > 1)
         walk_chunk<kStackDepth, kMode>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd, leaf_pos, nodes,
-                                       G, theta_, softening, Ri, bodyIdx, blc, elasticity, restitution, index_in, src,
-                                       ctr, tasks, task_cap, o);
+                                       G, theta_, softening, Ri, leaf_radius, bodyIdx, elasticity, restitution,
+                                       index_in, src, ctr, tasks, task_cap, o);
     const float ax = o.ax, ay = o.ay, az = o.az;
     const unsigned long long opened = o.opened, inter = o.inter;
     const uint32_t maxsp = o.maxsp, err = o.err;
@@ -260,7 +278,10 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
 #define BH_WALK(M)                                                                                               \
     k_walk<M><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
-                                                a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution)
+                                                a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
+                                                a.leaf_radius)
+    if (a.merging != 0u)
+        k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
     if (a.merging == 1u) BH_WALK(1u);
     else if (a.merging == 2u) BH_WALK(2u);
     else BH_WALK(0u);
@@ -307,7 +328,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
             walk_chunk<kDepth, 0u>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos, nodes, G,
-                                   theta, soft, 0.0f, bodyIdx, 0.0f, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+                                   theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
                 const float nvx = fadd(vd.x, fmul(o.ax, dt));
                 const float nvy = fadd(vd.y, fmul(o.ay, dt));
