@@ -244,6 +244,8 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
         TRY(dalloc(c, &T.xchg, 8 * ((size_t)n - 1)));
+        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n)));
+        TRY(dalloc(c, &T.frontier_count, 1));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
@@ -300,7 +302,7 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.nodes,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.xchg, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
+                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -207,7 +207,8 @@ void bh_ensemble_destroy(bh_ensemble* e) {
     drop_graphs(e);
     void* ptrs[] = {e->sims, e->tiles, e->tasks, e->bodies[0], e->bodies[1], e->keys[0], e->keys[1], e->index[0],
                     e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.nodes, e->tree.node_aabb,
-                    e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->ctr, e->queues,
+                    e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->tree.frontier,
+                    e->tree.frontier_count, e->ctr, e->queues,
                     e->partials, e->hist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -330,6 +331,8 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &T.node_range, ni));
         TRY(ealloc(e, &T.meet, ni));
         TRY(ealloc(e, &T.xchg, 8 * ni));
+        TRY(ealloc(e, &T.frontier, 4 * (size_t)frontier_cap(N)));
+        TRY(ealloc(e, &T.frontier_count, 1));
         TRY(ealloc(e, &e->ctr, 1));
         TRY(ealloc(e, &e->queues, kEnsQueues));
         TRY(ealloc(e, &e->partials, 6 * (size_t)e->ntiles));

@@ -655,13 +655,35 @@ __device__ __forceinline__ NodeState leaf_at(const LeafSource& L, uint32_t j, in
 // sees them), and a leaf whose sibling is internal simply stops. Only parents
 // of two internal children use the arrival exchange and the coherent
 // records; that is about a third of the nodes of a radix tree.
-__device__ __forceinline__ void tree_climb(int i, const TreeSeg& S, NodeState me, const uint64_t* __restrict__ keys,
-                                           const LeafSource& L, const TreeArrays& T,
-                                           uint32_t* __restrict__ height_out, bool height_max, int colors) {
+//
+// The build runs in two launches. A thread climbs at most until it completes
+// a node of kTreeFrontier or more bodies (stop = true); it then stores that
+// node's summary in the frontier list and ends, so a wave holds its slot for
+// a few levels only. The second launch runs one thread per frontier node from
+// there to the root (stop = false); records and arrival slots left by the
+// first launch are read by the second.
+struct FrontierEntry {
+    float4 com, a, b;   // NodeState as put_state packs it
+    uint4 range;        // lo, hi (local), segment (ensemble simulation), unused
+};
+
+__device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, NodeState me,
+                                           const uint64_t* __restrict__ keys, const LeafSource& L,
+                                           const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
+                                           int colors, bool stop, uint32_t segment) {
     const int num = (int)S.num;
-    int lo = i, hi = i;
     for (;;) {
         if (lo == 0 && hi == num - 1) break;
+        if (stop && hi - lo + 1 >= (int)kTreeFrontier) {
+            const uint32_t slot = atomicAdd(T.frontier_count, 1u);
+            FrontierEntry e;
+            e.com = me.com;
+            e.a = make_float4(me.mn.x, me.mn.y, me.mn.z, me.mx.x);
+            e.b = make_float4(me.mx.y, me.mx.z, __uint_as_float(me.height), __uint_as_float(me.id));
+            e.range = make_uint4((uint32_t)lo, (uint32_t)hi, segment, 0u);
+            reinterpret_cast<FrontierEntry*>(T.frontier)[slot] = e;
+            return;
+        }
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
@@ -785,7 +807,35 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
     }
     const TreeSeg S = {0u, (uint32_t)num, 0u, n};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
-    tree_climb(i, S, me, keys, L, T, &ctr->treeHeight, false, colors);
+    tree_climb(i, i, S, me, keys, L, T, &ctr->treeHeight, false, colors, true, 0u);
+}
+
+// (the colour of a frontier node, when colours are on, is in node_color)
+__device__ __forceinline__ NodeState frontier_state(const FrontierEntry& e) {
+    NodeState me;
+    me.com = e.com;
+    me.mn = make_float4(e.a.x, e.a.y, e.a.z, 0.f);
+    me.mx = make_float4(e.a.w, e.b.x, e.b.y, 0.f);
+    me.height = __float_as_uint(e.b.z);
+    me.id = __float_as_uint(e.b.w);
+    me.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    return me;
+}
+
+// The second launch: one thread per frontier node, climbing to the root.
+__global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                    const float4* __restrict__ src,
+                                                    const uint32_t* __restrict__ index_in,
+                                                    const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *T.frontier_count) return;
+    const uint32_t n = sim->initialNumBodies;
+    const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
+    NodeState me = frontier_state(e);
+    if (colors) me.col = T.node_color[me.id - n];
+    const TreeSeg S = {0u, sim->numBodies, 0u, n};
+    const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
+    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors, false, 0u);
 }
 
 // ------------------------------------------------------------------ merge
@@ -1025,7 +1075,9 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors) {
+    k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
     k_tree<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T, colors);
+    k_tree_upper<<<cdiv(frontier_cap(n), 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T, colors);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {
@@ -1275,7 +1327,22 @@ __global__ __launch_bounds__(256) void k_ens_tree(const EnsSim* __restrict__ sim
     T.leaf_pos[i] = pm;
     const TreeSeg seg = {S.off, S.num, S.ibase, N};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
-    tree_climb((int)(i - S.off), seg, leaf_state(pm, i), keys + S.off, L, T, &ctr->treeHeight, true, 0);
+    const int li = (int)(i - S.off);
+    tree_climb(li, li, seg, leaf_state(pm, i), keys + S.off, L, T, &ctr->treeHeight, true, 0, true, lo);
+}
+
+__global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict__ sims, uint32_t N,
+                                                        DevCounters* __restrict__ ctr, const float4* __restrict__ src,
+                                                        const uint32_t* __restrict__ index_in,
+                                                        const uint64_t* __restrict__ keys, TreeArrays T) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *T.frontier_count) return;
+    const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
+    const EnsSim& S = sims[e.range.z];
+    const TreeSeg seg = {S.off, S.num, S.ibase, N};
+    const LeafSource L = {src, index_in, 0.0f, 0.0f};
+    tree_climb((int)e.range.x, (int)e.range.y, seg, frontier_state(e), keys + S.off, L, T, &ctr->treeHeight,
+               true, 0, false, e.range.z);
 }
 
 __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430* __restrict__ out) {
@@ -1333,7 +1400,9 @@ int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t n
 
 void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t N, DevCounters* ctr,
                      const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T) {
+    k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
     k_ens_tree<<<cdiv(N, 256), 256, 0, s>>>(sims, nsims, N, ctr, src, index_in, keys, T);
+    k_ens_tree_upper<<<cdiv(frontier_cap(N), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T);
 }
 
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out) {

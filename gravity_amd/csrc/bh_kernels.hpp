@@ -29,7 +29,14 @@ struct TreeArrays {
     float4* node_color;      // [n-1]
     uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
     float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
+    float4* frontier;        // [4 frontier_cap(n)] nodes where the first build launch stopped
+    uint32_t* frontier_count;
 };
+
+// The first tree launch stops a climb at a node of this many bodies or more.
+// Those nodes are disjoint subtrees, so there are at most n / kTreeFrontier.
+constexpr uint32_t kTreeFrontier = 64;
+inline uint32_t frontier_cap(uint32_t n) { return n / kTreeFrontier + 1; }
 
 struct WalkArgs {
     const SimValues* sim;
