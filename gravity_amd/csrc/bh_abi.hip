@@ -244,7 +244,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
         TRY(dalloc(c, &T.xchg, 8 * ((size_t)n - 1)));
-        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n)));
+        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n, (n + kTreeBlock - 1) / kTreeBlock)));
         TRY(dalloc(c, &T.frontier_count, 1));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;

@@ -98,6 +98,14 @@ __device__ __forceinline__ float rcp_cr_normal(float y) {
     float e = __builtin_fmaf(-y, r, 1.0f);
     return __builtin_fmaf(e, r, r);
 }
+// Whether every d2 = |r|^2 + soft of a walk lies in the fast domain: r joins
+// two points of the root's box (bodies and centres of mass), so |r|^2 is at
+// most 12 half^2 for the root's half longest side; a NaN or infinite box
+// fails the test.
+__device__ __forceinline__ bool inv_sqrt_fast_domain(float root_half, float soft) {
+    return soft >= 1e-30f && fadd(fmul(fmul(root_half, root_half), 16.0f), soft) <= 1e29f;
+}
+__device__ __forceinline__ float inv_sqrt_fast(float d2) { return rcp_cr_normal(sqrt_cr_normal(d2)); }
 __device__ __forceinline__ float inv_sqrt_exact(float d2) {
     // fast sequence on [2^-100, 2^100], the general IEEE one elsewhere
     if (d2 >= 7.88860905e-31f && d2 <= 1.26765060e30f) return rcp_cr_normal(sqrt_cr_normal(d2));

@@ -138,8 +138,8 @@ int enqueue_step(bh_ensemble* e) {
     launch_ens_encode(e->stream, e->sims, e->tiles, e->ntiles, e->bodies[b], e->index[e->mi_in], e->keys[e->mi_in]);
     int rc = enqueue_sort(e, e->cfg.radix_bits);
     if (rc) return rc;
-    launch_ens_tree(e->stream, e->sims, e->S, e->N, e->ctr, e->bodies[b], e->index[e->mi_in], e->keys[e->mi_in],
-                    e->tree);
+    launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b], e->index[e->mi_in],
+                    e->keys[e->mi_in], e->tree);
     if ((rc = enqueue_walk(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, e->walk_blocks))) return rc;
     e->body_in = 1 - b;
     return BH_OK;
@@ -331,7 +331,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &T.node_range, ni));
         TRY(ealloc(e, &T.meet, ni));
         TRY(ealloc(e, &T.xchg, 8 * ni));
-        TRY(ealloc(e, &T.frontier, 4 * (size_t)frontier_cap(N)));
+        TRY(ealloc(e, &T.frontier, 4 * (size_t)frontier_cap(N, 2 * e->ntiles)));
         TRY(ealloc(e, &T.frontier_count, 1));
         TRY(ealloc(e, &e->ctr, 1));
         TRY(ealloc(e, &e->queues, kEnsQueues));
@@ -523,7 +523,7 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
         if (rc) break;
         // walk shape on the tree of the current state
         if ((rc = prepare()) || (rc = enqueue_sort(e, best.radix_bits))) break;
-        launch_ens_tree(e->stream, e->sims, e->S, e->N, e->ctr, e->bodies[b0], e->index[e->mi_in],
+        launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b0], e->index[e->mi_in],
                         e->keys[e->mi_in], e->tree);
         DevCounters d;
         if (hipMemcpyAsync(&d, e->ctr, sizeof d, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||

@@ -9,7 +9,7 @@
 //   k_encode               [COMPUTE_MORTON_ENCODE, morton/bh_morton.comp:6-34,238-262]
 //   k_sort_hist/scan/scatter
 //                          [COMPUTE_RADIX_*, radix/bh_radix.comp:12-166]
-//   k_tree                 [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
+//   k_tree/k_tree_upper    [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
 //   (walk: bh_walk.hip)    [COMPUTE_FORCE_COMPUTE]
 //   k_merge                [COMPUTE_MERGE_BODIES, merge/bh_merge.comp:5-71]
 //   k_node_to_std430 / k_std430_to_node: the 80-byte Node layout at the boundary.
@@ -656,34 +656,74 @@ __device__ __forceinline__ NodeState leaf_at(const LeafSource& L, uint32_t j, in
 // of two internal children use the arrival exchange and the coherent
 // records; that is about a third of the nodes of a radix tree.
 //
-// The build runs in two launches. A thread climbs at most until it completes
-// a node of kTreeFrontier or more bodies (stop = true); it then stores that
-// node's summary in the frontier list and ends, so a wave holds its slot for
-// a few levels only. The second launch runs one thread per frontier node from
-// there to the root (stop = false); records and arrival slots left by the
-// first launch are read by the second.
+// The build runs in two launches. The first (tree_block) builds, per block of
+// kTreeBlock sorted positions, every node inside the block with the exchange
+// in LDS, and lists the roots of the subtrees it leaves. The second runs
+// tree_climb from each of them to the root, with the exchange in HBM.
 struct FrontierEntry {
     float4 com, a, b;   // NodeState as put_state packs it
-    uint4 range;        // lo, hi (local), segment (ensemble simulation), unused
+    uint4 range;        // lo, hi (local positions), segment (ensemble simulation), unused
 };
+
+// The summary of the parent of A (left child) and B over local positions
+// [plo, phi] with Karras index pid, stored in the node arrays (bh_tree.comp:183-259).
+__device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeState& B, int plo, int phi,
+                                                 uint32_t pid, const TreeSeg& S, const TreeArrays& T,
+                                                 uint32_t* __restrict__ height_out, bool height_max, int colors) {
+    NodeState p;
+    const float M = fadd(A.com.w, B.com.w);
+    if (M > 0.0f) {
+        p.com.x = fdiv(fadd(fmul(A.com.w, A.com.x), fmul(B.com.w, B.com.x)), M);
+        p.com.y = fdiv(fadd(fmul(A.com.w, A.com.y), fmul(B.com.w, B.com.y)), M);
+        p.com.z = fdiv(fadd(fmul(A.com.w, A.com.z), fmul(B.com.w, B.com.z)), M);
+        p.col.x = fdiv(fadd(fmul(A.com.w, A.col.x), fmul(B.com.w, B.col.x)), M);
+        p.col.y = fdiv(fadd(fmul(A.com.w, A.col.y), fmul(B.com.w, B.col.y)), M);
+        p.col.z = fdiv(fadd(fmul(A.com.w, A.col.z), fmul(B.com.w, B.col.z)), M);
+    } else {
+        p.com.x = fmul(fadd(A.com.x, B.com.x), 0.5f);
+        p.com.y = fmul(fadd(A.com.y, B.com.y), 0.5f);
+        p.com.z = fmul(fadd(A.com.z, B.com.z), 0.5f);
+        p.col.x = fmul(fadd(A.col.x, B.col.x), 0.5f);
+        p.col.y = fmul(fadd(A.col.y, B.col.y), 0.5f);
+        p.col.z = fmul(fadd(A.col.z, B.col.z), 0.5f);
+    }
+    p.com.w = M;
+    p.col.w = 0.0f;
+    p.mn = make_float4(gmin(A.mn.x, B.mn.x), gmin(A.mn.y, B.mn.y), gmin(A.mn.z, B.mn.z), 0.f);
+    p.mx = make_float4(gmax(A.mx.x, B.mx.x), gmax(A.mx.y, B.mx.y), gmax(A.mx.z, B.mx.z), 0.f);
+    p.height = 1u + (A.height > B.height ? A.height : B.height);
+    const uint32_t k = S.ibase + pid;                 // storage index of the parent
+    p.id = S.n_ids + k;
+    WalkNode nd;
+    nd.comMass = p.com;
+    const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
+    nd.half = fdiv(ext, 2.0f);
+    nd.childA = A.id;
+    nd.childB = B.id;
+    nd.height = p.height;
+    // these arrays are read only by later kernels
+    T.nodes[k] = nd;
+    T.node_aabb[2 * k] = p.mn;
+    T.node_aabb[2 * k + 1] = p.mx;
+    if (colors) T.node_color[k] = p.col;
+    T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
+    if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
+    if (B.id < S.n_ids) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - S.n_ids] = p.id;
+    if (pid == 0) {
+        T.node_parent[k] = kEmpty;
+        if (height_max) atomicMax(height_out, p.height);
+        else *height_out = p.height;
+    }
+    return p;
+}
 
 __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, NodeState me,
                                            const uint64_t* __restrict__ keys, const LeafSource& L,
                                            const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
-                                           int colors, bool stop, uint32_t segment) {
+                                           int colors) {
     const int num = (int)S.num;
     for (;;) {
         if (lo == 0 && hi == num - 1) break;
-        if (stop && hi - lo + 1 >= (int)kTreeFrontier) {
-            const uint32_t slot = atomicAdd(T.frontier_count, 1u);
-            FrontierEntry e;
-            e.com = me.com;
-            e.a = make_float4(me.mn.x, me.mn.y, me.mn.z, me.mx.x);
-            e.b = make_float4(me.mx.y, me.mx.z, __uint_as_float(me.height), __uint_as_float(me.id));
-            e.range = make_uint4((uint32_t)lo, (uint32_t)hi, segment, 0u);
-            reinterpret_cast<FrontierEntry*>(T.frontier)[slot] = e;
-            return;
-        }
         const bool left = (lo == 0) || (hi != num - 1 && delta_at(keys, num, hi) > delta_at(keys, num, lo - 1));
         const int split = left ? hi : lo - 1;
         // each side of a split has its own record; the sibling's is the other one
@@ -719,77 +759,209 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
             const bool pl = (plo == 0) || (phi != num - 1 && delta_at(keys, num, phi) > delta_at(keys, num, plo - 1));
             pid = (uint32_t)(pl ? phi : plo);
         }
-        const NodeState& A = left ? me : sib;             // childA is the left child
-        const NodeState& B = left ? sib : me;
-        NodeState p;
-        const float M = fadd(A.com.w, B.com.w);
-        if (M > 0.0f) {
-            p.com.x = fdiv(fadd(fmul(A.com.w, A.com.x), fmul(B.com.w, B.com.x)), M);
-            p.com.y = fdiv(fadd(fmul(A.com.w, A.com.y), fmul(B.com.w, B.com.y)), M);
-            p.com.z = fdiv(fadd(fmul(A.com.w, A.com.z), fmul(B.com.w, B.com.z)), M);
-            p.col.x = fdiv(fadd(fmul(A.com.w, A.col.x), fmul(B.com.w, B.col.x)), M);
-            p.col.y = fdiv(fadd(fmul(A.com.w, A.col.y), fmul(B.com.w, B.col.y)), M);
-            p.col.z = fdiv(fadd(fmul(A.com.w, A.col.z), fmul(B.com.w, B.col.z)), M);
-        } else {
-            p.com.x = fmul(fadd(A.com.x, B.com.x), 0.5f);
-            p.com.y = fmul(fadd(A.com.y, B.com.y), 0.5f);
-            p.com.z = fmul(fadd(A.com.z, B.com.z), 0.5f);
-            p.col.x = fmul(fadd(A.col.x, B.col.x), 0.5f);
-            p.col.y = fmul(fadd(A.col.y, B.col.y), 0.5f);
-            p.col.z = fmul(fadd(A.col.z, B.col.z), 0.5f);
-        }
-        p.com.w = M;
-        p.col.w = 0.0f;
-        p.mn = make_float4(gmin(A.mn.x, B.mn.x), gmin(A.mn.y, B.mn.y), gmin(A.mn.z, B.mn.z), 0.f);
-        p.mx = make_float4(gmax(A.mx.x, B.mx.x), gmax(A.mx.y, B.mx.y), gmax(A.mx.z, B.mx.z), 0.f);
-        p.height = 1u + (A.height > B.height ? A.height : B.height);
-        const uint32_t k = S.ibase + pid;                 // storage index of the parent
-        p.id = S.n_ids + k;
-        WalkNode nd;
-        nd.comMass = p.com;
-        const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
-        nd.half = fdiv(ext, 2.0f);
-        nd.childA = A.id;
-        nd.childB = B.id;
-        nd.height = p.height;
-        // these arrays are read only by later kernels
-        T.nodes[k] = nd;
-        T.node_aabb[2 * k] = p.mn;
-        T.node_aabb[2 * k + 1] = p.mx;
-        if (colors) T.node_color[k] = p.col;
-        T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
-        if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
-        if (B.id < S.n_ids) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - S.n_ids] = p.id;
-        if (pid == 0) {
-            T.node_parent[k] = kEmpty;
-            if (height_max) atomicMax(height_out, p.height);
-            else *height_out = p.height;
-        }
-        me = p;
+        me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, colors)
+                  : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, colors);
         lo = plo;
         hi = phi;
     }
 }
 
-__global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                              const float4* __restrict__ src,
-                                              const uint32_t* __restrict__ index_in,
-                                              const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
+// ---- first launch: block-local build.
+// A block of kTreeBlock threads takes kTreeBlock consecutive sorted positions
+// [blo, bhi] of one segment and builds every internal node whose range lies
+// inside them, with the same agglomeration as tree_climb but with the
+// arrival slots, the records and the split lengths in LDS. A climb stops at a
+// node whose parent split is outside [blo, bhi - 1] (an edge node: at most one
+// per side), and a first arrival whose sibling never comes (its sibling
+// crosses the block's edge) is left in its slot. Both kinds go to the
+// frontier list for the second launch. Each side of a block is crossed by a
+// chain of nested nodes, one per level, so a block leaves at most
+// 2 * 98 + 2 entries (a tree over 96-bit keys is at most 98 levels high).
+//
+// Records are indexed by the child's own Karras index (hi for a left child,
+// lo for a right one), which is unique among internal nodes; the arrival slot
+// of split j holds the first arrival's far end, with kSideRight set for a
+// right child.
+constexpr uint32_t kSideRight = 0x80000000u;
+
+template <bool kColors>
+struct TreeBlockLds {
+    static constexpr int kRec = kColors ? 4 : 3;   // float4 words of one record
+    int d[kTreeBlock + 1];                         // delta(blo - 1 + j)
+    uint32_t meet[kTreeBlock];                     // arrival slot of split blo + j
+    float4 pm[kTreeBlock];                         // posMass of leaf blo + j
+    float4 col[kColors ? kTreeBlock : 1];          // colour of leaf blo + j
+    float4 rec[kRec * kTreeBlock];                 // summary of internal node blo + j
+    float4 edge[2][kRec];                          // edge nodes: left child at split bhi, right child at blo - 1
+    int2 edge_range[2];
+    uint32_t nf, base;
+};
+
+template <bool kColors>
+__device__ __forceinline__ void lds_put(float4* x, const NodeState& s) {
+    x[0] = s.com;
+    x[1] = make_float4(s.mn.x, s.mn.y, s.mn.z, s.mx.x);
+    x[2] = make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id));
+    if (kColors) x[3] = s.col;
+}
+
+template <bool kColors>
+__device__ __forceinline__ NodeState lds_get(const float4* x) {
+    NodeState s;
+    s.com = x[0];
+    const float4 a = x[1], b = x[2];
+    s.mn = make_float4(a.x, a.y, a.z, 0.f);
+    s.mx = make_float4(a.w, b.x, b.y, 0.f);
+    s.height = __float_as_uint(b.z);
+    s.id = __float_as_uint(b.w);
+    s.col = kColors ? x[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+    return s;
+}
+
+__device__ __forceinline__ FrontierEntry frontier_entry(const NodeState& me, int lo, int hi, uint32_t segment) {
+    FrontierEntry e;
+    e.com = me.com;
+    e.a = make_float4(me.mn.x, me.mn.y, me.mn.z, me.mx.x);
+    e.b = make_float4(me.mx.y, me.mx.z, __uint_as_float(me.height), __uint_as_float(me.id));
+    e.range = make_uint4((uint32_t)lo, (uint32_t)hi, segment, 0u);
+    return e;
+}
+
+// Positions [blo, blo + cnt) of segment S (local positions; keys given from
+// the segment's first key); thread t holds leaf blo + t. Called by every
+// thread of the block (it has barriers).
+template <bool kColors>
+__device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeSeg& S, int blo, int cnt,
+                                           const uint64_t* __restrict__ keys, const LeafSource& L,
+                                           const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
+                                           uint32_t segment) {
+    constexpr int R = TreeBlockLds<kColors>::kRec;
+    const int num = (int)S.num;
+    const int t = threadIdx.x;
+    const int bhi = blo + cnt - 1;
+    if (t < cnt) {
+        const uint32_t g = S.off + (uint32_t)(blo + t);
+        const uint32_t b = L.index_in[g];
+        const float4 pm = L.src[2 * b];
+        T.leaf_pos[g] = pm;
+        s.pm[t] = pm;
+        if (kColors) {
+            const float4 vd = L.src[2 * b + 1];
+            float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (pm.w > 0.0f && vd.w > 0.0f) c = star_color(fmul(pm.w, L.unit_mass), fmul(vd.w, L.unit_density));
+            T.leaf_color[g] = c;
+            s.col[t] = c;
+        }
+    }
+    s.d[t + 1] = delta_at(keys, num, blo + t);
+    if (t == 0) {
+        s.d[0] = delta_at(keys, num, blo - 1);
+        s.nf = 0;
+        s.edge_range[0] = s.edge_range[1] = make_int2(-1, -1);
+    }
+    s.meet[t] = kEmpty;
+    __syncthreads();
+    auto D = [&](int x) { return s.d[x - blo + 1]; };   // x in [blo - 1, bhi]
+    if (t < cnt) {
+        int lo = blo + t, hi = lo;
+        NodeState me = leaf_state(s.pm[t], S.off + (uint32_t)lo);
+        if (kColors) me.col = s.col[t];
+        for (;;) {
+            if (lo == 0 && hi == num - 1) break;
+            const bool left = (lo == 0) || (hi != num - 1 && D(hi) > D(lo - 1));
+            const int split = left ? hi : lo - 1;
+            if (split < blo || split >= bhi) {             // the parent crosses the block's edge
+                const int side = left ? 0 : 1;
+                lds_put<kColors>(s.edge[side], me);
+                s.edge_range[side] = make_int2(lo, hi);
+                break;
+            }
+            const bool sib_leaf = left ? (split + 1 == num - 1 || D(split + 1) <= D(split))
+                                       : (split == 0 || D(split) > D(split - 1));
+            int other;
+            NodeState sib;
+            if (sib_leaf) {
+                if (lo == hi && !left) break;              // two leaves: the left one builds the parent
+                other = left ? split + 1 : split;
+                sib = leaf_state(s.pm[other - blo], S.off + (uint32_t)other);
+                if (kColors) sib.col = s.col[other - blo];
+            } else {
+                if (lo == hi) break;                       // the internal sibling's thread builds the parent
+                // the exchange orders LDS only: the fences are for the local
+                // address space, so they do not wait for this wave's node stores
+                lds_put<kColors>(&s.rec[R * ((left ? hi : lo) - blo)], me);
+                const uint32_t v = left ? (uint32_t)lo : ((uint32_t)hi | kSideRight);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                const uint32_t old =
+                    __hip_atomic_exchange(&s.meet[split - blo], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old == kEmpty) break;                  // first to arrive
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                s.meet[split - blo] = kEmpty;              // only this thread touches the slot now
+                other = (int)(old & ~kSideRight);
+                sib = lds_get<kColors>(&s.rec[R * ((left ? split + 1 : split) - blo)]);
+            }
+            const int plo = left ? lo : other;
+            const int phi = left ? other : hi;
+            uint32_t pid;
+            if (plo == 0 && phi == num - 1) {
+                pid = 0;
+            } else {
+                const bool pl = (plo == 0) || (phi != num - 1 && D(phi) > D(plo - 1));
+                pid = (uint32_t)(pl ? phi : plo);
+            }
+            me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, kColors)
+                      : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, kColors);
+            lo = plo;
+            hi = phi;
+        }
+    }
+    __syncthreads();
+    // first arrivals left waiting, and the edge nodes
+    FrontierEntry e0, e1;
+    int slot0 = -1, slot1 = -1;
+    if (t < cnt - 1) {
+        const uint32_t v = s.meet[t];
+        if (v != kEmpty) {
+            const int split = blo + t, far = (int)(v & ~kSideRight);
+            const bool right = (v & kSideRight) != 0;
+            const int lo = right ? split + 1 : far, hi = right ? far : split;
+            e0 = frontier_entry(lds_get<kColors>(&s.rec[R * ((right ? lo : hi) - blo)]), lo, hi, segment);
+            slot0 = (int)atomicAdd(&s.nf, 1u);
+        }
+    }
+    if (t < 2 && s.edge_range[t].x >= 0) {
+        e1 = frontier_entry(lds_get<kColors>(s.edge[t]), s.edge_range[t].x, s.edge_range[t].y, segment);
+        slot1 = (int)atomicAdd(&s.nf, 1u);
+    }
+    __syncthreads();
+    if (t == 0 && s.nf) s.base = atomicAdd(T.frontier_count, s.nf);
+    __syncthreads();
+    FrontierEntry* F = reinterpret_cast<FrontierEntry*>(T.frontier);
+    if (slot0 >= 0) F[s.base + slot0] = e0;
+    if (slot1 >= 0) F[s.base + slot1] = e1;
+}
+
+template <bool kColors>
+__global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict__ sim,
+                                                     DevCounters* __restrict__ ctr, const float4* __restrict__ src,
+                                                     const uint32_t* __restrict__ index_in,
+                                                     const uint64_t* __restrict__ keys, TreeArrays T) {
+    __shared__ TreeBlockLds<kColors> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num) return;
-    const uint32_t b = index_in[i];
-    const float4 pm = src[2 * b];
-    NodeState me = leaf_state(pm, (uint32_t)i);
-    T.leaf_pos[i] = pm;
-    if (colors) {
-        float4 vd = src[2 * b + 1];
-        if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
-        T.leaf_color[i] = me.col;
-    }
+    const int blo = (int)(blockIdx.x * kTreeBlock);
+    if (blo >= num) return;
     if (num == 1) {
+        if (threadIdx.x != 0) return;
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
+        const float4 pm = src[2 * index_in[0]];
+        T.leaf_pos[0] = pm;
+        if (kColors) {
+            const float4 vd = src[2 * index_in[0] + 1];
+            float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (pm.w > 0.0f && vd.w > 0.0f) c = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
+            T.leaf_color[0] = c;
+        }
         WalkNode r;
         r.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
         r.half = -1e38f;
@@ -807,7 +979,8 @@ __global__ __launch_bounds__(256) void k_tree(const SimValues* __restrict__ sim,
     }
     const TreeSeg S = {0u, (uint32_t)num, 0u, n};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
-    tree_climb(i, i, S, me, keys, L, T, &ctr->treeHeight, false, colors, true, 0u);
+    const int cnt = num - blo < (int)kTreeBlock ? num - blo : (int)kTreeBlock;
+    tree_block<kColors>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
 }
 
 // (the colour of a frontier node, when colours are on, is in node_color)
@@ -832,10 +1005,10 @@ __global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict_
     const uint32_t n = sim->initialNumBodies;
     const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
     NodeState me = frontier_state(e);
-    if (colors) me.col = T.node_color[me.id - n];
+    if (colors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
     const TreeSeg S = {0u, sim->numBodies, 0u, n};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
-    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors, false, 0u);
+    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors);
 }
 
 // ------------------------------------------------------------------ merge
@@ -1076,8 +1249,10 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    k_tree<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T, colors);
-    k_tree_upper<<<cdiv(frontier_cap(n), 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T, colors);
+    if (colors) k_tree<true><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T);
+    else k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T);
+    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T,
+                                                                                  colors);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {
@@ -1309,26 +1484,24 @@ __global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restr
     scatter_tile<BITS, kEnsTile>(L, keys_in, idx_in, tl.start, tl.count, keys_out, idx_out, shift);
 }
 
-// Tree of every simulation in one launch: thread = global sorted position.
-__global__ __launch_bounds__(256) void k_ens_tree(const EnsSim* __restrict__ sims, uint32_t nsims, uint32_t N,
-                                                  DevCounters* __restrict__ ctr, const float4* __restrict__ src,
-                                                  const uint32_t* __restrict__ index_in,
-                                                  const uint64_t* __restrict__ keys, TreeArrays T) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    uint32_t lo = 0, hi = nsims - 1;   // last simulation whose first position is <= i
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (sims[mid].off <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    const EnsSim& S = sims[lo];
-    const float4 pm = src[2 * index_in[i]];
-    T.leaf_pos[i] = pm;
+// Tree of every simulation: the block-local first launch over two blocks per
+// tile (tiles never span simulations).
+__global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restrict__ sims,
+                                                         const EnsTile* __restrict__ tiles, uint32_t N,
+                                                         DevCounters* __restrict__ ctr,
+                                                         const float4* __restrict__ src,
+                                                         const uint32_t* __restrict__ index_in,
+                                                         const uint64_t* __restrict__ keys, TreeArrays T) {
+    __shared__ TreeBlockLds<false> s;
+    const EnsTile tl = tiles[blockIdx.x >> 1];
+    const uint32_t first = (blockIdx.x & 1u) * kTreeBlock;
+    if (first >= tl.count) return;
+    const EnsSim& S = sims[tl.sim];
+    const int cnt = tl.count - first < kTreeBlock ? (int)(tl.count - first) : (int)kTreeBlock;
     const TreeSeg seg = {S.off, S.num, S.ibase, N};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
-    const int li = (int)(i - S.off);
-    tree_climb(li, li, seg, leaf_state(pm, i), keys + S.off, L, T, &ctr->treeHeight, true, 0, true, lo);
+    tree_block<false>(s, seg, (int)(tl.start + first - S.off), cnt, keys + S.off, L, T, &ctr->treeHeight, true,
+                      tl.sim);
 }
 
 __global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict__ sims, uint32_t N,
@@ -1342,7 +1515,7 @@ __global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict
     const TreeSeg seg = {S.off, S.num, S.ibase, N};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
     tree_climb((int)e.range.x, (int)e.range.y, seg, frontier_state(e), keys + S.off, L, T, &ctr->treeHeight,
-               true, 0, false, e.range.z);
+               true, 0);
 }
 
 __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430* __restrict__ out) {
@@ -1398,11 +1571,12 @@ int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t n
     return 0;
 }
 
-void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t N, DevCounters* ctr,
-                     const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T) {
+void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, uint32_t N,
+                     DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
+                     const TreeArrays& T) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    k_ens_tree<<<cdiv(N, 256), 256, 0, s>>>(sims, nsims, N, ctr, src, index_in, keys, T);
-    k_ens_tree_upper<<<cdiv(frontier_cap(N), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T);
+    k_ens_tree<<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T);
+    k_ens_tree_upper<<<cdiv(frontier_cap(N, 2 * ntiles), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T);
 }
 
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out) {

@@ -29,14 +29,21 @@ struct TreeArrays {
     float4* node_color;      // [n-1]
     uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
     float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
-    float4* frontier;        // [4 frontier_cap(n)] nodes where the first build launch stopped
+    float4* frontier;        // [4 frontier_cap] roots of the subtrees the first build launch left
     uint32_t* frontier_count;
 };
 
-// The first tree launch stops a climb at a node of this many bodies or more.
-// Those nodes are disjoint subtrees, so there are at most n / kTreeFrontier.
-constexpr uint32_t kTreeFrontier = 64;
-inline uint32_t frontier_cap(uint32_t n) { return n / kTreeFrontier + 1; }
+// The first tree launch builds each block of kTreeBlock sorted positions on
+// its own. A block leaves at most one frontier entry per level of the two
+// chains of nodes crossing its edges, plus the two edge nodes (a tree over
+// 96-bit keys is at most 98 levels high); the entries are disjoint subtrees,
+// so there are also at most as many as positions.
+constexpr uint32_t kTreeBlock = 1024;
+constexpr uint32_t kTreeBlockFrontier = 2 * 98 + 2;
+inline uint32_t frontier_cap(uint32_t n, uint32_t blocks) {
+    const uint64_t b = (uint64_t)blocks * kTreeBlockFrontier;
+    return (uint32_t)(b < n ? b : n) + 1;
+}
 
 struct WalkArgs {
     const SimValues* sim;
@@ -78,6 +85,7 @@ struct EnsQueue {
 };
 static_assert(sizeof(EnsQueue) == 128, "EnsQueue layout");
 constexpr uint32_t kEnsTile = 2048;    // positions per sort/bounds/encode tile
+static_assert(kEnsTile == 2 * kTreeBlock, "an ensemble tile is two tree blocks");
 
 void launch_ens_reset(hipStream_t s, DevCounters* ctr, EnsQueue* queues);
 void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
@@ -88,8 +96,10 @@ void launch_ens_encode(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, 
 int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t nsims, const EnsTile* tiles,
                          uint32_t ntiles, const uint64_t* keys_in, const uint32_t* idx_in, uint64_t* keys_out,
                          uint32_t* idx_out, uint32_t shift, uint32_t* hist);
-void launch_ens_tree(hipStream_t s, const EnsSim* sims, uint32_t nsims, uint32_t N, DevCounters* ctr,
-                     const float4* src, const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T);
+// tree: two blocks of kTreeBlock positions per tile
+void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, uint32_t N,
+                     DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
+                     const TreeArrays& T);
 // persistent walk over the task queue; returns nonzero for an unsupported shape
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,

@@ -49,7 +49,7 @@ struct WalkAcc {
 // 1 = leaf visits emit merge tasks, 2 = an overlapping leaf changes this
 // lane's body at once, velocity impulse and position correction, and the rest
 // of the walk and the integration use the changed body (bh_force.comp:85-112).
-template <int kDepth, uint32_t mode>
+template <int kDepth, uint32_t mode, bool kFast>
 __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsigned long long* __restrict__ s_mask,
                                            uint32_t root, uint32_t n_ids, uint32_t base, uint32_t g,
                                            unsigned long long mask, float4& pm, float4& vd,
@@ -72,104 +72,113 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     uint32_t node = root;
     int sp = 0;
     const float theta_hi = theta * 1.0000153f;   // theta (1 + 2^-16), rounded up
+    const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
+    // Lane predicates are taken from 64-bit wave masks (inverse ballot: a
+    // lane's bit of a scalar mask) so that they cost no vector instructions.
     for (;;) {
-        const bool mine = (mask >> lane) & 1ull;
-        bool descend = false;
-        if (node >= n_ids) {
+        // one body-node interaction per lane, for an internal node (its centre
+        // of mass, if the lane accepts it) or a leaf (its body, unless it is
+        // the lane's own); r, d2 and the sum are shared by both kinds
+        const bool internal = node >= n_ids;
+        float4 sp4;                 // position and mass of the source
+        float half = 0.0f;
+        uint32_t childA = 0, childB = 0;
+        if (internal) {
             const WalkNode nd = nodes[node - n_ids];
-            const float rx = fsub(nd.comMass.x, pm.x);
-            const float ry = fsub(nd.comMass.y, pm.y);
-            const float rz = fsub(nd.comMass.z, pm.z);
-            const float d2 = fadd(dot3(rx, ry, rz), soft);
-            // The hardware 1/sqrt (within 2 ulp) decides a node that every
-            // lane of the mask opens by a wide margin without the correctly
-            // rounded inverse square root: half * rsq >= theta (1 + 2^-16)
-            // implies the exact ratio is >= theta. Only the decision is taken
-            // from it; every value that is accumulated uses the exact one.
-            const bool sure_open = fmul(nd.half, __builtin_amdgcn_rsqf(d2)) >= theta_hi;
-            unsigned long long open;
-            if (ballot(mine && !sure_open) == 0ull) {
+            sp4 = nd.comMass;
+            half = nd.half;
+            childA = nd.childA;
+            childB = nd.childB;
+        } else {
+            sp4 = leaf_pos[node];
+        }
+        const float rx = fsub(sp4.x, pm.x);
+        const float ry = fsub(sp4.y, pm.y);
+        const float rz = fsub(sp4.z, pm.z);
+        const float d2 = fadd(dot3(rx, ry, rz), soft);
+        unsigned long long open = 0ull, take;
+        // bh_force.comp:77-79 (self skip): the lane whose own leaf this is
+        const uint32_t rel = node - wbase;
+        const unsigned long long use_m = mask & ~(rel < (uint32_t)kWave ? (1ull << rel) : 0ull);
+        // The hardware 1/sqrt (within 2 ulp) decides a node that every lane of
+        // the mask opens by a wide margin without the correctly rounded
+        // inverse square root: half * rsq >= theta (1 + 2^-16) implies the
+        // exact ratio is >= theta. Only the decision is taken from it; every
+        // value that is accumulated uses the exact one.
+        float inv;
+        if (internal) {
+            if ((ballot(!(fmul(half, __builtin_amdgcn_rsqf(d2)) >= theta_hi)) & mask) == 0ull) {
                 open = mask;
+                take = 0ull;
             } else {
-                const float inv = inv_sqrt_exact(d2);
-                const bool accept = fmul(nd.half, inv) < theta;
-                open = ballot(mine && !accept);
-                if (mine && accept) {
-                    const float gm = fmul(G, nd.comMass.w);
-                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                }
-            }
-            inter += (unsigned long long)__popcll(mask & ~open);
-            if (open) {
-                if (sp + 1 > kDepth) {
-                    err |= 1u;
-                    break;
-                }
-                if (lane == 0) {
-                    s_node[sp] = nd.childA;
-                    s_mask[sp] = open;
-                }
-                sp += 1;
-                opened += (unsigned long long)__popcll(open);
-                if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
-                node = uniform(nd.childB);
-                mask = open;
-                descend = true;
+                inv = kFast ? inv_sqrt_fast(d2) : inv_sqrt_exact(d2);
+                take = ballot(fmul(half, inv) < theta) & mask;
+                open = mask & ~take;
             }
         } else {
-            const float4 lp = leaf_pos[node];
-            const bool use = mine && node != g;          // bh_force.comp:77-79 (self skip)
-            const float rx = fsub(lp.x, pm.x);
-            const float ry = fsub(lp.y, pm.y);
-            const float rz = fsub(lp.z, pm.z);
-            const float d2 = fadd(dot3(rx, ry, rz), soft);
-            const float inv = inv_sqrt_exact(d2);
-            if (use) {
-                const float gm = fmul(G, lp.w);
-                ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-            }
-            const unsigned long long selfbit =
-                (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
-            inter += (unsigned long long)__popcll(mask & ~selfbit);
-            if constexpr (mode != 0u) if (mask & ~selfbit) {
-                // bh_force.comp:85-112: overlap test against the leaf's body
-                const float Rj = leaf_radius[node];
-                const float dist = fsqrt(dot3(rx, ry, rz));
-                const float pen = fsub(fadd(Ri, Rj), dist);
-                const bool hit = use && pen > 0.0f;
-                if (ballot(hit)) {
-                    const uint32_t other = uniform(index_in[node]);
-                    if constexpr (mode == 1u) {
-                        // merge task from the lower-index body
-                        if (hit && bodyIdx < other) {
-                            uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
-                            if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                            else err |= 2u;
-                        }
-                    } else if (hit) {
-                        // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
-                        const float4 ov = src[2 * other + 1];
-                        const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
-                        const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
-                        const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
-                        if (vimp < 0.0f) {
-                            const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
-                            const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
-                            vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
-                            vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
-                            vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
-                        }
-                        const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
-                        pm.x = fsub(pm.x, fmul(cs, nx));
-                        pm.y = fsub(pm.y, fmul(cs, ny));
-                        pm.z = fsub(pm.z, fmul(cs, nz));
+            inv = kFast ? inv_sqrt_fast(d2) : inv_sqrt_exact(d2);
+            take = use_m;
+        }
+        if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+            const float gm = fmul(G, sp4.w);
+            ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+            ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+            az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+        }
+        inter += (unsigned long long)__popcll(take);
+        if constexpr (mode != 0u) if (!internal && use_m) {
+            // bh_force.comp:85-112: overlap test against the leaf's body
+            const bool use = __builtin_amdgcn_inverse_ballot_w64(use_m);
+            const float4 lp = sp4;
+            const float Rj = leaf_radius[node];
+            const float dist = fsqrt(dot3(rx, ry, rz));
+            const float pen = fsub(fadd(Ri, Rj), dist);
+            const bool hit = use && pen > 0.0f;
+            if (ballot(hit)) {
+                const uint32_t other = uniform(index_in[node]);
+                if constexpr (mode == 1u) {
+                    // merge task from the lower-index body
+                    if (hit && bodyIdx < other) {
+                        uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
+                        if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
+                        else err |= 2u;
                     }
+                } else if (hit) {
+                    // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
+                    const float4 ov = src[2 * other + 1];
+                    const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
+                    const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
+                    const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
+                    if (vimp < 0.0f) {
+                        const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
+                        const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
+                        vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
+                        vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
+                        vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
+                    }
+                    const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
+                    pm.x = fsub(pm.x, fmul(cs, nx));
+                    pm.y = fsub(pm.y, fmul(cs, ny));
+                    pm.z = fsub(pm.z, fmul(cs, nz));
                 }
             }
+        }
+        bool descend = false;
+        if (open) {
+            if (sp + 1 > kDepth) {
+                err |= 1u;
+                break;
+            }
+            if (lane == 0) {
+                s_node[sp] = childA;
+                s_mask[sp] = open;
+            }
+            sp += 1;
+            opened += (unsigned long long)__popcll(open);
+            if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
+            node = uniform(childB);
+            mask = open;
+            descend = true;
         }
         if (!descend) {
             if (sp == 0) break;
@@ -218,10 +227,16 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
     const float Ri = (kMode != 0u && active) ? leaf_radius[g] : 0.0f;
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-    if (num > 1)
-        walk_chunk<kStackDepth, kMode>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd, leaf_pos, nodes,
-                                       G, theta_, softening, Ri, leaf_radius, bodyIdx, elasticity, restitution,
-                                       index_in, src, ctr, tasks, task_cap, o);
+    if (num > 1) {
+        if (inv_sqrt_fast_domain(nodes[0].half, softening))
+            walk_chunk<kStackDepth, kMode, true>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd,
+                                                 leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
+                                                 elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+        else
+            walk_chunk<kStackDepth, kMode, false>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd,
+                                                  leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
+                                                  elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+    }
     const float ax = o.ax, ay = o.ay, az = o.az;
     const unsigned long long opened = o.opened, inter = o.inter;
     const uint32_t maxsp = o.maxsp, err = o.err;
@@ -327,8 +342,14 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-            walk_chunk<kDepth, 0u>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos, nodes, G,
-                                   theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+            if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
+                walk_chunk<kDepth, 0u, true>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
+                                             nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src,
+                                             ctr, nullptr, 0u, o);
+            else
+                walk_chunk<kDepth, 0u, false>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
+                                              nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src,
+                                              ctr, nullptr, 0u, o);
             if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
                 const float nvx = fadd(vd.x, fmul(o.ax, dt));
                 const float nvy = fadd(vd.y, fmul(o.ay, dt));
