@@ -237,7 +237,8 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.leaf_pos, (size_t)n));
         TRY(dalloc(c, &T.leaf_parent, (size_t)n));
         TRY(dalloc(c, &T.leaf_color, (size_t)n));
-        TRY(dalloc(c, &T.nodes, (size_t)n - 1));
+        TRY(dalloc(c, &T.walk, 2 * (size_t)n - 1));
+        T.nodes = T.walk + n;
         TRY(dalloc(c, &T.node_aabb, 2 * ((size_t)n - 1)));
         TRY(dalloc(c, &T.node_parent, (size_t)n - 1));
         TRY(dalloc(c, &T.node_range, (size_t)n - 1));
@@ -300,7 +301,7 @@ void bh_destroy(bh_context* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
-                    c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.nodes,
+                    c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius};

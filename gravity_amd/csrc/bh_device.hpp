@@ -45,7 +45,7 @@ struct alignas(32) WalkNode {
     float half;         // half of the longest AABB side (bh_force.comp:14-20)
     uint32_t childA;    // global node ids: < n leaf, >= n internal (n = initialNumBodies)
     uint32_t childB;
-    uint32_t height;    // nodeDepth after the fill: 1 + max(child heights)
+    float half2;        // half * half (the walk's cheap test for nodes every lane opens)
 };
 static_assert(sizeof(WalkNode) == 32, "WalkNode is 32 bytes");
 

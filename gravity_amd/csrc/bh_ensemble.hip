@@ -206,7 +206,7 @@ void bh_ensemble_destroy(bh_ensemble* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     void* ptrs[] = {e->sims, e->tiles, e->tasks, e->bodies[0], e->bodies[1], e->keys[0], e->keys[1], e->index[0],
-                    e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.nodes, e->tree.node_aabb,
+                    e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.walk, e->tree.node_aabb,
                     e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->tree.frontier,
                     e->tree.frontier_count, e->ctr, e->queues,
                     e->partials, e->hist};
@@ -325,7 +325,8 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TreeArrays& T = e->tree;
         TRY(ealloc(e, &T.leaf_pos, (size_t)N));
         TRY(ealloc(e, &T.leaf_parent, (size_t)N));
-        TRY(ealloc(e, &T.nodes, ni));
+        TRY(ealloc(e, &T.walk, (size_t)N + ni));
+        T.nodes = T.walk + N;
         TRY(ealloc(e, &T.node_aabb, 2 * ni));
         TRY(ealloc(e, &T.node_parent, ni));
         TRY(ealloc(e, &T.node_range, ni));

@@ -629,6 +629,18 @@ __device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
     return me;
 }
 
+// The walk's record of a leaf: its body, with a half side of 0 and a half2
+// of -1 (so that the test for nodes every lane opens never passes).
+__device__ __forceinline__ WalkNode leaf_record(float4 pm) {
+    WalkNode w;
+    w.comMass = pm;
+    w.half = 0.0f;
+    w.childA = 0;
+    w.childB = 0;
+    w.half2 = -1.0f;
+    return w;
+}
+
 struct LeafSource {
     const float4* src;          // bodies (written by earlier kernels: coherent)
     const uint32_t* index_in;   // sorted position -> body id
@@ -700,10 +712,10 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
     nd.half = fdiv(ext, 2.0f);
     nd.childA = A.id;
     nd.childB = B.id;
-    nd.height = p.height;
+    nd.half2 = fmul(nd.half, nd.half);
     // these arrays are read only by later kernels
     T.nodes[k] = nd;
-    T.node_aabb[2 * k] = p.mn;
+    T.node_aabb[2 * k] = make_float4(p.mn.x, p.mn.y, p.mn.z, __uint_as_float(p.height));
     T.node_aabb[2 * k + 1] = p.mx;
     if (colors) T.node_color[k] = p.col;
     T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
@@ -844,6 +856,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
         const uint32_t b = L.index_in[g];
         const float4 pm = L.src[2 * b];
         T.leaf_pos[g] = pm;
+        T.walk[g] = leaf_record(pm);
         s.pm[t] = pm;
         if (kColors) {
             const float4 vd = L.src[2 * b + 1];
@@ -956,6 +969,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
         const float4 pm = src[2 * index_in[0]];
         T.leaf_pos[0] = pm;
+        T.walk[0] = leaf_record(pm);
         if (kColors) {
             const float4 vd = src[2 * index_in[0] + 1];
             float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -967,7 +981,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         r.half = -1e38f;
         r.childA = 0;
         r.childB = kEmpty;
-        r.height = 0;
+        r.half2 = 0.0f;
         T.nodes[0] = r;
         T.node_aabb[0] = make_float4(1e38f, 1e38f, 1e38f, 0.f);
         T.node_aabb[1] = make_float4(-1e38f, -1e38f, -1e38f, 0.f);
@@ -1162,7 +1176,7 @@ __global__ void k_nodes_to_std430(const SimValues* __restrict__ sim, TreeArrays 
             o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
             o.childA = w.childA;
             o.childB = w.childB;
-            o.nodeDepth = w.height;
+            o.nodeDepth = __float_as_uint(mn.w);
             o.bodiesContained = T.node_range[k].y;
             o.readyChildren = kEmpty;
             o.parentId = T.node_parent[k];
@@ -1180,6 +1194,7 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
     if (which == 0 && k < n) {
         const NodeStd430& o = leaf_in[k];
         T.leaf_pos[k] = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], o.comMass[3]);
+        T.walk[k] = leaf_record(T.leaf_pos[k]);
         T.leaf_parent[k] = o.parentId;
         T.leaf_color[k] = make_float4(o.avgColor[0], o.avgColor[1], o.avgColor[2], o.avgColor[3]);
     }
@@ -1191,9 +1206,9 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
         w.half = fdiv(ext, 2.0f);
         w.childA = o.childA;
         w.childB = o.childB;
-        w.height = o.nodeDepth;
+        w.half2 = fmul(w.half, w.half);
         T.nodes[k] = w;
-        T.node_aabb[2 * k] = make_float4(o.aabb[0], o.aabb[1], o.aabb[2], 0.f);
+        T.node_aabb[2 * k] = make_float4(o.aabb[0], o.aabb[1], o.aabb[2], __uint_as_float(o.nodeDepth));
         T.node_aabb[2 * k + 1] = make_float4(o.aabb[3], o.aabb[4], o.aabb[5], 0.f);
         T.node_parent[k] = o.parentId;
         T.node_range[k] = make_uint2(0, o.bodiesContained);
@@ -1528,7 +1543,7 @@ __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430
     o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
     o.childA = w.childA;
     o.childB = w.childB;
-    o.nodeDepth = w.height;
+    o.nodeDepth = __float_as_uint(mn.w);
     o.bodiesContained = T.node_range[k].y;
     o.readyChildren = kEmpty;
     o.parentId = T.node_parent[k];

@@ -22,8 +22,10 @@ struct TreeArrays {
     float4* leaf_pos;        // [n] posMass in sorted order
     uint32_t* leaf_parent;   // [n]
     float4* leaf_color;      // [n]
-    WalkNode* nodes;         // [n-1]
-    float4* node_aabb;       // [2(n-1)] min, max
+    WalkNode* walk;          // [2n-1] the walk's records: leaf j at j (its body, half 0, half2 -1), then the
+                             // internal nodes; node id i is walk[i]
+    WalkNode* nodes;         // = walk + n: internal node k
+    float4* node_aabb;       // [2(n-1)] min, max; min.w holds the height (nodeDepth) as uint bits
     uint32_t* node_parent;   // [n-1]
     uint2* node_range;       // [n-1] (first sorted position, bodies contained)
     float4* node_color;      // [n-1]

@@ -41,6 +41,57 @@ struct WalkAcc {
     uint32_t maxsp, err;
 };
 
+// a += G m r inv^3 in the reference's operation order (bh_force.comp:22-25,74,82)
+__device__ __forceinline__ void add_force(float& ax, float& ay, float& az, float gm, float rx, float ry, float rz,
+                                          float inv) {
+    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
+    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
+    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+}
+
+// The overlap test of a leaf visit in the merging (1) and collision (2)
+// modes (bh_force.comp:85-112); use: this lane interacts with the leaf.
+template <uint32_t mode>
+__device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float4 lp, float rx, float ry, float rz,
+                                             float4& pm, float4& vd, float Ri, const float* __restrict__ leaf_radius,
+                                             uint32_t bodyIdx, float elasticity, float restitution,
+                                             const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
+                                             DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
+                                             uint32_t task_cap, uint32_t& err) {
+    const float Rj = leaf_radius[node];
+    const float dist = fsqrt(dot3(rx, ry, rz));
+    const float pen = fsub(fadd(Ri, Rj), dist);
+    const bool hit = use && pen > 0.0f;
+    if (ballot(hit)) {
+        const uint32_t other = uniform(index_in[node]);
+        if constexpr (mode == 1u) {
+            // merge task from the lower-index body
+            if (hit && bodyIdx < other) {
+                uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
+                if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
+                else err |= 2u;
+            }
+        } else if (hit) {
+            // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
+            const float4 ov = src[2 * other + 1];
+            const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
+            const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
+            const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
+            if (vimp < 0.0f) {
+                const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
+                const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
+                vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
+                vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
+                vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
+            }
+            const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
+            pm.x = fsub(pm.x, fmul(cs, nx));
+            pm.y = fsub(pm.y, fmul(cs, ny));
+            pm.z = fsub(pm.z, fmul(cs, nz));
+        }
+    }
+}
+
 // The masked-stack walk of one wave over the tree rooted at `root`: lanes in
 // `mask` walk for the bodies at sorted positions base + lane (position g,
 // position-mass pm, velocity-density vd); leaf ids are below n_ids, internal
@@ -119,50 +170,13 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             inv = kFast ? inv_sqrt_fast(d2) : inv_sqrt_exact(d2);
             take = use_m;
         }
-        if (__builtin_amdgcn_inverse_ballot_w64(take)) {
-            const float gm = fmul(G, sp4.w);
-            ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-            ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-            az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-        }
+        if (__builtin_amdgcn_inverse_ballot_w64(take)) add_force(ax, ay, az, fmul(G, sp4.w), rx, ry, rz, inv);
         inter += (unsigned long long)__popcll(take);
-        if constexpr (mode != 0u) if (!internal && use_m) {
-            // bh_force.comp:85-112: overlap test against the leaf's body
-            const bool use = __builtin_amdgcn_inverse_ballot_w64(use_m);
-            const float4 lp = sp4;
-            const float Rj = leaf_radius[node];
-            const float dist = fsqrt(dot3(rx, ry, rz));
-            const float pen = fsub(fadd(Ri, Rj), dist);
-            const bool hit = use && pen > 0.0f;
-            if (ballot(hit)) {
-                const uint32_t other = uniform(index_in[node]);
-                if constexpr (mode == 1u) {
-                    // merge task from the lower-index body
-                    if (hit && bodyIdx < other) {
-                        uint32_t slot = atomicAdd(&ctr->nTasks, 1u);
-                        if (slot < task_cap) tasks[slot] = make_uint2(bodyIdx, other);
-                        else err |= 2u;
-                    }
-                } else if (hit) {
-                    // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
-                    const float4 ov = src[2 * other + 1];
-                    const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
-                    const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
-                    const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
-                    if (vimp < 0.0f) {
-                        const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
-                        const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
-                        vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
-                        vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
-                        vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
-                    }
-                    const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
-                    pm.x = fsub(pm.x, fmul(cs, nx));
-                    pm.y = fsub(pm.y, fmul(cs, ny));
-                    pm.z = fsub(pm.z, fmul(cs, nz));
-                }
-            }
-        }
+        if constexpr (mode != 0u)
+            if (!internal && use_m)
+                leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(use_m), node, sp4, rx, ry, rz, pm, vd, Ri,
+                                   leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr, tasks, task_cap,
+                                   err);
         bool descend = false;
         if (open) {
             if (sp + 1 > kDepth) {
@@ -193,6 +207,97 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     o.az = az;
     o.opened = opened;
     o.inter = inter;
+    o.maxsp = maxsp;
+    o.err = err;
+}
+
+// The same walk for the common case, with less scalar bookkeeping per visit
+// (the walk issues about as many scalar as vector instructions, and each
+// SIMD issues at most one of each per four cycles): every d2 is in the fast
+// domain of the inverse square root, the tree is at most kLeanHeight high so
+// that the stack (never deeper than the tree) fits one entry per lane of
+// three registers (node, mask low, mask high: v_writelane / v_readlane
+// instead of LDS), nothing can overflow it, and the per-wave counts fit
+// 32 bits (at most 2^26 bodies). Same visits, same order, same sums.
+constexpr uint32_t kLeanHeight = 62;
+// v_writelane_b32 (the LLVM intrinsic; this compiler has no builtin for it)
+__device__ int lane_write(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+constexpr uint32_t kLeanBodies = 1u << 26;
+
+__device__ __forceinline__ bool walk_lean_ok(float root_half, float soft, uint32_t height, uint32_t num) {
+    return inv_sqrt_fast_domain(root_half, soft) && height <= kLeanHeight && num <= kLeanBodies;
+}
+
+template <uint32_t mode>
+__device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_t g, unsigned long long mask,
+                                          float4& pm, float4& vd, const WalkNode* __restrict__ nodes, float G,
+                                          float theta, float soft, float Ri, const float* __restrict__ leaf_radius,
+                                          uint32_t bodyIdx, float elasticity, float restitution,
+                                          const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
+                                          DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
+                                          uint32_t task_cap, WalkAcc& o) {
+    float ax = o.ax, ay = o.ay, az = o.az;
+    uint32_t opened = 0, inter = 0, maxsp = o.maxsp, err = o.err;
+    // One record per node id, leaves included (the tree arrays keep the leaf
+    // records right before the internal ones), addressed as a base plus a
+    // 32-bit byte offset (ids < 2^27).
+    const char* __restrict__ wbase = reinterpret_cast<const char*>(nodes - n_ids);
+    int st_node = 0, st_lo = 0, st_hi = 0;       // stack entry i in lane i
+    uint32_t node = root;
+    int sp = 0;
+    // Every lane of the mask opens a node by a wide margin if d2 theta^2
+    // (1 + 2^-10) <= half^2: then half / sqrt(d2) >= theta (1 + 2^-11), and
+    // the correctly rounded half * inv is >= theta (a few ulps of error).
+    // A leaf's half2 of -1 never passes.
+    const float open_k = fmul(fmul(theta, theta), 1.0009766f);
+    for (;;) {
+        const WalkNode nd = *reinterpret_cast<const WalkNode*>(wbase + (uint32_t)(node << 5));
+        const float rx = fsub(nd.comMass.x, pm.x);
+        const float ry = fsub(nd.comMass.y, pm.y);
+        const float rz = fsub(nd.comMass.z, pm.z);
+        const float d2 = fadd(dot3(rx, ry, rz), soft);
+        unsigned long long open = mask;
+        if (ballot(!(fmul(d2, open_k) <= nd.half2)) & mask) {
+            // a node some lane may accept, or a leaf (every lane takes it but
+            // the one whose own body it is: bh_force.comp:77-79)
+            const bool internal = node >= n_ids;
+            const float inv = inv_sqrt_fast(d2);
+            const unsigned long long accm = internal ? ballot(fmul(nd.half, inv) < theta) : ~0ull;
+            const unsigned long long take = accm & mask & ballot(node != g);
+            open = mask & ~accm;
+            if (__builtin_amdgcn_inverse_ballot_w64(take))
+                add_force(ax, ay, az, fmul(G, nd.comMass.w), rx, ry, rz, inv);
+            inter += (uint32_t)__popcll(take);
+            if constexpr (mode != 0u)
+                if (!internal && take)
+                    leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(take), node, nd.comMass, rx, ry, rz, pm,
+                                       vd, Ri, leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr,
+                                       tasks, task_cap, err);
+        }
+        // the entry goes to lane sp on every visit: unless it is pushed it
+        // lies above the top and is never read
+        st_node = lane_write((int)nd.childA, sp, st_node);
+        st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
+        st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
+        if (open) {
+            sp += 1;
+            opened += (uint32_t)__popcll(open);
+            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
+            node = nd.childB;
+            mask = open;
+        } else {
+            if (sp == 0) break;
+            sp -= 1;
+            node = (uint32_t)__builtin_amdgcn_readlane(st_node, sp);
+            mask = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, sp) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane(st_lo, sp);
+        }
+    }
+    o.ax = ax;
+    o.ay = ay;
+    o.az = az;
+    o.opened += opened;
+    o.inter += inter;
     o.maxsp = maxsp;
     o.err = err;
 }
@@ -228,7 +333,10 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
     if (num > 1) {
-        if (inv_sqrt_fast_domain(nodes[0].half, softening))
+        if (walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num))
+            walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
+                             elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+        else if (inv_sqrt_fast_domain(nodes[0].half, softening))
             walk_chunk<kStackDepth, kMode, true>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd,
                                                  leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                                                  elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
@@ -324,6 +432,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
     // workgroups are dealt to the 8 XCDs round robin: this block's XCD
     const uint32_t home = blockIdx.x % kEnsQueues;
+    const uint32_t height = ctr->treeHeight;   // the highest tree of the ensemble
     unsigned long long opened = 0, inter = 0;
     uint32_t maxsp = 1, err = 0;
     for (uint32_t qi = 0; qi < kEnsQueues; qi++) {
@@ -342,7 +451,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-            if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
+            if (walk_lean_ok(nodes[root - N].half, soft, height, N))
+                walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, G, theta, soft, 0.0f, nullptr, bodyIdx,
+                              0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+            else if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
                 walk_chunk<kDepth, 0u, true>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
                                              nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src,
                                              ctr, nullptr, 0u, o);
