@@ -274,24 +274,25 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
                                        vd, Ri, leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr,
                                        tasks, task_cap, err);
         }
-        // the entry goes to lane sp on every visit: unless it is pushed it
-        // lies above the top and is never read
+        // Push (childA, open) and go to childB if any lane opened the node,
+        // else pop; both without a branch. The top entry is read first (lane
+        // -1 reads as lane 63 when the stack is empty: not used), and the new
+        // entry goes to lane sp either way (above the top, it is never read).
+        const int top = sp - 1;
+        const uint32_t pop_node = (uint32_t)__builtin_amdgcn_readlane(st_node, top);
+        const unsigned long long pop_mask =
+            ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, top) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane(st_lo, top);
         st_node = lane_write((int)nd.childA, sp, st_node);
         st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
         st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
-        if (open) {
-            sp += 1;
-            opened += (uint32_t)__popcll(open);
-            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
-            node = nd.childB;
-            mask = open;
-        } else {
-            if (sp == 0) break;
-            sp -= 1;
-            node = (uint32_t)__builtin_amdgcn_readlane(st_node, sp);
-            mask = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, sp) << 32) |
-                   (uint32_t)__builtin_amdgcn_readlane(st_lo, sp);
-        }
+        opened += (uint32_t)__popcll(open);
+        const bool descend = open != 0ull;
+        node = descend ? nd.childB : pop_node;
+        mask = descend ? open : pop_mask;
+        sp = descend ? sp + 1 : top;
+        maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;   // a pop never raises it
+        if (sp < 0) break;
     }
     o.ax = ax;
     o.ay = ay;
