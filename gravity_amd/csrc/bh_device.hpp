@@ -78,20 +78,18 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
 
 // 1 / sqrt(d2) rounded exactly as RN(1 / RN(sqrt(d2))), the reference's
 // inversesqrt, for normal finite d2 (the walk's d2 >= softening > 0).
-// Hardware v_sqrt_f32 and v_rcp_f32 are within one ulp; one residual step
-// each (the same correction the compiler emits for a correctly rounded sqrt,
-// without its denormal and special-value handling) makes them correctly
-// rounded. bh_selftest_math checks both against the general sequences over
-// every float of the domain.
+// The square root starts from the hardware 1/sqrt y (a few ulps): s = x y is
+// then within a few ulps of sqrt(x), and one residual step with h = y / 2
+// (r = x - s s exactly, by fma; s + r h) rounds it correctly; the reciprocal
+// is the hardware v_rcp_f32 (within one ulp) plus one Newton step. For every
+// float of [2^-100, 2^100] both are the correctly rounded results, as
+// bh_selftest_math checks against the general IEEE sequences on the device.
 __device__ __forceinline__ float sqrt_cr_normal(float x) {
-    float s = __builtin_amdgcn_sqrtf(x);
-    uint32_t b = __float_as_uint(s);
-    float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
-    float rd = __builtin_fmaf(-sd, s, x);
-    float ru = __builtin_fmaf(-su, s, x);
-    s = (rd <= 0.0f) ? sd : s;
-    s = (ru > 0.0f) ? su : s;
-    return s;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float s = __fmul_rn(x, y);
+    const float h = __fmul_rn(0.5f, y);
+    const float r = __builtin_fmaf(-s, s, x);
+    return __builtin_fmaf(r, h, s);
 }
 __device__ __forceinline__ float rcp_cr_normal(float y) {
     float r = __builtin_amdgcn_rcpf(y);

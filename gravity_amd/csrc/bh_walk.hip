@@ -220,6 +220,7 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
 // instead of LDS), nothing can overflow it, and the per-wave counts fit
 // 32 bits (at most 2^26 bodies). Same visits, same order, same sums.
 constexpr uint32_t kLeanHeight = 62;
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
 // v_writelane_b32 (the LLVM intrinsic; this compiler has no builtin for it)
 __device__ int lane_write(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 constexpr uint32_t kLeanBodies = 1u << 26;
@@ -251,7 +252,16 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
     // A leaf's half2 of -1 never passes.
     const float open_k = fmul(fmul(theta, theta), 1.0009766f);
     for (;;) {
-        const WalkNode nd = *reinterpret_cast<const WalkNode*>(wbase + (uint32_t)(node << 5));
+        // one 32-byte scalar load (a vector type keeps the compiler from
+        // splitting it and sinking the half side into a later dependent load)
+        const u32x8 rec = *reinterpret_cast<const u32x8*>(wbase + (uint32_t)(node << 5));
+        WalkNode nd;
+        nd.comMass = make_float4(__uint_as_float(rec[0]), __uint_as_float(rec[1]), __uint_as_float(rec[2]),
+                                 __uint_as_float(rec[3]));
+        nd.half = __uint_as_float(rec[4]);
+        nd.childA = rec[5];
+        nd.childB = rec[6];
+        nd.half2 = __uint_as_float(rec[7]);
         const float rx = fsub(nd.comMass.x, pm.x);
         const float ry = fsub(nd.comMass.y, pm.y);
         const float rz = fsub(nd.comMass.z, pm.z);
