@@ -365,21 +365,51 @@ __global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ si
 // Stable LSD sort of (key, index) pairs over positions [0, numBodies), eight
 // passes of eight bits. Per pass: tile histograms, per-digit scan over tiles,
 // stable scatter with in-wave ballot ranking.
+// Digit histogram of one tile of up to kSortTile keys, per wave without
+// atomics: keys of a tile share their high digits (the input is nearly in
+// order), so per-key LDS atomics would serialize on one bin. Each row of 64
+// keys finds the lanes holding its digit (one ballot per digit bit) and the
+// lowest of them adds the group's size to the wave's own bins (distinct
+// digits: distinct addresses). Called by all 256 threads; returns thread t's
+// digit count for the tile.
+template <int BITS>
+__device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[4][256], const uint64_t* __restrict__ keys,
+                                                   uint32_t start, uint32_t count, uint32_t shift) {
+    constexpr uint32_t R = 1u << BITS;
+    constexpr int kPer = kSortTile / 4, kRows = kPer / 64;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    for (int k = 0; k < 4; k++) s_h[w][lane * 4 + k] = 0;
+    uint64_t key[kRows];
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t e = w * kPer + r * 64 + lane;
+        key[r] = e < count ? keys[start + e] : 0ull;
+    }
+    for (int r = 0; r < kRows; r++) {
+        const uint32_t e = w * kPer + r * 64 + lane;
+        const bool valid = e < count;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & (R - 1u);
+        uint64_t peers = ballot(valid);
+        for (int bit = 0; bit < BITS; bit++) {
+            const uint64_t bb = ballot(valid && ((d >> bit) & 1u));
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        if (valid && __popcll(peers & lt) == 0) s_h[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    return s_h[0][t] + s_h[1][t] + s_h[2][t] + s_h[3][t];
+}
+
 __global__ __launch_bounds__(256) void k_sort_hist(const SimValues* __restrict__ sim,
                                                    const uint64_t* __restrict__ keys, uint32_t shift,
                                                    uint32_t* __restrict__ tile_hist, uint32_t tiles_cap) {
-    __shared__ uint32_t s_h[256];
+    __shared__ uint32_t s_h[4][256];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kSortTile;
     if (base >= num) return;
-    s_h[threadIdx.x] = 0;
-    __syncthreads();
-    for (int r = 0; r < kSortTile / 256; r++) {
-        uint32_t e = base + r * 256 + threadIdx.x;
-        if (e < num) atomicAdd(&s_h[(uint32_t)(keys[e] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = s_h[threadIdx.x];
+    const uint32_t count = num - base < kSortTile ? num - base : kSortTile;
+    const uint32_t c = tile_histogram<8>(s_h, keys, base, count, shift);
+    tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = c;
 }
 
 // one block per digit: exclusive scan of that digit's tile counts
@@ -1446,14 +1476,10 @@ __global__ __launch_bounds__(256) void k_ens_sort_hist(const EnsTile* __restrict
                                                        const uint64_t* __restrict__ keys, uint32_t shift,
                                                        uint32_t* __restrict__ hist) {
     constexpr uint32_t R = 1u << BITS;
-    __shared__ uint32_t s_h[256];
+    __shared__ uint32_t s_h[4][256];
     const EnsTile tl = tiles[blockIdx.x];
-    s_h[threadIdx.x] = 0;
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < tl.count; e += blockDim.x)
-        atomicAdd(&s_h[(uint32_t)(keys[tl.start + e] >> shift) & (R - 1u)], 1u);
-    __syncthreads();
-    if (threadIdx.x < R) hist[(size_t)blockIdx.x * 256 + threadIdx.x] = s_h[threadIdx.x];
+    const uint32_t c = tile_histogram<BITS>(s_h, keys, tl.start, tl.count, shift);
+    if (threadIdx.x < R) hist[(size_t)blockIdx.x * 256 + threadIdx.x] = c;
 }
 
 template <int BITS>

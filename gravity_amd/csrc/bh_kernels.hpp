@@ -88,6 +88,7 @@ struct EnsQueue {
 static_assert(sizeof(EnsQueue) == 128, "EnsQueue layout");
 constexpr uint32_t kEnsTile = 2048;    // positions per sort/bounds/encode tile
 static_assert(kEnsTile == 2 * kTreeBlock, "an ensemble tile is two tree blocks");
+static_assert(kEnsTile == kSortTile, "ensemble tiles are histogrammed as sort tiles");
 
 void launch_ens_reset(hipStream_t s, DevCounters* ctr, EnsQueue* queues);
 void launch_ens_bounds(hipStream_t s, EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, const float4* src,
