@@ -27,7 +27,7 @@ MERGE_TASKS = 14
 
 NEITHER, MERGING, COLLISION = 0, 1, 2
 STATIC, DYNAMIC = 0, 1
-FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED = 1, 2, 4
+FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK = 1, 2, 4, 8
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
 
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "unsupported", -5: "too large",

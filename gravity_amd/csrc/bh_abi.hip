@@ -517,6 +517,7 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
     // multiplies by 0; bh_params.restitution is kept for the interface
     a.restitution = 0.0f;
     a.leaf_radius = c->leaf_radius;
+    a.general = (p->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
     if (p->flags & BH_FLAG_LEARNED) {
         if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
         if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)

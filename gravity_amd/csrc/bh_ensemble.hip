@@ -41,6 +41,7 @@ struct bh_ensemble {
     uint32_t* hist = nullptr;
 
     float theta = 0.5f, dt = 1e-3f, soft = 0.01f, G = 1.0f;
+    uint32_t general = 0;      // BH_FLAG_GENERAL_WALK: always the walk's general loop
     bh_ensemble_config cfg{8, 4, 192, 1};
     uint32_t walk_blocks = 0;
     int cus = 0;
@@ -126,7 +127,7 @@ int enqueue_walk(bh_ensemble* e, int waves, int depth, uint32_t blocks) {
     const int b = e->body_in;
     if (launch_ens_walk(e->stream, waves, depth, blocks, e->sims, e->tasks, e->queues, e->ctr,
                         e->bodies[b], e->bodies[1 - b], e->index[e->mi_in], e->tree.leaf_pos, e->tree.nodes, e->N,
-                        e->G, e->theta, e->dt, e->soft))
+                        e->G, e->theta, e->dt, e->soft, e->general))
         return efail(e, BH_ERR_INVALID_ARGUMENT, "walk shape %d x %d", waves, depth);
     return BH_OK;
 }
@@ -242,6 +243,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
     e->theta = params->theta;
     e->dt = params->dt;
     e->soft = params->softening;
+    e->general = (params->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
     float G = params->gravitational_constant;
     if (G == 0.0f && params->unit_mass != 0.0f) {
         // common/common.glsl:346-352, in FP32 as the shader does

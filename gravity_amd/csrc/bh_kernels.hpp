@@ -61,6 +61,7 @@ struct WalkArgs {
     uint32_t merging, dynamic, wrap;
     float elasticity, restitution;   // COLLISION mode
     float* leaf_radius;              // [n] scratch of the merging and collision modes
+    uint32_t general;                // 1: always the general walk loop
 };
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.
@@ -107,7 +108,7 @@ void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, ui
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
-                    float theta, float dt, float soft);
+                    float theta, float dt, float soft, uint32_t general);
 int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu);
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out);
 

@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                                               const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
-                                              const float* __restrict__ leaf_radius) {
+                                              const float* __restrict__ leaf_radius, uint32_t general) {
     __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
     __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
 
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
     if (num > 1) {
-        if (walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num))
+        if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num))
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                              elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
         else if (inv_sqrt_fast_domain(nodes[0].half, softening))
@@ -413,7 +413,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     k_walk<M><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
-                                                a.leaf_radius)
+                                                a.leaf_radius, a.general)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
     if (a.merging == 1u) BH_WALK(1u);
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
                                                           const uint32_t* __restrict__ index_in,
                                                           const float4* __restrict__ leaf_pos,
                                                           const WalkNode* __restrict__ nodes, uint32_t N, float G,
-                                                          float theta, float dt, float soft) {
+                                                          float theta, float dt, float soft, uint32_t general) {
     __shared__ uint32_t s_node[kWaves][kDepth];
     __shared__ unsigned long long s_mask[kWaves][kDepth];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
-            if (walk_lean_ok(nodes[root - N].half, soft, height, N))
+            if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
                 walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, G, theta, soft, 0.0f, nullptr, bodyIdx,
                               0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             else if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
@@ -501,11 +501,11 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
-                    float theta, float dt, float soft) {
+                    float theta, float dt, float soft, uint32_t general) {
 #define BH_LAUNCH(W, D)                                                                                            \
     if (waves == W && depth == D) {                                                                                \
         k_ens_walk<W, D><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos, nodes,  \
-                                                   N, G, theta, dt, soft);                                         \
+                                                   N, G, theta, dt, soft, general);                                \
         return 0;                                                                                                  \
     }
     BH_ENS_WALK_SHAPES(BH_LAUNCH)

@@ -69,6 +69,9 @@ extern "C" {
 #define BH_FLAG_LEARNED 4u     /* corrected walk: nodes are accepted at theta only with the target outside
                                   their box; the attached network corrects those that theta_plain would
                                   open (no reference counterpart) */
+#define BH_FLAG_GENERAL_WALK 8u /* run the walk's general loop (LDS stack, overflow check, IEEE inverse
+                                   square root outside its fast domain) even where the shorter loop
+                                   applies; same results (for tests of both loops) */
 
 #define BH_WALK_STACK_DEPTH 192
 

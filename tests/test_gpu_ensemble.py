@@ -111,6 +111,25 @@ def test_graph_replay_matches_eager_launches():
     graph.close()
 
 
+def test_general_walk_loop_gives_identical_results():
+    """BH_FLAG_GENERAL_WALK runs the walk's LDS-stack loop for every task: the
+    same bits as the shorter loop."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import ensemble as E
+    print("[gpu] ensemble general walk loop", flush=True)
+    sims = _sims([5000, 20000, 2, 3333], 5)
+    general = B.make_params(theta=0.5, dt=1e-3, softening=SOFT, G=1.0, flags=B.FLAG_GENERAL_WALK)
+    ref = E.Ensemble(sims, _params(), config=E.Config(8, 4, 192, 0))
+    alt = E.Ensemble(sims, general, config=E.Config(8, 4, 192, 0))
+    ref.step(3)
+    alt.step(3)
+    a, b = ref.bodies(), alt.bodies()
+    for s in range(len(sims)):
+        assert np.array_equal(_bits(a[s]), _bits(b[s])), s
+    ref.close()
+    alt.close()
+
+
 @pytest.mark.parametrize("bits,waves,depth", [(5, 1, 64), (6, 2, 128), (7, 8, 192), (7, 4, 64), (8, 8, 64)])
 def test_digit_widths_and_walk_shapes_give_identical_results(bits, waves, depth):
     from gravity_amd import ensemble as E

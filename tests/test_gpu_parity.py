@@ -180,13 +180,33 @@ def test_gpu_free_running_steps(name):
     bh.close()
 
 
-def test_gpu_merge_steps_match_oracle():
+@pytest.mark.parametrize("name", ["plummer_4096", "collision_4096", "static_kill_1024", "theta0_512"])
+def test_gpu_general_walk_loop(name):
+    """BH_FLAG_GENERAL_WALK (the walk's LDS-stack loop instead of the shorter
+    one) reproduces the reference trajectory bit for bit as well."""
+    from gravity_amd import barnes_hut as B
+    fx = load(name)
+    print(f"[gpu] general walk loop {name}", flush=True)
+    bh, prm, _, _, _ = make_bh(fx, flags=B.FLAG_GENERAL_WALK)
+    bh.init()
+    for s in range(steps(fx)):
+        bh.step()
+        c = bh.counters()
+        assert c["opened"] == int(fx[f"s{s}_opened"])
+        assert c["error_flags"] == 0
+    assert_bits(bh.bodies(), fx["final_bodies"], "final bodies")
+    bh.close()
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_gpu_merge_steps_match_oracle(general):
     """Merging on: bh_step and the oracle's step agree on every step (same
-    deterministic task order), and the reference agrees on step 0's tasks."""
+    deterministic task order), and the reference agrees on step 0's tasks;
+    with either walk loop."""
     from gravity_amd import barnes_hut as B
     fx = load("merge_4096")
-    print("[gpu] merge steps", flush=True)
-    bh, prm, _, G, blc = make_bh(fx)
+    print(f"[gpu] merge steps (general loop {general})", flush=True)
+    bh, prm, _, G, blc = make_bh(fx, flags=B.FLAG_GENERAL_WALK if general else 0)
     bh.init()
     sim = O.Simulation(fx["bodies0"], fx["bounds0"], 1.0, 1.0, G=G, body_len_const=blc)
     oprm = O.params(**params_of(fx))
