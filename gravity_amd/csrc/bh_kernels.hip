@@ -1489,12 +1489,14 @@ __global__ __launch_bounds__(256) void k_ens_sort_scan(const EnsSim* __restrict_
     const EnsSim S = sims[blockIdx.x];
     uint32_t* h = hist + (size_t)S.tile0 * 256;
     const uint32_t d = threadIdx.x, lane = lane_id();
+    // this digit's total over the simulation's tiles (8 loads in flight)
+    constexpr uint32_t kU = 8;
     uint32_t total = 0;
     if (d < R)
-        for (uint32_t k = 0; k < S.ntiles; k++) {
-            const uint32_t v = h[(size_t)k * 256 + d];
-            h[(size_t)k * 256 + d] = total;
-            total += v;
+        for (uint32_t k0 = 0; k0 < S.ntiles; k0 += kU) {
+            uint32_t v[kU];
+            for (uint32_t j = 0; j < kU; j++) v[j] = k0 + j < S.ntiles ? h[(size_t)(k0 + j) * 256 + d] : 0u;
+            for (uint32_t j = 0; j < kU; j++) total += v[j];
         }
     uint32_t incl = total;
     for (int off = 1; off < 64; off <<= 1) {
@@ -1505,9 +1507,19 @@ __global__ __launch_bounds__(256) void k_ens_sort_scan(const EnsSim* __restrict_
     __syncthreads();
     uint32_t before = S.off;
     for (uint32_t k = 0; k < (d >> 6); k++) before += s_w[k];
-    const uint32_t base = before + incl - total;
+    // each tile's start of this digit: the digit's base plus the counts of
+    // the simulation's earlier tiles
+    uint32_t run = before + incl - total;
     if (d < R)
-        for (uint32_t k = 0; k < S.ntiles; k++) h[(size_t)k * 256 + d] += base;
+        for (uint32_t k0 = 0; k0 < S.ntiles; k0 += kU) {
+            uint32_t v[kU];
+            for (uint32_t j = 0; j < kU; j++) v[j] = k0 + j < S.ntiles ? h[(size_t)(k0 + j) * 256 + d] : 0u;
+            for (uint32_t j = 0; j < kU; j++)
+                if (k0 + j < S.ntiles) {
+                    h[(size_t)(k0 + j) * 256 + d] = run;
+                    run += v[j];
+                }
+        }
 }
 
 template <int BITS>
