@@ -137,15 +137,51 @@ __device__ __forceinline__ void mlp_wave_forward(const MlpRegs& R, const __bf16*
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
-// A lane's features as two 16-byte LDS stores into its row of xs.
-__device__ __forceinline__ void mlp_stage_features(__bf16* xs, const float* f) {
+// mlp_wave_forward with the fragments read from LDS as each MFMA needs them
+// (wfrag, bias as for mlp_load_lds): a few fragments live at a time instead
+// of all 14, which keeps the calling kernel's register count low.
+__device__ __forceinline__ void mlp_wave_forward_lds(const bf16x8* wfrag, const f32x4* bias, const __bf16* xs,
+                                                     float* ys, uint32_t blocks) {
     const int lane = (int)lane_id();
+    const int r16 = lane & 15, grp = lane >> 4;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const bf16x8 zero = {};
+    for (int ns = 0; ns < 4; ns++) {
+        if (!((blocks >> ns) & 1u)) continue;
+        const bf16x8 xb = grp < 2 ? *(const bf16x8*)(xs + (16 * ns + r16) * kMlpXStride + 8 * grp) : zero;
+        f32x4 a[4];
+        for (int mo = 0; mo < 4; mo++)
+            a[mo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag[(kFragW1 + mo) * 64 + lane], xb,
+                                                            bias[4 * mo + grp], 0, 0, 0);
+        bf16x8 h0 = pack_silu(a[0], a[1]), h1 = pack_silu(a[2], a[3]);
+        for (int mo = 0; mo < 4; mo++) {
+            a[mo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag[(kFragW2 + 2 * mo) * 64 + lane], h0,
+                                                            bias[16 + 4 * mo + grp], 0, 0, 0);
+            a[mo] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag[(kFragW2 + 2 * mo + 1) * 64 + lane], h1, a[mo],
+                                                            0, 0, 0);
+        }
+        h0 = pack_silu(a[0], a[1]);
+        h1 = pack_silu(a[2], a[3]);
+        f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag[kFragW3 * 64 + lane], h0, bias[32 + grp], 0, 0, 0);
+        o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfrag[(kFragW3 + 1) * 64 + lane], h1, o, 0, 0, 0);
+        if (grp == 0) *(f32x4*)(ys + (16 * ns + r16) * 4) = o;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// A lane's features as two 16-byte LDS stores into row `row` of xs.
+__device__ __forceinline__ void mlp_stage_row(__bf16* xs, int row, const float* f) {
     bf16x8 lo, hi;
     for (int c = 0; c < 8; c++) lo[c] = (__bf16)f[c];
     for (int c = 0; c < 4; c++) hi[c] = (__bf16)f[8 + c];
     for (int c = 4; c < 8; c++) hi[c] = (__bf16)0.0f;
-    *(bf16x8*)(xs + lane * kMlpXStride) = lo;
-    *(bf16x8*)(xs + lane * kMlpXStride + 8) = hi;
+    *(bf16x8*)(xs + row * kMlpXStride) = lo;
+    *(bf16x8*)(xs + row * kMlpXStride + 8) = hi;
+}
+__device__ __forceinline__ void mlp_stage_features(__bf16* xs, const float* f) {
+    mlp_stage_row(xs, (int)lane_id(), f);
 }
 
 }  // namespace bh

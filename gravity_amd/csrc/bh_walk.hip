@@ -1,15 +1,17 @@
 // bh_walk.hip — tree walk, force and integration (shaders/compute/force/bh_force.comp:6-142).
 //
 // One wavefront walks the tree for 64 consecutive sorted bodies with one
-// shared LIFO stack in LDS. Each stack entry carries the 64-bit mask of the
-// lanes that still need that node; a node is loaded once per wave (its index
-// is wave-uniform, so the 32-byte record comes in through scalar loads), every
+// shared LIFO stack. Each stack entry carries the 64-bit mask of the lanes
+// that still need that node; a node is loaded once per wave (its index is
+// wave-uniform, so the 32-byte record comes in through a scalar load), every
 // lane in the mask evaluates its own acceptance test, and the children are
 // pushed with the mask of the lanes that opened it. Restricted to one lane,
 // this visits exactly the nodes of that body's own per-thread walk in exactly
 // the reference's order (push childA, push childB, pop childB first: here
-// childA is pushed and childB visited directly), so the
-// accepted set, the opened-node count and the FP32 sums are the reference's.
+// childA is pushed and childB visited directly), so the accepted set, the
+// opened-node count and the FP32 sums are the reference's. Two loops run it:
+// walk_lean (stack in lanes of three registers, for the common case) and
+// walk_chunk (stack in LDS, any tree, any distance).
 #include "bh_device.hpp"
 #include "bh_kernels.hpp"
 
@@ -555,8 +557,14 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
     __shared__ __attribute__((aligned(16))) __bf16 s_x[kWalkWaves][64 * kMlpXStride];
     __shared__ __attribute__((aligned(16))) float s_y[kWalkWaves][64 * 4];
 
-    MlpRegs R;
-    mlp_load(R, mlp_w, mlp_b);
+    // the network's fragments, for every wave of the block
+    __shared__ bf16x8 s_wf[kFrags * 64];
+    __shared__ f32x4 s_bias[kBiasTotal / 4];
+    for (uint32_t i = threadIdx.x; i < kFrags * 64; i += blockDim.x)
+        s_wf[i] = reinterpret_cast<const bf16x8*>(mlp_w)[i];
+    for (uint32_t i = threadIdx.x; i < kBiasTotal / 4; i += blockDim.x)
+        s_bias[i] = reinterpret_cast<const f32x4*>(mlp_b)[i];
+    __syncthreads();
 
     const uint32_t n = sim->initialNumBodies;
     const uint32_t num = sim->numBodies;
@@ -575,6 +583,31 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     unsigned long long opened = 0, inter = 0, corrected = 0;
     uint32_t maxsp = 1, err = 0;
+
+    // Corrections are batched per wave: a pair to correct stages its
+    // features in row `slot` of the wave's queue and remembers (slot, G m /
+    // d^2); a flush runs the network on the queued rows and every lane adds
+    // its own pairs' corrections, in the order it queued them, to a separate
+    // sum that joins the acceleration at the end.
+    constexpr uint32_t kPend = 4;
+    uint32_t qn = 0;                              // queued rows (wave-uniform)
+    uint32_t pend_n = 0, pend_slot[kPend];
+    float pend_s2[kPend];
+    float cax = 0.0f, cay = 0.0f, caz = 0.0f;
+    auto flush = [&]() {
+        if (qn == 0) return;
+        const uint32_t nb = (qn + 15u) >> 4;      // 16-sample blocks holding queued rows
+        mlp_wave_forward_lds(s_wf, s_bias, s_x[w], s_y[w], (1u << nb) - 1u);
+        for (uint32_t i = 0; i < kPend; i++)
+            if (i < pend_n) {
+                const float* y = &s_y[w][pend_slot[i] * 4];
+                cax += pend_s2[i] * y[0];
+                cay += pend_s2[i] * y[1];
+                caz += pend_s2[i] * y[2];
+            }
+        pend_n = 0;
+        qn = 0;
+    };
 
     if (num > 1) {
         uint32_t node = n;
@@ -599,32 +632,29 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                 const bool correct = accept && ratio >= theta_plain;
                 const unsigned long long open = ballot(mine && !accept);
                 const unsigned long long corr = ballot(mine && correct);
-                float cx = 0.0f, cy = 0.0f, cz = 0.0f;
                 if (corr) {
-                    const uint32_t cnt = node_range[k].y;
-                    float f[kMlpIn];
-                    mlp_features(rx, ry, rz, inv, soft, nd.comMass.w, cnt, mn, mx, nd.comMass, f);
-                    mlp_stage_features(s_x[w], f);
-                    // 16-sample blocks holding at least one lane to correct
-                    uint32_t blocks = 0;
-                    for (int b = 0; b < 4; b++) blocks |= ((corr >> (16 * b)) & 0xFFFFull) ? (1u << b) : 0u;
-                    mlp_wave_forward(R, s_x[w], s_y[w], blocks);
-                    cx = s_y[w][lane * 4 + 0];
-                    cy = s_y[w][lane * 4 + 1];
-                    cz = s_y[w][lane * 4 + 2];
-                    corrected += (unsigned long long)__popcll(corr);
+                    // queue the pairs to correct; the network runs on full
+                    // queues (64 samples, or a lane with kPend pairs waiting)
+                    const uint32_t nc = (uint32_t)__popcll(corr);
+                    if (qn + nc > 64u || ballot(((corr >> lane) & 1ull) && pend_n == kPend)) flush();
+                    if (mine && correct) {
+                        const uint32_t cnt = node_range[k].y;
+                        float f[kMlpIn];
+                        mlp_features(rx, ry, rz, inv, soft, nd.comMass.w, cnt, mn, mx, nd.comMass, f);
+                        const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
+                        mlp_stage_row(s_x[w], (int)slot, f);
+                        pend_slot[pend_n] = slot;
+                        pend_s2[pend_n] = fmul(G, nd.comMass.w) * inv * inv;
+                        pend_n++;
+                    }
+                    qn += nc;
+                    corrected += (unsigned long long)nc;
                 }
                 if (mine && accept) {
                     const float gm = fmul(G, nd.comMass.w);
                     ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
                     ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
                     az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                    if (correct) {
-                        const float s2 = gm * inv * inv;
-                        ax += s2 * cx;
-                        ay += s2 * cy;
-                        az += s2 * cz;
-                    }
                 }
                 inter += (unsigned long long)__popcll(mask & ~open);
                 if (open) {
@@ -670,6 +700,12 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
             }
         }
     }
+    flush();
+    // the corrections' sum joins the monopole sum (a zero correction leaves
+    // the walk's own bits, the sign of a zero included)
+    if (cax != 0.0f) ax += cax;
+    if (cay != 0.0f) ay += cay;
+    if (caz != 0.0f) az += caz;
     if (active) {
         const float dt = dt_;
         const float nvx = fadd(vd.x, fmul(ax, dt));
