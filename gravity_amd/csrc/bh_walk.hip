@@ -609,7 +609,79 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
         qn = 0;
     };
 
-    if (num > 1) {
+    if (num > 1 && walk_lean_ok(nodes[0].half, soft, ctr->treeHeight, num)) {
+        // walk_lean's loop (one record array for leaves and internal nodes,
+        // stack in lanes, branch-free push/pop, 32-bit wave counters) with
+        // the box-aware acceptance and the queued corrections
+        const char* __restrict__ wbase = reinterpret_cast<const char*>(nodes - n);
+        auto record = [&](uint32_t id) { return *reinterpret_cast<const u32x8*>(wbase + (uint32_t)(id << 5)); };
+        const float open_k = fmul(fmul(theta, theta), 1.0009766f);
+        int st_node = 0, st_lo = 0, st_hi = 0;
+        uint32_t node = n, op32 = 0, in32 = 0, co32 = 0;
+        unsigned long long mask = ballot(active);
+        int sp = 0;
+        for (;;) {
+            const u32x8 rec = record(node);
+            const bool internal = node >= n;
+            const uint32_t k = internal ? node - n : 0u;   // a leaf reads node 0's box, unused
+            const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
+            const float4 com = make_float4(__uint_as_float(rec[0]), __uint_as_float(rec[1]),
+                                           __uint_as_float(rec[2]), __uint_as_float(rec[3]));
+            const float half = __uint_as_float(rec[4]), half2 = __uint_as_float(rec[7]);
+            const float rx = fsub(com.x, pm.x);
+            const float ry = fsub(com.y, pm.y);
+            const float rz = fsub(com.z, pm.z);
+            const float d2 = fadd(dot3(rx, ry, rz), soft);
+            unsigned long long open = mask;
+            if (ballot(!(fmul(d2, open_k) <= half2)) & mask) {
+                const float inv = inv_sqrt_fast(d2);
+                const float ratio = fmul(half, inv);
+                const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y &&
+                                    pm.z >= mn.z && pm.z <= mx.z;
+                const bool accept = ratio < theta && !inside;
+                const unsigned long long accm = internal ? ballot(accept) : ~0ull;
+                const unsigned long long take = accm & mask & ballot(node != g);   // self skip at a leaf
+                open = mask & ~accm;
+                if (__builtin_amdgcn_inverse_ballot_w64(take))
+                    add_force(ax, ay, az, fmul(G, com.w), rx, ry, rz, inv);
+                in32 += (uint32_t)__popcll(take);
+                const unsigned long long corr = internal ? ballot(accept && ratio >= theta_plain) & mask : 0ull;
+                if (corr) {
+                    const uint32_t nc = (uint32_t)__popcll(corr);
+                    if (qn + nc > 64u || ballot(((corr >> lane) & 1ull) && pend_n == kPend)) flush();
+                    if ((corr >> lane) & 1ull) {
+                        float f[kMlpIn];
+                        mlp_features(rx, ry, rz, inv, soft, com.w, node_range[k].y, mn, mx, com, f);
+                        const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
+                        mlp_stage_row(s_x[w], (int)slot, f);
+                        pend_slot[pend_n] = slot;
+                        pend_s2[pend_n] = fmul(G, com.w) * inv * inv;
+                        pend_n++;
+                    }
+                    qn += nc;
+                    co32 += nc;
+                }
+            }
+            const int top = sp - 1;
+            const uint32_t pop_node = (uint32_t)__builtin_amdgcn_readlane(st_node, top);
+            const unsigned long long pop_mask =
+                ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, top) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane(st_lo, top);
+            st_node = lane_write((int)rec[5], sp, st_node);
+            st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
+            st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
+            op32 += (uint32_t)__popcll(open);
+            const bool descend = open != 0ull;
+            node = descend ? rec[6] : pop_node;
+            mask = descend ? open : pop_mask;
+            sp = descend ? sp + 1 : top;
+            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
+            if (sp < 0) break;
+        }
+        opened = op32;
+        inter = in32;
+        corrected = co32;
+    } else if (num > 1) {
         uint32_t node = n;
         unsigned long long mask = ballot(active);
         int sp = 0;
