@@ -215,12 +215,14 @@ __global__ void k_pack_weights(const float* __restrict__ p, __bf16* __restrict__
             const int sl = f - kFragW3;
             if (r16 < kMlpOut) v = p[kOffW3 + r16 * kMlpHidden + mlp_hid(sl, grp, j)];
         }
-        w[i] = (__bf16)v;
+        // log2(e) into the hidden pre-activations, ln(2) out of the last layer's inputs
+        const float scale = f < kFragW2 ? 1.44269504f : (f < kFragW3 ? 1.0f : 0.693147181f);
+        w[i] = (__bf16)(v * scale);
     }
     if (i < kBiasTotal) {
         float v = 0.0f;
-        if (i < 64) v = p[kOffB1 + i];
-        else if (i < 128) v = p[kOffB2 + i - 64];
+        if (i < 64) v = p[kOffB1 + i] * 1.44269504f;
+        else if (i < 128) v = p[kOffB2 + i - 64] * 1.44269504f;
         else if (i - 128 < kMlpOut) v = p[kOffB3 + i - 128];
         bias[i] = v;
     }

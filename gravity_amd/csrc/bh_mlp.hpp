@@ -16,6 +16,11 @@
 // loads its 14 fragments with coalesced 16-byte loads and holds them in
 // registers for the whole kernel. Biases (FP32, b1[64] b2[64] b3[16]) seed
 // the accumulators.
+//
+// The packed copy carries log2(e) in the hidden pre-activations: W1, b1 and
+// b2 are scaled by log2(e) and W3 by ln(2), so a hidden unit holds
+// a' = log2(e) a and SiLU becomes a' / (1 + 2^-a') = log2(e) silu(a), whose
+// factor the next layer's unscaled W2 (or the ln(2) in W3) takes back.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -69,10 +74,11 @@ __device__ __forceinline__ void mlp_features(float rx, float ry, float rz, float
     f[11] = soft * inv * inv;
 }
 
-// x * sigmoid(x) with the hardware exp2 and reciprocal (the result is
-// rounded to BF16 next, far coarser than their error)
+// SiLU of a pre-activation held as x = log2(e) a, returning log2(e) silu(a),
+// with the hardware exp2 and reciprocal (the result is rounded to BF16 next,
+// far coarser than their error)
 __device__ __forceinline__ float silu_fast(float x) {
-    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * x));
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-x));
 }
 
 __device__ __forceinline__ bf16x8 pack_silu(f32x4 lo, f32x4 hi) {

@@ -542,6 +542,7 @@ int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu) {
 
 namespace bh {
 
+template <bool kLean>
 __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                                       const float4* __restrict__ src, float4* __restrict__ dst,
                                                       const uint32_t* __restrict__ index_in,
@@ -551,9 +552,15 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                                                       const uint2* __restrict__ node_range,
                                                       const __bf16* __restrict__ mlp_w,
                                                       const float* __restrict__ mlp_b, float theta_, float theta_plain,
-                                                      float dt_, float softening) {
-    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
-    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+                                                      float dt_, float softening, uint32_t general) {
+    // Two instantiations, launched one after the other: the lean loop, and
+    // the general loop with its LDS stack for the trees the lean loop does
+    // not take; each returns at once on the other's trees, and the lean one
+    // holds no stack in LDS, which lets more of its blocks share a CU.
+    const uint32_t n = sim->initialNumBodies;
+    const uint32_t num = sim->numBodies;
+    const bool lean = !general && num > 1 && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num);
+    if (lean != kLean) return;
     __shared__ __attribute__((aligned(16))) __bf16 s_x[kWalkWaves][64 * kMlpXStride];
     __shared__ __attribute__((aligned(16))) float s_y[kWalkWaves][64 * 4];
 
@@ -566,8 +573,6 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
         s_bias[i] = reinterpret_cast<const f32x4*>(mlp_b)[i];
     __syncthreads();
 
-    const uint32_t n = sim->initialNumBodies;
-    const uint32_t num = sim->numBodies;
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t base = (lb * kWalkWaves + w) * kWave;
@@ -609,7 +614,7 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
         qn = 0;
     };
 
-    if (num > 1 && walk_lean_ok(nodes[0].half, soft, ctr->treeHeight, num)) {
+    if constexpr (kLean) {
         // walk_lean's loop (one record array for leaves and internal nodes,
         // stack in lanes, branch-free push/pop, 32-bit wave counters) with
         // the box-aware acceptance and the queued corrections
@@ -682,6 +687,8 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
         inter = in32;
         corrected = co32;
     } else if (num > 1) {
+        __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
+        __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
         uint32_t node = n;
         unsigned long long mask = ballot(active);
         int sp = 0;
@@ -802,9 +809,12 @@ void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const flo
                          const uint2* node_range, const __bf16* w, const float* bias, float theta_plain) {
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
-    k_walk_learned<<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in,
-                                                      a.leaf_pos, a.nodes, node_aabb, node_range, w, bias, a.theta,
-                                                      theta_plain, a.dt, a.softening);
+    k_walk_learned<true><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
+                                                            a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
+                                                            bias, a.theta, theta_plain, a.dt, a.softening, a.general);
+    k_walk_learned<false><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
+                                                             a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
+                                                             bias, a.theta, theta_plain, a.dt, a.softening, a.general);
 }
 
 }  // namespace bh

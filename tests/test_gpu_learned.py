@@ -110,13 +110,15 @@ def test_pair_labels_match_cpu_fp64():
     bh.close()
 
 
-def test_corrected_walk_with_zero_network_equals_box_aware_oracle_walk():
+@pytest.mark.parametrize("general", [False, True])
+def test_corrected_walk_with_zero_network_equals_box_aware_oracle_walk(general):
     """With a zero network the corrected walk is the box-aware walk: bit for
-    bit the oracle's walk that opens every node containing its target."""
+    bit the oracle's walk that opens every node containing its target, from
+    the lean loop and from the general loop alike."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
     from gravity_amd import learned as Lr
-    print("[gpu] zero-network corrected walk", flush=True)
+    print(f"[gpu] zero-network corrected walk, general loop {general}", flush=True)
     n = 1 << 16
     soft = 0.01
     bodies = ic.plummer(n, 5)
@@ -127,7 +129,7 @@ def test_corrected_walk_with_zero_network_equals_box_aware_oracle_walk():
     m = Lr.Mlp(params=np.zeros(Lr.num_params(), np.float32))
     Lr.attach(bh, m, 0.5)
     lp = B.Params.from_buffer_copy(prm)
-    lp.flags = B.FLAG_LEARNED
+    lp.flags = B.FLAG_LEARNED | (B.FLAG_GENERAL_WALK if general else 0)
     corr = bh.accelerations(lp)
     c = bh.counters()
     assert c["corrected"] > 0
