@@ -41,7 +41,7 @@ static_assert(sizeof(NodeStd430) == 80, "Node is 80 bytes");
 
 // What the walk reads per internal node: one 32-byte record, one scalar load.
 struct alignas(32) WalkNode {
-    float4 comMass;     // centre of mass, mass
+    float4 comMass;     // centre of mass, G * mass (the walk's factor, rounded once as each interaction would)
     float half;         // half of the longest AABB side (bh_force.comp:14-20)
     uint32_t childA;    // global node ids: < n leaf, >= n internal (n = initialNumBodies)
     uint32_t childB;

@@ -140,7 +140,7 @@ int enqueue_step(bh_ensemble* e) {
     int rc = enqueue_sort(e, e->cfg.radix_bits);
     if (rc) return rc;
     launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b], e->index[e->mi_in],
-                    e->keys[e->mi_in], e->tree);
+                    e->keys[e->mi_in], e->tree, e->G);
     if ((rc = enqueue_walk(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, e->walk_blocks))) return rc;
     e->body_in = 1 - b;
     return BH_OK;
@@ -527,7 +527,7 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
         // walk shape on the tree of the current state
         if ((rc = prepare()) || (rc = enqueue_sort(e, best.radix_bits))) break;
         launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b0], e->index[e->mi_in],
-                        e->keys[e->mi_in], e->tree);
+                        e->keys[e->mi_in], e->tree, e->G);
         DevCounters d;
         if (hipMemcpyAsync(&d, e->ctr, sizeof d, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
             hipStreamSynchronize(e->stream) != hipSuccess) {

@@ -646,6 +646,7 @@ __device__ __forceinline__ void get_state(const float4* x, NodeState& s, int col
 // segment off = 0, ibase = 0, n_ids = n; an ensemble has one per simulation.
 struct TreeSeg {
     uint32_t off, num, ibase, n_ids;
+    float G;   // the walk records carry G m
 };
 
 // The summary of leaf (sorted position) j as its own thread starts it.
@@ -659,11 +660,11 @@ __device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
     return me;
 }
 
-// The walk's record of a leaf: its body, with a half side of 0 and a half2
-// of -1 (so that the test for nodes every lane opens never passes).
-__device__ __forceinline__ WalkNode leaf_record(float4 pm) {
+// The walk's record of a leaf: its position and G m, with a half side of 0
+// and a half2 of -1 (so that the test for nodes every lane opens never passes).
+__device__ __forceinline__ WalkNode leaf_record(float4 pm, float G) {
     WalkNode w;
-    w.comMass = pm;
+    w.comMass = make_float4(pm.x, pm.y, pm.z, fmul(G, pm.w));
     w.half = 0.0f;
     w.childA = 0;
     w.childB = 0;
@@ -737,7 +738,7 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
     const uint32_t k = S.ibase + pid;                 // storage index of the parent
     p.id = S.n_ids + k;
     WalkNode nd;
-    nd.comMass = p.com;
+    nd.comMass = make_float4(p.com.x, p.com.y, p.com.z, fmul(S.G, M));
     const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
     nd.half = fdiv(ext, 2.0f);
     nd.childA = A.id;
@@ -746,7 +747,7 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
     // these arrays are read only by later kernels
     T.nodes[k] = nd;
     T.node_aabb[2 * k] = make_float4(p.mn.x, p.mn.y, p.mn.z, __uint_as_float(p.height));
-    T.node_aabb[2 * k + 1] = p.mx;
+    T.node_aabb[2 * k + 1] = make_float4(p.mx.x, p.mx.y, p.mx.z, M);
     if (colors) T.node_color[k] = p.col;
     T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
     if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
@@ -886,7 +887,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
         const uint32_t b = L.index_in[g];
         const float4 pm = L.src[2 * b];
         T.leaf_pos[g] = pm;
-        T.walk[g] = leaf_record(pm);
+        T.walk[g] = leaf_record(pm, S.G);
         s.pm[t] = pm;
         if (kColors) {
             const float4 vd = L.src[2 * b + 1];
@@ -999,7 +1000,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
         const float4 pm = src[2 * index_in[0]];
         T.leaf_pos[0] = pm;
-        T.walk[0] = leaf_record(pm);
+        T.walk[0] = leaf_record(pm, sim->gravitationalConstant);
         if (kColors) {
             const float4 vd = src[2 * index_in[0] + 1];
             float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1021,7 +1022,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         ctr->treeHeight = 0;
         return;
     }
-    const TreeSeg S = {0u, (uint32_t)num, 0u, n};
+    const TreeSeg S = {0u, (uint32_t)num, 0u, n, sim->gravitationalConstant};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
     const int cnt = num - blo < (int)kTreeBlock ? num - blo : (int)kTreeBlock;
     tree_block<kColors>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
@@ -1050,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict_
     const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
     NodeState me = frontier_state(e);
     if (colors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
-    const TreeSeg S = {0u, sim->numBodies, 0u, n};
+    const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
     tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors);
 }
@@ -1202,7 +1203,7 @@ __global__ void k_nodes_to_std430(const SimValues* __restrict__ sim, TreeArrays 
         if (k + 1 < num || (num == 1 && k == 0)) {
             WalkNode w = T.nodes[k];
             float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
-            o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = w.comMass.w;
+            o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = mx.w;
             o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
             o.childA = w.childA;
             o.childB = w.childB;
@@ -1224,14 +1225,14 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
     if (which == 0 && k < n) {
         const NodeStd430& o = leaf_in[k];
         T.leaf_pos[k] = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], o.comMass[3]);
-        T.walk[k] = leaf_record(T.leaf_pos[k]);
+        T.walk[k] = leaf_record(T.leaf_pos[k], sim->gravitationalConstant);
         T.leaf_parent[k] = o.parentId;
         T.leaf_color[k] = make_float4(o.avgColor[0], o.avgColor[1], o.avgColor[2], o.avgColor[3]);
     }
     if (which == 1 && k + 1 < n) {
         const NodeStd430& o = int_in[k];
         WalkNode w;
-        w.comMass = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], o.comMass[3]);
+        w.comMass = make_float4(o.comMass[0], o.comMass[1], o.comMass[2], fmul(sim->gravitationalConstant, o.comMass[3]));
         float ext = gmax(fsub(o.aabb[3], o.aabb[0]), gmax(fsub(o.aabb[4], o.aabb[1]), fsub(o.aabb[5], o.aabb[2])));
         w.half = fdiv(ext, 2.0f);
         w.childA = o.childA;
@@ -1239,7 +1240,7 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
         w.half2 = fmul(w.half, w.half);
         T.nodes[k] = w;
         T.node_aabb[2 * k] = make_float4(o.aabb[0], o.aabb[1], o.aabb[2], __uint_as_float(o.nodeDepth));
-        T.node_aabb[2 * k + 1] = make_float4(o.aabb[3], o.aabb[4], o.aabb[5], 0.f);
+        T.node_aabb[2 * k + 1] = make_float4(o.aabb[3], o.aabb[4], o.aabb[5], o.comMass[3]);
         T.node_parent[k] = o.parentId;
         T.node_range[k] = make_uint2(0, o.bodiesContained);
         T.node_color[k] = make_float4(o.avgColor[0], o.avgColor[1], o.avgColor[2], o.avgColor[3]);
@@ -1544,14 +1545,15 @@ __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restric
                                                          DevCounters* __restrict__ ctr,
                                                          const float4* __restrict__ src,
                                                          const uint32_t* __restrict__ index_in,
-                                                         const uint64_t* __restrict__ keys, TreeArrays T) {
+                                                         const uint64_t* __restrict__ keys, TreeArrays T,
+                                                         float G) {
     __shared__ TreeBlockLds<false> s;
     const EnsTile tl = tiles[blockIdx.x >> 1];
     const uint32_t first = (blockIdx.x & 1u) * kTreeBlock;
     if (first >= tl.count) return;
     const EnsSim& S = sims[tl.sim];
     const int cnt = tl.count - first < kTreeBlock ? (int)(tl.count - first) : (int)kTreeBlock;
-    const TreeSeg seg = {S.off, S.num, S.ibase, N};
+    const TreeSeg seg = {S.off, S.num, S.ibase, N, G};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
     tree_block<false>(s, seg, (int)(tl.start + first - S.off), cnt, keys + S.off, L, T, &ctr->treeHeight, true,
                       tl.sim);
@@ -1560,12 +1562,12 @@ __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restric
 __global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict__ sims, uint32_t N,
                                                         DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                         const uint32_t* __restrict__ index_in,
-                                                        const uint64_t* __restrict__ keys, TreeArrays T) {
+                                                        const uint64_t* __restrict__ keys, TreeArrays T, float G) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *T.frontier_count) return;
     const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
     const EnsSim& S = sims[e.range.z];
-    const TreeSeg seg = {S.off, S.num, S.ibase, N};
+    const TreeSeg seg = {S.off, S.num, S.ibase, N, G};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
     tree_climb((int)e.range.x, (int)e.range.y, seg, frontier_state(e), keys + S.off, L, T, &ctr->treeHeight,
                true, 0);
@@ -1577,7 +1579,7 @@ __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430
     NodeStd430 o = {};
     const WalkNode w = T.nodes[k];
     const float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
-    o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = w.comMass.w;
+    o.comMass[0] = w.comMass.x; o.comMass[1] = w.comMass.y; o.comMass[2] = w.comMass.z; o.comMass[3] = mx.w;
     o.aabb[0] = mn.x; o.aabb[1] = mn.y; o.aabb[2] = mn.z; o.aabb[3] = mx.x; o.aabb[4] = mx.y; o.aabb[5] = mx.z;
     o.childA = w.childA;
     o.childB = w.childB;
@@ -1626,10 +1628,11 @@ int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t n
 
 void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, uint32_t N,
                      DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
-                     const TreeArrays& T) {
+                     const TreeArrays& T, float G) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    k_ens_tree<<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T);
-    k_ens_tree_upper<<<cdiv(frontier_cap(N, 2 * ntiles), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T);
+    k_ens_tree<<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
+    k_ens_tree_upper<<<cdiv(frontier_cap(N, 2 * ntiles), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T,
+                                                                               G);
 }
 
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out) {

@@ -22,10 +22,10 @@ struct TreeArrays {
     float4* leaf_pos;        // [n] posMass in sorted order
     uint32_t* leaf_parent;   // [n]
     float4* leaf_color;      // [n]
-    WalkNode* walk;          // [2n-1] the walk's records: leaf j at j (its body, half 0, half2 -1), then the
-                             // internal nodes; node id i is walk[i]
+    WalkNode* walk;          // [2n-1] the walk's records (centre of mass and G m): leaf j at j (half 0,
+                             // half2 -1), then the internal nodes; node id i is walk[i]
     WalkNode* nodes;         // = walk + n: internal node k
-    float4* node_aabb;       // [2(n-1)] min, max; min.w holds the height (nodeDepth) as uint bits
+    float4* node_aabb;       // [2(n-1)] min, max; min.w holds the height (nodeDepth) as uint bits, max.w the mass
     uint32_t* node_parent;   // [n-1]
     uint2* node_range;       // [n-1] (first sorted position, bodies contained)
     float4* node_color;      // [n-1]
@@ -103,7 +103,7 @@ int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t n
 // tree: two blocks of kTreeBlock positions per tile
 void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, uint32_t N,
                      DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
-                     const TreeArrays& T);
+                     const TreeArrays& T, float G);
 // persistent walk over the task queue; returns nonzero for an unsupported shape
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,

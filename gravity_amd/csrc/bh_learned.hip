@@ -97,13 +97,14 @@ __global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ p
         const double rx = (double)nd.comMass.x - x.x, ry = (double)nd.comMass.y - x.y, rz = (double)nd.comMass.z - x.z;
         const double ds2 = rx * rx + ry * ry + rz * rz + (double)soft;
         const double ds = sqrt(ds2);
-        const double scale = ds2 / (G * (double)nd.comMass.w);   // 1 / (G M / d^2)
+        const float mass = node_aabb[2 * k + 1].w;                // (the record holds G M)
+        const double scale = ds2 / (G * (double)mass);           // 1 / (G M / d^2)
         targ[3 * p + 0] = (float)(ax * scale - rx / ds);
         targ[3 * p + 1] = (float)(ay * scale - ry / ds);
         targ[3 * p + 2] = (float)(az * scale - rz / ds);
         const float frx = nd.comMass.x - x.x, fry = nd.comMass.y - x.y, frz = nd.comMass.z - x.z;
         const float inv = inv_sqrt_exact(fadd(dot3(frx, fry, frz), soft));
-        mlp_features(frx, fry, frz, inv, soft, nd.comMass.w, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1],
+        mlp_features(frx, fry, frz, inv, soft, mass, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1],
                      nd.comMass, feat + 12 * (size_t)p);
     }
 }

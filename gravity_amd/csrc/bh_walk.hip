@@ -54,7 +54,7 @@ __device__ __forceinline__ void add_force(float& ax, float& ay, float& az, float
 // The overlap test of a leaf visit in the merging (1) and collision (2)
 // modes (bh_force.comp:85-112); use: this lane interacts with the leaf.
 template <uint32_t mode>
-__device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float4 lp, float rx, float ry, float rz,
+__device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float rx, float ry, float rz,
                                              float4& pm, float4& vd, float Ri, const float* __restrict__ leaf_radius,
                                              uint32_t bodyIdx, float elasticity, float restitution,
                                              const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
@@ -76,17 +76,18 @@ __device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float4 lp,
         } else if (hit) {
             // collision (bh_force.comp:93-107): normalize(r) = r * inversesqrt(r.r)
             const float4 ov = src[2 * other + 1];
+            const float mj = src[2 * other].w;                    // the leaf's mass
             const float il = fdiv(1.0f, fsqrt(dot3(rx, ry, rz)));
             const float nx = fmul(rx, il), ny = fmul(ry, il), nz = fmul(rz, il);
             const float vimp = dot3(fsub(ov.x, vd.x), fsub(ov.y, vd.y), fsub(ov.z, vd.z), nx, ny, nz);
             if (vimp < 0.0f) {
-                const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, lp.w)));
+                const float meff = fdiv(1.0f, fadd(fdiv(1.0f, pm.w), fdiv(1.0f, mj)));
                 const float imp = fmul(fmul(fadd(1.0f, elasticity), meff), vimp);
                 vd.x = fadd(vd.x, fdiv(fmul(nx, imp), pm.w));
                 vd.y = fadd(vd.y, fdiv(fmul(ny, imp), pm.w));
                 vd.z = fadd(vd.z, fdiv(fmul(nz, imp), pm.w));
             }
-            const float cs = fmul(fdiv(pen, fadd(pm.w, lp.w)), restitution);
+            const float cs = fmul(fdiv(pen, fadd(pm.w, mj)), restitution);
             pm.x = fsub(pm.x, fmul(cs, nx));
             pm.y = fsub(pm.y, fmul(cs, ny));
             pm.z = fsub(pm.z, fmul(cs, nz));
@@ -172,11 +173,13 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
             inv = kFast ? inv_sqrt_fast(d2) : inv_sqrt_exact(d2);
             take = use_m;
         }
-        if (__builtin_amdgcn_inverse_ballot_w64(take)) add_force(ax, ay, az, fmul(G, sp4.w), rx, ry, rz, inv);
+        // an internal record holds G m; a leaf's position-mass holds m
+        if (__builtin_amdgcn_inverse_ballot_w64(take))
+            add_force(ax, ay, az, internal ? sp4.w : fmul(G, sp4.w), rx, ry, rz, inv);
         inter += (unsigned long long)__popcll(take);
         if constexpr (mode != 0u)
             if (!internal && use_m)
-                leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(use_m), node, sp4, rx, ry, rz, pm, vd, Ri,
+                leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(use_m), node, rx, ry, rz, pm, vd, Ri,
                                    leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr, tasks, task_cap,
                                    err);
         bool descend = false;
@@ -278,11 +281,11 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
             const unsigned long long take = accm & mask & ballot(node != g);
             open = mask & ~accm;
             if (__builtin_amdgcn_inverse_ballot_w64(take))
-                add_force(ax, ay, az, fmul(G, nd.comMass.w), rx, ry, rz, inv);
+                add_force(ax, ay, az, nd.comMass.w, rx, ry, rz, inv);   // the record holds G m
             inter += (uint32_t)__popcll(take);
             if constexpr (mode != 0u)
                 if (!internal && take)
-                    leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(take), node, nd.comMass, rx, ry, rz, pm,
+                    leaf_contact<mode>(__builtin_amdgcn_inverse_ballot_w64(take), node, rx, ry, rz, pm,
                                        vd, Ri, leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr,
                                        tasks, task_cap, err);
         }
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                 const unsigned long long take = accm & mask & ballot(node != g);   // self skip at a leaf
                 open = mask & ~accm;
                 if (__builtin_amdgcn_inverse_ballot_w64(take))
-                    add_force(ax, ay, az, fmul(G, com.w), rx, ry, rz, inv);
+                    add_force(ax, ay, az, com.w, rx, ry, rz, inv);   // the record holds G m
                 in32 += (uint32_t)__popcll(take);
                 const unsigned long long corr = internal ? ballot(accept && ratio >= theta_plain) & mask : 0ull;
                 if (corr) {
@@ -656,11 +659,11 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                     if (qn + nc > 64u || ballot(((corr >> lane) & 1ull) && pend_n == kPend)) flush();
                     if ((corr >> lane) & 1ull) {
                         float f[kMlpIn];
-                        mlp_features(rx, ry, rz, inv, soft, com.w, node_range[k].y, mn, mx, com, f);
+                        mlp_features(rx, ry, rz, inv, soft, mx.w, node_range[k].y, mn, mx, com, f);
                         const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
                         mlp_stage_row(s_x[w], (int)slot, f);
                         pend_slot[pend_n] = slot;
-                        pend_s2[pend_n] = fmul(G, com.w) * inv * inv;
+                        pend_s2[pend_n] = com.w * inv * inv;
                         pend_n++;
                     }
                     qn += nc;
@@ -719,18 +722,18 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
                     if (mine && correct) {
                         const uint32_t cnt = node_range[k].y;
                         float f[kMlpIn];
-                        mlp_features(rx, ry, rz, inv, soft, nd.comMass.w, cnt, mn, mx, nd.comMass, f);
+                        mlp_features(rx, ry, rz, inv, soft, mx.w, cnt, mn, mx, nd.comMass, f);
                         const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
                         mlp_stage_row(s_x[w], (int)slot, f);
                         pend_slot[pend_n] = slot;
-                        pend_s2[pend_n] = fmul(G, nd.comMass.w) * inv * inv;
+                        pend_s2[pend_n] = nd.comMass.w * inv * inv;
                         pend_n++;
                     }
                     qn += nc;
                     corrected += (unsigned long long)nc;
                 }
                 if (mine && accept) {
-                    const float gm = fmul(G, nd.comMass.w);
+                    const float gm = nd.comMass.w;   // the record holds G m
                     ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
                     ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
                     az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
