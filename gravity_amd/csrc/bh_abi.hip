@@ -61,6 +61,13 @@ struct bh_context {
     uint32_t* task_done = nullptr;
     NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1
     float* leaf_radius = nullptr;             // body radius per sorted position (merging/collision walks)
+    // Positions the last bh_step's walk wrote, by sorted position (the same
+    // values it wrote to the body buffer by id). While valid, the next step's
+    // Morton encode reads them in order instead of gathering bodies through
+    // the index; any other call that can change bodies, index or order ends
+    // the validity, and a partition that removes bodies skips them on device.
+    float4* step_pos = nullptr;
+    bool step_pos_valid = false;
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
     float theta_plain = 0.5f;
@@ -257,6 +264,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->task_done, (size_t)n));
         TRY(dalloc(c, &c->staging_nodes, (size_t)n));
         TRY(dalloc(c, &c->leaf_radius, (size_t)n));
+        TRY(dalloc(c, &c->step_pos, (size_t)n));
         // zero everything the reference zero-initialises, then pack sim
         // (gpu/GPU.java:1096-1143)
         for (int k = 0; k < 2; k++) {
@@ -304,7 +312,7 @@ void bh_destroy(bh_context* c) {
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
-                    c->reserve, c->task_done, c->staging_nodes, c->leaf_radius};
+                    c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -356,6 +364,7 @@ static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
 
 int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) {
     if (!c || (!host && bytes)) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
@@ -423,18 +432,21 @@ int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {
 
 int bh_swap_body_buffers(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     c->body_in = 1 - c->body_in;
     return BH_OK;
 }
 
 int bh_swap_morton_and_index_buffers(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     c->mi_in = 1 - c->mi_in;
     return BH_OK;
 }
 
 int bh_init(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     HIPCHECK(c, hipSetDevice(c->device));
     launch_init(c->stream, c->index[c->mi_in], c->n);
     return check_launch(c, "init");
@@ -451,6 +463,7 @@ int bh_reset_values(bh_context* c, const bh_params* p) {
 
 int bh_partition_dead_bodies(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     launch_dead_partition(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->bodies[1 - c->body_in],
                           c->index[c->mi_in], c->index[1 - c->mi_in], c->part_scratch);
     int rc = check_launch(c, "dead partition");
@@ -465,14 +478,23 @@ int bh_update_bounds(bh_context* c) {
     return check_launch(c, "bounds");
 }
 
+namespace {
+int generate_morton_codes(bh_context* c, const float4* step_pos) {
+    launch_encode(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], step_pos,
+                  c->morton[c->mi_in]);
+    return check_launch(c, "encode");
+}
+}  // namespace
+
 int bh_generate_morton_codes(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
-    launch_encode(c->stream, c->n, c->sim, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in]);
-    return check_launch(c, "encode");
+    c->step_pos_valid = false;
+    return generate_morton_codes(c, nullptr);
 }
 
 int bh_radix_sort(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     for (uint32_t pass = 0; pass < kSortPasses; pass++) {
         launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], c->index[c->mi_in], c->morton[1 - c->mi_in],
                          c->index[1 - c->mi_in], pass * kSortBits, c->tile_hist, c->tiles_cap, c->digit_total);
@@ -485,14 +507,18 @@ int bh_radix_sort(bh_context* c) {
 
 int bh_build_tree(bh_context* c, const bh_params* p) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     c->colors = p && (p->flags & BH_FLAG_COLORS);
     launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
                 c->tree, c->colors ? 1 : 0);
     return check_launch(c, "tree");
 }
 
-int bh_compute_force(bh_context* c, const bh_params* p) {
-    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+namespace {
+// step_pos: where the plain walk also writes the new positions by sorted
+// position (nullptr: not written); *wrote tells whether it did.
+int compute_force(bh_context* c, const bh_params* p, float4* step_pos, bool* wrote) {
+    *wrote = false;
     if (p->merging_collision_or_neither > BH_COLLISION || p->static_or_dynamic > 1)
         return fail(c, BH_ERR_INVALID_ARGUMENT, "bad selector");
     WalkArgs a;
@@ -518,6 +544,7 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
     a.restitution = 0.0f;
     a.leaf_radius = c->leaf_radius;
     a.general = (p->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
+    a.pos_out = step_pos;
     if (p->flags & BH_FLAG_LEARNED) {
         if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
         if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)
@@ -527,11 +554,22 @@ int bh_compute_force(bh_context* c, const bh_params* p) {
         return check_launch(c, "walk (corrected)");
     }
     launch_walk(c->stream, c->n, a);
-    return check_launch(c, "walk");
+    const int rc = check_launch(c, "walk");
+    *wrote = rc == BH_OK && step_pos != nullptr;
+    return rc;
+}
+}  // namespace
+
+int bh_compute_force(bh_context* c, const bh_params* p) {
+    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
+    bool wrote;
+    return compute_force(c, p, nullptr, &wrote);
 }
 
 int bh_merge_bodies(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
     launch_merge(c->stream, c->bodies[1 - c->body_in], c->tasks, c->n, c->ctr, c->reserve, c->task_done);
     return check_launch(c, "merge");
 }
@@ -542,6 +580,9 @@ int bh_step(bh_context* c, const bh_params* p) {
     c->timing = (p->flags & BH_FLAG_TIMING) != 0;
     int rc;
     if (c->timing && (rc = ensure_events(c))) return rc;
+    // the entry points below end the validity; this step's encode may use
+    // what the previous step's walk left, if nothing else ran in between
+    const bool use_pos = c->step_pos_valid;
     stage_begin(c, 0);
     if ((rc = bh_reset_values(c, p))) return rc;
     stage_begin(c, 1);
@@ -549,18 +590,22 @@ int bh_step(bh_context* c, const bh_params* p) {
     stage_begin(c, 2);
     if (p->static_or_dynamic == BH_DYNAMIC && (rc = bh_update_bounds(c))) return rc;
     stage_begin(c, 3);
-    if ((rc = bh_generate_morton_codes(c))) return rc;
+    if ((rc = generate_morton_codes(c, use_pos ? c->step_pos : nullptr))) return rc;
     stage_begin(c, 4);
     if ((rc = bh_radix_sort(c))) return rc;
     stage_begin(c, 5);
     if ((rc = bh_build_tree(c, p))) return rc;
     stage_begin(c, 6);
-    if ((rc = bh_compute_force(c, p))) return rc;
+    bool wrote = false;
+    c->step_pos_valid = false;
+    if ((rc = compute_force(c, p, c->step_pos, &wrote))) return rc;
     stage_begin(c, 7);
     if (p->merging_collision_or_neither == BH_MERGING && (rc = bh_merge_bodies(c))) return rc;
     stage_begin(c, 8);
     if (c->timing && c->timed < bh_context::kTimedSteps) c->timed++;
-    return bh_swap_body_buffers(c);
+    rc = bh_swap_body_buffers(c);
+    c->step_pos_valid = rc == BH_OK && wrote;
+    return rc;
 }
 
 int bh_sync(bh_context* c) {

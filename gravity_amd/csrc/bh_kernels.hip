@@ -350,15 +350,22 @@ __device__ __forceinline__ uint64_t morton_of(float4 pm, const float* bmin, cons
     return (expand_bits21(q[0]) << 2) | (expand_bits21(q[1]) << 1) | expand_bits21(q[2]);
 }
 
-__global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ sim, const float4* __restrict__ src,
+// The body at sorted position g is src[index_in[g]]; step_pos, when given
+// and the partition before this encode removed nobody (so that the order of
+// positions is the one the last walk wrote them in), holds the same values
+// by position and is read in order instead of gathered.
+__global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ sim,
+                                                const DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                 const uint32_t* __restrict__ index_in,
+                                                const float4* __restrict__ step_pos,
                                                 uint64_t* __restrict__ morton_in) {
     const uint32_t num = sim->numBodies;
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= num) return;
     float bmin[3] = {sim->minCorner[0], sim->minCorner[1], sim->minCorner[2]};
     float bmax[3] = {sim->maxCorner[0], sim->maxCorner[1], sim->maxCorner[2]};
-    morton_in[g] = morton_of(src[2 * index_in[g]], bmin, bmax, bmin[0] > bmax[0]);
+    const float4 pm = (step_pos && ctr->died == 0) ? step_pos[g] : src[2 * index_in[g]];
+    morton_in[g] = morton_of(pm, bmin, bmax, bmin[0] > bmax[0]);
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -1279,9 +1286,9 @@ void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, 
     if (nb < 1) nb = 1;
     k_bounds<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, partials);
 }
-void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const float4* src, const uint32_t* index_in,
-                   uint64_t* morton_in) {
-    k_encode<<<cdiv(n, 256), 256, 0, s>>>(sim, src, index_in, morton_in);
+void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCounters* ctr, const float4* src,
+                   const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in) {
+    k_encode<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, step_pos, morton_in);
 }
 void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
                       const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,

@@ -62,6 +62,7 @@ struct WalkArgs {
     float elasticity, restitution;   // COLLISION mode
     float* leaf_radius;              // [n] scratch of the merging and collision modes
     uint32_t general;                // 1: always the general walk loop
+    float4* pos_out = nullptr;       // [n] new position-mass by sorted position as well, or nullptr
 };
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.
@@ -119,8 +120,10 @@ void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounter
                            const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch);
 void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, const float4* src,
                    const uint32_t* index_in, float* partials, uint32_t partial_cap);
-void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const float4* src, const uint32_t* index_in,
-                   uint64_t* morton_in);
+// step_pos: the bodies by sorted position as the previous step's walk wrote
+// them (read when the partition removed nobody), or nullptr: gathered from src
+void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCounters* ctr, const float4* src,
+                   const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in);
 void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
                       const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
                       uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total);

@@ -329,7 +329,8 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                                               const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
-                                              const float* __restrict__ leaf_radius, uint32_t general) {
+                                              const float* __restrict__ leaf_radius, uint32_t general,
+                                              float4* __restrict__ pos_out) {
     __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
     __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
 
@@ -388,14 +389,12 @@ __global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCo
                 kill = true;
             }
         }
-        if (kill) {
-            dst[2 * bodyIdx] = make_float4(0.f, 0.f, 0.f, 0.f);
-            dst[2 * bodyIdx + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            oob = 1;
-        } else {
-            dst[2 * bodyIdx] = make_float4(npx, npy, npz, pm.w);
-            dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
-        }
+        const float4 np = kill ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(npx, npy, npz, pm.w);
+        const float4 nv = kill ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(nvx, nvy, nvz, vd.w);
+        dst[2 * bodyIdx] = np;
+        dst[2 * bodyIdx + 1] = nv;
+        if (pos_out) pos_out[g] = np;   // the same value by sorted position
+        oob = kill ? 1u : 0u;
     }
     // one set of atomics per wave
     const uint32_t oob_w = __popcll(ballot(oob != 0));
@@ -418,7 +417,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     k_walk<M><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
-                                                a.leaf_radius, a.general)
+                                                a.leaf_radius, a.general, a.pos_out)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
     if (a.merging == 1u) BH_WALK(1u);
