@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <utility>
 
 #include "../../include/bh_abi.h"
 #include "bh_device.hpp"
@@ -68,6 +69,12 @@ struct bh_context {
     // the validity, and a partition that removes bodies skips them on device.
     float4* step_pos = nullptr;
     bool step_pos_valid = false;
+    // bh_step sorts positions instead of ids: sort_pos[1] ends up holding,
+    // per sorted position, the position the body had before the sort; the
+    // tree reads the bodies through it (from step_pos, nearly in order) and
+    // writes the ids in the new order to sort_pos[0], which then becomes
+    // IndexIn (the buffers trade places).
+    uint32_t* sort_pos[2] = {nullptr, nullptr};
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
     float theta_plain = 0.5f;
@@ -236,6 +243,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
             TRY(dalloc(c, &c->bodies[k], 2 * (size_t)n));
             TRY(dalloc(c, &c->morton[k], (size_t)n));
             TRY(dalloc(c, &c->index[k], (size_t)n));
+            TRY(dalloc(c, &c->sort_pos[k], (size_t)n));
         }
         TRY(dalloc(c, &c->sim, 1));
         TRY(dalloc(c, &c->tasks, (size_t)n));
@@ -312,7 +320,8 @@ void bh_destroy(bh_context* c) {
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
-                    c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos};
+                    c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
+                    c->sort_pos[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -492,12 +501,21 @@ int bh_generate_morton_codes(bh_context* c) {
     return generate_morton_codes(c, nullptr);
 }
 
-int bh_radix_sort(bh_context* c) {
-    if (!c) return BH_ERR_INVALID_ARGUMENT;
-    c->step_pos_valid = false;
+namespace {
+// Stable LSD sort of the keys in MortonIn, landing back in MortonIn (an even
+// number of passes), with the ids in IndexIn (positions: with the positions
+// 0..num-1 instead, into sort_pos[1], IndexIn untouched).
+static_assert(kSortPasses % 2 == 0, "the sort ends in the In buffers");
+int radix_sort(bh_context* c, bool positions) {
     for (uint32_t pass = 0; pass < kSortPasses; pass++) {
-        launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], c->index[c->mi_in], c->morton[1 - c->mi_in],
-                         c->index[1 - c->mi_in], pass * kSortBits, c->tile_hist, c->tiles_cap, c->digit_total);
+        if (positions)
+            launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], pass ? c->sort_pos[c->mi_in] : nullptr,
+                             c->morton[1 - c->mi_in], c->sort_pos[1 - c->mi_in], pass * kSortBits, c->tile_hist,
+                             c->tiles_cap, c->digit_total);
+        else
+            launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], c->index[c->mi_in],
+                             c->morton[1 - c->mi_in], c->index[1 - c->mi_in], pass * kSortBits, c->tile_hist,
+                             c->tiles_cap, c->digit_total);
         int rc = check_launch(c, "sort pass");
         if (rc) return rc;
         c->mi_in = 1 - c->mi_in;
@@ -505,13 +523,29 @@ int bh_radix_sort(bh_context* c) {
     return BH_OK;
 }
 
+// positions: the sort before sorted positions (radix_sort(c, true)); the
+// tree then puts the ids in the new order into IndexIn.
+int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* step_pos) {
+    c->colors = p && (p->flags & BH_FLAG_COLORS);
+    launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
+                c->tree, c->colors ? 1 : 0, positions ? step_pos : nullptr,
+                positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr);
+    const int rc = check_launch(c, "tree");
+    if (rc == BH_OK && positions) std::swap(c->index[c->mi_in], c->sort_pos[1 - c->mi_in]);
+    return rc;
+}
+}  // namespace
+
+int bh_radix_sort(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
+    return radix_sort(c, false);
+}
+
 int bh_build_tree(bh_context* c, const bh_params* p) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
     c->step_pos_valid = false;
-    c->colors = p && (p->flags & BH_FLAG_COLORS);
-    launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
-                c->tree, c->colors ? 1 : 0);
-    return check_launch(c, "tree");
+    return build_tree(c, p, false, nullptr);
 }
 
 namespace {
@@ -592,9 +626,9 @@ int bh_step(bh_context* c, const bh_params* p) {
     stage_begin(c, 3);
     if ((rc = generate_morton_codes(c, use_pos ? c->step_pos : nullptr))) return rc;
     stage_begin(c, 4);
-    if ((rc = bh_radix_sort(c))) return rc;
+    if ((rc = radix_sort(c, true))) return rc;
     stage_begin(c, 5);
-    if ((rc = bh_build_tree(c, p))) return rc;
+    if ((rc = build_tree(c, p, true, use_pos ? c->step_pos : nullptr))) return rc;
     stage_begin(c, 6);
     bool wrote = false;
     c->step_pos_valid = false;

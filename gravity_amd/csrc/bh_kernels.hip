@@ -466,6 +466,14 @@ struct ScatterLds {
     uint32_t w_tot[4];
 };
 
+// Positions travel through the sort multiplied by an odd constant (a
+// bijection of 32-bit words): on MI355X the scatter passes run ~7 % slower
+// when their values are the positions themselves (small, nearly ordered
+// words) than with these scrambled ones, measured over whole sorts.
+constexpr uint32_t kPosScramble = 0x9E3779B1u, kPosUnscramble = 0x0E8B2F51u;
+static_assert(kPosScramble * kPosUnscramble == 1u, "inverse mod 2^32");
+
+// idx_in nullptr: the values start as the scrambled positions start + e.
 template <int BITS, int TILE>
 __device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t* __restrict__ keys_in,
                                              const uint32_t* __restrict__ idx_in, uint32_t start, uint32_t count,
@@ -483,7 +491,7 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t
         const uint32_t e = w * kPer + r * 64 + lane;
         const bool valid = e < count;
         key[r] = valid ? keys_in[start + e] : 0ull;
-        idx[r] = valid ? idx_in[start + e] : 0u;
+        idx[r] = valid ? (idx_in ? idx_in[start + e] : (start + e) * kPosScramble) : 0u;
     }
     for (int r = 0; r < kRows; r++) {
         const uint32_t e = w * kPer + r * 64 + lane;
@@ -683,14 +691,35 @@ struct LeafSource {
     const float4* src;          // bodies (written by earlier kernels: coherent)
     const uint32_t* index_in;   // sorted position -> body id
     float unit_mass, unit_density;
+    // With from set, the sort has sorted positions, not ids: from[j] is the
+    // (scrambled) position before the sort of the body now at j (close to j
+    // for most bodies), index_in is still in that order, and the first launch writes
+    // the ids in the new order to ids_out. With step_pos set too, the body is
+    // step_pos[from[j]] (the previous walk's output by its positions: read
+    // nearly in order, where the body buffer by id is in no useful order).
+    const float4* step_pos = nullptr;
+    const uint32_t* from = nullptr;
+    uint32_t* ids_out = nullptr;
 };
 
-__device__ __forceinline__ NodeState leaf_at(const LeafSource& L, uint32_t j, int colors) {
-    const uint32_t b = L.index_in[j];
-    const float4 pm = L.src[2 * b];
+// the body at sorted position j and its id (the first launch)
+__device__ __forceinline__ float4 leaf_body(const LeafSource& L, uint32_t j, uint32_t& id) {
+    if (!L.from) {
+        id = L.index_in[j];
+        return L.src[2 * id];
+    }
+    const uint32_t f = L.from[j] * kPosUnscramble;
+    id = L.index_in[f];
+    L.ids_out[j] = id;
+    return L.step_pos ? L.step_pos[f] : L.src[2 * id];
+}
+
+// a leaf met during the climb: the first launch has stored every leaf's body in leaf_pos
+__device__ __forceinline__ NodeState leaf_at(const LeafSource& L, const TreeArrays& T, uint32_t j, int colors) {
+    const float4 pm = T.leaf_pos[j];
     NodeState me = leaf_state(pm, j);
     if (colors) {
-        const float4 vd = L.src[2 * b + 1];
+        const float4 vd = L.src[2 * L.index_in[j] + 1];
         if (pm.w > 0.0f && vd.w > 0.0f) me.col = star_color(fmul(pm.w, L.unit_mass), fmul(vd.w, L.unit_density));
     }
     return me;
@@ -787,7 +816,7 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
         if (sib_leaf) {
             if (lo == hi && !left) return;                // two leaves: the left one builds the parent
             other = (uint32_t)(left ? split + 1 : split);
-            sib = leaf_at(L, S.off + other, colors);
+            sib = leaf_at(L, T, S.off + other, colors);
         } else {
             if (lo == hi) return;                         // the internal sibling's thread builds the parent
             float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
@@ -891,8 +920,8 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
     const int bhi = blo + cnt - 1;
     if (t < cnt) {
         const uint32_t g = S.off + (uint32_t)(blo + t);
-        const uint32_t b = L.index_in[g];
-        const float4 pm = L.src[2 * b];
+        uint32_t b;
+        const float4 pm = leaf_body(L, g, b);
         T.leaf_pos[g] = pm;
         T.walk[g] = leaf_record(pm, S.G);
         s.pm[t] = pm;
@@ -996,7 +1025,10 @@ template <bool kColors>
 __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict__ sim,
                                                      DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                      const uint32_t* __restrict__ index_in,
-                                                     const uint64_t* __restrict__ keys, TreeArrays T) {
+                                                     const uint64_t* __restrict__ keys, TreeArrays T,
+                                                     const float4* __restrict__ step_pos,
+                                                     const uint32_t* __restrict__ from,
+                                                     uint32_t* __restrict__ ids_out) {
     __shared__ TreeBlockLds<kColors> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
@@ -1005,6 +1037,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
     if (num == 1) {
         if (threadIdx.x != 0) return;
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
+        if (from) ids_out[0] = index_in[0];
         const float4 pm = src[2 * index_in[0]];
         T.leaf_pos[0] = pm;
         T.walk[0] = leaf_record(pm, sim->gravitationalConstant);
@@ -1030,7 +1063,11 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         return;
     }
     const TreeSeg S = {0u, (uint32_t)num, 0u, n, sim->gravitationalConstant};
-    const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
+    // the previous walk's positions hold this step's bodies unless the
+    // partition before the sort removed some (then their order is another)
+    const bool by_pos = from && step_pos && ctr->died == 0;
+    const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity, by_pos ? step_pos : nullptr, from,
+                          ids_out};
     const int cnt = num - blo < (int)kTreeBlock ? num - blo : (int)kTreeBlock;
     tree_block<kColors>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
 }
@@ -1254,6 +1291,15 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
     }
 }
 
+// ids of the positions past numBodies (the dead bodies, which keep their
+// places) into the new index
+__global__ void k_copy_tail(const SimValues* __restrict__ sim, const uint32_t* __restrict__ index_in,
+                            uint32_t* __restrict__ ids_out) {
+    const uint32_t n = sim->initialNumBodies;
+    for (uint32_t j = sim->numBodies + blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        ids_out[j] = index_in[j];
+}
+
 // fills a u32 array (graph-safe replacement for a memset)
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1300,12 +1346,20 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
                                       digit_total);
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
-                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors) {
+                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
+                 const float4* step_pos, const uint32_t* from, uint32_t* ids_out) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    if (colors) k_tree<true><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T);
-    else k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T);
-    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src, index_in, keys, T,
-                                                                                  colors);
+    if (colors)
+        k_tree<true><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
+                                                                ids_out);
+    else
+        k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
+                                                                 ids_out);
+    if (from) k_copy_tail<<<64, 256, 0, s>>>(sim, index_in, ids_out);
+    // the second launch meets leaves by their ids in the new order
+    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src,
+                                                                                  from ? ids_out : index_in, keys,
+                                                                                  T, colors);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {

@@ -124,11 +124,19 @@ void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, 
 // them (read when the partition removed nobody), or nullptr: gathered from src
 void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCounters* ctr, const float4* src,
                    const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in);
+// One LSD pass over (key, value) at positions [0, numBodies); idx_in nullptr:
+// the values start as the positions (scrambled: times an odd constant).
 void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
                       const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
                       uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total);
+// from: when the sort sorted positions instead of ids, the (scrambled)
+// position before the sort of the body at each sorted position; index_in is then in the
+// order before the sort and the tree writes the ids in the new order to
+// ids_out. step_pos (with from): the previous walk's positions by its sorted
+// positions, read instead of the bodies when the partition removed nobody.
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
-                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors);
+                 const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
+                 const float4* step_pos = nullptr, const uint32_t* from = nullptr, uint32_t* ids_out = nullptr);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
                          const uint2* node_range, const __bf16* w, const float* bias, float theta_plain);
