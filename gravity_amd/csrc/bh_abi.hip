@@ -265,7 +265,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
-        c->tiles_cap = (n + kSortTile - 1) / kSortTile;
+        c->tiles_cap = (n + kSortTileOne - 1) / kSortTileOne;
         TRY(dalloc(c, &c->tile_hist, 256 * (size_t)c->tiles_cap));
         TRY(dalloc(c, &c->digit_total, 256));
         TRY(dalloc(c, &c->reserve, (size_t)n));

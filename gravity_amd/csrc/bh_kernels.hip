@@ -379,11 +379,11 @@ __global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ si
 // lowest of them adds the group's size to the wave's own bins (distinct
 // digits: distinct addresses). Called by all 256 threads; returns thread t's
 // digit count for the tile.
-template <int BITS>
+template <int BITS, int TILE = kSortTile>
 __device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[4][256], const uint64_t* __restrict__ keys,
                                                    uint32_t start, uint32_t count, uint32_t shift) {
     constexpr uint32_t R = 1u << BITS;
-    constexpr int kPer = kSortTile / 4, kRows = kPer / 64;
+    constexpr int kPer = TILE / 4, kRows = kPer / 64;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     const uint64_t lt = lanemask_lt();
     for (int k = 0; k < 4; k++) s_h[w][lane * 4 + k] = 0;
@@ -412,10 +412,10 @@ __global__ __launch_bounds__(256) void k_sort_hist(const SimValues* __restrict__
                                                    uint32_t* __restrict__ tile_hist, uint32_t tiles_cap) {
     __shared__ uint32_t s_h[4][256];
     const uint32_t num = sim->numBodies;
-    const uint32_t base = blockIdx.x * kSortTile;
+    const uint32_t base = blockIdx.x * kSortTileOne;
     if (base >= num) return;
-    const uint32_t count = num - base < kSortTile ? num - base : kSortTile;
-    const uint32_t c = tile_histogram<8>(s_h, keys, base, count, shift);
+    const uint32_t count = num - base < kSortTileOne ? num - base : kSortTileOne;
+    const uint32_t c = tile_histogram<8, kSortTileOne>(s_h, keys, base, count, shift);
     tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = c;
 }
 
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__
                                                    uint32_t tiles_cap, uint32_t* __restrict__ digit_total) {
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
-    const uint32_t T = (num + kSortTile - 1) / kSortTile;
+    const uint32_t T = (num + kSortTileOne - 1) / kSortTileOne;
     uint32_t* row = tile_hist + (size_t)blockIdx.x * tiles_cap;
     const uint32_t per = (T + 255) / 256;
     const uint32_t t = threadIdx.x;
@@ -552,10 +552,10 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restric
                                                       uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                                       uint32_t shift, const uint32_t* __restrict__ tile_hist,
                                                       uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
-    __shared__ ScatterLds<kSortTile> L;
+    __shared__ ScatterLds<kSortTileOne> L;
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
-    const uint32_t base = blockIdx.x * kSortTile;
+    const uint32_t base = blockIdx.x * kSortTileOne;
     if (base >= num) return;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     // digit bases: exclusive scan of the 256 digit totals
@@ -570,7 +570,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restric
     uint32_t before = 0;
     for (uint32_t k = 0; k < w; k++) before += s_w[k];
     L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
-    scatter_tile<8, kSortTile>(L, keys_in, idx_in, base, min(kSortTile, num - base), keys_out, idx_out, shift);
+    scatter_tile<8, kSortTileOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out, idx_out,
+                                  shift);
 }
 
 // ------------------------------------------------------------------ tree (build + fill)
@@ -1339,7 +1340,7 @@ void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCou
 void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
                       const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
                       uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total) {
-    uint32_t nt = cdiv(n, kSortTile);
+    uint32_t nt = cdiv(n, kSortTileOne);
     k_sort_hist<<<nt, 256, 0, s>>>(sim, keys_in, shift, tile_hist, tiles_cap);
     k_sort_scan<<<256, 256, 0, s>>>(sim, tile_hist, tiles_cap, digit_total);
     k_sort_scatter<<<nt, 256, 0, s>>>(sim, keys_in, idx_in, keys_out, idx_out, shift, tile_hist, tiles_cap,
