@@ -379,11 +379,11 @@ __global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ si
 // lowest of them adds the group's size to the wave's own bins (distinct
 // digits: distinct addresses). Called by all 256 threads; returns thread t's
 // digit count for the tile.
-template <int BITS, int TILE = kSortTile>
-__device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[4][256], const uint64_t* __restrict__ keys,
+template <int BITS, int TILE = kSortTile, int W = 4>
+__device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[W][256], const uint64_t* __restrict__ keys,
                                                    uint32_t start, uint32_t count, uint32_t shift) {
     constexpr uint32_t R = 1u << BITS;
-    constexpr int kPer = TILE / 4, kRows = kPer / 64;
+    constexpr int kPer = TILE / W, kRows = kPer / 64;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     const uint64_t lt = lanemask_lt();
     for (int k = 0; k < 4; k++) s_h[w][lane * 4 + k] = 0;
@@ -404,19 +404,23 @@ __device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[4][256], cons
         if (valid && __popcll(peers & lt) == 0) s_h[w][d] += (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    return s_h[0][t] + s_h[1][t] + s_h[2][t] + s_h[3][t];
+    uint32_t c = 0;
+    if (t < 256)
+        for (int k = 0; k < W; k++) c += s_h[k][t];
+    return c;
 }
 
-__global__ __launch_bounds__(256) void k_sort_hist(const SimValues* __restrict__ sim,
-                                                   const uint64_t* __restrict__ keys, uint32_t shift,
-                                                   uint32_t* __restrict__ tile_hist, uint32_t tiles_cap) {
-    __shared__ uint32_t s_h[4][256];
+__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_hist(const SimValues* __restrict__ sim,
+                                                                   const uint64_t* __restrict__ keys, uint32_t shift,
+                                                                   uint32_t* __restrict__ tile_hist,
+                                                                   uint32_t tiles_cap) {
+    __shared__ uint32_t s_h[kSortWavesOne][256];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kSortTileOne;
     if (base >= num) return;
     const uint32_t count = num - base < kSortTileOne ? num - base : kSortTileOne;
-    const uint32_t c = tile_histogram<8, kSortTileOne>(s_h, keys, base, count, shift);
-    tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = c;
+    const uint32_t c = tile_histogram<8, kSortTileOne, kSortWavesOne>(s_h, keys, base, count, shift);
+    if (threadIdx.x < 256) tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = c;
 }
 
 // one block per digit: exclusive scan of that digit's tile counts
@@ -457,13 +461,13 @@ __global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__
 // writes it out so that consecutive threads write consecutive positions of
 // each digit's run. The order within a digit is the element order: stable.
 // L.base[d]: the global position of the tile's first key with digit d.
-template <int TILE>
+template <int TILE, int W = 4>
 struct ScatterLds {
     uint64_t key[TILE];
     uint32_t idx[TILE];
     uint32_t base[256], start[256];
-    uint32_t cnt[4][256];   // per-wave digit counts, then per-wave offsets
-    uint32_t w_tot[4];
+    uint32_t cnt[W][256];   // per-wave digit counts, then per-wave offsets
+    uint32_t w_tot[4];      // the digit scan over 256 digits (4 waves)
 };
 
 // Positions travel through the sort multiplied by an odd constant (a
@@ -474,13 +478,13 @@ constexpr uint32_t kPosScramble = 0x9E3779B1u, kPosUnscramble = 0x0E8B2F51u;
 static_assert(kPosScramble * kPosUnscramble == 1u, "inverse mod 2^32");
 
 // idx_in nullptr: the values start as the scrambled positions start + e.
-template <int BITS, int TILE>
-__device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t* __restrict__ keys_in,
+template <int BITS, int TILE, int W = 4>
+__device__ __forceinline__ void scatter_tile(ScatterLds<TILE, W>& L, const uint64_t* __restrict__ keys_in,
                                              const uint32_t* __restrict__ idx_in, uint32_t start, uint32_t count,
                                              uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                              uint32_t shift) {
     constexpr uint32_t R = 1u << BITS;
-    constexpr int kPer = TILE / 4;         // elements per wave
+    constexpr int kPer = TILE / W;         // elements per wave
     constexpr int kRows = kPer / 64;       // rows of 64 per wave
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -508,23 +512,31 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t
         if (valid && in_row == 0) L.cnt[w][d] = before + __popcll(peers);
     }
     __syncthreads();
-    {   // thread t = digit t: per-wave offsets and the tile's digit starts
-        const uint32_t c0 = L.cnt[0][t], c1 = L.cnt[1][t], c2 = L.cnt[2][t], c3 = L.cnt[3][t];
-        const uint32_t tot = c0 + c1 + c2 + c3;
-        uint32_t incl = tot;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(incl, off, 64);
-            if (lane >= (uint32_t)off) incl += v;
+    {   // thread t < 256 = digit t: per-wave offsets and the tile's digit starts
+        uint32_t c[W], tot = 0, incl = 0;
+        if (t < 256) {
+            for (int k = 0; k < W; k++) {
+                c[k] = L.cnt[k][t];
+                tot += c[k];
+            }
+            incl = tot;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t v = __shfl_up(incl, off, 64);
+                if (lane >= (uint32_t)off) incl += v;
+            }
+            if (lane == 63) L.w_tot[w] = incl;
         }
-        if (lane == 63) L.w_tot[w] = incl;
         __syncthreads();
-        uint32_t before = 0;
-        for (uint32_t k = 0; k < w; k++) before += L.w_tot[k];
-        L.start[t] = before + incl - tot;
-        L.cnt[0][t] = 0;
-        L.cnt[1][t] = c0;
-        L.cnt[2][t] = c0 + c1;
-        L.cnt[3][t] = c0 + c1 + c2;
+        if (t < 256) {
+            uint32_t before = 0;
+            for (uint32_t k = 0; k < w; k++) before += L.w_tot[k];
+            L.start[t] = before + incl - tot;
+            uint32_t run = 0;
+            for (int k = 0; k < W; k++) {
+                L.cnt[k][t] = run;
+                run += c[k];
+            }
+        }
     }
     __syncthreads();
     for (int r = 0; r < kRows; r++) {
@@ -537,7 +549,7 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t
         }
     }
     __syncthreads();
-    for (uint32_t j = t; j < count; j += 256) {
+    for (uint32_t j = t; j < count; j += W * 64) {
         const uint64_t k = L.key[j];
         const uint32_t d = (uint32_t)(k >> shift) & (R - 1u);
         const uint32_t dest = L.base[d] + (j - L.start[d]);
@@ -546,32 +558,37 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE>& L, const uint64_t
     }
 }
 
-__global__ __launch_bounds__(256) void k_sort_scatter(const SimValues* __restrict__ sim,
+__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_scatter(const SimValues* __restrict__ sim,
                                                       const uint64_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ idx_in,
                                                       uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
                                                       uint32_t shift, const uint32_t* __restrict__ tile_hist,
                                                       uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
-    __shared__ ScatterLds<kSortTileOne> L;
+    __shared__ ScatterLds<kSortTileOne, kSortWavesOne> L;
     __shared__ uint32_t s_w[4];
     const uint32_t num = sim->numBodies;
     const uint32_t base = blockIdx.x * kSortTileOne;
     if (base >= num) return;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
-    // digit bases: exclusive scan of the 256 digit totals
-    uint32_t tot = digit_total[t];
-    uint32_t incl = tot;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t v = __shfl_up(incl, off, 64);
-        if (lane >= (uint32_t)off) incl += v;
+    // digit bases: exclusive scan of the 256 digit totals (threads 0..255)
+    uint32_t tot = 0, incl = 0;
+    if (t < 256) {
+        tot = digit_total[t];
+        incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += v;
+        }
+        if (lane == 63) s_w[w] = incl;
     }
-    if (lane == 63) s_w[w] = incl;
     __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t k = 0; k < w; k++) before += s_w[k];
-    L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
-    scatter_tile<8, kSortTileOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out, idx_out,
-                                  shift);
+    if (t < 256) {
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; k++) before += s_w[k];
+        L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
+    }
+    scatter_tile<8, kSortTileOne, kSortWavesOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out,
+                                                 idx_out, shift);
 }
 
 // ------------------------------------------------------------------ tree (build + fill)
@@ -1341,10 +1358,10 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
                       const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
                       uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total) {
     uint32_t nt = cdiv(n, kSortTileOne);
-    k_sort_hist<<<nt, 256, 0, s>>>(sim, keys_in, shift, tile_hist, tiles_cap);
+    k_sort_hist<<<nt, kSortWavesOne * 64, 0, s>>>(sim, keys_in, shift, tile_hist, tiles_cap);
     k_sort_scan<<<256, 256, 0, s>>>(sim, tile_hist, tiles_cap, digit_total);
-    k_sort_scatter<<<nt, 256, 0, s>>>(sim, keys_in, idx_in, keys_out, idx_out, shift, tile_hist, tiles_cap,
-                                      digit_total);
+    k_sort_scatter<<<nt, kSortWavesOne * 64, 0, s>>>(sim, keys_in, idx_in, keys_out, idx_out, shift, tile_hist,
+                                                     tiles_cap, digit_total);
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,

@@ -11,6 +11,7 @@ namespace bh {
 constexpr uint32_t kPartTile = 1024;   // dead partition tile (sorted positions per block)
 constexpr uint32_t kSortTile = 2048;   // radix sort tile of the ensembles (keys per block)
 constexpr uint32_t kSortTileOne = 4096; // radix sort tile of one simulation
+constexpr int kSortWavesOne = 8;        // waves per block of its histogram and scatter kernels
 constexpr uint32_t kSortBits = 8;
 constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
 constexpr int kWalkWaves = 4;          // waves per walk block
