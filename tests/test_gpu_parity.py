@@ -220,6 +220,41 @@ def test_gpu_merge_steps_match_oracle(general):
     bh.close()
 
 
+def test_gpu_steps_with_uploads_between_match_oracle():
+    """bh_step carries the walk's positions by sorted position into the next
+    step's encode and tree; an upload of new bodies between steps (and a
+    stage entry point called on its own) must drop them: with bodies
+    perturbed on both sides before step 3 the trajectory still equals the
+    oracle's bit for bit, and so does a trajectory with deaths (merging)."""
+    import ctypes
+    from gravity_amd import barnes_hut as B
+    for name in ("plummer_4096", "merge_4096"):
+        fx = load(name)
+        print(f"[gpu] steps with uploads between: {name}", flush=True)
+        bh, prm, _, G, blc = make_bh(fx)
+        bh.init()
+        sim = O.Simulation(fx["bodies0"], fx["bounds0"], 1.0, 1.0, G=G, body_len_const=blc)
+        oprm = O.params(**params_of(fx))
+        rng = np.random.default_rng(7)
+        for s in range(6):
+            if s == 3:
+                b = np.ascontiguousarray(bh.bodies(), np.float32)
+                alive = b[:, 3] != 0.0
+                b[alive, 0:3] += rng.normal(0.0, 1e-3, size=(int(alive.sum()), 3)).astype(np.float32)
+                bh.upload(B.BODIES_IN, b)
+                ctypes.memmove(sim.st.bodies[sim.st.cur], b.ctypes.data, b.nbytes)
+            if s == 4:
+                bh.generate_morton_codes()
+            bh.step()
+            assert sim.step(oprm) == 0
+            c = bh.counters()
+            assert c["num_bodies"] == sim.num_bodies
+            assert c["error_flags"] == 0
+            assert_bits(bh.bodies(), sim.bodies(), f"{name} bodies after step {s}")
+            np.testing.assert_array_equal(bh.index()[:sim.num_bodies], sim.index()[:sim.num_bodies])
+        bh.close()
+
+
 def test_gpu_plummer_65536_ten_steps():
     """The reference's CPU-runnable case end to end: 10 steps of bh_step."""
     from gravity_amd import barnes_hut as B
