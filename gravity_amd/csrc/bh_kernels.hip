@@ -1203,39 +1203,114 @@ __global__ __launch_bounds__(256) void k_merge_round(float4* __restrict__ dst, c
     if (applied && lane_id() == 0) atomicAdd(&ctr->mergesApplied, applied);
 }
 
-// The remaining chains, one block of 1024 threads.
+// The remaining chains, one block of 1024 threads. The pending tasks are
+// gathered into a list in LDS once, each round works on the list only and
+// compacts it; more pending tasks than the list holds go the long way: every
+// round scans the whole task array.
+constexpr uint32_t kMergeList = 16384;
+static_assert(kMergeList / 1024 <= 32, "one keep bit per list entry of a thread");
+
+// block-wide exclusive prefix of one flag per thread (1024 threads); returns
+// the thread's offset, *total the block's sum
+__device__ __forceinline__ uint32_t block_prefix_1024(bool flag, uint32_t* s_wsum, uint32_t* total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const unsigned long long b = ballot(flag);
+    if (lane == 0) s_wsum[w] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+    for (uint32_t k = 0; k < 16; k++) {
+        const uint32_t v = s_wsum[k];
+        before += k < w ? v : 0u;
+        sum += v;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + (uint32_t)__popcll(b & lanemask_lt());
+}
+
 __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const uint2* __restrict__ tasks,
                                                 uint32_t cap, DevCounters* __restrict__ ctr,
                                                 unsigned long long* __restrict__ reserve,
                                                 uint32_t* __restrict__ done) {
+    __shared__ uint32_t s_list[kMergeList];
+    __shared__ uint32_t s_wsum[16];
     __shared__ int s_pending;
     const uint32_t nt = min(ctr->nTasks, cap);
+    const uint32_t tid = threadIdx.x;
     uint32_t applied = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_pending = 0;
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            if (done[t]) continue;
-            const uint2 ab = tasks[t];
-            atomicMin(&reserve[ab.x], task_key(ab));
-            atomicMin(&reserve[ab.y], task_key(ab));
-            s_pending = 1;
+    // the pending tasks, in index order
+    uint32_t L = 0;
+    for (uint32_t base = 0; base < nt; base += 1024) {
+        const uint32_t t = base + tid;
+        const bool pend = t < nt && done[t] == 0u;
+        uint32_t tot;
+        const uint32_t off = block_prefix_1024(pend, s_wsum, &tot);
+        if (pend && L + off < kMergeList) s_list[L + off] = t;
+        L += tot;
+    }
+    __syncthreads();
+    if (L <= kMergeList) {
+        while (L) {
+            // a round: reserve, commit, release (each a barrier apart), then
+            // keep the tasks that did not commit
+            for (uint32_t i = tid; i < L; i += 1024) {
+                const uint2 ab = tasks[s_list[i]];
+                atomicMin(&reserve[ab.x], task_key(ab));
+                atomicMin(&reserve[ab.y], task_key(ab));
+            }
+            __syncthreads();
+            uint32_t keep = 0;   // bit k: entry tid + 1024 k stays pending
+            for (uint32_t i = tid, k = 0; i < L; i += 1024, k++)
+                if (!merge_commit(dst, tasks[s_list[i]], reserve, applied)) keep |= 1u << k;
+            __syncthreads();
+            for (uint32_t i = tid; i < L; i += 1024) {
+                const uint2 ab = tasks[s_list[i]];
+                __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            // in place, chunk by chunk: every entry of a chunk is read before
+            // the prefix's barrier and written at or below its own place
+            uint32_t kept = 0;
+            for (uint32_t base = 0, k = 0; base < L; base += 1024, k++) {
+                const uint32_t i = base + tid;
+                const uint32_t t = i < L ? s_list[i] : 0u;
+                const bool stay = i < L && ((keep >> k) & 1u);
+                uint32_t tot;
+                const uint32_t off = block_prefix_1024(stay, s_wsum, &tot);
+                if (stay) s_list[kept + off] = t;
+                kept += tot;
+            }
+            __syncthreads();
+            L = kept;
         }
-        __syncthreads();
-        if (!s_pending) break;
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            if (done[t]) continue;
-            if (merge_commit(dst, tasks[t], reserve, applied)) done[t] = 2u;
+    } else {
+        for (;;) {
+            if (tid == 0) s_pending = 0;
+            __syncthreads();
+            for (uint32_t t = tid; t < nt; t += blockDim.x) {
+                if (done[t]) continue;
+                const uint2 ab = tasks[t];
+                atomicMin(&reserve[ab.x], task_key(ab));
+                atomicMin(&reserve[ab.y], task_key(ab));
+                s_pending = 1;
+            }
+            __syncthreads();
+            if (!s_pending) break;
+            for (uint32_t t = tid; t < nt; t += blockDim.x) {
+                if (done[t]) continue;
+                if (merge_commit(dst, tasks[t], reserve, applied)) done[t] = 2u;
+            }
+            __syncthreads();
+            for (uint32_t t = tid; t < nt; t += blockDim.x) {
+                if (done[t] == 1u) continue;
+                const uint2 ab = tasks[t];
+                __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (done[t] == 2u) done[t] = 1u;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
-            if (done[t] == 1u) continue;
-            const uint2 ab = tasks[t];
-            __hip_atomic_store(&reserve[ab.x], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&reserve[ab.y], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (done[t] == 2u) done[t] = 1u;
-        }
-        __syncthreads();
     }
     for (int off = 32; off > 0; off >>= 1) applied += __shfl_xor(applied, off, 64);
     if (applied && lane_id() == 0) atomicAdd(&ctr->mergesApplied, applied);
