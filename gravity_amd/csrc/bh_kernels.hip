@@ -1450,7 +1450,7 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
     else
         k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
                                                                  ids_out);
-    if (from) k_copy_tail<<<64, 256, 0, s>>>(sim, index_in, ids_out);
+    if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
     // the second launch meets leaves by their ids in the new order
     k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src,
                                                                                   from ? ids_out : index_in, keys,
