@@ -470,14 +470,23 @@ int bh_reset_values(bh_context* c, const bh_params* p) {
     return check_launch(c, "reset");
 }
 
-int bh_partition_dead_bodies(bh_context* c) {
-    if (!c) return BH_ERR_INVALID_ARGUMENT;
-    c->step_pos_valid = false;
+namespace {
+// with_bounds: the partition's census pass also updates the bounds (what
+// bh_update_bounds does; the bounds skip the bodies the partition removes)
+int partition_dead_bodies(bh_context* c, bool with_bounds) {
     launch_dead_partition(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->bodies[1 - c->body_in],
-                          c->index[c->mi_in], c->index[1 - c->mi_in], c->part_scratch);
+                          c->index[c->mi_in], c->index[1 - c->mi_in], c->part_scratch,
+                          with_bounds ? c->bounds_partials : nullptr, c->bounds_partial_cap);
     int rc = check_launch(c, "dead partition");
     if (rc) return rc;
     return bh_swap_morton_and_index_buffers(c);   // BarnesHut.java:117
+}
+}  // namespace
+
+int bh_partition_dead_bodies(bh_context* c) {
+    if (!c) return BH_ERR_INVALID_ARGUMENT;
+    c->step_pos_valid = false;
+    return partition_dead_bodies(c, false);
 }
 
 int bh_update_bounds(bh_context* c) {
@@ -620,9 +629,10 @@ int bh_step(bh_context* c, const bh_params* p) {
     stage_begin(c, 0);
     if ((rc = bh_reset_values(c, p))) return rc;
     stage_begin(c, 1);
-    if ((rc = bh_partition_dead_bodies(c))) return rc;
+    // dynamic bounds are made by the partition's census pass (one read of
+    // the bodies for both); the bounds stage then records no work
+    if ((rc = partition_dead_bodies(c, p->static_or_dynamic == BH_DYNAMIC))) return rc;
     stage_begin(c, 2);
-    if (p->static_or_dynamic == BH_DYNAMIC && (rc = bh_update_bounds(c))) return rc;
     stage_begin(c, 3);
     if ((rc = generate_morton_codes(c, use_pos ? c->step_pos : nullptr))) return rc;
     stage_begin(c, 4);

@@ -54,29 +54,6 @@ __global__ void k_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ c
 // dead) and counts them. If that count equals n - numBodies nobody died and
 // the partition is the identity: the scatter copies the index. Otherwise the
 // stable partition runs in index order.
-__global__ __launch_bounds__(256) void k_dead_census(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                                     float4* __restrict__ src, float4* __restrict__ dst) {
-    __shared__ uint32_t s_w[4];
-    const uint32_t n = sim->initialNumBodies;
-    uint32_t zero = 0;
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
-        if (src[2 * b].w == 0.0f) {
-            // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
-            src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-            src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-            dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            zero++;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) zero += __shfl_xor(zero, off, 64);
-    if (lane_id() == 0) s_w[threadIdx.x >> 6] = zero;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-        if (t) atomicAdd(&ctr->zeroMassIds, t);
-    }
-}
 
 __device__ __forceinline__ bool nobody_died(const SimValues* sim, const DevCounters* ctr) {
     return ctr->zeroMassIds == sim->initialNumBodies - sim->numBodies;
@@ -287,12 +264,83 @@ __device__ __forceinline__ void bounds_collapse(const float* prev, const float* 
     }
 }
 
+// The block's min/max (r) to partials; the last block to finish reduces
+// every block's partial (min/max: order-free) and collapses the bounds. All
+// 256 threads call it.
+__device__ void bounds_finish(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                              float* __restrict__ partials, const float* prev, float* r) {
+    __shared__ float s_r[6][4];
+    __shared__ bool s_last;
+    bounds_block_reduce(r, s_r);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++)
+            __hip_atomic_store(&partials[6 * blockIdx.x + k], r[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        uint32_t ticket = __hip_atomic_fetch_add(&ctr->aabbTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+        for (int c = 0; c < 3; c++)
+            res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int c = 3; c < 6; c++)
+            res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    bounds_block_reduce(res, s_r);
+    if (threadIdx.x != 0) return;
+    ctr->aabbTicket = 0;
+    bounds_collapse(prev, res, sim->minCorner, sim->maxCorner);
+}
+
+// The census of the dead partition (see above), here because it can also
+// make the bounds.
+// With kBounds the same pass also makes the bounds of k_bounds (which skips
+// the zero-mass bodies the census counts) and collapses them.
+template <bool kBounds>
+__global__ __launch_bounds__(256) void k_dead_census(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                     float4* __restrict__ src, float4* __restrict__ dst,
+                                                     float* __restrict__ partials) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t n = sim->initialNumBodies;
+    float prev[6], r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
+    BoundsWindow W;
+    if constexpr (kBounds) {
+        for (int k = 0; k < 3; k++) {
+            prev[k] = sim->minCorner[k];
+            prev[3 + k] = sim->maxCorner[k];
+        }
+        W = bounds_window(prev, prev + 3);
+    }
+    uint32_t zero = 0;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+        const float4 pm = src[2 * b];
+        if constexpr (kBounds) bounds_accum(W, pm, r);
+        if (pm.w == 0.0f) {
+            // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
+            src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+            dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            zero++;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) zero += __shfl_xor(zero, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = zero;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        if (t) atomicAdd(&ctr->zeroMassIds, t);
+    }
+    if constexpr (kBounds) bounds_finish(sim, ctr, partials, prev, r);
+}
+
 __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                                 const float4* __restrict__ src,
                                                 const uint32_t* __restrict__ index_in,
                                                 float* __restrict__ partials) {
-    __shared__ float s_r[6][4];
-    __shared__ bool s_last;
     const uint32_t num = sim->numBodies;
     const float prev[6] = {sim->minCorner[0], sim->minCorner[1], sim->minCorner[2],
                            sim->maxCorner[0], sim->maxCorner[1], sim->maxCorner[2]};
@@ -306,29 +354,7 @@ __global__ __launch_bounds__(256) void k_bounds(SimValues* __restrict__ sim, Dev
     const uint32_t n = sim->initialNumBodies;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x)
         bounds_accum(W, src[2 * b], r);
-    bounds_block_reduce(r, s_r);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++)
-            __hip_atomic_store(&partials[6 * blockIdx.x + k], r[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        uint32_t ticket = __hip_atomic_fetch_add(&ctr->aabbTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = ticket == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    // the last block reduces every block's partial (min/max: order-free)
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    float res[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
-    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-        for (int c = 0; c < 3; c++)
-            res[c] = gmin(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        for (int c = 3; c < 6; c++)
-            res[c] = gmax(res[c], __hip_atomic_load(&partials[6 * k + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    bounds_block_reduce(res, s_r);
-    if (threadIdx.x != 0) return;
-    ctr->aabbTicket = 0;
-    bounds_collapse(prev, res, sim->minCorner, sim->maxCorner);
+    bounds_finish(sim, ctr, partials, prev, r);
 }
 
 // ------------------------------------------------------------------ Morton
@@ -1412,10 +1438,17 @@ void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units
     k_reset<<<1, 64, 0, s>>>(sim, ctr, set_units, mass, density, len, time, G, blc);
 }
 void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
-                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch) {
+                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch,
+                           float* bounds_partials, uint32_t partial_cap) {
     uint32_t nb = cdiv(n, kPartTile);
     uint32_t nc = cdiv(n, 256 * 8);
-    k_dead_census<<<nc < 1 ? 1 : nc, 256, 0, s>>>(sim, ctr, src, dst);
+    if (nc < 1) nc = 1;
+    if (bounds_partials) {
+        if (nc > partial_cap) nc = partial_cap;
+        k_dead_census<true><<<nc, 256, 0, s>>>(sim, ctr, src, dst, bounds_partials);
+    } else {
+        k_dead_census<false><<<nc, 256, 0, s>>>(sim, ctr, src, dst, nullptr);
+    }
     k_dead_count<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, block_scratch);
     k_dead_scan<<<1, 1024, 0, s>>>(sim, ctr, block_scratch);
     k_dead_scatter<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, index_out, block_scratch);

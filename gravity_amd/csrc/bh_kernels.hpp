@@ -118,8 +118,10 @@ void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_inte
 void launch_init(hipStream_t s, uint32_t* index_in, uint32_t n);
 void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units, float mass, float density,
                   float len, float time, float G, float blc);
+// bounds_partials: the census pass also updates the bounds (as launch_bounds)
 void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
-                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch);
+                           const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch,
+                           float* bounds_partials = nullptr, uint32_t partial_cap = 0);
 void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, const float4* src,
                    const uint32_t* index_in, float* partials, uint32_t partial_cap);
 // step_pos: the bodies by sorted position as the previous step's walk wrote
