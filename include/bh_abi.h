@@ -116,7 +116,10 @@ typedef struct bh_context bh_context;
 
 /* Context create/destroy: device memory for n_initial bodies, a HIP stream,
  * sim.numBodies = sim.initialNumBodies = n_initial, sim.bounds = bounds
- * (min xyz, max xyz), sim.units from the unit uniforms.
+ * (min xyz, max xyz), sim.units from the unit uniforms. n_initial must be
+ * at least 2 (the internal-node buffers hold n_initial - 1 nodes; a run whose
+ * bodies merge down to one takes the reference's one-body tree) and at most
+ * 2^30 (BH_ERR_TOO_LARGE).
  * Replaces GPU.initGPU (gpu/GPU.java:172-188) + BarnesHut.init (BarnesHut.java:170-173). */
 int bh_create(int device, uint32_t n_initial, const float *bounds6, const bh_params *units,
               bh_context **out);

@@ -255,6 +255,60 @@ def test_gpu_steps_with_uploads_between_match_oracle():
         bh.close()
 
 
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 1023, 1025, 4095, 4097, 8193])
+def test_gpu_step_sizes_match_oracle(n):
+    """bh_step at sizes around the wave (64), tree block (1,024) and sort
+    tile (4,096) boundaries equals the oracle's step bit for bit, with the
+    positions carried from step to step."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    print(f"[gpu] step sizes n={n}", flush=True)
+    bodies = ic.plummer(n, 100 + n)
+    bounds = ic.tight_bounds(bodies)
+    prm = B.make_params(theta=0.5, dt=1e-3, softening=0.01, G=1.0)
+    bh = B.BarnesHut(bodies, bounds, prm)
+    bh.init()
+    sim = O.Simulation(bodies, bounds, 1.0, 1.0, G=1.0, body_len_const=1.0)
+    oprm = O.params(theta=0.5, dt=1e-3, softening=0.01)
+    for s in range(3):
+        bh.step()
+        assert sim.step(oprm) == 0
+        c = bh.counters()
+        assert c["error_flags"] == 0
+        assert_bits(bh.bodies(), sim.bodies(), f"n={n} bodies after step {s}")
+        np.testing.assert_array_equal(bh.index(), sim.index())
+    bh.close()
+
+
+def test_gpu_two_bodies_merge_to_one():
+    """Two overlapping bodies merge in the first step; from the next step on
+    one body is left and the walk runs on the reference's one-body tree
+    (bh_tree.comp:46-62). Every step equals the oracle's bit for bit."""
+    from gravity_amd import barnes_hut as B
+    print("[gpu] two bodies merge to one", flush=True)
+    bodies = np.zeros((2, 8), np.float32)
+    bodies[0, :4] = [0.0, 0.0, 0.0, 0.5]
+    bodies[1, :4] = [0.01, 0.0, 0.0, 0.5]
+    bodies[:, 4:8] = [[0.0, 0.1, 0.0, 1.0], [0.0, -0.1, 0.0, 1.0]]
+    bounds = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+    prm = B.make_params(theta=0.5, dt=1e-3, softening=1e-6, merging=B.MERGING, G=1.0, body_length_constant=1.0)
+    bh = B.BarnesHut(bodies, bounds, prm)
+    bh.init()
+    sim = O.Simulation(bodies, bounds, 1.0, 1.0, G=1.0, body_len_const=1.0)
+    oprm = O.params(theta=0.5, dt=1e-3, softening=1e-6, merging=1)
+    alive = []
+    for s in range(4):
+        bh.step()
+        assert sim.step(oprm) == 0
+        c = bh.counters()
+        assert c["error_flags"] == 0
+        assert c["num_bodies"] == sim.num_bodies
+        alive.append(c["num_bodies"])
+        assert_bits(bh.bodies(), sim.bodies(), f"bodies after step {s}")
+    assert alive[0] == 2 and alive[-1] == 1
+    bh.close()
+
+
 def test_gpu_plummer_65536_ten_steps():
     """The reference's CPU-runnable case end to end: 10 steps of bh_step."""
     from gravity_amd import barnes_hut as B
