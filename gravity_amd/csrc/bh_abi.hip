@@ -34,6 +34,7 @@ struct bh_mlp {
     uint32_t cap = 0;
     float *x = nullptr, *t = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr, *y = nullptr,
           *dy = nullptr, *partial = nullptr;
+    float* gemm_ws = nullptr;   // split-K partial sums of the weight gradients
     char err[512] = {0};
 };
 
@@ -146,6 +147,8 @@ int mfail(bh_mlp* m, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return mfail((m), BH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
     } while (0)
 
+constexpr uint32_t kGemmWsFloats = (uint32_t)kGemmSlices * kMlpHidden * kMlpHidden;
+
 int mlp_workspace(bh_mlp* m, uint32_t rows) {
     if (rows <= m->cap) return BH_OK;
     float** bufs[] = {&m->x, &m->t, &m->z1, &m->h1, &m->z2, &m->h2, &m->y, &m->dy, &m->partial};
@@ -157,6 +160,7 @@ int mlp_workspace(bh_mlp* m, uint32_t rows) {
     size_t sizes[] = {r * kMlpIn, r * kMlpOut, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden,
                       r * kMlpOut, r * kMlpOut, (r * kMlpOut + 255) / 256 + 1};
     for (int k = 0; k < 9; k++) MHIP(m, hipMalloc((void**)bufs[k], sizes[k] * sizeof(float)));
+    if (!m->gemm_ws) MHIP(m, hipMalloc((void**)&m->gemm_ws, kGemmWsFloats * sizeof(float)));
     m->cap = rows;
     return BH_OK;
 }
@@ -181,17 +185,20 @@ uint32_t mlp_backward_fp32(bh_mlp* m, uint32_t rows) {
     uint32_t nparts = 0;
     launch_loss_grad(s, m->y, m->t, m->dy, B, m->partial, &nparts);
     // layer 3
-    launch_gemm(s, m->dy, 1, kMlpOut, m->h2, kMlpHidden, 1, g + kOffW3, kMlpOut, kMlpHidden, B, 0);
+    launch_gemm(s, m->dy, 1, kMlpOut, m->h2, kMlpHidden, 1, g + kOffW3, kMlpOut, kMlpHidden, B, 0, m->gemm_ws,
+                kGemmWsFloats);
     launch_colsum(s, m->dy, g + kOffB3, B, kMlpOut);
     launch_gemm(s, m->dy, kMlpOut, 1, m->params + kOffW3, kMlpHidden, 1, m->h2, B, kMlpHidden, kMlpOut, 0);
     launch_act_grad(s, m->z2, m->h2, (size_t)B * kMlpHidden);          // h2 now holds dZ2
     // layer 2
-    launch_gemm(s, m->h2, 1, kMlpHidden, m->h1, kMlpHidden, 1, g + kOffW2, kMlpHidden, kMlpHidden, B, 0);
+    launch_gemm(s, m->h2, 1, kMlpHidden, m->h1, kMlpHidden, 1, g + kOffW2, kMlpHidden, kMlpHidden, B, 0,
+                m->gemm_ws, kGemmWsFloats);
     launch_colsum(s, m->h2, g + kOffB2, B, kMlpHidden);
     launch_gemm(s, m->h2, kMlpHidden, 1, m->params + kOffW2, kMlpHidden, 1, m->h1, B, kMlpHidden, kMlpHidden, 0);
     launch_act_grad(s, m->z1, m->h1, (size_t)B * kMlpHidden);          // h1 now holds dZ1
     // layer 1
-    launch_gemm(s, m->h1, 1, kMlpHidden, m->x, kMlpIn, 1, g + kOffW1, kMlpHidden, kMlpIn, B, 0);
+    launch_gemm(s, m->h1, 1, kMlpHidden, m->x, kMlpIn, 1, g + kOffW1, kMlpHidden, kMlpIn, B, 0, m->gemm_ws,
+                kGemmWsFloats);
     launch_colsum(s, m->h1, g + kOffB1, B, kMlpHidden);
     return nparts;
 }
@@ -728,7 +735,7 @@ void bh_mlp_destroy(bh_mlp* m) {
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* ptrs[] = {m->params, m->grad, m->m1, m->m2, m->w, m->bias, m->x, m->t, m->z1, m->h1, m->z2, m->h2,
-                    m->y, m->dy, m->partial};
+                    m->y, m->dy, m->partial, m->gemm_ws};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);

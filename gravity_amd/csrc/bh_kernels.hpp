@@ -152,8 +152,11 @@ void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, cons
                         float soft, uint32_t max_bodies, float* feat, float* targ);
 void launch_gather_rows(hipStream_t s, const float* src, const uint32_t* perm, uint32_t offset, uint32_t rows,
                         int cols, float* dst);
+// ws (ws_floats floats, or nullptr): workspace for splitting a long K
+// (at most kGemmSlices slices of M x N partial sums)
+constexpr int kGemmSlices = 64, kGemmSliceK = 256;
 void launch_gemm(hipStream_t s, const float* A, int sam, int sak, const float* B, int sbk, int sbn, float* C, int M,
-                 int N, int K, int accumulate);
+                 int N, int K, int accumulate, float* ws = nullptr, uint32_t ws_floats = 0);
 void launch_bias_act(hipStream_t s, float* Z, float* H, const float* bias, int M, int N);
 void launch_loss_grad(hipStream_t s, const float* Y, const float* T, float* dY, int M, float* partial,
                       uint32_t* nblocks);

@@ -134,6 +134,39 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, int s
     }
 }
 
+// The same product with the K range cut into gridDim.z slices: slice z
+// writes its partial sums to ws[z][M][N] (the weight gradients have K = the
+// batch and only M x N <= 64 x 64 outputs, too few blocks for the plain one)
+__global__ __launch_bounds__(256) void k_gemm_splitk(const float* __restrict__ A, int sam, int sak,
+                                                     const float* __restrict__ B, int sbk, int sbn,
+                                                     float* __restrict__ ws, int M, int N, int K, int chunk) {
+    __shared__ float sA[16][17], sB[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m = blockIdx.y * 16 + ty, nn = blockIdx.x * 16 + tx;
+    const int k_begin = blockIdx.z * chunk, k_end = min(K, k_begin + chunk);
+    float acc = 0.0f;
+    for (int k0 = k_begin; k0 < k_end; k0 += 16) {
+        int ka = k0 + tx, kb = k0 + ty;
+        sA[ty][tx] = (m < M && ka < k_end) ? A[(size_t)m * sam + (size_t)ka * sak] : 0.0f;
+        int nb = blockIdx.x * 16 + tx;
+        sB[ty][tx] = (kb < k_end && nb < N) ? B[(size_t)kb * sbk + (size_t)nb * sbn] : 0.0f;
+        __syncthreads();
+        for (int kk = 0; kk < 16; kk++) acc += sA[ty][kk] * sB[kk][tx];
+        __syncthreads();
+    }
+    if (m < M && nn < N) ws[((size_t)blockIdx.z * M + m) * N + nn] = acc;
+}
+
+// C = (accumulate ? C : 0) + the slices' partial sums, added in slice order
+__global__ void k_splitk_reduce(const float* __restrict__ ws, float* __restrict__ C, int MN, int slices,
+                                int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= MN) return;
+    float acc = accumulate ? C[i] : 0.0f;
+    for (int z = 0; z < slices; z++) acc += ws[(size_t)z * MN + i];
+    C[i] = acc;
+}
+
 // Z = Z + bias (row broadcast); H = SiLU(Z) (if H)
 __global__ void k_bias_act(float* __restrict__ Z, float* __restrict__ H, const float* __restrict__ bias, int M,
                            int N) {
@@ -284,8 +317,19 @@ void launch_gather_rows(hipStream_t s, const float* src, const uint32_t* perm, u
     if (rows) k_gather_rows<<<cdiv2((size_t)rows * cols, 256), 256, 0, s>>>(src, perm, offset, rows, cols, dst);
 }
 void launch_gemm(hipStream_t s, const float* A, int sam, int sak, const float* B, int sbk, int sbn, float* C, int M,
-                 int N, int K, int accumulate) {
+                 int N, int K, int accumulate, float* ws, uint32_t ws_floats) {
     dim3 grid(cdiv2(N, 16), cdiv2(M, 16));
+    // few output tiles and a long K: split K over up to kGemmSlices slices
+    int slices = (K + kGemmSliceK - 1) / kGemmSliceK;
+    if (slices > kGemmSlices) slices = kGemmSlices;
+    if (ws && grid.x * grid.y < 64 && slices > 1 && (size_t)slices * M * N <= ws_floats) {
+        const int chunk = ((K + slices - 1) / slices + 15) / 16 * 16;
+        slices = (K + chunk - 1) / chunk;
+        grid.z = slices;
+        k_gemm_splitk<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, ws, M, N, K, chunk);
+        k_splitk_reduce<<<cdiv2((size_t)M * N, 256), 256, 0, s>>>(ws, C, M * N, slices, accumulate);
+        return;
+    }
     k_gemm<<<grid, 256, 0, s>>>(A, sam, sak, B, sbk, sbn, C, M, N, K, accumulate);
 }
 void launch_bias_act(hipStream_t s, float* Z, float* H, const float* bias, int M, int N) {
