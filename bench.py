@@ -53,6 +53,43 @@ def measured_traffic(kernel: str):
     return None
 
 
+def hbm_copy_gbps(nbytes: int = 1 << 30, reps: int = 10):
+    """Device-to-device copy bandwidth through the HIP runtime (read + write
+    bytes per second): the attainable HBM rate next to the 8 TB/s vendor peak.
+    None if the runtime calls fail."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    P = C.c_void_p
+    a, b = P(), P()
+    e0, e1 = P(), P()
+    ms = C.c_float()
+    try:
+        if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+            return None
+        if hip.hipEventCreate(C.byref(e0)) or hip.hipEventCreate(C.byref(e1)):
+            return None
+        for k in range(reps + 2):
+            if k == 2:
+                hip.hipEventRecord(e0, None)
+            if hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None):   # hipMemcpyDeviceToDevice
+                return None
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        if hip.hipEventElapsedTime(C.byref(ms), e0, e1) or ms.value <= 0:
+            return None
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        for ptr in (a, b):
+            if ptr.value:
+                hip.hipFree(ptr)
+        for ev in (e0, e1):
+            if ev.value:
+                hip.hipEventDestroy(ev)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -358,13 +395,26 @@ def main() -> int:
                      "traffic": measured_traffic("k_walk") if headline and n == 16_777_216 else None,
                      "traffic_source": "profiles/k_walk_plummer16M_step_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "flop_per_interaction": FLOP_PER_INTERACTION,
-                     "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms},
+                     "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms,
+                     "interactions_per_s": c["interactions"] / (walk_ms * 1e-3)},
         "stage_ms": {k: v / args.steps for k, v in stage.items()},
         "counters": {"opened": int(c["opened"]), "interactions_per_body": c["interactions"] / max(num, 1),
                      "tree_height": int(c["tree_height"]), "max_stack": int(c["max_stack"]),
                      "error_flags": int(c["error_flags"]), "alive_bodies": num,
                      "merges_last_step": int(c["merges_applied"])},
     }
+    # the HBM-bound stages at their algorithmic bytes per body (one sorted
+    # pass over 24 B per key, 8 passes plus 8 histogram reads; the tree's 172
+    # B; the encode's 24 B; census + bounds 16 B) against the measured copy
+    # rate and the vendor peak
+    copy = hbm_copy_gbps()
+    sm = result["stage_ms"]
+    alg = {"sort": 256, "tree": 172, "morton": 24, "dead": 16}
+    result["hbm_stages"] = {
+        "copy_GBps": copy, "peak_GBps": HBM_PEAK_GBS,
+        **{k: {"ms": sm[k], "alg_bytes_per_body": b, "alg_GBps": b * num / (sm[k] * 1e-3) / 1e9,
+               "frac_of_peak": b * num / (sm[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+           for k, b in alg.items() if sm.get(k, 0.0) > 0.0}}
     if not args.no_error:
         log("force error against the FP64 direct sum")
         result["force_error"] = force_error(bh, n, soft)
