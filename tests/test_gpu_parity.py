@@ -280,6 +280,41 @@ def test_gpu_step_sizes_match_oracle(n):
     bh.close()
 
 
+@pytest.mark.parametrize("name", ["plummer_4096", "merge_4096"])
+def test_gpu_step_with_colors_equals_stage_calls(name):
+    """bh_step (positions sorted, bodies read through them, new index written
+    by the tree) and the stage entry points called one by one (ids sorted,
+    bodies gathered by id) give the same bodies, index and std430 nodes,
+    colours included, step after step."""
+    from gravity_amd import barnes_hut as B
+    fx = load(name)
+    print(f"[gpu] bh_step with colours vs stage calls: {name}", flush=True)
+    a, prm, _, _, _ = make_bh(fx, flags=B.FLAG_COLORS)
+    b, _, _, _, _ = make_bh(fx, flags=B.FLAG_COLORS)
+    a.init()
+    b.init()
+    for s in range(4):
+        a.step()
+        b.reset_values()
+        b.partition_dead_bodies()
+        if prm.static_or_dynamic == B.DYNAMIC:
+            b.update_bounds()
+        b.generate_morton_codes()
+        b.radix_sort()
+        b.build_tree()
+        b.compute_force()
+        if prm.merging_collision_or_neither == B.MERGING:
+            b.merge_bodies()
+        np.testing.assert_array_equal(a.nodes(B.LEAF_NODES), b.nodes(B.LEAF_NODES), f"{name} leaves step {s}")
+        np.testing.assert_array_equal(a.nodes(B.INTERNAL_NODES), b.nodes(B.INTERNAL_NODES),
+                                      f"{name} internal nodes step {s}")
+        b.swap_body_buffers()
+        assert_bits(a.bodies(), b.bodies(), f"{name} bodies after step {s}")
+        np.testing.assert_array_equal(a.index(), b.index())
+    a.close()
+    b.close()
+
+
 def test_gpu_two_bodies_merge_to_one():
     """Two overlapping bodies merge in the first step; from the next step on
     one body is left and the walk runs on the reference's one-body tree
