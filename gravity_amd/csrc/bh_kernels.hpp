@@ -14,7 +14,8 @@ constexpr uint32_t kSortTileOne = 4096; // radix sort tile of one simulation
 constexpr int kSortWavesOne = 8;        // waves per block of its histogram and scatter kernels
 constexpr uint32_t kSortBits = 8;
 constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
-constexpr int kWalkWaves = 4;          // waves per walk block
+constexpr int kWalkWaves = 16;         // waves per walk block (adjacent body groups share a CU's scalar cache)
+constexpr int kLearnedWaves = 4;       // waves per block of the corrected walk
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 
 // Tree in HBM. Leaves are sorted positions 0..n-1, internal nodes 0..n-2

@@ -322,7 +322,7 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
 template <uint32_t kMode>
-__global__ __launch_bounds__(256) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+__global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
@@ -545,7 +545,7 @@ int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu) {
 namespace bh {
 
 template <bool kLean>
-__global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+__global__ __launch_bounds__(kLearnedWaves * 64) void k_walk_learned(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                                       const float4* __restrict__ src, float4* __restrict__ dst,
                                                       const uint32_t* __restrict__ index_in,
                                                       const float4* __restrict__ leaf_pos,
@@ -563,8 +563,8 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
     const uint32_t num = sim->numBodies;
     const bool lean = !general && num > 1 && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num);
     if (lean != kLean) return;
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[kWalkWaves][64 * kMlpXStride];
-    __shared__ __attribute__((aligned(16))) float s_y[kWalkWaves][64 * 4];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kLearnedWaves][64 * kMlpXStride];
+    __shared__ __attribute__((aligned(16))) float s_y[kLearnedWaves][64 * 4];
 
     // the network's fragments, for every wave of the block
     __shared__ bf16x8 s_wf[kFrags * 64];
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
 
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (lb * kWalkWaves + w) * kWave;
+    const uint32_t base = (lb * kLearnedWaves + w) * kWave;
     if (base >= num) return;
     const uint32_t g = base + lane;
     const bool active = g < num;
@@ -689,8 +689,8 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
         inter = in32;
         corrected = co32;
     } else if (num > 1) {
-        __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
-        __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+        __shared__ uint32_t s_node[kLearnedWaves][kStackDepth];
+        __shared__ unsigned long long s_mask[kLearnedWaves][kStackDepth];
         uint32_t node = n;
         unsigned long long mask = ballot(active);
         int sp = 0;
@@ -809,12 +809,12 @@ __global__ __launch_bounds__(256) void k_walk_learned(SimValues* __restrict__ si
 
 void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
                          const uint2* node_range, const __bf16* w, const float* bias, float theta_plain) {
-    const uint32_t per_block = kWalkWaves * kWave;
+    const uint32_t per_block = kLearnedWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
-    k_walk_learned<true><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
+    k_walk_learned<true><<<nb, kLearnedWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
                                                             a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
                                                             bias, a.theta, theta_plain, a.dt, a.softening, a.general);
-    k_walk_learned<false><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
+    k_walk_learned<false><<<nb, kLearnedWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
                                                              a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
                                                              bias, a.theta, theta_plain, a.dt, a.softening, a.general);
 }
