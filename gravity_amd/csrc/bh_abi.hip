@@ -49,6 +49,7 @@ struct bh_context {
     uint32_t* index[2] = {nullptr, nullptr};  // 7 IndexIn = index[mi_in], 8 IndexOut
     SimValues* sim = nullptr;                 // 2
     uint2* tasks = nullptr;                   // 14 (pairs)
+    uint32_t* walk_queues = nullptr;          // heads of the persistent walk's per-CU queues
     int body_in = 0, mi_in = 0;
 
     TreeArrays tree{};
@@ -254,6 +255,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         }
         TRY(dalloc(c, &c->sim, 1));
         TRY(dalloc(c, &c->tasks, (size_t)n));
+        TRY(dalloc(c, &c->walk_queues, (size_t)kWalkQueues * 32));
         TRY(dalloc(c, &c->ctr, 1));
         TreeArrays& T = c->tree;
         TRY(dalloc(c, &T.leaf_pos, (size_t)n));
@@ -328,7 +330,7 @@ void bh_destroy(bh_context* c) {
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
-                    c->sort_pos[1]};
+                    c->sort_pos[1], c->walk_queues};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -595,6 +597,7 @@ int compute_force(bh_context* c, const bh_params* p, float4* step_pos, bool* wro
     a.leaf_radius = c->leaf_radius;
     a.general = (p->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
     a.pos_out = step_pos;
+    a.queues = c->walk_queues;
     if (p->flags & BH_FLAG_LEARNED) {
         if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
         if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)

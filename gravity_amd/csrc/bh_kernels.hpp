@@ -15,6 +15,11 @@ constexpr int kSortWavesOne = 8;        // waves per block of its histogram and 
 constexpr uint32_t kSortBits = 8;
 constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
 constexpr int kWalkWaves = 16;         // waves per walk block (adjacent body groups share a CU's scalar cache)
+// persistent walk: one queue of 64-body groups per CU slot (8 XCDs x 4 shader
+// engines x 9 CU ids; 32 of the 36 slots of an XCD hold a CU), each head in
+// its own 128-byte line
+constexpr uint32_t kWalkSlots = 36, kWalkQueues = 8 * kWalkSlots;
+constexpr bool kWalkPersistent = true;
 constexpr int kLearnedWaves = 4;       // waves per block of the corrected walk
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 
@@ -66,6 +71,7 @@ struct WalkArgs {
     float* leaf_radius;              // [n] scratch of the merging and collision modes
     uint32_t general;                // 1: always the general walk loop
     float4* pos_out = nullptr;       // [n] new position-mass by sorted position as well, or nullptr
+    uint32_t* queues = nullptr;      // [kWalkQueues * 32] per-CU work queues (persistent walk), or nullptr
 };
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.

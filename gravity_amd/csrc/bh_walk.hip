@@ -318,28 +318,27 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
     o.err = err;
 }
 
-// Pointers are separate __restrict__ parameters so that the compiler can
-// prove the tree is not written during the kernel and fetch node records with
-// scalar loads.
-template <uint32_t kMode>
-__global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                              const float4* __restrict__ src, float4* __restrict__ dst,
-                                              const uint32_t* __restrict__ index_in,
-                                              const float4* __restrict__ leaf_pos,
-                                              const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
-                                              uint32_t task_cap, float theta_, float dt_, float softening,
-                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
-                                              const float* __restrict__ leaf_radius, uint32_t general,
-                                              float4* __restrict__ pos_out) {
-    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
-    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+// Totals a wave accumulates over its 64-body groups, added to the counters once.
+struct WalkTotals {
+    unsigned long long opened, inter;
+    uint32_t maxsp, err, oob;
+};
 
+// Walk and integrate the 64 sorted bodies [base, base + 64) of one wave.
+template <uint32_t kMode>
+__device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsigned long long* s_mask,
+                                           SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                           const float4* __restrict__ src, float4* __restrict__ dst,
+                                           const uint32_t* __restrict__ index_in,
+                                           const float4* __restrict__ leaf_pos,
+                                           const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                           uint32_t task_cap, float theta_, float dt_, float softening,
+                                           uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
+                                           const float* __restrict__ leaf_radius, uint32_t general,
+                                           float4* __restrict__ pos_out, WalkTotals& tot) {
     const uint32_t n = sim->initialNumBodies;
     const uint32_t num = sim->numBodies;
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (lb * kWalkWaves + w) * kWave;
-    if (base >= num) return;
+    const uint32_t lane = lane_id();
     const uint32_t g = base + lane;
     const bool active = g < num;
     const uint32_t bodyIdx = active ? index_in[g] : 0;
@@ -354,17 +353,15 @@ __global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict_
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                              elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
         else if (inv_sqrt_fast_domain(nodes[0].half, softening))
-            walk_chunk<kStackDepth, kMode, true>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd,
+            walk_chunk<kStackDepth, kMode, true>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
                                                  leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                                                  elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
         else
-            walk_chunk<kStackDepth, kMode, false>(s_node[w], s_mask[w], n, n, base, g, ballot(active), pm, vd,
+            walk_chunk<kStackDepth, kMode, false>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
                                                   leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                                                   elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
     }
     const float ax = o.ax, ay = o.ay, az = o.az;
-    const unsigned long long opened = o.opened, inter = o.inter;
-    const uint32_t maxsp = o.maxsp, err = o.err;
 
     // kick-drift into the output buffer (bh_force.comp:118-140)
     uint32_t oob = 0;
@@ -396,18 +393,131 @@ __global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict_
         if (pos_out) pos_out[g] = np;   // the same value by sorted position
         oob = kill ? 1u : 0u;
     }
-    // one set of atomics per wave
-    const uint32_t oob_w = __popcll(ballot(oob != 0));
-    const uint32_t err_w = (ballot(err & 1u) ? 1u : 0u) | (ballot(err & 2u) ? 2u : 0u);
-    if (lane == 0) {
-        atomicAdd(&ctr->opened, opened);
-        atomicAdd(&ctr->interactions, inter);
-        atomicMax(&ctr->maxStack, maxsp);
+    tot.opened += o.opened;
+    tot.inter += o.inter;
+    tot.maxsp = o.maxsp > tot.maxsp ? o.maxsp : tot.maxsp;
+    tot.err |= o.err;
+    tot.oob += __popcll(ballot(oob != 0));
+}
+
+// one set of atomics per wave
+__device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* ctr, const WalkTotals& t) {
+    const uint32_t oob_w = t.oob;   // wave-uniform
+    const uint32_t err_w = (ballot(t.err & 1u) ? 1u : 0u) | (ballot(t.err & 2u) ? 2u : 0u);
+    if (lane_id() == 0) {
+        atomicAdd(&ctr->opened, t.opened);
+        atomicAdd(&ctr->interactions, t.inter);
+        atomicMax(&ctr->maxStack, t.maxsp);
         if (oob_w) atomicAdd(&sim->outOfBounds, oob_w);
         if (err_w) atomicOr(&ctr->errorFlags, err_w);
         // sim.uintDebug[0] counts opened nodes as a 32-bit word (bh_force.comp:72)
-        atomicAdd(&sim->uintDebug[0], (uint32_t)opened);
+        atomicAdd(&sim->uintDebug[0], (uint32_t)t.opened);
     }
+}
+
+// Pointers are separate __restrict__ parameters so that the compiler can
+// prove the tree is not written during the kernel and fetch node records with
+// scalar loads.
+template <uint32_t kMode>
+__global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                              const float4* __restrict__ src, float4* __restrict__ dst,
+                                              const uint32_t* __restrict__ index_in,
+                                              const float4* __restrict__ leaf_pos,
+                                              const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                              uint32_t task_cap, float theta_, float dt_, float softening,
+                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
+                                              const float* __restrict__ leaf_radius, uint32_t general,
+                                              float4* __restrict__ pos_out) {
+    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
+    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = (lb * kWalkWaves + w) * kWave;
+    if (base >= sim->numBodies) return;
+    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u};
+    walk_group<kMode>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, tasks, task_cap,
+                      theta_, dt_, softening, dynamic, wrap, elasticity, restitution, leaf_radius, general, pos_out, t);
+    walk_totals_flush(sim, ctr, t);
+}
+
+// The CU slot (XCD x 36 + shader engine x 9 + CU id) the calling wave runs on,
+// read from the HW_ID and XCC_ID hardware registers.
+__device__ __forceinline__ uint32_t walk_home_queue() {
+    const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);          // HW_ID, 32 bits
+    const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7u;   // XCC_ID
+    const uint32_t f = (hw >> 8) & 0xFFu;   // CU_ID [3:0], SH_ID [4], SE_ID [7:5]
+    const uint32_t slot = (((f >> 5) & 7u) * 9u + (f & 15u)) % kWalkSlots;
+    return xcc * kWalkSlots + slot;
+}
+
+// 64-body groups [begin, end) of queue q: the sorted bodies split into
+// kWalkQueues consecutive ranges, XCD x owning queues 36x .. 36x + 35.
+__device__ __forceinline__ uint32_t walk_queue_begin(uint32_t groups, uint32_t q) {
+    return (uint32_t)(((unsigned long long)groups * q) / kWalkQueues);
+}
+
+__global__ void k_walk_queue_reset(uint32_t* __restrict__ queues) {
+    if (threadIdx.x < kWalkQueues) queues[threadIdx.x * 32] = 0;
+}
+
+// Persistent walk: each wave takes the next 64-body group of its own CU's
+// queue, so the waves resident on one CU walk neighbouring bodies and share
+// the records in that CU's scalar cache; a wave whose queue is drained moves
+// to the nearest queue (by slot, then XCD) that still holds groups. A queue
+// that a wave found drained (by its atomic or by a load) is never tried again
+// by that wave, so each wave ends after at most kWalkQueues failed takes.
+template <uint32_t kMode>
+__global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk_q(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                              const float4* __restrict__ src, float4* __restrict__ dst,
+                                              const uint32_t* __restrict__ index_in,
+                                              const float4* __restrict__ leaf_pos,
+                                              const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                              uint32_t task_cap, float theta_, float dt_, float softening,
+                                              uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
+                                              const float* __restrict__ leaf_radius, uint32_t general,
+                                              float4* __restrict__ pos_out, uint32_t* __restrict__ queues) {
+    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
+    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t groups = (sim->numBodies + kWave - 1) / kWave;
+    const uint32_t home = uniform(walk_home_queue());
+    uint32_t q = home;
+    uint32_t drained = 0;   // bit k: queue lane + 64 k is known to be drained
+    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u};
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&queues[q * 32], 1u);
+        k = uniform(__shfl(k, 0, 64));
+        const uint32_t qb = walk_queue_begin(groups, q);
+        if (k < walk_queue_begin(groups, q + 1) - qb) {
+            walk_group<kMode>((qb + k) * kWave, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes,
+                              tasks, task_cap, theta_, dt_, softening, dynamic, wrap, elasticity, restitution,
+                              leaf_radius, general, pos_out, t);
+            continue;
+        }
+        if (lane == (q & 63u)) drained |= 1u << (q >> 6);
+        // the nearest queue after home that may still hold groups
+        uint32_t best = 0xFFFFFFFFu;
+        for (uint32_t j = 0; j < (kWalkQueues + 63) / 64; j++) {
+            const uint32_t qq = lane + 64u * j;
+            if (qq >= kWalkQueues || ((drained >> j) & 1u)) continue;
+            const uint32_t h = __hip_atomic_load(&queues[qq * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (h >= walk_queue_begin(groups, qq + 1) - walk_queue_begin(groups, qq)) {
+                drained |= 1u << j;
+                continue;
+            }
+            const uint32_t d = (qq + kWalkQueues - home) % kWalkQueues;
+            best = d < best ? d : best;
+        }
+        for (uint32_t o = 32; o > 0; o >>= 1) {
+            const uint32_t v = __shfl_xor(best, o, 64);
+            best = v < best ? v : best;
+        }
+        best = uniform(best);
+        if (best == 0xFFFFFFFFu) break;
+        q = (home + best) % kWalkQueues;
+    }
+    walk_totals_flush(sim, ctr, t);
 }
 
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
@@ -420,7 +530,20 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
                                                 a.leaf_radius, a.general, a.pos_out)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
-    if (a.merging == 1u) BH_WALK(1u);
+#define BH_WALK_Q(M)                                                                                             \
+    k_walk_q<M><<<nq, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+                                                  a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
+                                                  a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
+                                                  a.leaf_radius, a.general, a.pos_out, a.queues)
+    if (kWalkPersistent && a.queues) {
+        // two blocks of 16 waves fill a CU (8 waves per SIMD); blocks beyond
+        // what is resident find the queues drained and end
+        const uint32_t nq = std::min(nb, 2u * 256u);
+        k_walk_queue_reset<<<1, kWalkQueues, 0, s>>>(a.queues);
+        if (a.merging == 1u) BH_WALK_Q(1u);
+        else if (a.merging == 2u) BH_WALK_Q(2u);
+        else BH_WALK_Q(0u);
+    } else if (a.merging == 1u) BH_WALK(1u);
     else if (a.merging == 2u) BH_WALK(2u);
     else BH_WALK(0u);
 #undef BH_WALK
