@@ -390,9 +390,9 @@ def main() -> int:
         "data": "synthetic",
         "config": dict(workload, n_bodies=n, theta=theta, dt=dt, softening=soft, bounds="dynamic",
                        method="plain Barnes-Hut", mac="half longest AABB side x inverse softened distance < theta"),
-        "roofline": {"bound": "valu", "kernel": "k_walk", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "kernel": "k_walk_q", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": measured_traffic("k_walk") if headline and n == 16_777_216 else None,
+                     "traffic": measured_traffic("k_walk_q") if headline and n == 16_777_216 else None,
                      "traffic_source": "profiles/k_walk_plummer16M_step_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms,

@@ -20,6 +20,12 @@ constexpr int kWalkWaves = 16;         // waves per walk block (adjacent body gr
 // its own 128-byte line
 constexpr uint32_t kWalkSlots = 36, kWalkQueues = 8 * kWalkSlots;
 constexpr bool kWalkPersistent = true;
+// below this many bodies a queue holds too few groups to repay its claims:
+// the walk runs as one block of kWalkSmallWaves waves per 4 groups instead
+// (MI355X, Plummer: that is faster at 524,288 bodies, slower at 1,048,576),
+// small blocks spreading a small walk over all CUs
+constexpr uint32_t kWalkPersistentMin = 1u << 20;
+constexpr int kWalkSmallWaves = 4;
 constexpr int kLearnedWaves = 4;       // waves per block of the corrected walk
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 

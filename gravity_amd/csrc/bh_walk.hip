@@ -418,8 +418,8 @@ __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* c
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
-template <uint32_t kMode>
-__global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+template <uint32_t kMode, int kW>
+__global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
@@ -428,11 +428,11 @@ __global__ __launch_bounds__(kWalkWaves * 64) void k_walk(SimValues* __restrict_
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                               const float* __restrict__ leaf_radius, uint32_t general,
                                               float4* __restrict__ pos_out) {
-    __shared__ uint32_t s_node[kWalkWaves][kStackDepth];
-    __shared__ unsigned long long s_mask[kWalkWaves][kStackDepth];
+    __shared__ uint32_t s_node[kW][kStackDepth];
+    __shared__ unsigned long long s_mask[kW][kStackDepth];
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (lb * kWalkWaves + w) * kWave;
+    const uint32_t base = (lb * kW + w) * kWave;
     if (base >= sim->numBodies) return;
     WalkTotals t = {0ull, 0ull, 1u, 0u, 0u};
     walk_group<kMode>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, tasks, task_cap,
@@ -523,8 +523,9 @@ __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
+    const uint32_t nb_small = (n + kWalkSmallWaves * kWave - 1) / (kWalkSmallWaves * kWave);
 #define BH_WALK(M)                                                                                               \
-    k_walk<M><<<nb, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+    k_walk<M, kWalkSmallWaves><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                 a.leaf_radius, a.general, a.pos_out)
@@ -535,7 +536,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
                                                   a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
                                                   a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                   a.leaf_radius, a.general, a.pos_out, a.queues)
-    if (kWalkPersistent && a.queues) {
+    if (kWalkPersistent && a.queues && n >= kWalkPersistentMin) {
         // two blocks of 16 waves fill a CU (8 waves per SIMD); blocks beyond
         // what is resident find the queues drained and end
         const uint32_t nq = std::min(nb, 2u * 256u);
