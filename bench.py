@@ -105,7 +105,7 @@ def force_error(bh, n: int, softening: float, n_sample: int = 1024, seed: int = 
     ref = bh.direct_sum_f64(sample, softening)
     err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
     return {"p50": float(np.percentile(err, 50)), "p99": float(np.percentile(err, 99)),
-            "max": float(err.max()), "sample": int(sample.size), "judge": "fp64 direct sum (HIP kernel)"}
+            "max": float(err.max()), "sample": int(sample.size), "exact_force": "fp64 direct sum (HIP kernel)"}
 
 
 def variant_time(bh, prm, steps: int = 3) -> float:
@@ -320,7 +320,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a multi-GPU split of one simulation is design-only, see DESIGN.md)", file=sys.stderr)
+              "(a split of one simulation over several GPUs is designed but not built)", file=sys.stderr)
         return 2
 
     from gravity_amd import barnes_hut as B

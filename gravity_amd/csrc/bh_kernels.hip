@@ -1513,7 +1513,7 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
 
 namespace bh {
 
-// ------------------------------------------------------------------ FP64 judge
+// ------------------------------------------------------------ FP64 direct sum
 // a_i = sum_{j != i, m_j != 0} G m_j r_ij (|r_ij|^2 + softening)^-3/2 in FP64 for
 // sampled bodies i; one block per sampled body, fixed-order reduction.
 __global__ __launch_bounds__(256) void k_direct_sum_f64(const float4* __restrict__ bodies, uint32_t n,

@@ -168,8 +168,8 @@ int bh_get_stage_times(bh_context *ctx, float *ms, int n);
 /* FP64 direct summation over the bodies of BodiesIn for n_sample sampled
  * body ids: accel_out[3k..3k+2] = sum_{j != i, m_j != 0} G m_j r_ij
  * (|r_ij|^2 + softening)^-3/2 with i = sample[k], in double precision, G and
- * softening taken from sim and from `softening`. This is the judge of the
- * walk's force error (no reference counterpart; the reference publishes no
+ * softening taken from sim and from `softening`. This exact force is what the
+ * walk's force error is measured against (no reference counterpart; the reference publishes no
  * error figure). Host arrays in and out; waits for the stream. */
 int bh_direct_sum_f64(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
                       double *accel_out);

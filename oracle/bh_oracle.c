@@ -686,7 +686,7 @@ int bho_step(bho_state *s, const bho_params *p) {
     return rc;
 }
 
-/* ------------------------------------------------------------ FP64 judge */
+/* -------------------------------------------------------- FP64 direct sum */
 void bho_direct_sum_f64(const float *bodies, uint32_t n, const uint32_t *sample, uint32_t n_sample,
                         double G, double soft, double *out, int threads) {
 #ifdef _OPENMP
