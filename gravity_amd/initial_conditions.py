@@ -67,7 +67,7 @@ def plummer_pos_vel(n: int, rng: np.random.Generator, a: float = 1.0, total_mass
     """Plummer sphere by Aarseth, Henon & Wielen (1974), float64.
 
     Radii follow the cumulative mass ``X = r^3/(r^2+a^2)^(3/2)`` with X drawn
-    uniformly in (0, mass_cut] (the radial cutoff: 99.9% of the mass, r < 22.8a).
+    uniformly in (0, mass_cut] (the radial cutoff: 99.9% of the mass, r < 38.7a).
     Speeds are ``q * v_esc`` with q drawn by rejection from
     ``g(q) = q^2 (1-q^2)^(7/2)``, ``v_esc = sqrt(2GM) (r^2+a^2)^(-1/4)``.
     The centre of mass and its velocity are moved to the origin.
@@ -159,27 +159,35 @@ def two_galaxy(n_per_galaxy: int, seed: int, separation: float = 10.0, bulge_fra
     """Two exponential disks (R_d=1, z0=0.1, v_c=1) with Hernquist bulges (a=0.2).
 
     Each galaxy has mass 1 (disk 0.8, bulge 0.2 by body count and equal
-    masses), the pair starts ``separation`` apart on a parabolic orbit
-    (relative speed sqrt(2 G M_total / d)). Total 2*n_per_galaxy bodies.
+    masses). The centres start ``separation`` apart with an impact parameter
+    of 2 (the offset across the initial relative velocity), on a parabolic
+    orbit: relative speed sqrt(2 G M_total / separation), zero orbital
+    energy. Total 2*n_per_galaxy bodies.
     """
     rng = np.random.default_rng(seed)
     n_bulge = int(round(n_per_galaxy * bulge_fraction))
     n_disk = n_per_galaxy - n_bulge
     parts = []
     v_rel = math.sqrt(2.0 * 2.0 / separation)
+    impact = 2.0
+    along = math.sqrt(separation * separation - impact * impact)
     for k, sign in enumerate((-1.0, 1.0)):
         dp, dv = _exp_disk(n_disk, rng, 1.0, 0.1, 1.0)
         bp, bv = _hernquist(n_bulge, rng, 0.2, bulge_fraction)
         pos = np.concatenate([dp, bp])
         vel = np.concatenate([dv, bv])
+        # the galaxy's own centre of mass and mean motion (equal masses) at
+        # rest at its origin, so that the pair's orbit is the stated one
+        pos -= pos.mean(axis=0)
+        vel -= vel.mean(axis=0)
         if k == 1:
             # tilt the second disk by 30 degrees about x
             c, s = math.cos(math.pi / 6), math.sin(math.pi / 6)
             rot = np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
             pos = pos @ rot.T
             vel = vel @ rot.T
-        pos[:, 0] += sign * separation / 2.0
-        pos[:, 1] += sign * 1.0
+        pos[:, 0] += sign * along / 2.0
+        pos[:, 1] += sign * impact / 2.0
         vel[:, 1] += -sign * v_rel / 2.0
         parts.append((pos, vel))
     pos = np.concatenate([p for p, _ in parts])

@@ -45,8 +45,8 @@ MAGIC = 0x4E494842
 
 
 def write_input(path, bodies, steps, theta, dt, soft, merging, dynamic, wrap, flags, dump_from, bounds,
-                elasticity=0.5):
-    u = ic.g_one_units()
+                elasticity=0.5, units=None):
+    u = ic.g_one_units() if units is None else units
     hdr = struct.pack("<8I9f6f", MAGIC, bodies.shape[0], steps, merging, dynamic, wrap, flags, dump_from,
                       theta, dt, soft, elasticity, u.mass, u.density, u.len, u.time, 1.0,
                       *[float(x) for x in bounds])
@@ -149,7 +149,48 @@ def scenario_list():
     # theta = 0 opens every internal node: the walk is a direct sum in walk order
     sc.append(dict(name="theta0_512", bodies=ic.plummer(512, 4), steps=1, theta=0.0, dt=1e-3, soft=0.01,
                    merging=0, dynamic=1, wrap=0))
+    # the two-galaxy workload (disks with bulges, 10 apart, parabolic orbit)
+    sc.append(dict(name="two_galaxy_4096", bodies=ic.two_galaxy(2048, 23), steps=2, theta=0.5, dt=1e-3,
+                   soft=0.01, merging=0, dynamic=1, wrap=0))
     return sc
+
+
+def unit_sets():
+    """(name, Units) pairs whose G and body-length constant the reference's
+    UPDATE program derives in-shader (common/common.glsl:345-352): the G = 1
+    units, the reference's named unit sets (simulation/UnitSet.java) and a few
+    arbitrary ones, including ones whose G is an FP32 subnormal."""
+    from gravity_amd import planets as pl
+    out = [("g_one", ic.g_one_units())]
+    for us in (pl.SOLAR_SYSTEM_SECOND, pl.SOLAR_SYSTEM_HOUR, pl.ASTRONOMICAL, pl.METRIC, pl.GALACTIC_MERGE):
+        m, d, ln, t = us.uniforms()
+        out.append((us.name, ic.Units(mass=m, density=d, len=ln, time=t)))
+    for k, (m, d, ln, t) in enumerate([(1e10, 1e3, 1e3, 1e5), (3.0, 0.25, 0.5, 2.0), (1e20, 5e2, 1e6, 10.0),
+                                       (7.0e29, 2.0e3, 3.0e10, 60.0), (1.0, 1.0, 1.0, 1.0)]):
+        out.append((f"custom_{k}", ic.Units(mass=m, density=d, len=ln, time=t)))
+    return out
+
+
+def units_fixture(runner, shaders):
+    """tests/golden/units.npz: per unit set the uniforms and the G and
+    body-length constant bits the reference's UPDATE program leaves in
+    SimulationValues (one step of two bodies per unit set)."""
+    names, uni, G, blc = [], [], [], []
+    bodies = ic.plummer(2, 1)
+    for name, u in unit_sets():
+        with tempfile.TemporaryDirectory() as td:
+            inp = os.path.join(td, "in.bin")
+            write_input(inp, bodies, 1, 0.5, 1e-3, 0.01, 0, 1, 0, 1, 0, ic.tight_bounds(bodies), units=u)
+            run(runner, shaders, inp, td)
+            sim = ld(td, "s0_dead_sim.bin").view(np.float32)
+            names.append(name)
+            uni.append([u.mass, u.density, u.len, u.time])
+            G.append(sim[21])
+            blc.append(sim[22])
+    path = os.path.join(HERE, "units.npz")
+    np.savez_compressed(path, names=np.array(names), uniforms=np.array(uni, np.float32),
+                        G=np.array(G, np.float32), blc=np.array(blc, np.float32))
+    print(f"units: {len(names)} unit sets -> {path}")
 
 
 def main():
@@ -189,6 +230,8 @@ def main():
         np.savez_compressed(path, **arrays)
         print(f"{sc['name']}: N={n} steps={sc['steps']} -> {os.path.getsize(path) // 1024} KiB  ({info['total_ms']:.0f} ms)")
 
+    if args.only in (None, "units"):
+        units_fixture(runner, shaders)
     if args.only in (None, "plummer_65536") and not args.skip_config1:
         config1(runner, shaders)
 

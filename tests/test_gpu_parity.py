@@ -162,6 +162,7 @@ def test_gpu_stage_parity(name, s):
 
 
 @pytest.mark.parametrize("name", ["plummer_4096", "collision_4096", "edge_dead_nan_1000", "edge_duplicates_512",
+                                  "two_galaxy_4096",
                                   "static_kill_1024", "static_wrap_1024", "theta0_512"])
 def test_gpu_free_running_steps(name):
     """bh_step from the initial bodies reproduces the reference trajectory bit for bit."""

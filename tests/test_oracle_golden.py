@@ -130,6 +130,7 @@ def test_stage_parity(name, s):
 
 
 @pytest.mark.parametrize("name", ["plummer_4096", "collision_4096", "edge_dead_nan_1000", "edge_duplicates_512",
+                                  "two_galaxy_4096",
                                   "static_kill_1024", "static_wrap_1024", "theta0_512"])
 def test_free_running_trajectory(name):
     """Without merges the whole multi-step trajectory is deterministic: the
