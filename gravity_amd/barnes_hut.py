@@ -29,6 +29,7 @@ NEITHER, MERGING, COLLISION = 0, 1, 2
 STATIC, DYNAMIC = 0, 1
 FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK = 1, 2, 4, 8
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
+NOALLOC_MAX_SAMPLE = 65536   # BH_NOALLOC_MAX_SAMPLE
 
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "unsupported", -5: "too large",
           -6: "walk stack overflow", -7: "merge task overflow"}
@@ -86,6 +87,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.bh_get_counters.argtypes = [ctx, C.POINTER(Counters)]
     L.bh_get_stage_times.argtypes = [ctx, C.POINTER(C.c_float), C.c_int]
     L.bh_direct_sum_f64.argtypes = [ctx, P, C.c_uint32, C.c_double, P]
+    L.bh_direct_sum_f64_noalloc.argtypes = [ctx, P, C.c_uint32, C.c_double, P]
     L.bh_selftest_math.argtypes = [ctx, P]
     _lib = L
     return L
@@ -196,6 +198,15 @@ class BarnesHut:
         out = np.zeros((s.shape[0], 3), np.float64)
         self._check(self.L.bh_direct_sum_f64(self.ctx, s.ctypes.data_as(C.c_void_p), s.shape[0], softening,
                                              out.ctypes.data_as(C.c_void_p)), "direct_sum_f64")
+        return out
+
+    def direct_sum_f64_noalloc(self, sample: np.ndarray, softening: float) -> np.ndarray:
+        """direct_sum_f64 through the entry point that allocates no device memory
+        (at most NOALLOC_MAX_SAMPLE samples per call)."""
+        s = np.ascontiguousarray(sample, dtype=np.uint32)
+        out = np.zeros((s.shape[0], 3), np.float64)
+        self._check(self.L.bh_direct_sum_f64_noalloc(self.ctx, s.ctypes.data_as(C.c_void_p), s.shape[0], softening,
+                                                     out.ctypes.data_as(C.c_void_p)), "direct_sum_f64_noalloc")
         return out
 
     def selftest_math(self) -> list:

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <utility>
 
@@ -204,6 +205,42 @@ uint32_t mlp_backward_fp32(bh_mlp* m, uint32_t rows) {
     return nparts;
 }
 
+// Static device scratch of bh_direct_sum_f64_noalloc: the sampled ids
+// (BH_NOALLOC_MAX_SAMPLE x 4 bytes) followed by the FP64 sums (x 24 bytes),
+// 1.75 MiB. Every GPU holds its own copy at its own address. In host code the
+// name `scratch` is only a stand-in for the symbol: kernels, copies and
+// memsets get the device address that hipGetSymbolAddress returned on that
+// device (scratch_ptr), never the name itself.
+constexpr size_t kNoAllocIdsBytes = (size_t)BH_NOALLOC_MAX_SAMPLE * 4;
+constexpr size_t kNoAllocBytes = kNoAllocIdsBytes + (size_t)BH_NOALLOC_MAX_SAMPLE * 24;
+static_assert(kNoAllocIdsBytes % 256 == 0, "the sums start 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[kMaxDevices] = {};   // device address of `scratch` per device, set once
+
+// With `device` current: resolve that device's address of the scratch array
+// (once per device, at context creation).
+int scratch_setup(bh_context* c, int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(c, BH_ERR_INVALID_ARGUMENT, "device %d outside 0..%d", device, kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return BH_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(c, BH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    scratch_ptr[device] = p;
+    return BH_OK;
+}
+
+unsigned char* scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return static_cast<unsigned char*>(scratch_ptr[device]);
+}
+
 // stage events are created on the first timed step
 int ensure_events(bh_context* c) {
     if (c->ev) return BH_OK;
@@ -245,6 +282,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
     {
         hipError_t e = hipSetDevice(device);
         if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e)); goto bad; }
+        TRY(scratch_setup(c, device));
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) { rc = fail(c, BH_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); goto bad; }
         for (int k = 0; k < 2; k++) {
@@ -950,6 +988,35 @@ int bh_direct_sum_f64(bh_context* c, const uint32_t* sample, uint32_t n_sample, 
     (void)hipFree(d_sample);
     if (d_out) (void)hipFree(d_out);
     return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "direct sum: %s", hipGetErrorString(e));
+}
+
+int bh_direct_sum_f64_noalloc(bh_context* c, const uint32_t* sample, uint32_t n_sample, double softening,
+                              double* out) {
+    if (!c || (!sample && n_sample) || (!out && n_sample)) return BH_ERR_INVALID_ARGUMENT;
+    if (n_sample > BH_NOALLOC_MAX_SAMPLE)
+        return fail(c, BH_ERR_TOO_LARGE, "n_sample %u > %u, the static scratch of the no-allocation direct sum",
+                    n_sample, BH_NOALLOC_MAX_SAMPLE);
+    if (n_sample == 0) return BH_OK;
+    HIPCHECK(c, hipSetDevice(c->device));
+    for (uint32_t k = 0; k < n_sample; k++)
+        if (sample[k] >= c->n) return fail(c, BH_ERR_INVALID_ARGUMENT, "sample id %u >= n", sample[k]);
+    unsigned char* base = scratch_of(c->device);
+    if (!base) return fail(c, BH_ERR_HIP, "no scratch address for device %d", c->device);
+    uint32_t* d_sample = reinterpret_cast<uint32_t*>(base);
+    double* d_out = reinterpret_cast<double*>(base + kNoAllocIdsBytes);
+    SimValues h;
+    HIPCHECK(c, hipMemcpyAsync(&h, c->sim, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(d_sample, sample, 4 * (size_t)n_sample, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    // every output word is written by the kernel: nothing in the shared
+    // array has to start at zero
+    launch_direct_sum_f64(c->stream, c->bodies[c->body_in], c->n, d_sample, n_sample,
+                          (double)h.gravitationalConstant, softening, d_out);
+    int rc = check_launch(c, "direct sum (no allocation)");
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpyAsync(out, d_out, 24 * (size_t)n_sample, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return BH_OK;
 }
 
 }  // extern "C"

@@ -174,6 +174,21 @@ int bh_get_stage_times(bh_context *ctx, float *ms, int n);
 int bh_direct_sum_f64(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
                       double *accel_out);
 
+/* bh_direct_sum_f64 without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving or monitoring loops that forbid hipMalloc).
+ * Same kernel, same arguments, same results bit for bit; the sampled ids and
+ * the FP64 sums are staged in one static device array of the library
+ * (BH_NOALLOC_MAX_SAMPLE samples, 28 bytes each, held by every GPU the library
+ * runs on until the process ends) instead of per-call hipMalloc buffers.
+ * n_sample > BH_NOALLOC_MAX_SAMPLE returns BH_ERR_TOO_LARGE. Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other contexts of the same device) race and must be serialised by the
+ * caller. Waits for the stream. */
+#define BH_NOALLOC_MAX_SAMPLE 65536u
+int bh_direct_sum_f64_noalloc(bh_context *ctx, const uint32_t *sample, uint32_t n_sample, double softening,
+                              double *accel_out);
+
 /* ---- learned far-field correction (no reference counterpart) ----
  * A 12-64-64-3 SiLU network predicts, for a (target, node) pair accepted by
  * the wide opening angle, the residual between the node's exact field and its
