@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
-"""Body-steps/s of the MI355X Barnes-Hut step on a 16M-body Plummer sphere.
+"""Body-steps/s of the MI355X Barnes-Hut step at 16M bodies, plain against
+learned-corrected at equal p99 force error.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--theta T]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 
-Workload: Plummer sphere, N = 16,777,216 bodies, equal masses 1/N, G = 1,
-scale radius 1, softening 0.01, theta 0.5 (the reference's half-longest-side
-opening criterion), dt 1e-3, merging off, bounds recomputed every step; the
-plain method (no learned correction). Bodies are generated on the host by
-gravity_amd.initial_conditions.plummer and uploaded once; the timed region
-is K calls of bh_step on resident device buffers, bracketed by stream
-synchronisation.
+Headline workload: Plummer sphere, N = 16,777,216, equal masses 1/N, G = 1,
+scale radius 1, softening 0.01, dt 1e-3, merging off, bounds recomputed every
+step. Force error: |a - a_fp64| / |a_fp64| over 16,384 sampled bodies against
+an FP64 direct sum (the GPU's FP64 kernel), p50 and p99.
 
-Printed on one JSON line: value = alive bodies x K / wall time of the K
-steps; the roofline entry for the dominant kernel (the walk, VALU-bound:
-20 FLOP per body-node interaction / its mean duration from HIP events on the
-library's stream); p50/p99 force error of the walk against an FP64 direct
-sum for 1,024 sampled bodies; and the CPU baseline: the oracle's C port of
-the same step timed on this host over a bounded sample (full-size O(N)
-stages, the walk for 8 evenly spaced runs of sorted bodies, scaled to N).
+1. The reference's configuration, the plain step at theta 0.5, is timed.
+2. The correction's network is trained on the GPU on (body, node) pairs with
+   FP64 labels from a 1M-body Plummer and a 1M-body two-galaxy snapshot.
+3. On one frozen state, sweeps of the opening angle: plain, the exact
+   quadrupole (no network) and learned-corrected (quadrupole + the network's
+   residual), each with its p50/p99 and its step time; the sweeps stop at the
+   metric's error level, p99 <= 1e-3.
+4. The timed loop: K steps of the learned-corrected method at the widest
+   angle whose p99 is <= 1e-3, bracketed by stream synchronisation, inputs
+   resident in HBM. value = alive bodies x K / wall time of the K steps.
+   equal_p99 compares it with the fastest plain angle at the same level.
+5. The other workloads (the reference's CPU-runnable Plummer 65,536 x 10
+   steps, cube + core merging 4,194,304 x 20 steps, two galaxies 33,554,432 x
+   5 steps), the 256-simulation ensemble, and the CPU baseline (the oracle's C
+   port of the plain step on this host over a bounded sample).
+
+The roofline entry is the dominant kernel of the timed step, the corrected
+walk (VALU-bound: 20 FLOP per body-node interaction plus 40 per quadrupole
+term, over its mean duration from HIP events on the library's stream).
 """
 from __future__ import annotations
 
@@ -36,12 +46,15 @@ METRIC = "body-steps/sec @16M bodies at equal p99 force err; plain vs learned-co
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
+FLOP_PER_QUAD_TERM = 40        # the quadrupole term of an accepted internal node (add_quad)
+P99_TARGET = 1e-3
+N_SAMPLE = 16384
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+def measured_traffic(kernel: str, profile: str):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary
     (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, calibration in the file), or None."""
-    path = os.path.join(ROOT, "profiles", "k_walk_plummer16M_step_pmc.md")
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         with open(path) as fh:
             for line in fh:
@@ -94,28 +107,38 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def force_error(bh, n: int, softening: float, n_sample: int = 1024, seed: int = 7) -> dict:
-    """p50/p99 of |a - a_fp64| / |a_fp64| over sampled bodies."""
-    from gravity_amd import barnes_hut as B
+def sample_ids(bodies_or_n, n_sample: int = N_SAMPLE, seed: int = 7) -> np.ndarray:
+    """Sorted random ids of alive bodies."""
     rng = np.random.default_rng(seed)
-    bodies = bh.bodies(B.BODIES_IN)
-    alive = np.nonzero(bodies[:, 3] != 0)[0]
-    sample = np.sort(rng.choice(alive, size=min(n_sample, alive.size), replace=False)).astype(np.uint32)
-    acc = bh.accelerations()[sample].astype(np.float64)
-    ref = bh.direct_sum_f64(sample, softening)
-    err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    return {"p50": float(np.percentile(err, 50)), "p99": float(np.percentile(err, 99)),
-            "max": float(err.max()), "sample": int(sample.size), "exact_force": "fp64 direct sum (HIP kernel)"}
+    if isinstance(bodies_or_n, np.ndarray):
+        alive = np.nonzero(bodies_or_n[:, 3] != 0)[0]
+    else:
+        alive = np.arange(bodies_or_n)
+    return np.sort(rng.choice(alive, size=min(n_sample, alive.size), replace=False)).astype(np.uint32)
 
 
-def variant_time(bh, prm, steps: int = 3) -> float:
-    """Mean ms per step of `steps` steps with parameters prm (HIP-event stage sum)."""
+def error_stats(acc: np.ndarray, exact: np.ndarray) -> dict:
+    e = np.linalg.norm(acc.astype(np.float64) - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    return {"p50": float(np.percentile(e, 50)), "p99": float(np.percentile(e, 99)), "max": float(e.max()),
+            "sample": int(e.size)}
+
+
+def force_error(bh, softening: float, n_sample: int = N_SAMPLE, seed: int = 7, prm=None) -> dict:
+    """p50/p99 of |a - a_fp64| / |a_fp64| over sampled alive bodies of bh's current state."""
+    from gravity_amd import barnes_hut as B
+    sample = sample_ids(bh.bodies(B.BODIES_IN), n_sample, seed)
+    exact = bh.direct_sum_f64(sample, softening)
+    out = error_stats(bh.accelerations(prm)[sample], exact)
+    out["exact_force"] = "fp64 direct sum (HIP kernel)"
+    return out
+
+
+def variant_time(bh, prm, steps: int = 2) -> float:
+    """Mean ms per step of `steps` steps with parameters prm (after one untimed step)."""
     from gravity_amd import barnes_hut as B
     p = B.Params.from_buffer_copy(prm)
-    p.flags |= B.FLAG_TIMING
-    bh.step(p)                          # warm the variant
+    bh.step(p)
     bh.sync()
-    bh.stage_times()
     t0 = time.perf_counter()
     for _ in range(steps):
         bh.step(p)
@@ -123,23 +146,42 @@ def variant_time(bh, prm, steps: int = 3) -> float:
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, seed: int) -> dict:
-    """Train the far-field network on 2M pairs from two 1M-body snapshots
-    (Plummer, two-disk galaxy) with FP64 labels, then compare on the bench
-    workload: plain walks over a sweep of angles vs the corrected walk at
-    theta_wide and 0.7 (and the same box-aware walk with no network), each
-    with its p50/p99 force error against the FP64 direct sum."""
+def timed_loop(bh, prm, steps: int, warmup: int) -> dict:
+    """W untimed steps, then exactly K steps bracketed by stream synchronisation."""
+    from gravity_amd import barnes_hut as B
+    for k in range(warmup):
+        bh.step(prm)
+        bh.sync()
+    timed = B.Params.from_buffer_copy(prm)
+    timed.flags |= B.FLAG_TIMING
+    bh.stage_times()                      # clear the event ring
+    bh.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bh.step(timed)
+    bh.sync()
+    wall = time.perf_counter() - t0
+    stage = bh.stage_times()
+    c = bh.counters()
+    num = int(c["num_bodies"])
+    return {"wall_s": wall, "ms_per_step": wall * 1e3 / steps, "body_steps_per_s": num * steps / wall,
+            "stage_ms": {k: v / steps for k, v in stage.items()}, "counters": c, "num": num}
+
+
+def train_network(soft: float, theta: float, seed: int) -> tuple:
+    """The correction's network trained on the GPU: (body, accepted node)
+    pairs of the walk at `theta` on a 1M-body Plummer and a 1M-body two-galaxy
+    snapshot, FP64 labels (the residual after monopole and exact quadrupole)."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
     from gravity_amd import learned as Lr
     t0 = time.perf_counter()
     feats, targs = [], []
     for k, snap in enumerate((ic.plummer(1 << 20, seed + 11), ic.two_galaxy(1 << 19, seed + 12))):
-        sb = B.BarnesHut(snap, ic.tight_bounds(snap), B.make_params(theta=theta_wide, softening=soft))
+        sb = B.BarnesHut(snap, ic.tight_bounds(snap), B.make_params(theta=theta, softening=soft))
         sb.init()
         sb.build_current_tree()
-        f, t = Lr.collect_training_pairs(sb, theta_wide, theta_plain, 3, soft, max_node_bodies=1 << 20,
-                                         cap=1 << 20)
+        f, t = Lr.collect_training_pairs(sb, theta, 5, soft, max_node_bodies=1 << 20, cap=1 << 21)
         feats.append(f)
         targs.append(t)
         sb.close()
@@ -148,51 +190,86 @@ def learned_leg(bh, n: int, soft: float, theta_plain: float, theta_wide: float, 
     t = np.concatenate(targs)
     t_data = time.perf_counter() - t0
     mlp = Lr.Mlp(params=Lr.init_params(seed))
-    loss0, _ = mlp.loss_grad(x[:65536], t[:65536])
+    rows = np.random.default_rng(seed).permutation(x.shape[0])
+    hold, train = rows[:65536], rows[65536:]
+    loss0, _ = mlp.loss_grad(x[hold], t[hold])
     t1 = time.perf_counter()
-    mlp.train(x, t, epochs=20, batch=4096, lr=2e-3, seed=seed)
+    mlp.train(x[train], t[train], epochs=8, batch=4096, lr=1e-3, seed=seed)
     t_train = time.perf_counter() - t1
-    loss1, _ = mlp.loss_grad(x[:65536], t[:65536])
-    log(f"learned: trained on {x.shape[0]} pairs in {t_train:.1f} s, loss {loss0:.4g} -> {loss1:.4g}")
-    zero = Lr.Mlp(params=np.zeros(Lr.num_params(), np.float32))
+    loss1, _ = mlp.loss_grad(x[hold], t[hold])
+    log(f"learned: trained on {train.size} pairs in {t_train:.1f} s, held-out loss {loss0:.4g} -> {loss1:.4g}")
+    t2 = time.perf_counter()
+    alpha, cal = Lr.calibrate(mlp, ic.plummer(1 << 20, seed + 13), theta, soft)
+    log(f"learned: output scale {alpha} on a validation snapshot ({time.perf_counter() - t2:.1f} s): {cal}")
+    return mlp, {"train_pairs": int(train.size), "held_out_pairs": int(hold.size), "data_s": t_data,
+                 "train_s": t_train, "held_out_loss_zero_network": loss0, "held_out_loss_trained": loss1,
+                 "train_theta": theta, "output_scale": alpha,
+                 "calibration": [{"alpha": a, "p50": p50, "p99": p99} for a, p50, p99 in cal],
+                 "calibration_snapshot": "Plummer 1,048,576 (a seed of its own), 4,096 sampled bodies"}
+
+
+def sweep(bh, saved, sample, exact, soft: float, method: str, thetas, flags: int) -> list:
+    """p50/p99 on the frozen state `saved` and step time of each angle, in
+    order, until the first angle whose p99 is at the target."""
+    from gravity_amd import barnes_hut as B
     rows = []
-
-    def run(method, th, net):
-        flags = 0 if net is None else B.FLAG_LEARNED
-        Lr.attach(bh, net, theta_plain)
+    for th in thetas:
         prm = B.make_params(theta=th, softening=soft, flags=flags)
+        bh.upload(B.BODIES_IN, saved)
+        err = error_stats(bh.accelerations(prm)[sample], exact)
         ms = variant_time(bh, prm)
-        bh.params = prm
-        err = force_error(bh, n, soft, n_sample=1024)
-        row = {"method": method, "theta": th, "ms_per_step": ms, "body_steps_per_s": n * 1e3 / ms,
-               "p50": err["p50"], "p99": err["p99"]}
-        if net is not None:
-            row["theta_plain"] = theta_plain
-            row["network_evals_per_body"] = bh.counters()["corrected"] / n
+        c = bh.counters()
+        row = {"method": method, "theta": th, "ms_per_step": ms, "body_steps_per_s": bh.n * 1e3 / ms,
+               "p50": err["p50"], "p99": err["p99"], "interactions_per_body": c["interactions"] / bh.n}
         rows.append(row)
-        log(f"learned leg: {method} theta {th}: {ms:.1f} ms, p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+        log(f"sweep {method} theta {th}: {ms:.1f} ms, p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+        if err["p99"] <= P99_TARGET:
+            break
+    return rows
 
-    for th in (0.3, 0.4, theta_plain, 0.6, 0.7, 0.8, theta_wide):
-        run("plain", th, None)
-    for th in (theta_wide, 0.7):
-        run("box walk, no network", th, zero)
-        run("learned", th, mlp)
-    bh.params = B.make_params(theta=theta_plain, softening=soft)
-    Lr.attach(bh, None, theta_plain)
-    # equal-p99 comparison: the fastest plain angle whose p99 is at most the corrected walk's
-    best = {}
-    for r in rows:
-        if r["method"] != "learned":
-            continue
-        ok = [q for q in rows if q["method"] == "plain" and q["p99"] <= r["p99"]]
-        if ok:
-            q = max(ok, key=lambda q: q["body_steps_per_s"])
-            best[str(r["theta"])] = {"plain_theta_at_equal_p99": q["theta"],
-                                     "speedup_vs_plain_at_equal_p99": r["body_steps_per_s"] / q["body_steps_per_s"]}
-    mlp.close()
-    zero.close()
-    return {"train_pairs": int(x.shape[0]), "data_s": t_data, "train_s": t_train, "loss_before": loss0,
-            "loss_after": loss1, "variants": rows, "equal_p99": best}
+
+def fastest_at_target(rows):
+    ok = [r for r in rows if r["p99"] <= P99_TARGET]
+    return max(ok, key=lambda r: r["body_steps_per_s"]) if ok else None
+
+
+def other_workload(name: str, steps: int, seed: int) -> dict:
+    """One of the other configurations: timed steps and the force error after them."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    soft = 0.01
+    if name == "plummer_65536":
+        bodies, dt = ic.plummer(65536, seed), 1e-3
+        prm = B.make_params(theta=0.5, dt=dt, softening=soft)
+        desc = {"n_bodies": 65536, "theta": 0.5, "dt": dt, "softening": soft, "merging": "off", "steps": steps}
+    elif name == "cube_core_merging":
+        # a softened distance (softening = eps^2) accepts every node smaller
+        # than ~eps before its leaves are reached, and merges are only found
+        # at leaves: this workload uses eps^2 = 1e-6 so that overlaps merge
+        bodies, dt, soft = ic.cube_plus_core(4_194_304, seed), 5e-4, 1e-6
+        prm = B.make_params(theta=0.5, dt=dt, softening=soft, merging=B.MERGING)
+        desc = {"n_bodies": 4_194_304, "theta": 0.5, "dt": dt, "softening": soft, "merging": "on",
+                "masses": "log-normal(0, 0.5), total 1", "density": 64.0, "steps": steps}
+    else:
+        bodies, dt = ic.two_galaxy(16_777_216, seed), 1e-3
+        prm = B.make_params(theta=0.5, dt=dt, softening=soft)
+        desc = {"n_bodies": int(bodies.shape[0]), "theta": 0.5, "dt": dt, "softening": soft, "merging": "off",
+                "steps": steps}
+    log(f"{name}: {bodies.shape[0]} bodies, {steps} steps")
+    bh = B.BarnesHut(bodies, ic.tight_bounds(bodies), prm)
+    del bodies
+    bh.init()
+    r = timed_loop(bh, prm, steps, 1)
+    err = force_error(bh, soft)
+    c = r["counters"]
+    out = dict(desc, ms_per_step=r["ms_per_step"], body_steps_per_s=r["body_steps_per_s"],
+               alive_bodies=r["num"], stage_ms=r["stage_ms"], p50=err["p50"], p99=err["p99"],
+               error_sample=err["sample"], error_flags=int(c["error_flags"]), tree_height=int(c["tree_height"]),
+               merges_last_step=int(c["merges_applied"]))
+    bh.close()
+    log(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['body_steps_per_s']:.3g} body-steps/s, "
+        f"p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+    return out
 
 
 def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, seed: int) -> dict:
@@ -237,7 +314,7 @@ def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, 
         bh.sync()
         t_seq += time.perf_counter() - t0
         if k < 4:
-            errs.append(force_error(bh, b.shape[0], soft, n_sample=256))
+            errs.append(force_error(bh, soft, n_sample=1024))
         bh.close()
     log(f"ensemble: sequential {steps} steps of each simulation in {t_seq:.3f} s")
     batched = total * steps / t_batched
@@ -303,17 +380,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("plummer", "cube_core_merging", "two_galaxy"), default="plummer",
-                    help="plummer: the headline 16M Plummer sphere (default); cube_core_merging: uniform cube "
-                         "plus Plummer core, log-normal masses, merging on, dt 5e-4 (default N 4,194,304); "
-                         "two_galaxy: two disk+bulge galaxies 10 apart on a parabolic orbit (default N 33,554,432)")
-    ap.add_argument("--n", type=int, default=None)
-    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--workload", choices=("plummer", "plummer_65536", "cube_core_merging", "two_galaxy"),
+                    default="plummer",
+                    help="plummer: the headline 16M comparison (default, with every other leg); the others run "
+                         "one of the other configurations alone")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--ensemble-sims", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
@@ -325,109 +399,124 @@ def main() -> int:
 
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
+    from gravity_amd import learned as Lr
 
-    theta, soft = args.theta, 0.01
-    if args.workload == "plummer":
-        n, dt = args.n or 16_777_216, 1e-3
-        log(f"generating Plummer N={n}")
-        bodies = ic.plummer(n, args.seed)
-        prm = B.make_params(theta=theta, dt=dt, softening=soft)
-        workload = {"workload": f"plummer_{n}_plain_theta{theta}", "merging": "off"}
-    elif args.workload == "cube_core_merging":
-        # a softened distance (softening = eps^2) accepts every node smaller
-        # than ~eps before its leaves are reached, and merges are only found
-        # at leaves: this workload uses eps^2 = 1e-6 so that overlaps merge
-        n, dt, soft = args.n or 4_194_304, 5e-4, 1e-6
-        log(f"generating cube + Plummer core N={n}")
-        bodies = ic.cube_plus_core(n, args.seed)
-        prm = B.make_params(theta=theta, dt=dt, softening=soft, merging=B.MERGING)
-        workload = {"workload": f"cube_plus_core_{n}_merging_theta{theta}", "merging": "on",
-                    "masses": "log-normal(0, 0.5), total 1", "density": 64.0}
-    else:
-        n, dt = args.n or 33_554_432, 1e-3
-        log(f"generating two galaxies N={n}")
-        bodies = ic.two_galaxy(n // 2, args.seed)
-        n = bodies.shape[0]
-        prm = B.make_params(theta=theta, dt=dt, softening=soft)
-        workload = {"workload": f"two_galaxy_{n}_plain_theta{theta}", "merging": "off"}
+    if args.workload != "plummer":
+        steps = {"plummer_65536": 10, "cube_core_merging": 20, "two_galaxy": 5}[args.workload]
+        print(json.dumps(other_workload(args.workload, steps, args.seed)), flush=True)
+        return 0
+
+    n, soft, dt = 16_777_216, 0.01, 1e-3
+    log(f"generating Plummer N={n}")
+    bodies = ic.plummer(n, args.seed)
     bounds = ic.tight_bounds(bodies)
-    headline = args.workload == "plummer"
-    log("creating context and uploading")
-    bh = B.BarnesHut(bodies, bounds, prm)
+    plain_ref = B.make_params(theta=0.5, dt=dt, softening=soft)
+    bh = B.BarnesHut(bodies, bounds, plain_ref)
     bh.init()
-    for k in range(args.warmup):
-        bh.step()
-        bh.sync()
-        log(f"warmup step {k + 1}/{args.warmup}")
-    timed = B.Params.from_buffer_copy(prm)
-    timed.flags = B.FLAG_TIMING
-    bh.stage_times()                      # clear the event ring
-    bh.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        bh.step(timed)
-    bh.sync()
-    wall = time.perf_counter() - t0
-    stage = bh.stage_times()
-    c = bh.counters()
-    log(f"timed {args.steps} steps in {wall:.3f} s; counters {c}")
-    num = int(c["num_bodies"])
-    value = num * args.steps / wall
-    walk_ms = stage["force"] / args.steps
-    achieved = c["interactions"] * FLOP_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+
+    # 1. the reference's configuration: plain, theta 0.5
+    ref = timed_loop(bh, plain_ref, args.steps, args.warmup)
+    log(f"plain theta 0.5: {ref['ms_per_step']:.2f} ms/step, {ref['body_steps_per_s']:.4g} body-steps/s")
+
+    # 2. the network
+    mlp, train_info = train_network(soft, 0.3, args.seed)
+    Lr.attach(bh, mlp)
+
+    # 3. sweeps on one frozen state against one FP64 judge
+    saved = np.ascontiguousarray(bh.bodies(B.BODIES_IN))
+    sample = sample_ids(saved)
+    exact = bh.direct_sum_f64(sample, soft)
+    plain_rows = sweep(bh, saved, sample, exact, soft, "plain", (0.5, 0.3, 0.2, 0.15, 0.12, 0.1, 0.08, 0.06), 0)
+    quad_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", (0.5, 0.4, 0.35, 0.3, 0.27, 0.25, 0.22, 0.2),
+                      B.FLAG_QUADRUPOLE)
+    learned_rows = sweep(bh, saved, sample, exact, soft, "learned", (0.5, 0.4, 0.35, 0.3, 0.27, 0.25, 0.22, 0.2),
+                         B.FLAG_LEARNED)
+    plain_best = fastest_at_target(plain_rows)
+    quad_best = fastest_at_target(quad_rows)
+    learned_best = fastest_at_target(learned_rows)
+
+    # 4. the timed loop: learned-corrected at its widest angle within the target
+    pick = learned_best or learned_rows[-1]
+    headline = B.make_params(theta=pick["theta"], dt=dt, softening=soft, flags=B.FLAG_LEARNED)
+    bh.upload(B.BODIES_IN, saved)
+    run = timed_loop(bh, headline, args.steps, args.warmup)
+    c = run["counters"]
+    num = run["num"]
+    log(f"timed {args.steps} learned-corrected steps (theta {pick['theta']}): {run['ms_per_step']:.2f} ms/step; "
+        f"counters {c}")
+    walk_ms = run["stage_ms"]["force"]
+    flops = c["interactions"] * FLOP_PER_INTERACTION + c["quad_terms"] * FLOP_PER_QUAD_TERM
+    achieved = flops / (walk_ms * 1e-3) / 1e12
     result = {
         "metric": METRIC,
-        "value": value,
+        "value": run["body_steps_per_s"],
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": dict(workload, n_bodies=n, theta=theta, dt=dt, softening=soft, bounds="dynamic",
-                       method="plain Barnes-Hut", mac="half longest AABB side x inverse softened distance < theta"),
-        "roofline": {"bound": "valu", "kernel": "k_walk_q", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": measured_traffic("k_walk_q") if headline and n == 16_777_216 else None,
-                     "traffic_source": "profiles/k_walk_plummer16M_step_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
-                     "flop_per_interaction": FLOP_PER_INTERACTION,
-                     "interactions_per_step": int(c["interactions"]), "walk_ms": walk_ms,
-                     "interactions_per_s": c["interactions"] / (walk_ms * 1e-3)},
-        "stage_ms": {k: v / args.steps for k, v in stage.items()},
+        "config": {"workload": f"plummer_{n}_learned_corrected_at_p99_1e-3", "n_bodies": n, "theta": pick["theta"],
+                   "dt": dt, "softening": soft, "merging": "off", "bounds": "dynamic",
+                   "method": "learned-corrected Barnes-Hut: monopole + softened quadrupole of the exact second "
+                             "moment plus the network's residual, one network evaluation per node and step",
+                   "mac": "half longest AABB side x inverse softened distance < theta (the reference's)",
+                   "p50": pick["p50"], "p99": pick["p99"], "error_sample": int(sample.size),
+                   "p99_target": P99_TARGET, "p99_met": bool(pick["p99"] <= P99_TARGET)},
+        "roofline": {"bound": "valu", "kernel": "k_walk_q (corrected)", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic("k_walk_q_corrected", "k_walk_q_corrected_plummer16M_pmc.md"),
+                     "traffic_source": "profiles/k_walk_q_corrected_plummer16M_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+                     "flop_per_interaction": FLOP_PER_INTERACTION, "flop_per_quad_term": FLOP_PER_QUAD_TERM,
+                     "interactions_per_step": int(c["interactions"]), "quad_terms_per_step": int(c["quad_terms"]),
+                     "walk_ms": walk_ms, "interactions_per_s": c["interactions"] / (walk_ms * 1e-3)},
+        "stage_ms": run["stage_ms"],
         "counters": {"opened": int(c["opened"]), "interactions_per_body": c["interactions"] / max(num, 1),
-                     "tree_height": int(c["tree_height"]), "max_stack": int(c["max_stack"]),
-                     "error_flags": int(c["error_flags"]), "alive_bodies": num,
-                     "merges_last_step": int(c["merges_applied"])},
+                     "quad_terms_per_body": c["quad_terms"] / max(num, 1),
+                     "network_evaluations": int(c["corrected"]), "tree_height": int(c["tree_height"]),
+                     "max_stack": int(c["max_stack"]), "error_flags": int(c["error_flags"]), "alive_bodies": num},
+        "equal_p99": {
+            "p99_target": P99_TARGET,
+            "plain": plain_best, "quadrupole": quad_best, "learned": learned_best,
+            "speedup_learned_vs_plain": (learned_best["body_steps_per_s"] / plain_best["body_steps_per_s"]
+                                         if learned_best and plain_best else None),
+            "speedup_quadrupole_vs_plain": (quad_best["body_steps_per_s"] / plain_best["body_steps_per_s"]
+                                            if quad_best and plain_best else None),
+            "speedup_learned_vs_plain_theta_0_5": (learned_best["body_steps_per_s"] / ref["body_steps_per_s"]
+                                                   if learned_best else None),
+            "sweeps": plain_rows + quad_rows + learned_rows},
+        "learned_training": train_info,
+        "plain_theta_0_5": {"ms_per_step": ref["ms_per_step"], "body_steps_per_s": ref["body_steps_per_s"],
+                            "stage_ms": ref["stage_ms"], "p50": plain_rows[0]["p50"], "p99": plain_rows[0]["p99"],
+                            "interactions": int(ref["counters"]["interactions"]),
+                            "walk_tflops": ref["counters"]["interactions"] * FLOP_PER_INTERACTION
+                            / (ref["stage_ms"]["force"] * 1e-3) / 1e12},
     }
-    # the HBM-bound stages at their algorithmic bytes per body (one sorted
-    # pass over 24 B per key, 8 passes plus 8 histogram reads; the tree's 172
-    # B; the encode's 24 B; census + bounds 16 B) against the measured copy
-    # rate and the vendor peak
     copy = hbm_copy_gbps()
-    sm = result["stage_ms"]
+    sm = ref["stage_ms"]
     alg = {"sort": 256, "tree": 172, "morton": 24, "dead": 16}
-    result["hbm_stages"] = {
+    result["hbm_stages_plain_theta_0_5"] = {
         "copy_GBps": copy, "peak_GBps": HBM_PEAK_GBS,
-        **{k: {"ms": sm[k], "alg_bytes_per_body": b, "alg_GBps": b * num / (sm[k] * 1e-3) / 1e9,
-               "frac_of_peak": b * num / (sm[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        **{k: {"ms": sm[k], "alg_bytes_per_body": b, "alg_GBps": b * n / (sm[k] * 1e-3) / 1e9,
+               "frac_of_peak": b * n / (sm[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
            for k, b in alg.items() if sm.get(k, 0.0) > 0.0}}
-    if not args.no_error:
-        log("force error against the FP64 direct sum")
-        result["force_error"] = force_error(bh, n, soft)
-    if not args.no_learned and headline:
-        log("learned-correction leg")
-        result["learned"] = learned_leg(bh, n, soft, theta, 0.9, args.seed)
+    Lr.attach(bh, None)
     bh.close()
-    if not args.no_ensemble and headline:
+    mlp.close()
+    if not args.no_configs:
+        result["configs"] = {name: other_workload(name, steps, args.seed)
+                             for name, steps in (("plummer_65536", 10), ("cube_core_merging", 20),
+                                                 ("two_galaxy", 5))}
+    if not args.no_ensemble:
         log("ensemble leg")
-        result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, theta, dt, args.seed)
-    if not args.no_cpu_baseline and headline:
+        result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, 0.5, dt, args.seed)
+    if not args.no_cpu_baseline:
         log("CPU baseline (oracle port)")
-        result["cpu_baseline"] = cpu_baseline(bodies, bounds, theta, dt, soft, args.cpu_threads,
+        result["cpu_baseline"] = cpu_baseline(bodies, bounds, 0.5, dt, soft, args.cpu_threads,
                                               args.cpu_walk_sample)
     print(json.dumps(result), flush=True)
     return 0

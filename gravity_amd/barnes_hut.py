@@ -24,10 +24,12 @@ LIB_PATH = os.path.join(HERE, "libgravity_amd.so")
 LEAF_NODES, INTERNAL_NODES, SIMULATION_VALUES, BODIES_IN, BODIES_OUT = 0, 1, 2, 3, 4
 MORTON_IN, MORTON_OUT, INDEX_IN, INDEX_OUT = 5, 6, 7, 8
 MERGE_TASKS = 14
+NODE_QUADRUPOLES = 16   # the corrected walk's records (no reference counterpart)
 
 NEITHER, MERGING, COLLISION = 0, 1, 2
 STATIC, DYNAMIC = 0, 1
-FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK = 1, 2, 4, 8
+FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK, FLAG_QUADRUPOLE = 1, 2, 4, 8, 16
+FLAG_CORRECTED = FLAG_LEARNED | FLAG_QUADRUPOLE
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
 NOALLOC_MAX_SAMPLE = 65536   # BH_NOALLOC_MAX_SAMPLE
 
@@ -51,7 +53,7 @@ class Counters(C.Structure):
                 ("out_of_bounds", C.c_uint32), ("opened", C.c_uint64), ("interactions", C.c_uint64),
                 ("merge_tasks", C.c_uint32), ("merges_applied", C.c_uint32), ("tree_height", C.c_uint32),
                 ("max_stack", C.c_uint32), ("error_flags", C.c_uint32), ("reserved", C.c_uint32),
-                ("corrected", C.c_uint64)]
+                ("corrected", C.c_uint64), ("quad_terms", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -178,6 +180,11 @@ class BarnesHut:
         rows = self.n if which == LEAF_NODES else self.n - 1
         return self.download(which, np.uint32).reshape(rows, 20)
 
+    def node_quadrupoles(self) -> np.ndarray:
+        """The corrected walk's records of the internal nodes, (n - 1, 8) float32:
+        G T xx xy xz yy yz zz, -1.5 G tr T, 0."""
+        return self.download(NODE_QUADRUPOLES, np.float32).reshape(self.n - 1, 8)
+
     def merge_tasks(self) -> np.ndarray:
         raw = self.download(MERGE_TASKS, np.uint32)
         return raw[2: 2 + 2 * int(raw[1])].reshape(-1, 2)
@@ -235,7 +242,7 @@ class BarnesHut:
         z[:, 4:7] = 0.0
         p1 = Params.from_buffer_copy(params or self.params)
         p1.dt = 1.0
-        p1.flags &= FLAG_LEARNED
+        p1.flags &= FLAG_CORRECTED | FLAG_GENERAL_WALK
         p1.merging_collision_or_neither = NEITHER
         p1.static_or_dynamic = DYNAMIC
         self.upload(BODIES_IN, z)

@@ -80,7 +80,11 @@ struct bh_context {
     uint32_t* sort_pos[2] = {nullptr, nullptr};
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
-    float theta_plain = 0.5f;
+    // corrected walk (allocated on first use): the quadrupole records of the
+    // internal nodes and the FP64 moment prefix sums they are made from
+    float4* qrec = nullptr;                   // [2(n-1)]
+    double* mom_prefix = nullptr;             // [mom_prefix_doubles(n)]
+    double* mom_tiles = nullptr;              // [mom_tile_doubles(n)]
 
     // per-stage HIP events of up to kTimedSteps steps, read without
     // synchronising inside the stepping loop
@@ -159,8 +163,8 @@ int mlp_workspace(bh_mlp* m, uint32_t rows) {
         *b = nullptr;
     }
     size_t r = rows;
-    size_t sizes[] = {r * kMlpIn, r * kMlpOut, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden,
-                      r * kMlpOut, r * kMlpOut, (r * kMlpOut + 255) / 256 + 1};
+    size_t sizes[] = {r * kMlpIn, r * kMlpTarget, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden, r * kMlpHidden,
+                      r * kMlpOut, r * kMlpOut, (r + 255) / 256 + 1};
     for (int k = 0; k < 9; k++) MHIP(m, hipMalloc((void**)bufs[k], sizes[k] * sizeof(float)));
     if (!m->gemm_ws) MHIP(m, hipMalloc((void**)&m->gemm_ws, kGemmWsFloats * sizeof(float)));
     m->cap = rows;
@@ -368,7 +372,7 @@ void bh_destroy(bh_context* c) {
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
-                    c->sort_pos[1], c->walk_queues};
+                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -394,6 +398,7 @@ size_t bh_binding_size(const bh_context* c, int b) {
         case BH_BINDING_INDEX_IN:
         case BH_BINDING_INDEX_OUT: return n * 4;
         case BH_BINDING_MERGE_TASKS: return 8 + n * 8;
+        case BH_BINDING_NODE_QUADRUPOLES: return (n - 1) * 32;
         default: return 0;
     }
 }
@@ -407,6 +412,7 @@ static void* binding_ptr(bh_context* c, int b) {
         case BH_BINDING_MORTON_OUT: return c->morton[1 - c->mi_in];
         case BH_BINDING_INDEX_IN: return c->index[c->mi_in];
         case BH_BINDING_INDEX_OUT: return c->index[1 - c->mi_in];
+        case BH_BINDING_NODE_QUADRUPOLES: return c->qrec;
         default: return nullptr;
     }
 }
@@ -418,12 +424,24 @@ static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
     return BH_OK;
 }
 
+// the corrected walk's arrays, allocated on first use (about 110 bytes per body)
+static int ensure_quad(bh_context* c) {
+    if (c->qrec) return BH_OK;
+    int rc;
+    if ((rc = dalloc(c, &c->mom_prefix, mom_prefix_doubles(c->n)))) return rc;
+    if ((rc = dalloc(c, &c->mom_tiles, mom_tile_doubles(c->n)))) return rc;
+    if ((rc = dalloc(c, &c->qrec, 2 * ((size_t)c->n - 1)))) return rc;
+    HIPCHECK(c, hipMemsetAsync(c->qrec, 0, 32 * ((size_t)c->n - 1), c->stream));
+    return BH_OK;
+}
+
 int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) {
     if (!c || (!host && bytes)) return BH_ERR_INVALID_ARGUMENT;
     c->step_pos_valid = false;
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
+    if (b == BH_BINDING_NODE_QUADRUPOLES && (rc = ensure_quad(c))) return rc;
     if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
         if (off != 0 || bytes != bh_binding_size(c, b))
             return fail(c, BH_ERR_INVALID_ARGUMENT, "node bindings are uploaded whole");
@@ -454,6 +472,7 @@ int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
+    if (b == BH_BINDING_NODE_QUADRUPOLES && (rc = ensure_quad(c))) return rc;
     if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
         launch_nodes_to_std430(c->stream, c->n, c->sim, c->tree, c->colors, b == BH_BINDING_LEAF_NODES ? 0 : 1,
                                c->staging_nodes);
@@ -583,11 +602,22 @@ int radix_sort(bh_context* c, bool positions) {
 // tree then puts the ids in the new order into IndexIn.
 int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* step_pos) {
     c->colors = p && (p->flags & BH_FLAG_COLORS);
+    const bool quad = p && (p->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED));
+    const bool learned = p && (p->flags & BH_FLAG_LEARNED);
+    if (learned && !c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
+    int rc;
+    if (quad && (rc = ensure_quad(c))) return rc;
     launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
                 c->tree, c->colors ? 1 : 0, positions ? step_pos : nullptr,
                 positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr);
-    const int rc = check_launch(c, "tree");
+    rc = check_launch(c, "tree");
     if (rc == BH_OK && positions) std::swap(c->index[c->mi_in], c->sort_pos[1 - c->mi_in]);
+    if (rc == BH_OK && quad) {
+        launch_node_quadrupoles(c->stream, c->n, c->sim, c->ctr, c->tree, c->mom_prefix, c->mom_tiles,
+                                learned ? c->correction->w : nullptr, learned ? c->correction->bias : nullptr,
+                                c->qrec);
+        rc = check_launch(c, "node quadrupoles");
+    }
     return rc;
 }
 }  // namespace
@@ -636,13 +666,13 @@ int compute_force(bh_context* c, const bh_params* p, float4* step_pos, bool* wro
     a.general = (p->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
     a.pos_out = step_pos;
     a.queues = c->walk_queues;
-    if (p->flags & BH_FLAG_LEARNED) {
-        if (!c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
-        if (p->merging_collision_or_neither != BH_NEITHER || p->static_or_dynamic != BH_DYNAMIC)
-            return fail(c, BH_ERR_UNSUPPORTED, "the corrected walk supports dynamic bounds without merging");
-        launch_walk_learned(c->stream, c->n, a, c->tree.node_aabb, c->tree.node_range, c->correction->w,
-                            c->correction->bias, c->theta_plain);
-        return check_launch(c, "walk (corrected)");
+    if (p->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED)) {
+        if (p->merging_collision_or_neither != BH_NEITHER)
+            return fail(c, BH_ERR_UNSUPPORTED, "the corrected walk runs without merging or collisions");
+        if (!c->qrec)
+            return fail(c, BH_ERR_INVALID_ARGUMENT,
+                        "no quadrupole records: build the tree with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED first");
+        a.qrec = c->qrec;
     }
     launch_walk(c->stream, c->n, a);
     const int rc = check_launch(c, "walk");
@@ -726,6 +756,7 @@ int bh_get_counters(bh_context* c, bh_counters* out) {
     out->max_stack = d.maxStack;
     out->error_flags = d.errorFlags | (d.nTasks > c->n ? 2u : 0u);
     out->corrected = d.corrected;
+    out->quad_terms = d.quadTerms;
     return BH_OK;
 }
 
@@ -839,7 +870,7 @@ int bh_mlp_loss_grad(bh_mlp* m, const float* x, const float* t, uint32_t rows, f
     int rc = mlp_workspace(m, rows);
     if (rc) return rc;
     MHIP(m, hipMemcpyAsync(m->x, x, (size_t)rows * kMlpIn * 4, hipMemcpyHostToDevice, m->stream));
-    MHIP(m, hipMemcpyAsync(m->t, t, (size_t)rows * kMlpOut * 4, hipMemcpyHostToDevice, m->stream));
+    MHIP(m, hipMemcpyAsync(m->t, t, (size_t)rows * kMlpTarget * 4, hipMemcpyHostToDevice, m->stream));
     mlp_forward_fp32(m, rows);
     uint32_t nparts = mlp_backward_fp32(m, rows);
     MHIP(m, hipGetLastError());
@@ -857,12 +888,12 @@ int bh_mlp_train(bh_mlp* m, const float* x, const float* t, uint32_t rows, uint3
     float *dx = nullptr, *dt = nullptr;
     uint32_t* dperm = nullptr;
     MHIP(m, hipMalloc((void**)&dx, (size_t)rows * kMlpIn * 4));
-    MHIP(m, hipMalloc((void**)&dt, (size_t)rows * kMlpOut * 4));
+    MHIP(m, hipMalloc((void**)&dt, (size_t)rows * kMlpTarget * 4));
     MHIP(m, hipMalloc((void**)&dperm, (size_t)rows * 4));
     uint32_t* perm = new (std::nothrow) uint32_t[rows];
     hipError_t e = perm ? hipSuccess : hipErrorOutOfMemory;
     if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)rows * kMlpIn * 4, hipMemcpyHostToDevice, m->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, (size_t)rows * kMlpOut * 4, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, (size_t)rows * kMlpTarget * 4, hipMemcpyHostToDevice, m->stream);
     uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
     uint32_t nparts = 0, last = batch;
     for (uint32_t ep = 0; ep < epochs && e == hipSuccess; ep++) {
@@ -876,7 +907,7 @@ int bh_mlp_train(bh_mlp* m, const float* x, const float* t, uint32_t rows, uint3
         if (e == hipSuccess) e = hipStreamSynchronize(m->stream);   // perm is reused next epoch
         for (uint32_t off = 0; off + batch <= rows && e == hipSuccess; off += batch) {
             launch_gather_rows(m->stream, dx, dperm, off, batch, kMlpIn, m->x);
-            launch_gather_rows(m->stream, dt, dperm, off, batch, kMlpOut, m->t);
+            launch_gather_rows(m->stream, dt, dperm, off, batch, kMlpTarget, m->t);
             mlp_forward_fp32(m, batch);
             nparts = mlp_backward_fp32(m, batch);
             m->adam_t++;
@@ -897,7 +928,7 @@ int bh_mlp_train(bh_mlp* m, const float* x, const float* t, uint32_t rows, uint3
     return rc;
 }
 
-int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uint32_t stride, float softening,
+int bh_collect_training_pairs(bh_context* c, float theta, uint32_t stride, float softening,
                               uint32_t max_node_bodies, uint32_t cap, float* features, float* targets,
                               uint32_t* pairs_out, uint32_t* n_out) {
     if (!c || !features || !targets || !n_out || stride == 0 || cap == 0) return BH_ERR_INVALID_ARGUMENT;
@@ -911,23 +942,24 @@ int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uin
     HIPCHECK(c, hipMalloc((void**)&count, 4));
     hipError_t e = hipMemsetAsync(count, 0, 4, c->stream);
     if (e == hipSuccess) {
-        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb, theta,
-                             theta_plain, softening, stride, pairs, cap, count);
+        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, theta, softening, stride, pairs,
+                             cap, count);
         e = hipGetLastError();
     }
     uint32_t found = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&found, count, 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     const uint32_t np = found > cap ? cap : found;
-    if (e == hipSuccess) e = hipMalloc((void**)&f, (size_t)(np ? np : 1) * 12 * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&t, (size_t)(np ? np : 1) * 3 * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&f, (size_t)(np ? np : 1) * kMlpIn * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&t, (size_t)(np ? np : 1) * kMlpTarget * 4);
     if (e == hipSuccess) {
         launch_pair_labels(c->stream, pairs, np, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb,
                            c->tree.node_range, (double)h.gravitationalConstant, softening, max_node_bodies, f, t);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(features, f, (size_t)np * 12 * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(targets, t, (size_t)np * 3 * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(features, f, (size_t)np * kMlpIn * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(targets, t, (size_t)np * kMlpTarget * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && pairs_out)
         e = hipMemcpyAsync(pairs_out, pairs, (size_t)np * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -939,10 +971,9 @@ int bh_collect_training_pairs(bh_context* c, float theta, float theta_plain, uin
     return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "collect pairs: %s", hipGetErrorString(e));
 }
 
-int bh_attach_correction(bh_context* c, const bh_mlp* m, float theta_plain) {
+int bh_attach_correction(bh_context* c, const bh_mlp* m) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
     c->correction = m;
-    c->theta_plain = theta_plain;
     return BH_OK;
 }
 

@@ -63,8 +63,9 @@ struct DevCounters {
     uint32_t aabbTicket;       // last-block-done ticket of the bounds reduction
     uint32_t mergeRound;
     uint32_t zeroMassIds;      // body ids with zero mass found by the dead-body census
-    unsigned long long corrected;   // network evaluations used by the corrected walk
-    uint32_t pad[4];
+    unsigned long long corrected;   // network evaluations of the corrected walk's tree stage
+    unsigned long long quadTerms;   // quadrupole terms evaluated by the corrected walk
+    uint32_t pad[2];
 };
 
 // ---- exact FP32 operations (no contraction, IEEE rounding)

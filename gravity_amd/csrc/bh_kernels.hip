@@ -34,6 +34,7 @@ __global__ void k_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ c
     ctr->maxStack = 0;
     ctr->errorFlags = 0;
     ctr->corrected = 0;
+    ctr->quadTerms = 0;
     if (set_units) {
         sim->unitMass = mass;
         sim->unitDensity = density;
@@ -1600,6 +1601,7 @@ __global__ void k_ens_reset(DevCounters* __restrict__ ctr, EnsQueue* __restrict_
     ctr->maxStack = 0;
     ctr->errorFlags = 0;
     ctr->corrected = 0;
+    ctr->quadTerms = 0;
     ctr->treeHeight = 0;
 }
 

@@ -26,7 +26,6 @@ constexpr bool kWalkPersistent = true;
 // small blocks spreading a small walk over all CUs
 constexpr uint32_t kWalkPersistentMin = 1u << 20;
 constexpr int kWalkSmallWaves = 4;
-constexpr int kLearnedWaves = 4;       // waves per block of the corrected walk
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 
 // Tree in HBM. Leaves are sorted positions 0..n-1, internal nodes 0..n-2
@@ -78,7 +77,16 @@ struct WalkArgs {
     uint32_t general;                // 1: always the general walk loop
     float4* pos_out = nullptr;       // [n] new position-mass by sorted position as well, or nullptr
     uint32_t* queues = nullptr;      // [kWalkQueues * 32] per-CU work queues (persistent walk), or nullptr
+    // the corrected walk: [2(n-1)] quadrupole records of the internal nodes
+    // (two float4 each: G T xx xy xz yy, G T yz zz, -1.5 G tr T, 0), or nullptr
+    const float4* qrec = nullptr;
 };
+
+// Second moments of the corrected walk: positions per tile of the FP64
+// prefix sums (m, m x, m x x^T: 10 components per position).
+constexpr uint32_t kMomTile = 1024;
+inline size_t mom_prefix_doubles(uint32_t n) { return ((size_t)n + 1) * 10; }
+inline size_t mom_tile_doubles(uint32_t n) { return ((size_t)n / kMomTile + 1) * 10; }
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.
 // Simulation s owns sorted positions and body ids [off, off + num) and
@@ -155,11 +163,16 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos = nullptr, const uint32_t* from = nullptr, uint32_t* ids_out = nullptr);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
-void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
-                         const uint2* node_range, const __bf16* w, const float* bias, float theta_plain);
+// The corrected walk's node records on the current tree: FP64 prefix sums of
+// the moments (prefix: mom_prefix_doubles(n), tiles: mom_tile_doubles(n)),
+// then per internal node the second moment about its centre of mass plus the
+// network's residual (w, bias: the packed network, or nullptr for none) into
+// qrec [2(n-1)]; ctr->corrected counts the network evaluations.
+void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
+                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, const float4* node_aabb, float theta, float theta_plain, float soft,
-                          uint32_t stride, uint2* pairs, uint32_t cap, uint32_t* count);
+                          const WalkNode* nodes, float theta, float soft, uint32_t stride, uint2* pairs, uint32_t cap,
+                          uint32_t* count);
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
                         float soft, uint32_t max_bodies, float* feat, float* targ);

@@ -1,30 +1,211 @@
-// bh_learned.hip — the learned far-field correction (no reference counterpart).
+// bh_learned.hip — the quadrupole-corrected walk's node records and the
+// learned residual (no reference counterpart).
 //
-//  * k_collect_pairs: for sampled bodies, walk the tree with the wide opening
-//    angle and record every accepted internal node (the pairs the corrected
-//    walk will evaluate the network on).
-//  * k_pair_labels: FP64 label of each pair: the exact field of the node's
-//    bodies on the target minus the node's monopole, in units of G M / d^2.
+//  * second moments: FP64 prefix sums of m, m x and m x x^T over the sorted
+//    bodies (k_mom_tile, k_mom_scan, k_mom_prefix); every internal node covers
+//    a contiguous range of sorted positions, so its moments are a difference of
+//    two prefix entries, with no bottom-up pass.
+//  * k_qnodes: per internal node, the second moment about its centre of mass,
+//    the network's residual (BF16 MFMA, one evaluation per node), and the
+//    32-byte quadrupole record the walk reads.
+//  * k_collect_pairs / k_pair_labels: (target, accepted node) pairs of the walk
+//    and their FP64 labels: the exact field of the node's bodies minus its
+//    monopole and exact quadrupole, in units of (h / s)^2 G M / s^2.
 //  * training: FP32 GEMM / activation / Adam kernels (forward and backward of
-//    the 12-64-64-3 SiLU network), checked against the FP64 oracle.
-//  * k_mlp_forward_mfma: BF16 MFMA inference of 64 samples per wave, the same
-//    device routine the corrected walk uses.
+//    the 12-64-64-6 SiLU network, the loss taken through the quadrupole
+//    basis), checked against the FP64 oracle.
 #include "bh_device.hpp"
 #include "bh_kernels.hpp"
 #include "bh_mlp.hpp"
 
 namespace bh {
 
+// ------------------------------------------------------------------ second moments
+// The 10 moment components of a body: m, m x, m y, m z, m xx, m xy, m xz,
+// m yy, m yz, m zz, in double precision.
+__device__ __forceinline__ void body_moments(float4 pm, double* v) {
+    const double m = pm.w, x = pm.x, y = pm.y, z = pm.z;
+    v[0] = m;
+    v[1] = m * x;
+    v[2] = m * y;
+    v[3] = m * z;
+    v[4] = v[1] * x;
+    v[5] = v[1] * y;
+    v[6] = v[1] * z;
+    v[7] = v[2] * y;
+    v[8] = v[2] * z;
+    v[9] = v[3] * z;
+}
+
+// Exclusive scan of 10 doubles per thread over a block of kMomTile threads
+// (fixed order: lanes, then waves); *total receives the block's sums.
+__device__ __forceinline__ void block_excl_scan10(const double* v, double* ex, double* s_w, double* total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    constexpr uint32_t kW = kMomTile / 64;
+    for (int c = 0; c < 10; c++) {
+        double x = v[c];
+        for (int o = 1; o < 64; o <<= 1) {
+            const double y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w * 10 + c] = x;
+        const double e = __shfl_up(x, 1, 64);
+        ex[c] = lane == 0 ? 0.0 : e;
+    }
+    __syncthreads();
+    for (int c = 0; c < 10; c++) {
+        double off = 0.0, all = 0.0;
+        for (uint32_t k = 0; k < kW; k++) {
+            const double t = s_w[k * 10 + c];
+            if (k < w) off += t;
+            all += t;
+        }
+        ex[c] += off;
+        total[c] = all;
+    }
+}
+
+// tile t: sums of the moments of positions [t kMomTile, (t + 1) kMomTile) below numBodies
+__global__ __launch_bounds__(kMomTile) void k_mom_tile(const SimValues* __restrict__ sim,
+                                                       const float4* __restrict__ leaf_pos,
+                                                       double* __restrict__ tiles) {
+    __shared__ double s_w[(kMomTile / 64) * 10];
+    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
+    if (g < sim->numBodies) body_moments(leaf_pos[g], v);
+    block_excl_scan10(v, ex, s_w, tot);
+    if (threadIdx.x < 10) tiles[blockIdx.x * 10 + threadIdx.x] = tot[threadIdx.x];
+}
+
+// one block: the tile sums become exclusive tile offsets (in place)
+__global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tiles, uint32_t ntiles) {
+    __shared__ double s_w[(kMomTile / 64) * 10];
+    const uint32_t per = (ntiles + kMomTile - 1) / kMomTile;
+    const uint32_t t0 = threadIdx.x * per, t1 = min(ntiles, t0 + per);
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
+    for (uint32_t t = t0; t < t1; t++)
+        for (int c = 0; c < 10; c++) v[c] += tiles[t * 10 + c];
+    block_excl_scan10(v, ex, s_w, tot);
+    for (uint32_t t = t0; t < t1; t++)
+        for (int c = 0; c < 10; c++) {
+            const double x = tiles[t * 10 + c];
+            tiles[t * 10 + c] = ex[c];
+            ex[c] += x;
+        }
+}
+
+// prefix[g] = the moments of positions [0, g), for g = 0 .. numBodies
+__global__ __launch_bounds__(kMomTile) void k_mom_prefix(const SimValues* __restrict__ sim,
+                                                         const float4* __restrict__ leaf_pos,
+                                                         const double* __restrict__ tiles,
+                                                         double* __restrict__ prefix) {
+    __shared__ double s_w[(kMomTile / 64) * 10];
+    const uint32_t num = sim->numBodies;
+    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
+    if (g < num) body_moments(leaf_pos[g], v);
+    block_excl_scan10(v, ex, s_w, tot);
+    if (g <= num)
+        for (int c = 0; c < 10; c++) prefix[(size_t)g * 10 + c] = tiles[blockIdx.x * 10 + c] + ex[c];
+}
+
+// Second moment about the point c of the bodies whose prefix entries are a
+// (first) and b (one past the last): M2 = S2 - c S1^T - S1 c^T + S0 c c^T.
+__device__ __forceinline__ void range_m2(const double* __restrict__ a, const double* __restrict__ b, double cx,
+                                         double cy, double cz, double* m2) {
+    const double S0 = b[0] - a[0], S1x = b[1] - a[1], S1y = b[2] - a[2], S1z = b[3] - a[3];
+    const double c[3] = {cx, cy, cz}, S1[3] = {S1x, S1y, S1z};
+    const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; q++) {
+        const int i = I[q], j = J[q];
+        m2[q] = (b[4 + q] - a[4 + q]) - c[i] * S1[j] - S1[i] * c[j] + S0 * c[i] * c[j];
+    }
+}
+
+// The corrected walk's records: internal node k gets
+//   qrec[2k]     = G (T_xx, T_xy, T_xz, T_yy),
+//   qrec[2k + 1] = G (T_yz, T_zz), -1.5 G tr T, 0,
+// T = M2 + M h^2 o with o the network's output for the node's features (o = 0
+// without a network). Each wave takes 64 consecutive nodes per turn; the grid
+// strides over the nodes so that every wave loads the network's fragments once.
+__global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+                                                const WalkNode* __restrict__ nodes,
+                                                const float4* __restrict__ node_aabb,
+                                                const uint2* __restrict__ node_range,
+                                                const double* __restrict__ prefix, const __bf16* __restrict__ wg,
+                                                const float* __restrict__ bg, float4* __restrict__ qrec) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
+    __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t num = sim->numBodies;
+    const uint32_t ninternal = num > 1 ? num - 1 : 0;
+    const float G = sim->gravitationalConstant;
+    MlpRegs R;
+    if (wg) mlp_load(R, wg, bg);
+    uint32_t evals = 0;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t base = (blockIdx.x * 4 + w) * 64; base < ninternal; base += waves * 64) {
+        const uint32_t k = base + lane;
+        const bool active = k < ninternal;
+        double m2[6] = {0, 0, 0, 0, 0, 0};
+        float M = 0.0f, h = 0.0f;
+        float4 mn = make_float4(0.f, 0.f, 0.f, 0.f), mx = mn, com = mn;
+        if (active) {
+            const WalkNode nd = nodes[k];
+            const uint2 rg = node_range[k];
+            com = nd.comMass;
+            h = nd.half;
+            mn = node_aabb[2 * k];
+            mx = node_aabb[2 * k + 1];
+            M = mx.w;
+            range_m2(prefix + (size_t)rg.x * 10, prefix + ((size_t)rg.x + rg.y) * 10, com.x, com.y, com.z, m2);
+        }
+        const bool ok = active && M > 0.0f && h > 0.0f;
+        double o[6] = {0, 0, 0, 0, 0, 0};
+        if (wg && ballot(ok)) {
+            // every lane takes part in the MFMAs; lanes without a node stage zeros
+            float f[kMlpIn];
+            double nm[6];
+            const double mh2 = ok ? (double)M * (double)h * (double)h : 1.0;
+            for (int c = 0; c < 6; c++) nm[c] = ok ? m2[c] / mh2 : 0.0;
+            if (ok) node_features(mn, mx, com.x, com.y, com.z, h, nm, f);
+            else
+                for (int c = 0; c < kMlpIn; c++) f[c] = 0.0f;
+            mlp_stage_features(s_x[w], f);
+            mlp_wave_forward(R, s_x[w], s_y[w], 0xFu);
+            if (ok)
+                for (int c = 0; c < 6; c++) o[c] = (double)s_y[w][lane * kMlpYStride + c] * mh2;
+            evals += (uint32_t)__popcll(ballot(ok));
+        }
+        if (active) {
+            float T[6];
+            for (int c = 0; c < 6; c++) T[c] = (float)(m2[c] + o[c]);
+            const float tr = T[0] + T[3] + T[5];
+            qrec[2 * k] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
+            qrec[2 * k + 1] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
+        }
+    }
+    if (evals && lane == 0) atomicAdd(&ctr->corrected, (unsigned long long)evals);
+}
+
+void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
+                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec) {
+    const uint32_t ntiles = n / kMomTile + 1;   // positions 0 .. n
+    k_mom_tile<<<ntiles, kMomTile, 0, s>>>(sim, T.leaf_pos, tiles);
+    k_mom_scan<<<1, kMomTile, 0, s>>>(tiles, ntiles);
+    k_mom_prefix<<<ntiles, kMomTile, 0, s>>>(sim, T.leaf_pos, tiles, prefix);
+    const uint32_t groups = (n + 63) / 64;
+    const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048u);
+    k_qnodes<<<blocks, 256, 0, s>>>(sim, ctr, T.nodes, T.node_aabb, T.node_range, prefix, w, bias, qrec);
+}
+
 // ------------------------------------------------------------------ pairs
-// Records the pairs the corrected walk evaluates the network on: accepted at
-// the wide angle theta with the target outside the node's box, but opened at
-// theta_plain (half * inv >= theta_plain).
+// The pairs of the corrected walk: every stride-th sorted body walks with the
+// reference's opening test at theta and records each internal node it accepts.
 __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restrict__ sim,
                                                        const float4* __restrict__ leaf_pos,
-                                                       const WalkNode* __restrict__ nodes,
-                                                       const float4* __restrict__ node_aabb, float theta,
-                                                       float theta_plain, float soft, uint32_t stride,
-                                                       uint2* __restrict__ pairs, uint32_t cap,
+                                                       const WalkNode* __restrict__ nodes, float theta, float soft,
+                                                       uint32_t stride, uint2* __restrict__ pairs, uint32_t cap,
                                                        uint32_t* __restrict__ count) {
     const uint32_t n = sim->initialNumBodies, num = sim->numBodies;
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) * stride;
@@ -37,17 +218,11 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
         const uint32_t node = stack[--sp];
         if (node < n) continue;
         const WalkNode nd = nodes[node - n];
-        const float rx = nd.comMass.x - pm.x, ry = nd.comMass.y - pm.y, rz = nd.comMass.z - pm.z;
+        const float rx = fsub(nd.comMass.x, pm.x), ry = fsub(nd.comMass.y, pm.y), rz = fsub(nd.comMass.z, pm.z);
         const float inv = inv_sqrt_exact(fadd(dot3(rx, ry, rz), soft));
-        const float ratio = fmul(nd.half, inv);
-        const float4 mn = node_aabb[2 * (node - n)], mx = node_aabb[2 * (node - n) + 1];
-        const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y && pm.z >= mn.z &&
-                            pm.z <= mx.z;
-        if (ratio < theta && !inside) {
-            if (ratio >= theta_plain) {
-                uint32_t slot = atomicAdd(count, 1u);
-                if (slot < cap) pairs[slot] = make_uint2(g, node - n);
-            }
+        if (fmul(nd.half, inv) < theta) {
+            uint32_t slot = atomicAdd(count, 1u);
+            if (slot < cap) pairs[slot] = make_uint2(g, node - n);
         } else if (sp + 2 <= kStackDepth) {
             stack[sp++] = nd.childA;
             stack[sp++] = nd.childB;
@@ -55,9 +230,12 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
     }
 }
 
-// One wave per pair (grid-stride over the pairs): exact field of the node's
-// bodies (sorted positions [lo, lo + count), the target itself excluded) in
-// FP64, minus the monopole, in units of G M / d^2.
+// One wave per pair (grid-stride over the pairs): in FP64 over the node's
+// bodies (sorted positions [lo, lo + count)), the exact field on the target
+// (itself excluded) and the node's second moment about its centre of mass;
+// the label is the exact field minus the monopole and the quadrupole of that
+// moment, in units of unit = (h / s)^2 G M / s^2, with u = r / s:
+// targ = (u, label, unit). (The host turns the unit into the row's weight.)
 __global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ pairs, uint32_t npairs,
                                                      const float4* __restrict__ leaf_pos,
                                                      const WalkNode* __restrict__ nodes,
@@ -72,40 +250,64 @@ __global__ __launch_bounds__(256) void k_pair_labels(const uint2* __restrict__ p
         const float4 x = leaf_pos[g];
         const WalkNode nd = nodes[k];
         const uint2 rg = node_range[k];
-        if (rg.y > max_bodies) {   // too large to label exactly: marked, dropped by the host
-            if (lane == 0) targ[3 * p] = targ[3 * p + 1] = targ[3 * p + 2] = __builtin_nanf("");
+        // Too large to label exactly, or a node that holds the target itself
+        // (its monopole then counts the target's own mass, a residual of the
+        // pair, not of the node's far field): marked, dropped by the host.
+        if (rg.y > max_bodies || (g >= rg.x && g < rg.x + rg.y)) {
+            if (lane == 0)
+                for (int c = 0; c < kMlpTarget; c++) targ[(size_t)kMlpTarget * p + c] = __builtin_nanf("");
             continue;
         }
-        double ax = 0, ay = 0, az = 0;
+        const double cx = nd.comMass.x, cy = nd.comMass.y, cz = nd.comMass.z;
+        double a[3] = {0, 0, 0}, m2[6] = {0, 0, 0, 0, 0, 0};
         for (uint32_t j = rg.x + lane; j < rg.x + rg.y; j += kWave) {
-            if (j == g) continue;
             const float4 b = leaf_pos[j];
+            const double dx = (double)b.x - cx, dy = (double)b.y - cy, dz = (double)b.z - cz, m = b.w;
+            m2[0] += m * dx * dx;
+            m2[1] += m * dx * dy;
+            m2[2] += m * dx * dz;
+            m2[3] += m * dy * dy;
+            m2[4] += m * dy * dz;
+            m2[5] += m * dz * dz;
+            if (j == g) continue;
             const double rx = (double)b.x - x.x, ry = (double)b.y - x.y, rz = (double)b.z - x.z;
             const double d2 = rx * rx + ry * ry + rz * rz + (double)soft;
             const double inv = 1.0 / sqrt(d2);
-            const double f = G * (double)b.w * inv * inv * inv;
-            ax += f * rx;
-            ay += f * ry;
-            az += f * rz;
+            const double f = G * m * inv * inv * inv;
+            a[0] += f * rx;
+            a[1] += f * ry;
+            a[2] += f * rz;
         }
         for (int off = 32; off > 0; off >>= 1) {
-            ax += __shfl_xor(ax, off, 64);
-            ay += __shfl_xor(ay, off, 64);
-            az += __shfl_xor(az, off, 64);
+            for (int c = 0; c < 3; c++) a[c] += __shfl_xor(a[c], off, 64);
+            for (int c = 0; c < 6; c++) m2[c] += __shfl_xor(m2[c], off, 64);
         }
         if (lane != 0) continue;
-        const double rx = (double)nd.comMass.x - x.x, ry = (double)nd.comMass.y - x.y, rz = (double)nd.comMass.z - x.z;
-        const double ds2 = rx * rx + ry * ry + rz * rz + (double)soft;
-        const double ds = sqrt(ds2);
-        const float mass = node_aabb[2 * k + 1].w;                // (the record holds G M)
-        const double scale = ds2 / (G * (double)mass);           // 1 / (G M / d^2)
-        targ[3 * p + 0] = (float)(ax * scale - rx / ds);
-        targ[3 * p + 1] = (float)(ay * scale - ry / ds);
-        targ[3 * p + 2] = (float)(az * scale - rz / ds);
-        const float frx = nd.comMass.x - x.x, fry = nd.comMass.y - x.y, frz = nd.comMass.z - x.z;
-        const float inv = inv_sqrt_exact(fadd(dot3(frx, fry, frz), soft));
-        mlp_features(frx, fry, frz, inv, soft, mass, rg.y, node_aabb[2 * k], node_aabb[2 * k + 1],
-                     nd.comMass, feat + 12 * (size_t)p);
+        const double r[3] = {cx - x.x, cy - x.y, cz - x.z};
+        const double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + (double)soft;
+        const double s = sqrt(s2);
+        const double M = node_aabb[2 * k + 1].w;                  // (the record holds G M)
+        const double h = nd.half;
+        // monopole and exact quadrupole, double precision
+        const double i2 = 1.0 / s2, i3 = i2 / s, i5 = i3 * i2;
+        const double tr[3] = {m2[0] * r[0] + m2[1] * r[1] + m2[2] * r[2], m2[1] * r[0] + m2[3] * r[1] + m2[4] * r[2],
+                              m2[2] * r[0] + m2[4] * r[1] + m2[5] * r[2]};
+        const double rtr = r[0] * tr[0] + r[1] * tr[1] + r[2] * tr[2];
+        const double c = (7.5 * rtr * i2 - 1.5 * (m2[0] + m2[3] + m2[5])) * i5;
+        const double unit = G * M / s2 * (h * h / s2);            // (h / s)^2 G M / s^2
+        float* t = targ + (size_t)kMlpTarget * p;
+        for (int q = 0; q < 3; q++) {
+            const double mono = G * M * r[q] * i3;
+            const double quad = G * (c * r[q] - 3.0 * tr[q] * i5);
+            t[q] = (float)(r[q] / s);
+            t[3 + q] = (float)((a[q] - mono - quad) / unit);
+        }
+        t[6] = (float)unit;
+        double nm[6];
+        const double mh2 = M * h * h;
+        for (int q = 0; q < 6; q++) nm[q] = m2[q] / mh2;
+        node_features(node_aabb[2 * k], node_aabb[2 * k + 1], nd.comMass.x, nd.comMass.y, nd.comMass.z, nd.half, nm,
+                      feat + (size_t)kMlpIn * p);
     }
 }
 
@@ -177,16 +379,27 @@ __global__ void k_bias_act(float* __restrict__ Z, float* __restrict__ H, const f
     if (H) H[i] = z / (1.0f + expf(-z));
 }
 
-// dY = 2 (Y - T) / batch; per-block partial sums of the squared error
+// Loss through the quadrupole basis: row m has network outputs Y[m][6] and
+// target T[m] = (u (3), label (3), weight w); the prediction is B(u) Y[m],
+// the loss w times the squared error summed over its 3 components.
+// dY = 2 w B(u)^T (B(u) Y - label) / batch; per-block partial sums of the loss.
 __global__ void k_loss_grad(const float* __restrict__ Y, const float* __restrict__ T, float* __restrict__ dY,
                             int M, float* __restrict__ loss_partial) {
     __shared__ float s[256];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
     float e2 = 0.0f;
-    if (i < (size_t)M * kMlpOut) {
-        float e = Y[i] - T[i];
-        dY[i] = 2.0f * e / (float)M;
-        e2 = e * e;
+    if (m < M) {
+        const float* t = T + (size_t)m * kMlpTarget;
+        float o[6], pr[3], e[3], g[6];
+        for (int c = 0; c < 6; c++) o[c] = Y[(size_t)m * kMlpOut + c];
+        quad_basis(t, o, pr);
+        const float wgt = t[6];
+        for (int c = 0; c < 3; c++) {
+            e[c] = pr[c] - t[3 + c];
+            e2 += wgt * e[c] * e[c];
+        }
+        quad_basis_t(t, e, g);
+        for (int c = 0; c < 6; c++) dY[(size_t)m * kMlpOut + c] = 2.0f * wgt * g[c] / (float)M;
     }
     s[threadIdx.x] = e2;
     __syncthreads();
@@ -263,12 +476,12 @@ __global__ void k_pack_weights(const float* __restrict__ p, __bf16* __restrict__
 }
 
 // ------------------------------------------------------------------ MFMA inference
-// 64 samples per wave, 4 waves per block; x is [n][12] FP32, y is [n][3].
+// 64 samples per wave, 4 waves per block; x is [n][12] FP32, y is [n][6].
 __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restrict__ x, uint32_t n,
                                                           const __bf16* __restrict__ wg,
                                                           const float* __restrict__ bg, float* __restrict__ y) {
     __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
-    __shared__ __attribute__((aligned(16))) float s_y[4][64 * 4];
+    __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
     const int w = threadIdx.x >> 6, lane = (int)lane_id();
     MlpRegs R;
     mlp_load(R, wg, bg);
@@ -282,18 +495,17 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restric
     const uint32_t blocks = live >= 64 ? 0xFu : (1u << ((live + 15) / 16)) - 1u;
     mlp_wave_forward(R, s_x[w], s_y[w], blocks);
     if (row < n)
-        for (int j = 0; j < kMlpOut; j++) y[(size_t)row * 3 + j] = s_y[w][lane * 4 + j];
+        for (int j = 0; j < kMlpOut; j++) y[(size_t)row * kMlpOut + j] = s_y[w][lane * kMlpYStride + j];
 }
 
 // ------------------------------------------------------------------ launchers
 static inline uint32_t cdiv2(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, const float4* node_aabb, float theta, float theta_plain, float soft,
-                          uint32_t stride, uint2* pairs, uint32_t cap, uint32_t* count) {
+                          const WalkNode* nodes, float theta, float soft, uint32_t stride, uint2* pairs, uint32_t cap,
+                          uint32_t* count) {
     uint32_t threads = (n + stride - 1) / stride;
-    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, node_aabb, theta, theta_plain, soft,
-                                                        stride, pairs, cap, count);
+    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, theta, soft, stride, pairs, cap, count);
 }
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
@@ -337,7 +549,7 @@ void launch_bias_act(hipStream_t s, float* Z, float* H, const float* bias, int M
 }
 void launch_loss_grad(hipStream_t s, const float* Y, const float* T, float* dY, int M, float* partial,
                       uint32_t* nblocks) {
-    *nblocks = cdiv2((size_t)M * kMlpOut, 256);
+    *nblocks = cdiv2((size_t)M, 256);
     k_loss_grad<<<*nblocks, 256, 0, s>>>(Y, T, dY, M, partial);
 }
 void launch_act_grad(hipStream_t s, const float* Z, float* dH, size_t n) {

@@ -41,6 +41,8 @@ struct WalkAcc {
     float ax, ay, az;
     unsigned long long opened, inter;
     uint32_t maxsp, err;
+    float qx, qy, qz;   // quadrupole terms of the corrected walk, summed apart from the monopoles
+    unsigned long long quads;   // quadrupole terms evaluated
 };
 
 // a += G m r inv^3 in the reference's operation order (bh_force.comp:22-25,74,82)
@@ -49,6 +51,27 @@ __device__ __forceinline__ void add_force(float& ax, float& ay, float& az, float
     ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
     ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
     az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
+}
+
+// The softened quadrupole term of an accepted internal node (no reference
+// counterpart): q holds G T (xx, xy, xz, yy, yz, zz) and -1.5 G tr(T) of the
+// node's effective second-moment tensor T; with inv = 1 / s, s^2 = |r|^2 + eps^2,
+//     a_q = G T-term = (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5.
+// Written with explicit fused multiply-adds (the oracle evaluates the same
+// sequence), summed into a separate accumulator.
+__device__ __forceinline__ void add_quad(float& qx, float& qy, float& qz, const float* q, float rx, float ry, float rz,
+                                         float inv) {
+    const float i2 = inv * inv;
+    const float i5 = i2 * i2 * inv;
+    const float tx = __builtin_fmaf(q[0], rx, __builtin_fmaf(q[1], ry, q[2] * rz));
+    const float ty = __builtin_fmaf(q[1], rx, __builtin_fmaf(q[3], ry, q[4] * rz));
+    const float tz = __builtin_fmaf(q[2], rx, __builtin_fmaf(q[4], ry, q[5] * rz));
+    const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
+    const float c = __builtin_fmaf(7.5f * rtr, i2, q[6]) * i5;
+    const float m3 = -3.0f * i5;
+    qx = __builtin_fmaf(c, rx, __builtin_fmaf(m3, tx, qx));
+    qy = __builtin_fmaf(c, ry, __builtin_fmaf(m3, ty, qy));
+    qz = __builtin_fmaf(c, rz, __builtin_fmaf(m3, tz, qz));
 }
 
 // The overlap test of a leaf visit in the merging (1) and collision (2)
@@ -103,12 +126,12 @@ __device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float rx, 
 // 1 = leaf visits emit merge tasks, 2 = an overlapping leaf changes this
 // lane's body at once, velocity impulse and position correction, and the rest
 // of the walk and the integration use the changed body (bh_force.comp:85-112).
-template <int kDepth, uint32_t mode, bool kFast>
+template <int kDepth, uint32_t mode, bool kFast, bool kQuad = false>
 __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsigned long long* __restrict__ s_mask,
                                            uint32_t root, uint32_t n_ids, uint32_t base, uint32_t g,
                                            unsigned long long mask, float4& pm, float4& vd,
                                            const float4* __restrict__ leaf_pos, const WalkNode* __restrict__ nodes,
-                                           float G, float theta, float soft, float Ri,
+                                           const float4* __restrict__ qrec, float G, float theta, float soft, float Ri,
                                            const float* __restrict__ leaf_radius, uint32_t bodyIdx,
                                            float elasticity, float restitution,
                                            const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
@@ -118,7 +141,8 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     // the sums and counters live in locals for the loop (a struct carried
     // around the loop costs register copies on every iteration)
     float ax = o.ax, ay = o.ay, az = o.az;
-    unsigned long long opened = o.opened, inter = o.inter;
+    float qx = o.qx, qy = o.qy, qz = o.qz;
+    unsigned long long opened = o.opened, inter = o.inter, quads = o.quads;
     uint32_t maxsp = o.maxsp, err = o.err;
     // current node in scalar registers; the stack holds the pending childA
     // entries (after opening a node its childB, the top of the reference's
@@ -176,6 +200,14 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
         // an internal record holds G m; a leaf's position-mass holds m
         if (__builtin_amdgcn_inverse_ballot_w64(take))
             add_force(ax, ay, az, internal ? sp4.w : fmul(G, sp4.w), rx, ry, rz, inv);
+        if constexpr (kQuad) {
+            if (internal && take) {
+                const float4 q0 = qrec[2 * (node - n_ids)], q1 = qrec[2 * (node - n_ids) + 1];
+                const float q[7] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z};
+                if (__builtin_amdgcn_inverse_ballot_w64(take)) add_quad(qx, qy, qz, q, rx, ry, rz, inv);
+                quads += (unsigned long long)__popcll(take);
+            }
+        }
         inter += (unsigned long long)__popcll(take);
         if constexpr (mode != 0u)
             if (!internal && use_m)
@@ -210,6 +242,10 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     o.ax = ax;
     o.ay = ay;
     o.az = az;
+    o.qx = qx;
+    o.qy = qy;
+    o.qz = qz;
+    o.quads = quads;
     o.opened = opened;
     o.inter = inter;
     o.maxsp = maxsp;
@@ -234,16 +270,20 @@ __device__ __forceinline__ bool walk_lean_ok(float root_half, float soft, uint32
     return inv_sqrt_fast_domain(root_half, soft) && height <= kLeanHeight && num <= kLeanBodies;
 }
 
-template <uint32_t mode>
+template <uint32_t mode, bool kQuad = false>
 __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_t g, unsigned long long mask,
-                                          float4& pm, float4& vd, const WalkNode* __restrict__ nodes, float G,
+                                          float4& pm, float4& vd, const WalkNode* __restrict__ nodes,
+                                          const float4* __restrict__ qrec, float G,
                                           float theta, float soft, float Ri, const float* __restrict__ leaf_radius,
                                           uint32_t bodyIdx, float elasticity, float restitution,
                                           const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
                                           DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
                                           uint32_t task_cap, WalkAcc& o) {
     float ax = o.ax, ay = o.ay, az = o.az;
-    uint32_t opened = 0, inter = 0, maxsp = o.maxsp, err = o.err;
+    float qx = o.qx, qy = o.qy, qz = o.qz;
+    uint32_t opened = 0, inter = 0, quads = 0, maxsp = o.maxsp, err = o.err;
+    // quadrupole records of the internal nodes (32 bytes each, internal node k at k)
+    const char* __restrict__ qbase = reinterpret_cast<const char*>(qrec);
     // One record per node id, leaves included (the tree arrays keep the leaf
     // records right before the internal ones), addressed as a base plus a
     // 32-bit byte offset (ids < 2^27).
@@ -260,6 +300,11 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
         // one 32-byte scalar load (a vector type keeps the compiler from
         // splitting it and sinking the half side into a later dependent load)
         const u32x8 rec = *reinterpret_cast<const u32x8*>(wbase + (uint32_t)(node << 5));
+        // the quadrupole record, fetched alongside (a leaf reads internal node
+        // 0's record, never used)
+        u32x8 qr;
+        if constexpr (kQuad)
+            qr = *reinterpret_cast<const u32x8*>(qbase + (uint32_t)((node >= n_ids ? node - n_ids : 0u) << 5));
         WalkNode nd;
         nd.comMass = make_float4(__uint_as_float(rec[0]), __uint_as_float(rec[1]), __uint_as_float(rec[2]),
                                  __uint_as_float(rec[3]));
@@ -282,6 +327,15 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
             open = mask & ~accm;
             if (__builtin_amdgcn_inverse_ballot_w64(take))
                 add_force(ax, ay, az, nd.comMass.w, rx, ry, rz, inv);   // the record holds G m
+            if constexpr (kQuad) {
+                if (internal && take) {
+                    const float q[7] = {__uint_as_float(qr[0]), __uint_as_float(qr[1]), __uint_as_float(qr[2]),
+                                        __uint_as_float(qr[3]), __uint_as_float(qr[4]), __uint_as_float(qr[5]),
+                                        __uint_as_float(qr[6])};
+                    if (__builtin_amdgcn_inverse_ballot_w64(take)) add_quad(qx, qy, qz, q, rx, ry, rz, inv);
+                    quads += (uint32_t)__popcll(take);
+                }
+            }
             inter += (uint32_t)__popcll(take);
             if constexpr (mode != 0u)
                 if (!internal && take)
@@ -312,6 +366,10 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
     o.ax = ax;
     o.ay = ay;
     o.az = az;
+    o.qx = qx;
+    o.qy = qy;
+    o.qz = qz;
+    o.quads += quads;
     o.opened += opened;
     o.inter += inter;
     o.maxsp = maxsp;
@@ -322,16 +380,18 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
 struct WalkTotals {
     unsigned long long opened, inter;
     uint32_t maxsp, err, oob;
+    unsigned long long quads;
 };
 
 // Walk and integrate the 64 sorted bodies [base, base + 64) of one wave.
-template <uint32_t kMode>
+template <uint32_t kMode, bool kQuad>
 __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsigned long long* s_mask,
                                            SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                            const float4* __restrict__ src, float4* __restrict__ dst,
                                            const uint32_t* __restrict__ index_in,
                                            const float4* __restrict__ leaf_pos,
-                                           const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                           const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
+                                           uint2* __restrict__ tasks,
                                            uint32_t task_cap, float theta_, float dt_, float softening,
                                            uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                            const float* __restrict__ leaf_radius, uint32_t general,
@@ -347,21 +407,31 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
     const float G = sim->gravitationalConstant;
     const float Ri = (kMode != 0u && active) ? leaf_radius[g] : 0.0f;
 
-    WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
+    WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0.0f, 0.0f, 0.0f, 0ull};
     if (num > 1) {
         if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num))
-            walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
-                             elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+            walk_lean<kMode, kQuad>(n, n, g, ballot(active), pm, vd, nodes, qrec, G, theta_, softening, Ri,
+                                    leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
+                                    task_cap, o);
         else if (inv_sqrt_fast_domain(nodes[0].half, softening))
-            walk_chunk<kStackDepth, kMode, true>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
-                                                 leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
-                                                 elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+            walk_chunk<kStackDepth, kMode, true, kQuad>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
+                                                        leaf_pos, nodes, qrec, G, theta_, softening, Ri, leaf_radius,
+                                                        bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
+                                                        task_cap, o);
         else
-            walk_chunk<kStackDepth, kMode, false>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
-                                                  leaf_pos, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
-                                                  elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+            walk_chunk<kStackDepth, kMode, false, kQuad>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
+                                                         leaf_pos, nodes, qrec, G, theta_, softening, Ri, leaf_radius,
+                                                         bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
+                                                         task_cap, o);
     }
-    const float ax = o.ax, ay = o.ay, az = o.az;
+    float ax = o.ax, ay = o.ay, az = o.az;
+    // the quadrupole terms join the monopole sum (all-zero records leave the
+    // plain walk's bits, the sign of a zero included)
+    if constexpr (kQuad) {
+        if (o.qx != 0.0f) ax = fadd(ax, o.qx);
+        if (o.qy != 0.0f) ay = fadd(ay, o.qy);
+        if (o.qz != 0.0f) az = fadd(az, o.qz);
+    }
 
     // kick-drift into the output buffer (bh_force.comp:118-140)
     uint32_t oob = 0;
@@ -395,6 +465,7 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
     }
     tot.opened += o.opened;
     tot.inter += o.inter;
+    tot.quads += o.quads;
     tot.maxsp = o.maxsp > tot.maxsp ? o.maxsp : tot.maxsp;
     tot.err |= o.err;
     tot.oob += __popcll(ballot(oob != 0));
@@ -407,6 +478,7 @@ __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* c
     if (lane_id() == 0) {
         atomicAdd(&ctr->opened, t.opened);
         atomicAdd(&ctr->interactions, t.inter);
+        if (t.quads) atomicAdd(&ctr->quadTerms, t.quads);
         atomicMax(&ctr->maxStack, t.maxsp);
         if (oob_w) atomicAdd(&sim->outOfBounds, oob_w);
         if (err_w) atomicOr(&ctr->errorFlags, err_w);
@@ -418,12 +490,13 @@ __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* c
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
-template <uint32_t kMode, int kW>
+template <uint32_t kMode, int kW, bool kQuad>
 __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
-                                              const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                              const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
+                                              uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                               const float* __restrict__ leaf_radius, uint32_t general,
@@ -434,9 +507,10 @@ __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, D
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t base = (lb * kW + w) * kWave;
     if (base >= sim->numBodies) return;
-    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u};
-    walk_group<kMode>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, tasks, task_cap,
-                      theta_, dt_, softening, dynamic, wrap, elasticity, restitution, leaf_radius, general, pos_out, t);
+    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u, 0ull};
+    walk_group<kMode, kQuad>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, qrec, tasks,
+                             task_cap, theta_, dt_, softening, dynamic, wrap, elasticity, restitution, leaf_radius,
+                             general, pos_out, t);
     walk_totals_flush(sim, ctr, t);
 }
 
@@ -466,12 +540,13 @@ __global__ void k_walk_queue_reset(uint32_t* __restrict__ queues) {
 // to the nearest queue (by slot, then XCD) that still holds groups. A queue
 // that a wave found drained (by its atomic or by a load) is never tried again
 // by that wave, so each wave ends after at most kWalkQueues failed takes.
-template <uint32_t kMode>
+template <uint32_t kMode, bool kQuad>
 __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk_q(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
-                                              const WalkNode* __restrict__ nodes, uint2* __restrict__ tasks,
+                                              const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
+                                              uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                               const float* __restrict__ leaf_radius, uint32_t general,
@@ -483,16 +558,16 @@ __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu
     const uint32_t home = uniform(walk_home_queue());
     uint32_t q = home;
     uint32_t drained = 0;   // bit k: queue lane + 64 k is known to be drained
-    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u};
+    WalkTotals t = {0ull, 0ull, 1u, 0u, 0u, 0ull};
     for (;;) {
         uint32_t k = 0;
         if (lane == 0) k = atomicAdd(&queues[q * 32], 1u);
         k = uniform(__shfl(k, 0, 64));
         const uint32_t qb = walk_queue_begin(groups, q);
         if (k < walk_queue_begin(groups, q + 1) - qb) {
-            walk_group<kMode>((qb + k) * kWave, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes,
-                              tasks, task_cap, theta_, dt_, softening, dynamic, wrap, elasticity, restitution,
-                              leaf_radius, general, pos_out, t);
+            walk_group<kMode, kQuad>((qb + k) * kWave, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos,
+                                     nodes, qrec, tasks, task_cap, theta_, dt_, softening, dynamic, wrap, elasticity,
+                                     restitution, leaf_radius, general, pos_out, t);
             continue;
         }
         if (lane == (q & 63u)) drained |= 1u << (q >> 6);
@@ -524,30 +599,34 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
     const uint32_t nb_small = (n + kWalkSmallWaves * kWave - 1) / (kWalkSmallWaves * kWave);
-#define BH_WALK(M)                                                                                               \
-    k_walk<M, kWalkSmallWaves><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
-                                                a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
+#define BH_WALK(M, Q)                                                                                            \
+    k_walk<M, kWalkSmallWaves, Q><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+                                                a.leaf_pos, a.nodes, a.qrec, a.tasks, a.task_cap, a.theta, a.dt, \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                 a.leaf_radius, a.general, a.pos_out)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
-#define BH_WALK_Q(M)                                                                                             \
-    k_walk_q<M><<<nq, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
-                                                  a.leaf_pos, a.nodes, a.tasks, a.task_cap, a.theta, a.dt,         \
+#define BH_WALK_Q(M, Q)                                                                                          \
+    k_walk_q<M, Q><<<nq, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+                                                  a.leaf_pos, a.nodes, a.qrec, a.tasks, a.task_cap, a.theta, a.dt, \
                                                   a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                   a.leaf_radius, a.general, a.pos_out, a.queues)
+    // the corrected walk (quadrupole records) runs without merging or collisions
     if (kWalkPersistent && a.queues && n >= kWalkPersistentMin) {
         // two blocks of 16 waves fill a CU (8 waves per SIMD); blocks beyond
         // what is resident find the queues drained and end
         const uint32_t nq = std::min(nb, 2u * 256u);
         k_walk_queue_reset<<<1, kWalkQueues, 0, s>>>(a.queues);
-        if (a.merging == 1u) BH_WALK_Q(1u);
-        else if (a.merging == 2u) BH_WALK_Q(2u);
-        else BH_WALK_Q(0u);
-    } else if (a.merging == 1u) BH_WALK(1u);
-    else if (a.merging == 2u) BH_WALK(2u);
-    else BH_WALK(0u);
+        if (a.qrec) BH_WALK_Q(0u, true);
+        else if (a.merging == 1u) BH_WALK_Q(1u, false);
+        else if (a.merging == 2u) BH_WALK_Q(2u, false);
+        else BH_WALK_Q(0u, false);
+    } else if (a.qrec) BH_WALK(0u, true);
+    else if (a.merging == 1u) BH_WALK(1u, false);
+    else if (a.merging == 2u) BH_WALK(2u, false);
+    else BH_WALK(0u, false);
 #undef BH_WALK
+#undef BH_WALK_Q
 }
 
 // Ensemble walk (no reference counterpart): a persistent grid whose waves
@@ -589,18 +668,18 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             const uint32_t bodyIdx = active ? index_in[g] : 0;
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
-            WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u};
+            WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0.0f, 0.0f, 0.0f, 0ull};
             if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
-                walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, G, theta, soft, 0.0f, nullptr, bodyIdx,
-                              0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
+                walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, nullptr, G, theta, soft, 0.0f, nullptr,
+                              bodyIdx, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             else if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
                 walk_chunk<kDepth, 0u, true>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
-                                             nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src,
-                                             ctr, nullptr, 0u, o);
+                                             nodes, nullptr, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
+                                             index_in, src, ctr, nullptr, 0u, o);
             else
                 walk_chunk<kDepth, 0u, false>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
-                                              nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f, index_in, src,
-                                              ctr, nullptr, 0u, o);
+                                              nodes, nullptr, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
+                                              index_in, src, ctr, nullptr, 0u, o);
             if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
                 const float nvx = fadd(vd.x, fmul(o.ax, dt));
                 const float nvy = fadd(vd.y, fmul(o.ay, dt));
@@ -651,296 +730,6 @@ int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu) {
     BH_ENS_WALK_SHAPES(BH_OCC)
 #undef BH_OCC
     return 1;
-}
-
-}  // namespace bh
-
-// ---------------------------------------------------------------------------
-// The corrected walk (no reference counterpart): the same masked-stack walk
-// with a wide opening angle `theta`, where a node is accepted only if the
-// target also lies outside the node's box (a node that contains its target
-// has a residual the network cannot know from the node summary); an internal
-// node accepted at `theta` that the plain angle `theta_plain` would have
-// opened gets, besides its monopole, the network's residual:
-// a += G M / d^2 * MLP(features). The wave stages the 64 lanes' features in
-// LDS and evaluates them together with BF16 MFMA (FP32 accumulation).
-#include "bh_mlp.hpp"
-
-namespace bh {
-
-template <bool kLean>
-__global__ __launch_bounds__(kLearnedWaves * 64) void k_walk_learned(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                                      const float4* __restrict__ src, float4* __restrict__ dst,
-                                                      const uint32_t* __restrict__ index_in,
-                                                      const float4* __restrict__ leaf_pos,
-                                                      const WalkNode* __restrict__ nodes,
-                                                      const float4* __restrict__ node_aabb,
-                                                      const uint2* __restrict__ node_range,
-                                                      const __bf16* __restrict__ mlp_w,
-                                                      const float* __restrict__ mlp_b, float theta_, float theta_plain,
-                                                      float dt_, float softening, uint32_t general) {
-    // Two instantiations, launched one after the other: the lean loop, and
-    // the general loop with its LDS stack for the trees the lean loop does
-    // not take; each returns at once on the other's trees, and the lean one
-    // holds no stack in LDS, which lets more of its blocks share a CU.
-    const uint32_t n = sim->initialNumBodies;
-    const uint32_t num = sim->numBodies;
-    const bool lean = !general && num > 1 && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num);
-    if (lean != kLean) return;
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[kLearnedWaves][64 * kMlpXStride];
-    __shared__ __attribute__((aligned(16))) float s_y[kLearnedWaves][64 * 4];
-
-    // the network's fragments, for every wave of the block
-    __shared__ bf16x8 s_wf[kFrags * 64];
-    __shared__ f32x4 s_bias[kBiasTotal / 4];
-    for (uint32_t i = threadIdx.x; i < kFrags * 64; i += blockDim.x)
-        s_wf[i] = reinterpret_cast<const bf16x8*>(mlp_w)[i];
-    for (uint32_t i = threadIdx.x; i < kBiasTotal / 4; i += blockDim.x)
-        s_bias[i] = reinterpret_cast<const f32x4*>(mlp_b)[i];
-    __syncthreads();
-
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (lb * kLearnedWaves + w) * kWave;
-    if (base >= num) return;
-    const uint32_t g = base + lane;
-    const bool active = g < num;
-    const uint32_t bodyIdx = active ? index_in[g] : 0;
-    const float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
-    const float G = sim->gravitationalConstant;
-    const float theta = theta_, soft = softening;
-
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    unsigned long long opened = 0, inter = 0, corrected = 0;
-    uint32_t maxsp = 1, err = 0;
-
-    // Corrections are batched per wave: a pair to correct stages its
-    // features in row `slot` of the wave's queue and remembers (slot, G m /
-    // d^2); a flush runs the network on the queued rows and every lane adds
-    // its own pairs' corrections, in the order it queued them, to a separate
-    // sum that joins the acceleration at the end.
-    constexpr uint32_t kPend = 4;
-    uint32_t qn = 0;                              // queued rows (wave-uniform)
-    uint32_t pend_n = 0, pend_slot[kPend];
-    float pend_s2[kPend];
-    float cax = 0.0f, cay = 0.0f, caz = 0.0f;
-    auto flush = [&]() {
-        if (qn == 0) return;
-        const uint32_t nb = (qn + 15u) >> 4;      // 16-sample blocks holding queued rows
-        mlp_wave_forward_lds(s_wf, s_bias, s_x[w], s_y[w], (1u << nb) - 1u);
-        for (uint32_t i = 0; i < kPend; i++)
-            if (i < pend_n) {
-                const float* y = &s_y[w][pend_slot[i] * 4];
-                cax += pend_s2[i] * y[0];
-                cay += pend_s2[i] * y[1];
-                caz += pend_s2[i] * y[2];
-            }
-        pend_n = 0;
-        qn = 0;
-    };
-
-    if constexpr (kLean) {
-        // walk_lean's loop (one record array for leaves and internal nodes,
-        // stack in lanes, branch-free push/pop, 32-bit wave counters) with
-        // the box-aware acceptance and the queued corrections
-        const char* __restrict__ wbase = reinterpret_cast<const char*>(nodes - n);
-        auto record = [&](uint32_t id) { return *reinterpret_cast<const u32x8*>(wbase + (uint32_t)(id << 5)); };
-        const float open_k = fmul(fmul(theta, theta), 1.0009766f);
-        int st_node = 0, st_lo = 0, st_hi = 0;
-        uint32_t node = n, op32 = 0, in32 = 0, co32 = 0;
-        unsigned long long mask = ballot(active);
-        int sp = 0;
-        for (;;) {
-            const u32x8 rec = record(node);
-            const bool internal = node >= n;
-            const uint32_t k = internal ? node - n : 0u;   // a leaf reads node 0's box, unused
-            const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
-            const float4 com = make_float4(__uint_as_float(rec[0]), __uint_as_float(rec[1]),
-                                           __uint_as_float(rec[2]), __uint_as_float(rec[3]));
-            const float half = __uint_as_float(rec[4]), half2 = __uint_as_float(rec[7]);
-            const float rx = fsub(com.x, pm.x);
-            const float ry = fsub(com.y, pm.y);
-            const float rz = fsub(com.z, pm.z);
-            const float d2 = fadd(dot3(rx, ry, rz), soft);
-            unsigned long long open = mask;
-            if (ballot(!(fmul(d2, open_k) <= half2)) & mask) {
-                const float inv = inv_sqrt_fast(d2);
-                const float ratio = fmul(half, inv);
-                const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y &&
-                                    pm.z >= mn.z && pm.z <= mx.z;
-                const bool accept = ratio < theta && !inside;
-                const unsigned long long accm = internal ? ballot(accept) : ~0ull;
-                const unsigned long long take = accm & mask & ballot(node != g);   // self skip at a leaf
-                open = mask & ~accm;
-                if (__builtin_amdgcn_inverse_ballot_w64(take))
-                    add_force(ax, ay, az, com.w, rx, ry, rz, inv);   // the record holds G m
-                in32 += (uint32_t)__popcll(take);
-                const unsigned long long corr = internal ? ballot(accept && ratio >= theta_plain) & mask : 0ull;
-                if (corr) {
-                    const uint32_t nc = (uint32_t)__popcll(corr);
-                    if (qn + nc > 64u || ballot(((corr >> lane) & 1ull) && pend_n == kPend)) flush();
-                    if ((corr >> lane) & 1ull) {
-                        float f[kMlpIn];
-                        mlp_features(rx, ry, rz, inv, soft, mx.w, node_range[k].y, mn, mx, com, f);
-                        const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
-                        mlp_stage_row(s_x[w], (int)slot, f);
-                        pend_slot[pend_n] = slot;
-                        pend_s2[pend_n] = com.w * inv * inv;
-                        pend_n++;
-                    }
-                    qn += nc;
-                    co32 += nc;
-                }
-            }
-            const int top = sp - 1;
-            const uint32_t pop_node = (uint32_t)__builtin_amdgcn_readlane(st_node, top);
-            const unsigned long long pop_mask =
-                ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, top) << 32) |
-                (uint32_t)__builtin_amdgcn_readlane(st_lo, top);
-            st_node = lane_write((int)rec[5], sp, st_node);
-            st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
-            st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
-            op32 += (uint32_t)__popcll(open);
-            const bool descend = open != 0ull;
-            node = descend ? rec[6] : pop_node;
-            mask = descend ? open : pop_mask;
-            sp = descend ? sp + 1 : top;
-            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
-            if (sp < 0) break;
-        }
-        opened = op32;
-        inter = in32;
-        corrected = co32;
-    } else if (num > 1) {
-        __shared__ uint32_t s_node[kLearnedWaves][kStackDepth];
-        __shared__ unsigned long long s_mask[kLearnedWaves][kStackDepth];
-        uint32_t node = n;
-        unsigned long long mask = ballot(active);
-        int sp = 0;
-        for (;;) {
-            const bool mine = (mask >> lane) & 1ull;
-            bool descend = false;
-            if (node >= n) {
-                const WalkNode nd = nodes[node - n];
-                const float rx = fsub(nd.comMass.x, pm.x);
-                const float ry = fsub(nd.comMass.y, pm.y);
-                const float rz = fsub(nd.comMass.z, pm.z);
-                const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = inv_sqrt_exact(d2);
-                const float ratio = fmul(nd.half, inv);
-                const uint32_t k = node - n;
-                const float4 mn = node_aabb[2 * k], mx = node_aabb[2 * k + 1];
-                const bool inside = pm.x >= mn.x && pm.x <= mx.x && pm.y >= mn.y && pm.y <= mx.y && pm.z >= mn.z &&
-                                    pm.z <= mx.z;
-                const bool accept = ratio < theta && !inside;
-                const bool correct = accept && ratio >= theta_plain;
-                const unsigned long long open = ballot(mine && !accept);
-                const unsigned long long corr = ballot(mine && correct);
-                if (corr) {
-                    // queue the pairs to correct; the network runs on full
-                    // queues (64 samples, or a lane with kPend pairs waiting)
-                    const uint32_t nc = (uint32_t)__popcll(corr);
-                    if (qn + nc > 64u || ballot(((corr >> lane) & 1ull) && pend_n == kPend)) flush();
-                    if (mine && correct) {
-                        const uint32_t cnt = node_range[k].y;
-                        float f[kMlpIn];
-                        mlp_features(rx, ry, rz, inv, soft, mx.w, cnt, mn, mx, nd.comMass, f);
-                        const uint32_t slot = qn + (uint32_t)__popcll(corr & lanemask_lt());
-                        mlp_stage_row(s_x[w], (int)slot, f);
-                        pend_slot[pend_n] = slot;
-                        pend_s2[pend_n] = nd.comMass.w * inv * inv;
-                        pend_n++;
-                    }
-                    qn += nc;
-                    corrected += (unsigned long long)nc;
-                }
-                if (mine && accept) {
-                    const float gm = nd.comMass.w;   // the record holds G m
-                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                }
-                inter += (unsigned long long)__popcll(mask & ~open);
-                if (open) {
-                    if (sp + 1 > kStackDepth) {
-                        err |= 1u;
-                        break;
-                    }
-                    if (lane == 0) {
-                        s_node[w][sp] = nd.childA;
-                        s_mask[w][sp] = open;
-                    }
-                    sp += 1;
-                    opened += (unsigned long long)__popcll(open);
-                    if ((uint32_t)sp + 1 > maxsp) maxsp = (uint32_t)sp + 1;
-                    node = uniform(nd.childB);
-                    mask = open;
-                    descend = true;
-                }
-            } else {
-                const float4 lp = leaf_pos[node];
-                const bool use = mine && node != g;
-                const float rx = fsub(lp.x, pm.x);
-                const float ry = fsub(lp.y, pm.y);
-                const float rz = fsub(lp.z, pm.z);
-                const float d2 = fadd(dot3(rx, ry, rz), soft);
-                const float inv = inv_sqrt_exact(d2);
-                if (use) {
-                    const float gm = fmul(G, lp.w);
-                    ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
-                    ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
-                    az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-                }
-                const unsigned long long selfbit =
-                    (node >= base && node < base + kWave) ? (1ull << (node - base)) : 0ull;
-                inter += (unsigned long long)__popcll(mask & ~selfbit);
-            }
-            if (!descend) {
-                if (sp == 0) break;
-                sp--;
-                node = uniform(s_node[w][sp]);
-                const unsigned long long mraw = s_mask[w][sp];
-                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
-            }
-        }
-    }
-    flush();
-    // the corrections' sum joins the monopole sum (a zero correction leaves
-    // the walk's own bits, the sign of a zero included)
-    if (cax != 0.0f) ax += cax;
-    if (cay != 0.0f) ay += cay;
-    if (caz != 0.0f) az += caz;
-    if (active) {
-        const float dt = dt_;
-        const float nvx = fadd(vd.x, fmul(ax, dt));
-        const float nvy = fadd(vd.y, fmul(ay, dt));
-        const float nvz = fadd(vd.z, fmul(az, dt));
-        dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)), fadd(pm.z, fmul(nvz, dt)),
-                                       pm.w);
-        dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
-    }
-    const uint32_t err_w = (ballot(err & 1u) ? 1u : 0u);
-    if (lane == 0) {
-        atomicAdd(&ctr->opened, opened);
-        atomicAdd(&ctr->interactions, inter);
-        atomicAdd(&ctr->corrected, corrected);
-        atomicMax(&ctr->maxStack, maxsp);
-        if (err_w) atomicOr(&ctr->errorFlags, err_w);
-        atomicAdd(&sim->uintDebug[0], (uint32_t)opened);
-    }
-}
-
-void launch_walk_learned(hipStream_t s, uint32_t n, const WalkArgs& a, const float4* node_aabb,
-                         const uint2* node_range, const __bf16* w, const float* bias, float theta_plain) {
-    const uint32_t per_block = kLearnedWaves * kWave;
-    const uint32_t nb = (n + per_block - 1) / per_block;
-    k_walk_learned<true><<<nb, kLearnedWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
-                                                            a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
-                                                            bias, a.theta, theta_plain, a.dt, a.softening, a.general);
-    k_walk_learned<false><<<nb, kLearnedWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst,
-                                                             a.index_in, a.leaf_pos, a.nodes, node_aabb, node_range, w,
-                                                             bias, a.theta, theta_plain, a.dt, a.softening, a.general);
 }
 
 }  // namespace bh

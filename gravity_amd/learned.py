@@ -1,12 +1,17 @@
 """Host side of the learned far-field correction (no reference counterpart).
 
-A 12-64-64-3 SiLU network predicts, for a (target body, tree node) pair that
-the wide opening angle accepts but the plain angle would open, the residual
-between the node's exact field and its monopole in units of G M / d^2. This
-module wraps the ``bh_mlp_*`` and ``bh_collect_training_pairs`` entry points
-of ``include/bh_abi.h``: pair generation with FP64 labels on the GPU, FP32
-training with Adam on the GPU, BF16 MFMA inference, and attaching the network
-to a simulation so that ``BH_FLAG_LEARNED`` steps use it.
+The corrected walk adds, for every accepted internal node, the softened
+quadrupole of T = M2 + M h^2 o: M2 the node's exact second moment about its
+centre of mass, h half its longest side, and o the output of a 12-64-64-6
+SiLU network evaluated once per node and step from the node's shape (AABB
+extents, centre-of-mass offset, normalised second moment). The network learns
+the residual of the node's exact field after its monopole and exact
+quadrupole. This module wraps the ``bh_mlp_*`` and
+``bh_collect_training_pairs`` entry points of ``include/bh_abi.h``: pair
+generation with FP64 labels on the GPU, FP32 training with Adam on the GPU
+through the quadrupole basis, BF16 MFMA inference, and attaching the network
+to a simulation so that ``BH_FLAG_LEARNED`` steps use it
+(``BH_FLAG_QUADRUPOLE`` is the same walk without a network).
 """
 from __future__ import annotations
 
@@ -17,7 +22,8 @@ import numpy as np
 from . import barnes_hut as B
 
 N_FEATURES = 12
-N_OUT = 3
+N_OUT = 6       # the residual tensor o: xx, xy, xz, yy, yz, zz
+N_TARGET = 7    # a training row's target: u = r / s (3), the scaled residual (3), the row's weight
 
 
 def _declare(L):
@@ -34,9 +40,9 @@ def _declare(L):
     L.bh_mlp_loss_grad.argtypes = [P, P, P, C.c_uint32, P, C.POINTER(C.c_float)]
     L.bh_mlp_train.argtypes = [P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
                                C.POINTER(C.c_float)]
-    L.bh_collect_training_pairs.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_float, C.c_uint32,
-                                            C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]
-    L.bh_attach_correction.argtypes = [P, P, C.c_float]
+    L.bh_collect_training_pairs.argtypes = [P, C.c_float, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, P, P, P,
+                                            C.POINTER(C.c_uint32)]
+    L.bh_attach_correction.argtypes = [P, P]
 
 
 def _lib():
@@ -60,7 +66,7 @@ def init_params(seed: int = 0) -> np.ndarray:
     rng = np.random.default_rng(seed)
     p = np.zeros(num_params(), np.float32)
     o = 0
-    for fan_out, fan_in, zero in ((64, 12, False), (64, 64, False), (3, 64, True)):
+    for fan_out, fan_in, zero in ((64, 12, False), (64, 64, False), (N_OUT, 64, True)):
         w = np.zeros((fan_out, fan_in)) if zero else rng.normal(0, np.sqrt(2.0 / fan_in), (fan_out, fan_in))
         p[o:o + fan_out * fan_in] = w.ravel()
         o += fan_out * fan_in + fan_out
@@ -126,21 +132,30 @@ class Mlp:
         return float(loss.value)
 
 
-def collect_training_pairs(bh: B.BarnesHut, theta: float, theta_plain: float, stride: int, softening: float,
+def collect_training_pairs(bh: B.BarnesHut, theta: float, stride: int, softening: float,
                            max_node_bodies: int = 1 << 20, cap: int = 1 << 22, with_pairs: bool = False):
-    """Features and FP64-labelled targets of the pairs on bh's current tree
-    (and the (sorted body, internal node) pairs). When the walkers at
-    `stride` find more than `cap` pairs, the stride is raised until they fit,
-    so the sample always spans the whole sorted order. Pairs whose node is
-    too large to label are dropped."""
+    """Features and FP64-labelled targets of the (body, accepted node) pairs
+    of the walk at `theta` on bh's current tree (and the (sorted body,
+    internal node) pairs). When the walkers at `stride` find more than `cap`
+    pairs, the stride is raised until they fit, so the sample always spans the
+    whole sorted order. Pairs whose node is too large to label, whose node
+    holds the target itself (its residual is the target's own mass, not the
+    node's far field), or whose label is not finite (a node of coincident
+    bodies) are dropped.
+
+    Each row's weight (target column 6) is (unit / |a|)^2, clipped at 1000
+    times its median and normalised to mean 1: unit = (h / s)^2 G M / s^2 is
+    the scale of the node's correction and |a| the target's total acceleration
+    (FP64 direct sum), so the loss measures the rows' errors relative to the
+    force they perturb."""
     L = _lib()
     f = np.zeros((cap, N_FEATURES), np.float32)
-    t = np.zeros((cap, N_OUT), np.float32)
+    t = np.zeros((cap, N_TARGET), np.float32)
     pr = np.zeros((cap, 2), np.uint32)
     n = C.c_uint32()
     for _ in range(8):
-        rc = L.bh_collect_training_pairs(bh.ctx, theta, theta_plain, stride, softening, max_node_bodies, cap,
-                                         _ptr(f), _ptr(t), _ptr(pr), C.byref(n))
+        rc = L.bh_collect_training_pairs(bh.ctx, theta, stride, softening, max_node_bodies, cap, _ptr(f), _ptr(t),
+                                         _ptr(pr), C.byref(n))
         bh._check(rc, "collect_training_pairs")
         if n.value <= cap:
             break
@@ -149,12 +164,61 @@ def collect_training_pairs(bh: B.BarnesHut, theta: float, theta_plain: float, st
         raise RuntimeError("collect_training_pairs: pairs do not fit in cap")
     f, t, pr = f[: n.value], t[: n.value], pr[: n.value]
     keep = np.isfinite(t).all(axis=1) & np.isfinite(f).all(axis=1)
+    f, t, pr = f[keep], t[keep], pr[keep]
+    # |a| of every target (sorted positions -> body ids of the current order)
+    targets, inverse = np.unique(pr[:, 0], return_inverse=True)
+    ids = bh.index()[targets]
+    a = np.linalg.norm(bh.direct_sum_f64(ids, softening), axis=1)
+    w = (t[:, 6].astype(np.float64) / a[inverse]) ** 2
+    if w.size:
+        w = np.minimum(w, 1000.0 * np.median(w))
+        t[:, 6] = (w / w.mean()).astype(np.float32)
     if with_pairs:
-        return f[keep], t[keep], pr[keep]
-    return f[keep], t[keep]
+        return f, t, pr
+    return f, t
 
 
-def attach(bh: B.BarnesHut, mlp: Mlp | None, theta_plain: float) -> None:
+def attach(bh: B.BarnesHut, mlp: Mlp | None) -> None:
     L = _lib()
-    bh._check(L.bh_attach_correction(bh.ctx, mlp.h if mlp is not None else None, theta_plain), "attach_correction")
+    bh._check(L.bh_attach_correction(bh.ctx, mlp.h if mlp is not None else None), "attach_correction")
     bh._correction = mlp   # keep the network alive while attached
+
+
+def scaled_params(params: np.ndarray, alpha: float) -> np.ndarray:
+    """The parameters with the last layer (W3, b3) scaled by alpha: the
+    network's residual shrunk by alpha (0: the exact quadrupole)."""
+    p = np.array(params, dtype=np.float32, copy=True)
+    p[-(N_OUT * 64 + N_OUT):] *= np.float32(alpha)
+    return p
+
+
+def calibrate(mlp: Mlp, bodies: np.ndarray, theta: float, softening: float, n_sample: int = 4096,
+              alphas=(0.0, 0.25, 0.5, 0.75, 1.0), seed: int = 0) -> tuple[float, list]:
+    """Choose the output scale alpha on a validation snapshot: the alpha whose
+    corrected walk at `theta` has the lowest p99 force error against the FP64
+    direct sum (then p50). The network is left with that scale. Returns
+    (alpha, [(alpha, p50, p99), ...])."""
+    base = mlp.params()
+    bh = B.BarnesHut(bodies, _bounds(bodies), B.make_params(theta=theta, softening=softening))
+    bh.init()
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(bodies.shape[0], min(n_sample, bodies.shape[0]), replace=False)).astype(np.uint32)
+    exact = bh.direct_sum_f64(sample, softening)
+    attach(bh, mlp)
+    rows = []
+    for a in alphas:
+        mlp.set_params(scaled_params(base, a))
+        acc = bh.accelerations(B.make_params(theta=theta, softening=softening, flags=B.FLAG_LEARNED))
+        e = np.linalg.norm(acc[sample].astype(np.float64) - exact, axis=1) / np.linalg.norm(exact, axis=1)
+        rows.append((float(a), float(np.percentile(e, 50)), float(np.percentile(e, 99))))
+    attach(bh, None)
+    bh.close()
+    best = min(rows, key=lambda r: (r[2], r[1]))
+    mlp.set_params(scaled_params(base, best[0]))
+    return best[0], rows
+
+
+def _bounds(bodies: np.ndarray) -> np.ndarray:
+    alive = bodies[:, 3] != 0
+    p = bodies[alive, 0:3]
+    return np.concatenate([p.min(axis=0), p.max(axis=0)]).astype(np.float32)

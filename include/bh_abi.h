@@ -54,6 +54,11 @@ extern "C" {
 #define BH_BINDING_RADIX_BUCKET_TOTALS 13
 #define BH_BINDING_MERGE_TASKS 14
 #define BH_BINDING_MERGE_BODY_LOCKS 15
+/* no reference counterpart: the corrected walk's quadrupole records of the
+ * internal nodes, 32 bytes each (float G T_xx, T_xy, T_xz, T_yy, T_yz, T_zz,
+ * -1.5 G tr T, 0), written by bh_build_tree under BH_FLAG_QUADRUPOLE or
+ * BH_FLAG_LEARNED and read by bh_compute_force under the same flags */
+#define BH_BINDING_NODE_QUADRUPOLES 16
 
 /* mergingCollisionOrNeither selector (common/common.glsl:136-138,
  * settings/defaultProperties.json:64-70) and staticOrDynamic (:71-77) */
@@ -66,12 +71,15 @@ extern "C" {
 /* bh_params.flags */
 #define BH_FLAG_COLORS 1u      /* compute leaf star colours and node avgColor (render interop) */
 #define BH_FLAG_TIMING 2u      /* record per-stage HIP-event timings */
-#define BH_FLAG_LEARNED 4u     /* corrected walk: nodes are accepted at theta only with the target outside
-                                  their box; the attached network corrects those that theta_plain would
-                                  open (no reference counterpart) */
+#define BH_FLAG_LEARNED 4u     /* corrected walk with the attached network: every accepted internal node adds
+                                  the softened quadrupole of T = M2 + M h^2 net(node), M2 its exact second
+                                  moment about its centre of mass, h half its longest side, net the
+                                  network's residual (no reference counterpart) */
 #define BH_FLAG_GENERAL_WALK 8u /* run the walk's general loop (LDS stack, overflow check, IEEE inverse
                                    square root outside its fast domain) even where the shorter loop
                                    applies; same results (for tests of both loops) */
+#define BH_FLAG_QUADRUPOLE 16u /* corrected walk with the exact quadrupole (T = M2, no network); the
+                                  opening test stays the reference's (no reference counterpart) */
 
 #define BH_WALK_STACK_DEPTH 192
 
@@ -109,7 +117,8 @@ typedef struct {
     uint32_t max_stack;           /* deepest walk stack of the last walk */
     uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow */
     uint32_t reserved;
-    uint64_t corrected;           /* network evaluations of the last corrected walk */
+    uint64_t corrected;           /* network evaluations (one per internal node) of the last tree build */
+    uint64_t quad_terms;          /* quadrupole terms added by the last corrected walk (accepted internal nodes) */
 } bh_counters;
 
 typedef struct bh_context bh_context;
@@ -190,11 +199,19 @@ int bh_direct_sum_f64_noalloc(bh_context *ctx, const uint32_t *sample, uint32_t 
                               double *accel_out);
 
 /* ---- learned far-field correction (no reference counterpart) ----
- * A 12-64-64-3 SiLU network predicts, for a (target, node) pair accepted by
- * the wide opening angle, the residual between the node's exact field and its
- * monopole in units of G M / d^2. Parameters: FP32 master copy (5,187 values,
- * order W1[64][12], b1, W2[64][64], b2, W3[3][64], b3); training uses FP32
- * GEMM kernels and Adam; the corrected walk evaluates it with BF16 MFMA. */
+ * A 12-64-64-6 SiLU network predicts, once per internal node and step, a
+ * residual o (6 values: xx, xy, xz, yy, yz, zz) to the node's quadrupole
+ * tensor: T = M2 + M h^2 o. Its 12 inputs describe the node: AABB extents /
+ * (2h) (3), (centre of mass - box centre) / h (3), M2 / (M h^2) (6). The
+ * target of a training row is (u, y, w): u = r / s (3) of a (target body,
+ * node) pair with r = com - x, s^2 = |r|^2 + softening, y (3) the residual of
+ * the node's exact field after its monopole and exact quadrupole in units of
+ * (h / s)^2 G M / s^2, and w the row's weight; the network's prediction for
+ * the row is B(u) o = -3 o u - 1.5 tr(o) u + 7.5 (u.o.u) u, and the loss
+ * w |B(u) o - y|^2 averaged over rows.
+ * Parameters: FP32 master copy (5,382 values, order W1[64][12], b1,
+ * W2[64][64], b2, W3[6][64], b3); training uses FP32 GEMM kernels and Adam;
+ * the tree stage evaluates it with BF16 MFMA. */
 typedef struct bh_mlp bh_mlp;
 int bh_mlp_create(int device, bh_mlp **out);
 void bh_mlp_destroy(bh_mlp *m);
@@ -202,29 +219,27 @@ const char *bh_mlp_last_error(const bh_mlp *m);
 uint32_t bh_mlp_num_params(void);
 int bh_mlp_set_params(bh_mlp *m, const float *params, uint32_t n);
 int bh_mlp_get_params(bh_mlp *m, float *params, uint32_t n);
-/* y[rows][3] = network(x[rows][12]); use_mfma = 0: FP32 kernels, 1: BF16 MFMA (the walk's routine) */
+/* y[rows][6] = network(x[rows][12]); use_mfma = 0: FP32 kernels, 1: BF16 MFMA (the tree stage's routine) */
 int bh_mlp_forward(bh_mlp *m, const float *x, uint32_t rows, float *y, int use_mfma);
-/* mean over rows of the squared error summed over the 3 outputs, and its gradient (FP32) */
+/* the loss above for targets t[rows][7] = (u, y, w), and its gradient (FP32) */
 int bh_mlp_loss_grad(bh_mlp *m, const float *x, const float *t, uint32_t rows, float *grad, float *loss);
 /* Adam (beta 0.9/0.999) over shuffled minibatches; loss of the last minibatch */
 int bh_mlp_train(bh_mlp *m, const float *x, const float *t, uint32_t rows, uint32_t epochs, uint32_t batch,
                  float lr, uint64_t seed, float *final_loss);
 /* On the context's current tree (after bh_build_tree): every stride-th sorted
- * body walks with angle theta; pairs with a node accepted at theta (target
- * outside the node's box) and opened at theta_plain get FP32 features and
- * FP64-labelled targets (nodes holding more than max_node_bodies bodies get
- * NaN targets). n_out = pairs found; when it exceeds cap only the first cap
- * (in no particular order) are written, and a caller that wants an unbiased
- * sample raises stride and calls again.
- * Features: extents / d (3), (COM - box centre) / d (3), r / d (3),
- * ln(mass) / 16, 1 / sqrt(bodies in node), softening / d^2, with
- * d^2 = |r|^2 + softening and r = COM - target. */
-int bh_collect_training_pairs(bh_context *ctx, float theta, float theta_plain, uint32_t stride, float softening,
+ * body walks with the reference's opening test at theta; every internal node
+ * it accepts makes a pair with FP32 node features [12] and an FP64-labelled
+ * target [7] = (u, y, unit) as above, unit = (h / s)^2 G M / s^2 (a caller
+ * turns it into the row's weight; nodes holding more than max_node_bodies bodies,
+ * and nodes holding the target body itself, get NaN targets). n_out = pairs found; when it exceeds cap only the first
+ * cap (in no particular order) are written, and a caller that wants an
+ * unbiased sample raises stride and calls again. */
+int bh_collect_training_pairs(bh_context *ctx, float theta, uint32_t stride, float softening,
                               uint32_t max_node_bodies, uint32_t cap, float *features, float *targets,
                               uint32_t *pairs_out /* optional: (sorted body, internal node) */, uint32_t *n_out);
 /* Attach (or with m = NULL detach) the network used under BH_FLAG_LEARNED;
  * it must outlive the steps that use it. */
-int bh_attach_correction(bh_context *ctx, const bh_mlp *m, float theta_plain);
+int bh_attach_correction(bh_context *ctx, const bh_mlp *m);
 
 /* Self-test of the walk's arithmetic: counts, over every float of the walk's
  * fast domain, the results of its correctly rounded sqrt [0], reciprocal [1]

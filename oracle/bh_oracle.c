@@ -396,16 +396,40 @@ static float body_radius(const float *b, float blc) {   /* common/common.glsl:22
     return blc * powf(b[3] / b[7], 1.0f / 3.0f);
 }
 
-/* bh_force.comp:40-142 for the body at sorted position g */
+/* The quadrupole term of an accepted internal node in the corrected walk (no
+ * reference counterpart; the GPU kernel evaluates the same sequence): Q holds
+ * G T (xx, xy, xz, yy, yz, zz) and -1.5 G tr T of the node's tensor T; with
+ * inv = 1 / s, q += (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5.
+ * Pinned by tests/test_oracle_quadrupole.py: the error of monopole + this term
+ * against the exact field of a cluster falls as the cube of its size over
+ * distance (one order faster than the monopole alone). */
+static void add_quad(float *q, const float *Q, const float *r, float inv) {
+    float i2 = inv * inv;
+    float i5 = i2 * i2 * inv;
+    float tx = fmaf(Q[0], r[0], fmaf(Q[1], r[1], Q[2] * r[2]));
+    float ty = fmaf(Q[1], r[0], fmaf(Q[3], r[1], Q[4] * r[2]));
+    float tz = fmaf(Q[2], r[0], fmaf(Q[4], r[1], Q[5] * r[2]));
+    float rtr = fmaf(r[0], tx, fmaf(r[1], ty, r[2] * tz));
+    float c = fmaf(7.5f * rtr, i2, Q[6]) * i5;
+    float m3 = -3.0f * i5;
+    q[0] = fmaf(c, r[0], fmaf(m3, tx, q[0]));
+    q[1] = fmaf(c, r[1], fmaf(m3, ty, q[1]));
+    q[2] = fmaf(c, r[2], fmaf(m3, tz, q[2]));
+}
+
+/* bh_force.comp:40-142 for the body at sorted position g; with quad (8 floats
+ * per internal node) the corrected walk: accepted internal nodes also add
+ * their quadrupole term, summed apart and joined to the sum at the end */
 static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *index, uint32_t n_init,
                      const bho_node *leaf, const bho_node *internal, const bho_params *p,
                      float G, float blc, const float *bounds, uint32_t *stack, int stack_cap,
                      float *accel_out, uint32_t *inter_out, uint32_t *tasks, uint32_t task_cap,
-                     uint32_t *n_tasks, bho_walk_stats *st, int box_mac) {
+                     uint32_t *n_tasks, bho_walk_stats *st, const float *quad) {
     uint32_t bodyIdx = index[g];
     float body[8];
     memcpy(body, src + (size_t)bodyIdx * 8, 32);
     float acc[3] = {0.0f, 0.0f, 0.0f};
+    float q[3] = {0.0f, 0.0f, 0.0f};
     uint32_t sp = 0;
     uint32_t opened = 0, inter = 0, tests = 0, maxsp = 1;
     stack[sp++] = n_init;
@@ -419,11 +443,10 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
             tests++;
             float ex = nd->aabb[3] - nd->aabb[0], ey = nd->aabb[4] - nd->aabb[1], ez = nd->aabb[5] - nd->aabb[2];
             float longest = gmax(ex, gmax(ey, ez)) / 2.0f;
-            int inside = box_mac && body[0] >= nd->aabb[0] && body[0] <= nd->aabb[3] && body[1] >= nd->aabb[1] &&
-                         body[1] <= nd->aabb[4] && body[2] >= nd->aabb[2] && body[2] <= nd->aabb[5];
-            if (longest * inv < p->theta && !inside) {
+            if (longest * inv < p->theta) {
                 float gm = G * nd->comMass[3];
                 for (int k = 0; k < 3; k++) acc[k] = acc[k] + gm * r[k] * inv * inv * inv;
+                if (quad) add_quad(q, quad + 8 * (size_t)(nodeIdx - n_init), r, inv);
                 inter++;
             } else {
                 if ((int)sp + 2 > stack_cap) { st->max_stack = 0xFFFFFFFFu; break; }
@@ -464,6 +487,9 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
             }
         }
     }
+    if (quad)
+        for (int k = 0; k < 3; k++)
+            if (q[k] != 0.0f) acc[k] = acc[k] + q[k];
     if (accel_out) { accel_out[3 * g] = acc[0]; accel_out[3 * g + 1] = acc[1]; accel_out[3 * g + 2] = acc[2]; }
     if (inter_out) inter_out[g] = inter;
 #pragma omp atomic
@@ -519,7 +545,7 @@ static int force_impl(const float *src, float *dst, const uint32_t *index, uint3
                       const bho_node *leaf, const bho_node *internal, const bho_params *p,
                       float G, float blc, const float *bounds, int stack_cap,
                       float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
-                      bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end, int box_mac) {
+                      bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end, const float *quad) {
     memset(st, 0, sizeof *st);
     if (g_end > num) g_end = num;
     if (stack_cap < 2) stack_cap = 2;
@@ -549,7 +575,7 @@ static int force_impl(const float *src, float *dst, const uint32_t *index, uint3
 #pragma omp for schedule(dynamic, 256)
         for (uint32_t g = g_begin; g < g_end; g++)
             walk_one(g, src, dst, index, n_init, leaf, internal, p, G, blc, bounds, stack, stack_cap,
-                     accel, interactions, tasks, task_cap, &nt, st, box_mac);
+                     accel, interactions, tasks, task_cap, &nt, st, quad);
         free(stack);
     }
     st->n_tasks = nt < task_cap ? nt : task_cap;
@@ -564,18 +590,47 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
                     float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
                     bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
     return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
-                      interactions, tasks, task_cap, st, threads, g_begin, g_end, 0);
+                      interactions, tasks, task_cap, st, threads, g_begin, g_end, NULL);
 }
 
-/* The walk with the box-aware acceptance test of the learned-correction
- * method (no reference counterpart): an internal node is accepted only if
- * half its longest side * inv < theta AND the target lies outside its AABB. */
-int bho_force_boxmac(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
-                     const bho_node *leaf, const bho_node *internal, const bho_params *p,
-                     float G, float blc, const float *bounds, int stack_cap,
-                     float *accel, uint32_t *interactions, bho_walk_stats *st, int threads) {
+/* The corrected walk (no reference counterpart): the reference's walk in
+ * which every accepted internal node k also adds the quadrupole term of its
+ * record quad[8k .. 8k+6] (G T xx xy xz yy yz zz, -1.5 G tr T). */
+int bho_force_quad(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                   const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                   float G, float blc, const float *bounds, int stack_cap, const float *quad,
+                   float *accel, uint32_t *interactions, bho_walk_stats *st, int threads) {
     return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
-                      interactions, NULL, 0, st, threads, 0, num, 1);
+                      interactions, NULL, 0, st, threads, 0, num, quad);
+}
+
+/* Second moment of the bodies of each internal node about its centre of
+ * mass, FP64, by direct summation over the node's leaves (sorted positions
+ * [lo, lo + bodiesContained), lo its leftmost leaf): m2[6k..6k+5] = xx, xy,
+ * xz, yy, yz, zz. No reference counterpart; pinned by closed forms (two and
+ * three bodies) in tests/test_oracle_quadrupole.py. */
+void bho_node_moments(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init, double *m2,
+                      int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    if (num < 2) return;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint32_t k = 0; k < num - 1; k++) {
+        uint32_t lo = n_init + k;
+        while (lo >= n_init) lo = internal[lo - n_init].childA;
+        const bho_node *nd = &internal[k];
+        double c[3] = {nd->comMass[0], nd->comMass[1], nd->comMass[2]}, acc[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t j = lo; j < lo + nd->bodiesContained; j++) {
+            const float *b = leaf[j].comMass;
+            double d[3] = {b[0] - c[0], b[1] - c[1], b[2] - c[2]}, m = b[3];
+            acc[0] += m * d[0] * d[0]; acc[1] += m * d[0] * d[1]; acc[2] += m * d[0] * d[2];
+            acc[3] += m * d[1] * d[1]; acc[4] += m * d[1] * d[2]; acc[5] += m * d[2] * d[2];
+        }
+        for (int q = 0; q < 6; q++) m2[6 * (size_t)k + q] = acc[q];
+    }
 }
 
 /* ------------------------------------------------------------ merge */

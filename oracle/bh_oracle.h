@@ -112,12 +112,17 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
                     float G, float body_len_const, const float *bounds, int stack_cap,
                     float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
                     bho_walk_stats *stats, int threads, uint32_t g_begin, uint32_t g_end);
-/* the walk with the box-aware acceptance test of the learned-correction method
- * (no reference counterpart): a node containing the target is always opened */
-int bho_force_boxmac(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
-                     const bho_node *leaf, const bho_node *internal, const bho_params *p,
-                     float G, float blc, const float *bounds, int stack_cap,
-                     float *accel, uint32_t *interactions, bho_walk_stats *st, int threads);
+/* the corrected walk (no reference counterpart): every accepted internal node
+ * k also adds the quadrupole term of quad[8k .. 8k+6] (G T xx xy xz yy yz zz,
+ * -1.5 G tr T), summed apart and joined to the monopole sum at the end */
+int bho_force_quad(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
+                   const bho_node *leaf, const bho_node *internal, const bho_params *p,
+                   float G, float blc, const float *bounds, int stack_cap, const float *quad,
+                   float *accel, uint32_t *interactions, bho_walk_stats *st, int threads);
+/* FP64 second moment of each internal node's bodies about its centre of mass
+ * (m2 [num-1][6]: xx xy xz yy yz zz), by direct summation */
+void bho_node_moments(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init, double *m2,
+                      int threads);
 
 /* merge resolution: shaders/compute/merge/bh_merge.comp:5-71, resolved in
  * ascending (a, b) task order. Returns the number of merges applied. */

@@ -96,9 +96,11 @@ def _declare(L: C.CDLL) -> None:
     L.bho_force.restype = C.c_int
     L.bho_force_range.argtypes = L.bho_force.argtypes + [U32, U32]
     L.bho_force_range.restype = C.c_int
-    L.bho_force_boxmac.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int,
-                                   P, P, C.POINTER(WalkStats), C.c_int]
-    L.bho_force_boxmac.restype = C.c_int
+    L.bho_force_quad.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int, P,
+                                 P, P, C.POINTER(WalkStats), C.c_int]
+    L.bho_force_quad.restype = C.c_int
+    L.bho_node_moments.argtypes = [P, P, U32, U32, P, C.c_int]
+    L.bho_node_moments.restype = None
     L.bho_merge.argtypes = [P, P, U32]
     L.bho_merge.restype = U32
     L.bho_merge_pair.argtypes = [P, P, P]
@@ -110,6 +112,8 @@ def _declare(L: C.CDLL) -> None:
     L.bho_direct_sum_f64.argtypes = [P, U32, P, U32, C.c_double, C.c_double, P, C.c_int]
     L.mlp_num_params.restype = C.c_int
     L.mlp_forward_f64.argtypes = [P, P, C.c_int, P]
+    L.mlp_quad_basis_f64.argtypes = [P, P, P]
+    L.mlp_quad_basis_f64.restype = None
     L.mlp_loss_grad_f64.argtypes = [P, P, P, C.c_int, P]
     L.mlp_loss_grad_f64.restype = C.c_double
 
@@ -194,16 +198,27 @@ def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, 
     return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
 
 
-def force_boxmac(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, stack_cap=256, threads=0):
-    """Accelerations of the walk whose acceptance also requires the target to
-    lie outside the node's box (the learned-correction method's walk)."""
+def force_quad(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, quad, stack_cap=256,
+               threads=0):
+    """Accelerations of the corrected walk: the reference's walk in which every
+    accepted internal node k also adds the quadrupole term of its record
+    quad[k] (8 floats: G T xx xy xz yy yz zz, -1.5 G tr T, unused)."""
     accel = np.zeros((n_init, 3), np.float32)
     inter = np.zeros(n_init, np.uint32)
     st = WalkStats()
     b = np.ascontiguousarray(bounds, dtype=np.float32)
-    rc = lib().bho_force_boxmac(_p(src), None, _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm),
-                                float(G), float(blc), _p(b), stack_cap, _p(accel), _p(inter), C.byref(st), threads)
+    q = np.ascontiguousarray(quad, dtype=np.float32)
+    assert q.shape == (n_init - 1, 8)
+    rc = lib().bho_force_quad(_p(src), None, _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm), float(G),
+                              float(blc), _p(b), stack_cap, _p(q), _p(accel), _p(inter), C.byref(st), threads)
     return {"rc": rc, "accel": accel, "interactions": inter, "stats": st}
+
+
+def node_moments(leaf, internal, num, n_init, threads=0) -> np.ndarray:
+    """FP64 second moment (xx xy xz yy yz zz) of each internal node's bodies about its centre of mass."""
+    m2 = np.zeros((max(num - 1, 1), 6), np.float64)
+    lib().bho_node_moments(_p(leaf), _p(internal), num, n_init, _p(m2), threads)
+    return m2[: max(num - 1, 0)]
 
 
 def force_range(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, g_begin, g_end,
@@ -234,16 +249,25 @@ def mlp_num_params() -> int:
 
 
 def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
-    """FP64 forward of the 12-64-64-3 SiLU network (oracle/mlp_oracle.c)."""
+    """FP64 forward of the 12-64-64-6 SiLU network (oracle/mlp_oracle.c)."""
     p = np.ascontiguousarray(params, dtype=np.float32)
     xx = np.ascontiguousarray(x, dtype=np.float32)
-    y = np.zeros((xx.shape[0], 3), np.float64)
+    y = np.zeros((xx.shape[0], 6), np.float64)
     lib().mlp_forward_f64(_p(p), _p(xx), xx.shape[0], _p(y))
     return y
 
 
+def mlp_quad_basis(u, o) -> np.ndarray:
+    """B(u) o = -3 o u - 1.5 tr(o) u + 7.5 (u.o.u) u for one direction u (3) and tensor o (6)."""
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    oo = np.ascontiguousarray(o, dtype=np.float64)
+    out = np.zeros(3, np.float64)
+    lib().mlp_quad_basis_f64(_p(uu), _p(oo), _p(out))
+    return out
+
+
 def mlp_loss_grad(params: np.ndarray, x: np.ndarray, t: np.ndarray):
-    """FP64 mean squared error and its gradient."""
+    """FP64 loss through the quadrupole basis (targets t[rows][7] = (u, y, w)) and its gradient."""
     p = np.ascontiguousarray(params, dtype=np.float32)
     xx = np.ascontiguousarray(x, dtype=np.float32)
     tt = np.ascontiguousarray(t, dtype=np.float32)

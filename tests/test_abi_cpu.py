@@ -30,7 +30,7 @@ def test_params_struct_matches_header():
     from gravity_amd import barnes_hut
     # 12 four-byte fields before flags... sizes must agree with the C struct
     assert ctypes.sizeof(barnes_hut.Params) == 15 * 4
-    assert ctypes.sizeof(barnes_hut.Counters) == 4 * 4 + 2 * 8 + 6 * 4 + 8
+    assert ctypes.sizeof(barnes_hut.Counters) == 4 * 4 + 2 * 8 + 6 * 4 + 8 + 8
 
 
 def test_library_rejects_bad_arguments_without_gpu():
