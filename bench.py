@@ -26,7 +26,7 @@ an FP64 direct sum (the GPU's FP64 kernel), p50 and p99.
    port of the plain step on this host over a bounded sample).
 
 The roofline entry is the dominant kernel of the timed step, the corrected
-walk (VALU-bound: 20 FLOP per body-node interaction plus 40 per quadrupole
+walk (VALU-bound: 20 FLOP per body-node interaction plus 33 per quadrupole
 term, over its mean duration from HIP events on the library's stream).
 """
 from __future__ import annotations
@@ -46,7 +46,11 @@ METRIC = "body-steps/sec @16M bodies at equal p99 force err; plain vs learned-co
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20
-FLOP_PER_QUAD_TERM = 40        # the quadrupole term of an accepted internal node (add_quad)
+# the quadrupole term of an accepted internal node beyond its monopole: G T r (15
+# with a fused multiply-add as 2), r.GT.r (5), 1/s^5 (1), the coefficient
+# (7.5/s^2, one fma, the -3/s^5 factor: 4), its fma with the monopole's G M/s^3
+# (1 more than a product) and the three fused updates of the sum (6)
+FLOP_PER_QUAD_TERM = 33
 P99_TARGET = 1e-3
 N_SAMPLE = 16384
 
@@ -391,6 +395,9 @@ def main() -> int:
     ap.add_argument("--ensemble-sims", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
+    ap.add_argument("--profile-step", type=float, default=None, metavar="THETA",
+                    help="only the headline's timed loop: the learned-corrected step at THETA with an untrained "
+                         "(zero-output) network, which runs the same kernels; for rocprofv3 profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -413,6 +420,23 @@ def main() -> int:
     plain_ref = B.make_params(theta=0.5, dt=dt, softening=soft)
     bh = B.BarnesHut(bodies, bounds, plain_ref)
     bh.init()
+    if args.profile_step is not None:
+        mlp = Lr.Mlp(params=Lr.init_params(args.seed))      # zero output layer: no residual
+        Lr.attach(bh, mlp)
+        prm = B.make_params(theta=args.profile_step, dt=dt, softening=soft, flags=B.FLAG_LEARNED)
+        run = timed_loop(bh, prm, args.steps, args.warmup)
+        c = run["counters"]
+        walk_ms = run["stage_ms"]["force"]
+        flops = c["interactions"] * FLOP_PER_INTERACTION + c["quad_terms"] * FLOP_PER_QUAD_TERM
+        print(json.dumps({"profile_step": args.profile_step, "ms_per_step": run["ms_per_step"],
+                          "body_steps_per_s": run["body_steps_per_s"], "stage_ms": run["stage_ms"],
+                          "walk_ms": walk_ms, "walk_tflops": flops / (walk_ms * 1e-3) / 1e12,
+                          "interactions": int(c["interactions"]), "quad_terms": int(c["quad_terms"])}),
+              flush=True)
+        Lr.attach(bh, None)
+        bh.close()
+        mlp.close()
+        return 0
 
     # 1. the reference's configuration: plain, theta 0.5
     ref = timed_loop(bh, plain_ref, args.steps, args.warmup)

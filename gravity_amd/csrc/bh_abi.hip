@@ -80,9 +80,12 @@ struct bh_context {
     uint32_t* sort_pos[2] = {nullptr, nullptr};
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
-    // corrected walk (allocated on first use): the quadrupole records of the
-    // internal nodes and the FP64 moment prefix sums they are made from
-    float4* qrec = nullptr;                   // [2(n-1)]
+    // corrected walk (allocated on first use): the fused 64-byte records of the
+    // internal nodes (walk record, then quadrupole terms) and the FP64 moment
+    // prefix sums they are made from. quad_valid: the records belong to the
+    // current tree (made by its tree stage under a corrected-walk flag).
+    float4* qrec = nullptr;                   // [4(n-1)]
+    bool quad_valid = false;
     double* mom_prefix = nullptr;             // [mom_prefix_doubles(n)]
     double* mom_tiles = nullptr;              // [mom_tile_doubles(n)]
 
@@ -412,7 +415,6 @@ static void* binding_ptr(bh_context* c, int b) {
         case BH_BINDING_MORTON_OUT: return c->morton[1 - c->mi_in];
         case BH_BINDING_INDEX_IN: return c->index[c->mi_in];
         case BH_BINDING_INDEX_OUT: return c->index[1 - c->mi_in];
-        case BH_BINDING_NODE_QUADRUPOLES: return c->qrec;
         default: return nullptr;
     }
 }
@@ -424,14 +426,29 @@ static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
     return BH_OK;
 }
 
-// the corrected walk's arrays, allocated on first use (about 110 bytes per body)
+// the corrected walk's arrays, allocated on first use (about 210 bytes per body)
 static int ensure_quad(bh_context* c) {
     if (c->qrec) return BH_OK;
     int rc;
     if ((rc = dalloc(c, &c->mom_prefix, mom_prefix_doubles(c->n)))) return rc;
     if ((rc = dalloc(c, &c->mom_tiles, mom_tile_doubles(c->n)))) return rc;
-    if ((rc = dalloc(c, &c->qrec, 2 * ((size_t)c->n - 1)))) return rc;
-    HIPCHECK(c, hipMemsetAsync(c->qrec, 0, 32 * ((size_t)c->n - 1), c->stream));
+    if ((rc = dalloc(c, &c->qrec, 4 * corr_records(c->n)))) return rc;
+    // the quadrupole halves of the leaf records stay zero
+    HIPCHECK(c, hipMemsetAsync(c->qrec, 0, 64 * corr_records(c->n), c->stream));
+    return BH_OK;
+}
+
+// Binding 16 is the quadrupole half (bytes 32..63) of every internal node's
+// record: copied whole, with a 64-byte pitch on the device.
+static int quad_binding_copy(bh_context* c, void* host, size_t off, size_t bytes, bool upload) {
+    if (off != 0 || bytes != bh_binding_size(c, BH_BINDING_NODE_QUADRUPOLES))
+        return fail(c, BH_ERR_INVALID_ARGUMENT, "the quadrupole records are copied whole");
+    char* dev = reinterpret_cast<char*>(c->qrec + 4 * (size_t)corr_internal_id(c->n, 0)) + 32;
+    if (upload)
+        HIPCHECK(c, hipMemcpy2DAsync(dev, 64, host, 32, 32, (size_t)c->n - 1, hipMemcpyHostToDevice, c->stream));
+    else
+        HIPCHECK(c, hipMemcpy2DAsync(host, 32, dev, 64, 32, (size_t)c->n - 1, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     return BH_OK;
 }
 
@@ -441,8 +458,12 @@ int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) 
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
-    if (b == BH_BINDING_NODE_QUADRUPOLES && (rc = ensure_quad(c))) return rc;
+    if (b == BH_BINDING_NODE_QUADRUPOLES) {
+        if ((rc = ensure_quad(c))) return rc;
+        return quad_binding_copy(c, const_cast<void*>(host), off, bytes, true);
+    }
     if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
+        c->quad_valid = false;   // the tree changes under the fused records
         if (off != 0 || bytes != bh_binding_size(c, b))
             return fail(c, BH_ERR_INVALID_ARGUMENT, "node bindings are uploaded whole");
         HIPCHECK(c, hipMemcpyAsync(c->staging_nodes, host, bytes, hipMemcpyHostToDevice, c->stream));
@@ -472,7 +493,10 @@ int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
-    if (b == BH_BINDING_NODE_QUADRUPOLES && (rc = ensure_quad(c))) return rc;
+    if (b == BH_BINDING_NODE_QUADRUPOLES) {
+        if ((rc = ensure_quad(c))) return rc;
+        return quad_binding_copy(c, host, off, bytes, false);
+    }
     if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
         launch_nodes_to_std430(c->stream, c->n, c->sim, c->tree, c->colors, b == BH_BINDING_LEAF_NODES ? 0 : 1,
                                c->staging_nodes);
@@ -612,11 +636,13 @@ int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* 
                 positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr);
     rc = check_launch(c, "tree");
     if (rc == BH_OK && positions) std::swap(c->index[c->mi_in], c->sort_pos[1 - c->mi_in]);
+    c->quad_valid = false;
     if (rc == BH_OK && quad) {
         launch_node_quadrupoles(c->stream, c->n, c->sim, c->ctr, c->tree, c->mom_prefix, c->mom_tiles,
                                 learned ? c->correction->w : nullptr, learned ? c->correction->bias : nullptr,
                                 c->qrec);
         rc = check_launch(c, "node quadrupoles");
+        c->quad_valid = rc == BH_OK;
     }
     return rc;
 }
@@ -669,9 +695,11 @@ int compute_force(bh_context* c, const bh_params* p, float4* step_pos, bool* wro
     if (p->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED)) {
         if (p->merging_collision_or_neither != BH_NEITHER)
             return fail(c, BH_ERR_UNSUPPORTED, "the corrected walk runs without merging or collisions");
-        if (!c->qrec)
+        if (!(p->softening >= 1e-14f))
+            return fail(c, BH_ERR_UNSUPPORTED, "the corrected walk needs softening >= 1e-14");
+        if (!c->qrec || !c->quad_valid)
             return fail(c, BH_ERR_INVALID_ARGUMENT,
-                        "no quadrupole records: build the tree with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED first");
+                        "no records of the current tree: build it with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED first");
         a.qrec = c->qrec;
     }
     launch_walk(c->stream, c->n, a);

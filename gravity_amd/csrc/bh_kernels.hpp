@@ -77,10 +77,22 @@ struct WalkArgs {
     uint32_t general;                // 1: always the general walk loop
     float4* pos_out = nullptr;       // [n] new position-mass by sorted position as well, or nullptr
     uint32_t* queues = nullptr;      // [kWalkQueues * 32] per-CU work queues (persistent walk), or nullptr
-    // the corrected walk: [2(n-1)] quadrupole records of the internal nodes
-    // (two float4 each: G T xx xy xz yy, G T yz zz, -1.5 G tr T, 0), or nullptr
+    // the corrected walk: its records [4 corr_records(n)], or nullptr
     const float4* qrec = nullptr;
 };
+
+// The corrected walk's records, 64 bytes per node id (four float4):
+//   (centre of mass, G M), (half, childA, childB, half^2),
+//   G (T_xx, T_xy, T_xz, T_yy), (G T_yz, G T_zz, -1.5 G tr T, 0).
+// Leaf j is record j (its position and G m, half 0, half^2 -1, no children,
+// zero quadrupole terms); internal node k is record n + kCorrGap + k, and the
+// child ids are in the same numbering, so every internal id is at least 64
+// above every leaf id. The kCorrGap records in between are never read.
+constexpr uint32_t kCorrGap = 64;
+__host__ __device__ inline uint32_t corr_internal_id(uint32_t n, uint32_t k) { return n + kCorrGap + k; }
+__host__ __device__ inline size_t corr_records(uint32_t n) { return 2 * (size_t)n - 1 + kCorrGap; }
+// every record's byte offset fits 32 bits
+__host__ __device__ inline bool corr_offsets_32bit(uint32_t n) { return corr_records(n) <= (1ull << 26); }
 
 // Second moments of the corrected walk: positions per tile of the FP64
 // prefix sums (m, m x, m x x^T: 10 components per position).
@@ -166,8 +178,10 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 // The corrected walk's node records on the current tree: FP64 prefix sums of
 // the moments (prefix: mom_prefix_doubles(n), tiles: mom_tile_doubles(n)),
 // then per internal node the second moment about its centre of mass plus the
-// network's residual (w, bias: the packed network, or nullptr for none) into
-// qrec [2(n-1)]; ctr->corrected counts the network evaluations.
+// network's residual (w, bias: the packed network, or nullptr for none), into
+// the records qrec [4 corr_records(n)] (the leaves' quadrupole halves must
+// hold zeros: they are never written); ctr->corrected counts the network
+// evaluations.
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
                              double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,

@@ -122,14 +122,14 @@ __device__ __forceinline__ void range_m2(const double* __restrict__ a, const dou
     }
 }
 
-// The corrected walk's records: internal node k gets
-//   qrec[2k]     = G (T_xx, T_xy, T_xz, T_yy),
-//   qrec[2k + 1] = G (T_yz, T_zz), -1.5 G tr T, 0,
-// T = M2 + M h^2 o with o the network's output for the node's features (o = 0
-// without a network). Each wave takes 64 consecutive nodes per turn; the grid
-// strides over the nodes so that every wave loads the network's fragments once.
+// The corrected walk's records (layout with corr_records): the first half of
+// every leaf's record (its walk record), and the whole record of every
+// internal node, T = M2 + M h^2 o with o the network's output for the node's
+// features (o = 0 without a network), children renumbered. Each wave takes 64
+// consecutive nodes per turn; the grid strides over the nodes so that every
+// wave loads the network's fragments once.
 __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                                const WalkNode* __restrict__ nodes,
+                                                const WalkNode* __restrict__ walk,
                                                 const float4* __restrict__ node_aabb,
                                                 const uint2* __restrict__ node_range,
                                                 const double* __restrict__ prefix, const __bf16* __restrict__ wg,
@@ -138,20 +138,28 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
     __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
     const uint32_t num = sim->numBodies;
+    const uint32_t n = sim->initialNumBodies;
     const uint32_t ninternal = num > 1 ? num - 1 : 0;
     const float G = sim->gravitationalConstant;
+    const WalkNode* __restrict__ nodes = walk + n;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t j = (blockIdx.x * 4 + w) * 64 + lane; j < num; j += waves * 64) {
+        const WalkNode lf = walk[j];
+        qrec[4 * (size_t)j] = lf.comMass;
+        qrec[4 * (size_t)j + 1] = make_float4(lf.half, __uint_as_float(0u), __uint_as_float(0u), lf.half2);
+    }
     MlpRegs R;
     if (wg) mlp_load(R, wg, bg);
     uint32_t evals = 0;
-    const uint32_t waves = gridDim.x * 4;
     for (uint32_t base = (blockIdx.x * 4 + w) * 64; base < ninternal; base += waves * 64) {
         const uint32_t k = base + lane;
         const bool active = k < ninternal;
         double m2[6] = {0, 0, 0, 0, 0, 0};
         float M = 0.0f, h = 0.0f;
         float4 mn = make_float4(0.f, 0.f, 0.f, 0.f), mx = mn, com = mn;
+        WalkNode nd;
         if (active) {
-            const WalkNode nd = nodes[k];
+            nd = nodes[k];
             const uint2 rg = node_range[k];
             com = nd.comMass;
             h = nd.half;
@@ -181,8 +189,13 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
             float T[6];
             for (int c = 0; c < 6; c++) T[c] = (float)(m2[c] + o[c]);
             const float tr = T[0] + T[3] + T[5];
-            qrec[2 * k] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
-            qrec[2 * k + 1] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
+            const uint32_t a = nd.childA >= n ? nd.childA + kCorrGap : nd.childA;
+            const uint32_t b = nd.childB >= n ? nd.childB + kCorrGap : nd.childB;
+            float4* rec = qrec + 4 * (size_t)corr_internal_id(n, k);
+            rec[0] = nd.comMass;
+            rec[1] = make_float4(nd.half, __uint_as_float(a), __uint_as_float(b), nd.half2);
+            rec[2] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
+            rec[3] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
         }
     }
     if (evals && lane == 0) atomicAdd(&ctr->corrected, (unsigned long long)evals);
@@ -196,7 +209,7 @@ void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, De
     k_mom_prefix<<<ntiles, kMomTile, 0, s>>>(sim, T.leaf_pos, tiles, prefix);
     const uint32_t groups = (n + 63) / 64;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048u);
-    k_qnodes<<<blocks, 256, 0, s>>>(sim, ctr, T.nodes, T.node_aabb, T.node_range, prefix, w, bias, qrec);
+    k_qnodes<<<blocks, 256, 0, s>>>(sim, ctr, T.walk, T.node_aabb, T.node_range, prefix, w, bias, qrec);
 }
 
 // ------------------------------------------------------------------ pairs

@@ -41,8 +41,7 @@ struct WalkAcc {
     float ax, ay, az;
     unsigned long long opened, inter;
     uint32_t maxsp, err;
-    float qx, qy, qz;   // quadrupole terms of the corrected walk, summed apart from the monopoles
-    unsigned long long quads;   // quadrupole terms evaluated
+    unsigned long long quads;   // quadrupole terms evaluated (corrected walk)
 };
 
 // a += G m r inv^3 in the reference's operation order (bh_force.comp:22-25,74,82)
@@ -51,27 +50,6 @@ __device__ __forceinline__ void add_force(float& ax, float& ay, float& az, float
     ax = fadd(ax, fmul(fmul(fmul(fmul(gm, rx), inv), inv), inv));
     ay = fadd(ay, fmul(fmul(fmul(fmul(gm, ry), inv), inv), inv));
     az = fadd(az, fmul(fmul(fmul(fmul(gm, rz), inv), inv), inv));
-}
-
-// The softened quadrupole term of an accepted internal node (no reference
-// counterpart): q holds G T (xx, xy, xz, yy, yz, zz) and -1.5 G tr(T) of the
-// node's effective second-moment tensor T; with inv = 1 / s, s^2 = |r|^2 + eps^2,
-//     a_q = G T-term = (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5.
-// Written with explicit fused multiply-adds (the oracle evaluates the same
-// sequence), summed into a separate accumulator.
-__device__ __forceinline__ void add_quad(float& qx, float& qy, float& qz, const float* q, float rx, float ry, float rz,
-                                         float inv) {
-    const float i2 = inv * inv;
-    const float i5 = i2 * i2 * inv;
-    const float tx = __builtin_fmaf(q[0], rx, __builtin_fmaf(q[1], ry, q[2] * rz));
-    const float ty = __builtin_fmaf(q[1], rx, __builtin_fmaf(q[3], ry, q[4] * rz));
-    const float tz = __builtin_fmaf(q[2], rx, __builtin_fmaf(q[4], ry, q[5] * rz));
-    const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
-    const float c = __builtin_fmaf(7.5f * rtr, i2, q[6]) * i5;
-    const float m3 = -3.0f * i5;
-    qx = __builtin_fmaf(c, rx, __builtin_fmaf(m3, tx, qx));
-    qy = __builtin_fmaf(c, ry, __builtin_fmaf(m3, ty, qy));
-    qz = __builtin_fmaf(c, rz, __builtin_fmaf(m3, tz, qz));
 }
 
 // The overlap test of a leaf visit in the merging (1) and collision (2)
@@ -126,12 +104,12 @@ __device__ __forceinline__ void leaf_contact(bool use, uint32_t node, float rx, 
 // 1 = leaf visits emit merge tasks, 2 = an overlapping leaf changes this
 // lane's body at once, velocity impulse and position correction, and the rest
 // of the walk and the integration use the changed body (bh_force.comp:85-112).
-template <int kDepth, uint32_t mode, bool kFast, bool kQuad = false>
+template <int kDepth, uint32_t mode, bool kFast>
 __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsigned long long* __restrict__ s_mask,
                                            uint32_t root, uint32_t n_ids, uint32_t base, uint32_t g,
                                            unsigned long long mask, float4& pm, float4& vd,
                                            const float4* __restrict__ leaf_pos, const WalkNode* __restrict__ nodes,
-                                           const float4* __restrict__ qrec, float G, float theta, float soft, float Ri,
+                                           float G, float theta, float soft, float Ri,
                                            const float* __restrict__ leaf_radius, uint32_t bodyIdx,
                                            float elasticity, float restitution,
                                            const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
@@ -141,8 +119,7 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     // the sums and counters live in locals for the loop (a struct carried
     // around the loop costs register copies on every iteration)
     float ax = o.ax, ay = o.ay, az = o.az;
-    float qx = o.qx, qy = o.qy, qz = o.qz;
-    unsigned long long opened = o.opened, inter = o.inter, quads = o.quads;
+    unsigned long long opened = o.opened, inter = o.inter;
     uint32_t maxsp = o.maxsp, err = o.err;
     // current node in scalar registers; the stack holds the pending childA
     // entries (after opening a node its childB, the top of the reference's
@@ -200,14 +177,6 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
         // an internal record holds G m; a leaf's position-mass holds m
         if (__builtin_amdgcn_inverse_ballot_w64(take))
             add_force(ax, ay, az, internal ? sp4.w : fmul(G, sp4.w), rx, ry, rz, inv);
-        if constexpr (kQuad) {
-            if (internal && take) {
-                const float4 q0 = qrec[2 * (node - n_ids)], q1 = qrec[2 * (node - n_ids) + 1];
-                const float q[7] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z};
-                if (__builtin_amdgcn_inverse_ballot_w64(take)) add_quad(qx, qy, qz, q, rx, ry, rz, inv);
-                quads += (unsigned long long)__popcll(take);
-            }
-        }
         inter += (unsigned long long)__popcll(take);
         if constexpr (mode != 0u)
             if (!internal && use_m)
@@ -242,10 +211,6 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
     o.ax = ax;
     o.ay = ay;
     o.az = az;
-    o.qx = qx;
-    o.qy = qy;
-    o.qz = qz;
-    o.quads = quads;
     o.opened = opened;
     o.inter = inter;
     o.maxsp = maxsp;
@@ -262,6 +227,8 @@ __device__ __forceinline__ void walk_chunk(uint32_t* __restrict__ s_node, unsign
 // 32 bits (at most 2^26 bodies). Same visits, same order, same sums.
 constexpr uint32_t kLeanHeight = 62;
 typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+// a corrected-walk record (64 bytes, 64-byte aligned)
+typedef uint32_t u32x16 __attribute__((ext_vector_type(16)));
 // v_writelane_b32 (the LLVM intrinsic; this compiler has no builtin for it)
 __device__ int lane_write(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 constexpr uint32_t kLeanBodies = 1u << 26;
@@ -270,20 +237,16 @@ __device__ __forceinline__ bool walk_lean_ok(float root_half, float soft, uint32
     return inv_sqrt_fast_domain(root_half, soft) && height <= kLeanHeight && num <= kLeanBodies;
 }
 
-template <uint32_t mode, bool kQuad = false>
+template <uint32_t mode>
 __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_t g, unsigned long long mask,
-                                          float4& pm, float4& vd, const WalkNode* __restrict__ nodes,
-                                          const float4* __restrict__ qrec, float G,
+                                          float4& pm, float4& vd, const WalkNode* __restrict__ nodes, float G,
                                           float theta, float soft, float Ri, const float* __restrict__ leaf_radius,
                                           uint32_t bodyIdx, float elasticity, float restitution,
                                           const uint32_t* __restrict__ index_in, const float4* __restrict__ src,
                                           DevCounters* __restrict__ ctr, uint2* __restrict__ tasks,
                                           uint32_t task_cap, WalkAcc& o) {
     float ax = o.ax, ay = o.ay, az = o.az;
-    float qx = o.qx, qy = o.qy, qz = o.qz;
-    uint32_t opened = 0, inter = 0, quads = 0, maxsp = o.maxsp, err = o.err;
-    // quadrupole records of the internal nodes (32 bytes each, internal node k at k)
-    const char* __restrict__ qbase = reinterpret_cast<const char*>(qrec);
+    uint32_t opened = 0, inter = 0, maxsp = o.maxsp, err = o.err;
     // One record per node id, leaves included (the tree arrays keep the leaf
     // records right before the internal ones), addressed as a base plus a
     // 32-bit byte offset (ids < 2^27).
@@ -300,11 +263,6 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
         // one 32-byte scalar load (a vector type keeps the compiler from
         // splitting it and sinking the half side into a later dependent load)
         const u32x8 rec = *reinterpret_cast<const u32x8*>(wbase + (uint32_t)(node << 5));
-        // the quadrupole record, fetched alongside (a leaf reads internal node
-        // 0's record, never used)
-        u32x8 qr;
-        if constexpr (kQuad)
-            qr = *reinterpret_cast<const u32x8*>(qbase + (uint32_t)((node >= n_ids ? node - n_ids : 0u) << 5));
         WalkNode nd;
         nd.comMass = make_float4(__uint_as_float(rec[0]), __uint_as_float(rec[1]), __uint_as_float(rec[2]),
                                  __uint_as_float(rec[3]));
@@ -327,15 +285,6 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
             open = mask & ~accm;
             if (__builtin_amdgcn_inverse_ballot_w64(take))
                 add_force(ax, ay, az, nd.comMass.w, rx, ry, rz, inv);   // the record holds G m
-            if constexpr (kQuad) {
-                if (internal && take) {
-                    const float q[7] = {__uint_as_float(qr[0]), __uint_as_float(qr[1]), __uint_as_float(qr[2]),
-                                        __uint_as_float(qr[3]), __uint_as_float(qr[4]), __uint_as_float(qr[5]),
-                                        __uint_as_float(qr[6])};
-                    if (__builtin_amdgcn_inverse_ballot_w64(take)) add_quad(qx, qy, qz, q, rx, ry, rz, inv);
-                    quads += (uint32_t)__popcll(take);
-                }
-            }
             inter += (uint32_t)__popcll(take);
             if constexpr (mode != 0u)
                 if (!internal && take)
@@ -366,12 +315,134 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
     o.ax = ax;
     o.ay = ay;
     o.az = az;
-    o.qx = qx;
-    o.qy = qy;
-    o.qz = qz;
-    o.quads += quads;
     o.opened += opened;
     o.inter += inter;
+    o.maxsp = maxsp;
+    o.err = err;
+}
+
+// The corrected walk (no reference counterpart): the same masked-stack walk
+// and visiting order, in which every accepted internal node adds its softened
+// monopole and quadrupole,
+//     a += G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5,
+// s^2 = |r|^2 + eps^2, and every leaf its monopole. It reads one 64-byte
+// record per node id (see corr_records in bh_kernels.hpp: leaves have zero
+// quadrupole terms and half^2 = -1, so one sequence serves both kinds). It
+// needs no bit parity with the reference, so it takes the shortest FP32
+// sequence: s^2 by fused multiply-adds, the acceptance test half < theta s as
+// theta^2 s^2 > half^2 (no root; exactly reproducible on the host), the
+// hardware 1/sqrt (within 1 ulp) and one fused update of both terms. The
+// lane whose own leaf a record is skips it (its r is 0, so it would add
+// exactly 0, but it does not count as an interaction). kLds: stack in LDS
+// (any tree height, overflow checked, 64-bit record offsets) instead of lanes
+// of three registers (trees at most kLeanHeight high, offsets below 2^32).
+// Needs eps^2 >= 1e-14 (1/s^5 stays finite) and the fast domain's bounds.
+__device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
+    return soft >= 1e-14f && inv_sqrt_fast_domain(root_half, soft);
+}
+
+template <bool kLds>
+__device__ __forceinline__ void walk_corrected(uint32_t n_ids, uint32_t base, unsigned long long mask,
+                                               const float4& pm, const float4* __restrict__ crec, float theta,
+                                               float soft, uint32_t* __restrict__ s_node,
+                                               unsigned long long* __restrict__ s_mask, WalkAcc& o) {
+    typedef typename std::conditional<kLds, unsigned long long, uint32_t>::type Count;
+    float ax = o.ax, ay = o.ay, az = o.az;
+    Count inter = 0, leaves = 0, opened = 0;
+    uint32_t maxsp = o.maxsp, err = o.err;
+    const float th2 = theta * theta;
+    const char* __restrict__ cbase = reinterpret_cast<const char*>(crec);
+    const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
+    const uint32_t first_internal = corr_internal_id(n_ids, 0);
+    int st_node = 0, st_lo = 0, st_hi = 0;   // lane stack: entry i in lane i
+    uint32_t node = first_internal;          // the root
+    int sp = 0;
+    for (;;) {
+        const u32x16 r = kLds ? *reinterpret_cast<const u32x16*>(cbase + ((size_t)node << 6))
+                              : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(node << 6));
+        const float rx = __uint_as_float(r[0]) - pm.x;
+        const float ry = __uint_as_float(r[1]) - pm.y;
+        const float rz = __uint_as_float(r[2]) - pm.z;
+        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, soft)));
+        const unsigned long long acc = ballot(d2 * th2 > __uint_as_float(r[7]));
+        unsigned long long take = acc & mask;
+        const unsigned long long open = mask & ~acc;
+        if (node < first_internal) {
+            // a leaf (rare): not for the lane whose own body it is
+            const uint32_t rel = node - wbase;
+            if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
+            leaves += (Count)__popcll(take);
+        }
+        {
+            if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+                const float inv = __builtin_amdgcn_rsqf(d2);
+                const float i2 = inv * inv;
+                const float i3 = i2 * inv;
+                const float i5 = i3 * i2;
+                const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
+                const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
+                const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
+                const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
+                const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
+                const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
+                const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
+                const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
+                const float m3 = -3.0f * i5;
+                ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
+                ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
+                az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
+            }
+            inter += (Count)__popcll(take);
+        }
+        if constexpr (kLds) {
+            if (open) {
+                if (sp + 1 > kStackDepth) {
+                    err |= 1u;
+                    break;
+                }
+                if (lane_id() == 0) {
+                    s_node[sp] = r[5];
+                    s_mask[sp] = open;
+                }
+                sp += 1;
+                opened += (Count)__popcll(open);
+                maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
+                node = r[6];
+                mask = open;
+            } else {
+                if (sp == 0) break;
+                sp -= 1;
+                node = uniform(s_node[sp]);
+                const unsigned long long mraw = s_mask[sp];
+                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
+            }
+        } else {
+            // push (childA, open) and go to childB if any lane opened the node,
+            // else pop; both without a branch (as in walk_lean)
+            const int top = sp - 1;
+            const uint32_t pop_node = (uint32_t)__builtin_amdgcn_readlane(st_node, top);
+            const unsigned long long pop_mask =
+                ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, top) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane(st_lo, top);
+            st_node = lane_write((int)r[5], sp, st_node);
+            st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
+            st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
+            opened += (Count)__popcll(open);
+            const bool descend = open != 0ull;
+            node = descend ? r[6] : pop_node;
+            mask = descend ? open : pop_mask;
+            sp = descend ? sp + 1 : top;
+            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
+            if (sp < 0) break;
+        }
+    }
+    const Count quads = inter - leaves;
+    o.ax = ax;
+    o.ay = ay;
+    o.az = az;
+    o.quads += quads;
+    o.inter += inter;
+    o.opened += opened;
     o.maxsp = maxsp;
     o.err = err;
 }
@@ -407,31 +478,29 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
     const float G = sim->gravitationalConstant;
     const float Ri = (kMode != 0u && active) ? leaf_radius[g] : 0.0f;
 
-    WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0.0f, 0.0f, 0.0f, 0ull};
+    WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0ull};
     if (num > 1) {
-        if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num))
-            walk_lean<kMode, kQuad>(n, n, g, ballot(active), pm, vd, nodes, qrec, G, theta_, softening, Ri,
-                                    leaf_radius, bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
-                                    task_cap, o);
-        else if (inv_sqrt_fast_domain(nodes[0].half, softening))
-            walk_chunk<kStackDepth, kMode, true, kQuad>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
-                                                        leaf_pos, nodes, qrec, G, theta_, softening, Ri, leaf_radius,
-                                                        bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
-                                                        task_cap, o);
-        else
-            walk_chunk<kStackDepth, kMode, false, kQuad>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd,
-                                                         leaf_pos, nodes, qrec, G, theta_, softening, Ri, leaf_radius,
-                                                         bodyIdx, elasticity, restitution, index_in, src, ctr, tasks,
-                                                         task_cap, o);
+        if constexpr (kQuad) {
+            if (!walk_corrected_ok(nodes[0].half, softening))
+                o.err |= 4u;   // outside the corrected walk's range: no forces
+            else if (!general && ctr->treeHeight <= kLeanHeight && corr_offsets_32bit(n))
+                walk_corrected<false>(n, base, ballot(active), pm, qrec, theta_, softening, s_node, s_mask, o);
+            else
+                walk_corrected<true>(n, base, ballot(active), pm, qrec, theta_, softening, s_node, s_mask, o);
+        } else if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num)) {
+            walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
+                             elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
+        } else if (inv_sqrt_fast_domain(nodes[0].half, softening)) {
+            walk_chunk<kStackDepth, kMode, true>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd, leaf_pos, nodes,
+                                                 G, theta_, softening, Ri, leaf_radius, bodyIdx, elasticity,
+                                                 restitution, index_in, src, ctr, tasks, task_cap, o);
+        } else {
+            walk_chunk<kStackDepth, kMode, false>(s_node, s_mask, n, n, base, g, ballot(active), pm, vd, leaf_pos,
+                                                  nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx, elasticity,
+                                                  restitution, index_in, src, ctr, tasks, task_cap, o);
+        }
     }
-    float ax = o.ax, ay = o.ay, az = o.az;
-    // the quadrupole terms join the monopole sum (all-zero records leave the
-    // plain walk's bits, the sign of a zero included)
-    if constexpr (kQuad) {
-        if (o.qx != 0.0f) ax = fadd(ax, o.qx);
-        if (o.qy != 0.0f) ay = fadd(ay, o.qy);
-        if (o.qz != 0.0f) az = fadd(az, o.qz);
-    }
+    const float ax = o.ax, ay = o.ay, az = o.az;
 
     // kick-drift into the output buffer (bh_force.comp:118-140)
     uint32_t oob = 0;
@@ -474,7 +543,8 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
 // one set of atomics per wave
 __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* ctr, const WalkTotals& t) {
     const uint32_t oob_w = t.oob;   // wave-uniform
-    const uint32_t err_w = (ballot(t.err & 1u) ? 1u : 0u) | (ballot(t.err & 2u) ? 2u : 0u);
+    const uint32_t err_w =
+        (ballot(t.err & 1u) ? 1u : 0u) | (ballot(t.err & 2u) ? 2u : 0u) | (ballot(t.err & 4u) ? 4u : 0u);
     if (lane_id() == 0) {
         atomicAdd(&ctr->opened, t.opened);
         atomicAdd(&ctr->interactions, t.inter);
@@ -668,17 +738,17 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             const uint32_t bodyIdx = active ? index_in[g] : 0;
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
-            WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0.0f, 0.0f, 0.0f, 0ull};
+            WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0ull};
             if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
-                walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, nullptr, G, theta, soft, 0.0f, nullptr,
+                walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, G, theta, soft, 0.0f, nullptr,
                               bodyIdx, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             else if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
                 walk_chunk<kDepth, 0u, true>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
-                                             nodes, nullptr, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
+                                             nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
                                              index_in, src, ctr, nullptr, 0u, o);
             else
                 walk_chunk<kDepth, 0u, false>(s_node[w], s_mask[w], root, N, base, g, ballot(active), pm, vd, leaf_pos,
-                                              nodes, nullptr, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
+                                              nodes, G, theta, soft, 0.0f, nullptr, bodyIdx, 0.0f, 0.0f,
                                               index_in, src, ctr, nullptr, 0u, o);
             if (active) {   // kick-drift (bh_force.comp:118-140), dynamic bounds
                 const float nvx = fadd(vd.x, fmul(o.ax, dt));

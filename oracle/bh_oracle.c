@@ -396,40 +396,16 @@ static float body_radius(const float *b, float blc) {   /* common/common.glsl:22
     return blc * powf(b[3] / b[7], 1.0f / 3.0f);
 }
 
-/* The quadrupole term of an accepted internal node in the corrected walk (no
- * reference counterpart; the GPU kernel evaluates the same sequence): Q holds
- * G T (xx, xy, xz, yy, yz, zz) and -1.5 G tr T of the node's tensor T; with
- * inv = 1 / s, q += (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5.
- * Pinned by tests/test_oracle_quadrupole.py: the error of monopole + this term
- * against the exact field of a cluster falls as the cube of its size over
- * distance (one order faster than the monopole alone). */
-static void add_quad(float *q, const float *Q, const float *r, float inv) {
-    float i2 = inv * inv;
-    float i5 = i2 * i2 * inv;
-    float tx = fmaf(Q[0], r[0], fmaf(Q[1], r[1], Q[2] * r[2]));
-    float ty = fmaf(Q[1], r[0], fmaf(Q[3], r[1], Q[4] * r[2]));
-    float tz = fmaf(Q[2], r[0], fmaf(Q[4], r[1], Q[5] * r[2]));
-    float rtr = fmaf(r[0], tx, fmaf(r[1], ty, r[2] * tz));
-    float c = fmaf(7.5f * rtr, i2, Q[6]) * i5;
-    float m3 = -3.0f * i5;
-    q[0] = fmaf(c, r[0], fmaf(m3, tx, q[0]));
-    q[1] = fmaf(c, r[1], fmaf(m3, ty, q[1]));
-    q[2] = fmaf(c, r[2], fmaf(m3, tz, q[2]));
-}
-
-/* bh_force.comp:40-142 for the body at sorted position g; with quad (8 floats
- * per internal node) the corrected walk: accepted internal nodes also add
- * their quadrupole term, summed apart and joined to the sum at the end */
+/* bh_force.comp:40-142 for the body at sorted position g */
 static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *index, uint32_t n_init,
                      const bho_node *leaf, const bho_node *internal, const bho_params *p,
                      float G, float blc, const float *bounds, uint32_t *stack, int stack_cap,
                      float *accel_out, uint32_t *inter_out, uint32_t *tasks, uint32_t task_cap,
-                     uint32_t *n_tasks, bho_walk_stats *st, const float *quad) {
+                     uint32_t *n_tasks, bho_walk_stats *st) {
     uint32_t bodyIdx = index[g];
     float body[8];
     memcpy(body, src + (size_t)bodyIdx * 8, 32);
     float acc[3] = {0.0f, 0.0f, 0.0f};
-    float q[3] = {0.0f, 0.0f, 0.0f};
     uint32_t sp = 0;
     uint32_t opened = 0, inter = 0, tests = 0, maxsp = 1;
     stack[sp++] = n_init;
@@ -446,7 +422,6 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
             if (longest * inv < p->theta) {
                 float gm = G * nd->comMass[3];
                 for (int k = 0; k < 3; k++) acc[k] = acc[k] + gm * r[k] * inv * inv * inv;
-                if (quad) add_quad(q, quad + 8 * (size_t)(nodeIdx - n_init), r, inv);
                 inter++;
             } else {
                 if ((int)sp + 2 > stack_cap) { st->max_stack = 0xFFFFFFFFu; break; }
@@ -487,9 +462,6 @@ static void walk_one(uint32_t g, const float *src, float *dst, const uint32_t *i
             }
         }
     }
-    if (quad)
-        for (int k = 0; k < 3; k++)
-            if (q[k] != 0.0f) acc[k] = acc[k] + q[k];
     if (accel_out) { accel_out[3 * g] = acc[0]; accel_out[3 * g + 1] = acc[1]; accel_out[3 * g + 2] = acc[2]; }
     if (inter_out) inter_out[g] = inter;
 #pragma omp atomic
@@ -545,7 +517,7 @@ static int force_impl(const float *src, float *dst, const uint32_t *index, uint3
                       const bho_node *leaf, const bho_node *internal, const bho_params *p,
                       float G, float blc, const float *bounds, int stack_cap,
                       float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
-                      bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end, const float *quad) {
+                      bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
     memset(st, 0, sizeof *st);
     if (g_end > num) g_end = num;
     if (stack_cap < 2) stack_cap = 2;
@@ -575,7 +547,7 @@ static int force_impl(const float *src, float *dst, const uint32_t *index, uint3
 #pragma omp for schedule(dynamic, 256)
         for (uint32_t g = g_begin; g < g_end; g++)
             walk_one(g, src, dst, index, n_init, leaf, internal, p, G, blc, bounds, stack, stack_cap,
-                     accel, interactions, tasks, task_cap, &nt, st, quad);
+                     accel, interactions, tasks, task_cap, &nt, st);
         free(stack);
     }
     st->n_tasks = nt < task_cap ? nt : task_cap;
@@ -590,18 +562,116 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
                     float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
                     bho_walk_stats *st, int threads, uint32_t g_begin, uint32_t g_end) {
     return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
-                      interactions, tasks, task_cap, st, threads, g_begin, g_end, NULL);
+                      interactions, tasks, task_cap, st, threads, g_begin, g_end);
 }
 
-/* The corrected walk (no reference counterpart): the reference's walk in
- * which every accepted internal node k also adds the quadrupole term of its
- * record quad[8k .. 8k+6] (G T xx xy xz yy yz zz, -1.5 G tr T). */
-int bho_force_quad(const float *src, float *dst, const uint32_t *index, uint32_t num, uint32_t n_init,
-                   const bho_node *leaf, const bho_node *internal, const bho_params *p,
-                   float G, float blc, const float *bounds, int stack_cap, const float *quad,
-                   float *accel, uint32_t *interactions, bho_walk_stats *st, int threads) {
-    return force_impl(src, dst, index, num, n_init, leaf, internal, p, G, blc, bounds, stack_cap, accel,
-                      interactions, NULL, 0, st, threads, 0, num, quad);
+/* The corrected walk (no reference counterpart) for the bodies at sorted
+ * positions [0, num): the reference's walk order (childB before childA) with
+ * the opening test half < theta s taken as fl(fl(s^2) theta^2) > half^2, s^2 =
+ * fma(x, x, fma(y, y, fma(z, z, eps^2))) in FP32 with r = com - position in
+ * FP32 (the GPU kernel's exact sequence, so both accept the same nodes), and
+ * every accepted node's term evaluated in FP64 from those FP32 inputs:
+ *     G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5
+ * with quad[8k .. 8k+6] = G T (xx xy xz yy yz zz), -1.5 G tr T for internal
+ * node k and no quadrupole for a leaf (the body's own leaf skipped). Outputs
+ * per sorted position: accel[3g..] the FP64 sum, absum[g] the sum over the
+ * accepted nodes of |G M r / s^3| + |quadrupole term| (the scale of the FP32
+ * rounding a GPU sum may carry), inter[g] the accepted nodes. The stats get
+ * the opened nodes, the quadrupole terms (in mac_tests) and the stack depth.
+ * Pinned by tests/test_oracle_quadrupole.py (exact field of clusters, error
+ * falling one order faster than the plain walk's, zero records = plain). */
+int bho_force_corrected(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init,
+                        const float *quad, float theta, float softening, float G, int stack_cap, double *accel,
+                        double *absum, uint32_t *inter, bho_walk_stats *st, int threads) {
+    memset(st, 0, sizeof *st);
+    if (stack_cap < 2) stack_cap = 2;
+    if (num < 2) {
+        for (uint32_t g = 0; g < num; g++) {
+            accel[3 * g] = accel[3 * g + 1] = accel[3 * g + 2] = 0.0;
+            absum[g] = 0.0;
+            inter[g] = 0;
+        }
+        return 0;
+    }
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    const float th2 = theta * theta;
+    int overflow = 0;
+    uint64_t opened_all = 0, quads_all = 0;
+    uint32_t maxsp_all = 0;
+#pragma omp parallel reduction(+ : opened_all, quads_all) reduction(max : maxsp_all) reduction(| : overflow)
+    {
+        uint32_t *stack = malloc((size_t)stack_cap * 4);
+#pragma omp for schedule(dynamic, 256)
+        for (uint32_t g = 0; g < num; g++) {
+            const float *p = leaf[g].comMass;
+            double a[3] = {0.0, 0.0, 0.0}, sabs = 0.0;
+            uint32_t sp = 0, n_acc = 0, maxsp = 1;
+            stack[sp++] = n_init;
+            while (sp > 0) {
+                const uint32_t id = stack[--sp];
+                const int internal_node = id >= n_init;
+                const bho_node *nd = get_node(leaf, internal, n_init, id);
+                if (!internal_node && id == g) continue;
+                const float rf[3] = {nd->comMass[0] - p[0], nd->comMass[1] - p[1], nd->comMass[2] - p[2]};
+                const float d2f = fmaf(rf[0], rf[0], fmaf(rf[1], rf[1], fmaf(rf[2], rf[2], softening)));
+                if (internal_node) {
+                    float ex = nd->aabb[3] - nd->aabb[0], ey = nd->aabb[4] - nd->aabb[1], ez = nd->aabb[5] - nd->aabb[2];
+                    float half = gmax(ex, gmax(ey, ez)) / 2.0f;
+                    float half2 = half * half;
+                    float lhs = d2f * th2;
+                    if (!(lhs > half2)) {
+                        if ((int)sp + 2 > stack_cap) { overflow = 1; break; }
+                        stack[sp++] = nd->childA;
+                        stack[sp++] = nd->childB;
+                        if (sp > maxsp) maxsp = sp;
+                        opened_all++;
+                        continue;
+                    }
+                }
+                const double r[3] = {rf[0], rf[1], rf[2]};
+                const double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + (double)softening;
+                const double inv = 1.0 / sqrt(s2);
+                const double i2 = inv * inv, i3 = i2 * inv, i5 = i3 * i2;
+                const double gm = (double)(G * nd->comMass[3]);
+                double term[3], mono = 0.0, qabs = 0.0;
+                for (int k = 0; k < 3; k++) { term[k] = gm * i3 * r[k]; mono += term[k] * term[k]; }
+                if (internal_node) {
+                    const float *Q = quad + 8 * (size_t)(id - n_init);
+                    const double t[3] = {Q[0] * r[0] + Q[1] * r[1] + Q[2] * r[2],
+                                         Q[1] * r[0] + Q[3] * r[1] + Q[4] * r[2],
+                                         Q[2] * r[0] + Q[4] * r[1] + Q[5] * r[2]};
+                    const double rtr = r[0] * t[0] + r[1] * t[1] + r[2] * t[2];
+                    const double c = (7.5 * rtr * i2 + (double)Q[6]) * i5;
+                    for (int k = 0; k < 3; k++) {
+                        const double qk = c * r[k] - 3.0 * i5 * t[k];
+                        term[k] += qk;
+                        qabs += qk * qk;
+                    }
+                    quads_all++;
+                }
+                for (int k = 0; k < 3; k++) a[k] += term[k];
+                sabs += sqrt(mono) + sqrt(qabs);
+                n_acc++;
+            }
+            accel[3 * g] = a[0];
+            accel[3 * g + 1] = a[1];
+            accel[3 * g + 2] = a[2];
+            absum[g] = sabs;
+            inter[g] = n_acc;
+            if (maxsp > maxsp_all) maxsp_all = maxsp;
+        }
+        free(stack);
+    }
+    st->opened = opened_all;
+    st->mac_tests = quads_all;
+    st->max_stack = maxsp_all;
+    st->interactions = 0;
+    for (uint32_t g = 0; g < num; g++) st->interactions += inter[g];
+    return overflow ? -1 : 0;
 }
 
 /* Second moment of the bodies of each internal node about its centre of

@@ -96,9 +96,9 @@ def _declare(L: C.CDLL) -> None:
     L.bho_force.restype = C.c_int
     L.bho_force_range.argtypes = L.bho_force.argtypes + [U32, U32]
     L.bho_force_range.restype = C.c_int
-    L.bho_force_quad.argtypes = [P, P, P, U32, U32, P, P, C.POINTER(Params), C.c_float, C.c_float, P, C.c_int, P,
-                                 P, P, C.POINTER(WalkStats), C.c_int]
-    L.bho_force_quad.restype = C.c_int
+    L.bho_force_corrected.argtypes = [P, P, U32, U32, P, C.c_float, C.c_float, C.c_float, C.c_int, P, P, P,
+                                      C.POINTER(WalkStats), C.c_int]
+    L.bho_force_corrected.restype = C.c_int
     L.bho_node_moments.argtypes = [P, P, U32, U32, P, C.c_int]
     L.bho_node_moments.restype = None
     L.bho_merge.argtypes = [P, P, U32]
@@ -198,20 +198,22 @@ def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, 
     return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
 
 
-def force_quad(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, quad, stack_cap=256,
-               threads=0):
-    """Accelerations of the corrected walk: the reference's walk in which every
-    accepted internal node k also adds the quadrupole term of its record
-    quad[k] (8 floats: G T xx xy xz yy yz zz, -1.5 G tr T, unused)."""
-    accel = np.zeros((n_init, 3), np.float32)
-    inter = np.zeros(n_init, np.uint32)
-    st = WalkStats()
-    b = np.ascontiguousarray(bounds, dtype=np.float32)
+def force_corrected(leaf, internal, num, n_init, quad, theta, softening, G=1.0, stack_cap=256, threads=0):
+    """The corrected walk in FP64 over the node set that the GPU's FP32 opening
+    test accepts: per sorted position the acceleration (float64), the sum of
+    the accepted terms' magnitudes and the accepted-node count; quad[k] holds
+    internal node k's G T xx xy xz yy yz zz, -1.5 G tr T (8 floats, the last
+    unused)."""
     q = np.ascontiguousarray(quad, dtype=np.float32)
     assert q.shape == (n_init - 1, 8)
-    rc = lib().bho_force_quad(_p(src), None, _p(index), num, n_init, _p(leaf), _p(internal), C.byref(prm), float(G),
-                              float(blc), _p(b), stack_cap, _p(q), _p(accel), _p(inter), C.byref(st), threads)
-    return {"rc": rc, "accel": accel, "interactions": inter, "stats": st}
+    accel = np.zeros((max(num, 1), 3), np.float64)
+    absum = np.zeros(max(num, 1), np.float64)
+    inter = np.zeros(max(num, 1), np.uint32)
+    st = WalkStats()
+    rc = lib().bho_force_corrected(_p(leaf), _p(internal), num, n_init, _p(q), float(theta), float(softening),
+                                   float(G), stack_cap, _p(accel), _p(absum), _p(inter), C.byref(st), threads)
+    return {"rc": rc, "accel": accel[:num], "absum": absum[:num], "interactions": inter[:num],
+            "opened": int(st.opened), "quad_terms": int(st.mac_tests), "max_stack": int(st.max_stack)}
 
 
 def node_moments(leaf, internal, num, n_init, threads=0) -> np.ndarray:

@@ -7,9 +7,10 @@
   oracle's direct FP64 sums; with a network whose last layer is only a bias
   b, exactly M2 + M h^2 b; with a full network, M2 + M h^2 net(features)
   within BF16 precision.
-* The corrected walk: with all-zero records it is the plain walk bit for bit;
-  with the GPU's records it equals the oracle's corrected walk bit for bit
-  (both loops).
+* The corrected walk (both loops, zero records and the GPU's records): the
+  same accepted nodes as the oracle's corrected walk (its FP32 opening test is
+  reproduced exactly) and accelerations within FP32 rounding of the oracle's
+  FP64 sum over them.
 * Training pairs: FP64 labels against a CPU recomputation.
 * What the correction gains: on a held-out Plummer sphere the quadrupole
   records cut the force error of the plain walk at the same angle, and the
@@ -192,41 +193,36 @@ def _force_with_records(bh, prm, records=None):
 
 
 @pytest.mark.parametrize("general", [False, True])
-def test_corrected_walk_zero_records_equal_plain_walk(general):
+@pytest.mark.parametrize("records", ["zero", "moments"])
+def test_corrected_walk_matches_oracle(general, records):
+    """Counts exact; each body's error at most 2e-5 of the sum of its accepted
+    terms' magnitudes (FP32 accumulation of ~2,000 terms with a 1-ulp inverse
+    square root), and the median relative error below 2e-6. A wrong sign or a
+    dropped term of the quadrupole would leave errors of the quadrupole's own
+    size, ~1e-3 of the acceleration."""
     from gravity_amd import barnes_hut as B
     n = 1 << 16
-    print(f"[gpu] corrected walk with zero records, general loop {general}", flush=True)
-    extra = B.FLAG_GENERAL_WALK if general else 0
-    bh, _, prm = _tree_ctx(n, 7, B.FLAG_QUADRUPOLE | extra)
-    plain = B.Params.from_buffer_copy(prm)
-    plain.flags = extra
-    a_plain = _force_with_records(bh, plain)
-    c_plain = bh.counters()
-    a_zero = _force_with_records(bh, prm, np.zeros((n - 1, 8), np.float32))
-    c_zero = bh.counters()
-    np.testing.assert_array_equal(a_zero.view(np.uint32), a_plain.view(np.uint32))
-    assert c_zero["interactions"] == c_plain["interactions"] and c_zero["opened"] == c_plain["opened"]
-    bh.close()
-
-
-@pytest.mark.parametrize("general", [False, True])
-def test_corrected_walk_matches_oracle(general):
-    from gravity_amd import barnes_hut as B
-    n = 1 << 16
-    print(f"[gpu] corrected walk vs oracle, general loop {general}", flush=True)
+    print(f"[gpu] corrected walk vs oracle, general loop {general}, records {records}", flush=True)
     extra = B.FLAG_GENERAL_WALK if general else 0
     bh, bodies, prm = _tree_ctx(n, 8, B.FLAG_QUADRUPOLE | extra)
-    acc = _force_with_records(bh, prm)
+    acc = _force_with_records(bh, prm, np.zeros((n - 1, 8), np.float32) if records == "zero" else None)
     c = bh.counters()
     q = bh.node_quadrupoles()
+    if records == "zero":
+        assert not q.any()
     idx = bh.index()
     leaf = bh.nodes(B.LEAF_NODES)
     internal = bh.nodes(B.INTERNAL_NODES)
-    r = O.force_quad(bodies, idx, n, n, leaf, internal, O.params(theta=0.3, dt=1.0, softening=0.01), 1.0, 1.0,
-                     bh.bounds(), q, threads=16)
-    assert r["rc"] == 0
+    r = O.force_corrected(leaf, internal, n, n, q, 0.3, 0.01, threads=16)
+    assert r["rc"] == 0 and c["error_flags"] == 0
     assert c["interactions"] == int(r["interactions"].astype(np.int64).sum())
-    np.testing.assert_array_equal(acc[idx].view(np.uint32), r["accel"].view(np.uint32))
+    assert c["quad_terms"] == r["quad_terms"] and c["opened"] == r["opened"]
+    a = acc[idx].astype(np.float64)
+    err = np.linalg.norm(a - r["accel"], axis=1)
+    worst = (err / r["absum"]).max()
+    med = np.median(err / np.linalg.norm(r["accel"], axis=1))
+    print(f"[gpu] corrected walk error: max {worst:.3g} of the term sum, median relative {med:.3g}", flush=True)
+    assert worst <= 2e-5 and med <= 2e-6
     bh.close()
 
 

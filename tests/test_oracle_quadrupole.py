@@ -9,7 +9,8 @@ and the quadrupole term, pinned by closed forms and by physics.
   and the corrected walk's error falls one order faster with distance than
   the plain walk's (a wrong coefficient, sign or trace term would leave an
   error of the monopole's order).
-* With all-zero records the corrected walk is the plain walk bit for bit.
+* With all-zero records the corrected walk accepts the plain walk's nodes and
+  gives its accelerations within FP32 rounding.
 """
 import numpy as np
 
@@ -130,7 +131,7 @@ def test_corrected_walk_error_falls_one_order_faster():
         t = int(np.nonzero(sidx == 63)[0][0])
         exact = O.direct_sum_f64(b, np.array([63], np.uint32), G=1.0, softening=1e-6)[0]
         plain = O.force(b, sidx, 64, 64, leaf, internal, prm, 1.0, 1.0, bounds, stack_cap=256)
-        corr = O.force_quad(b, sidx, 64, 64, leaf, internal, prm, 1.0, 1.0, bounds, q)
+        corr = O.force_corrected(leaf, internal, 64, 64, q, 0.3, 1e-6)
         assert plain["rc"] == 0 and corr["rc"] == 0
         assert plain["interactions"][t] == 1 and corr["interactions"][t] == 1   # one node: the whole cluster
         ep = np.linalg.norm(plain["accel"][t] - exact) / np.linalg.norm(exact)
@@ -142,7 +143,11 @@ def test_corrected_walk_error_falls_one_order_faster():
     assert 0.07 < q2 / q1 < 0.18          # corrected: ~ (b / s)^3
 
 
-def test_zero_records_give_the_plain_walk_bit_for_bit():
+def test_zero_records_give_the_plain_walk():
+    """With zero records the corrected walk's opening test (theta^2 s^2 >
+    half^2 in FP32) accepts the nodes of the reference's (half / s < theta)
+    except at ties, and its FP64 sum is the plain walk's FP32 sum within the
+    FP32 rounding of the terms."""
     rng = np.random.default_rng(5)
     n = 3000
     b = np.zeros((n, 8), np.float32)
@@ -151,14 +156,35 @@ def test_zero_records_give_the_plain_walk_bit_for_bit():
     sidx, leaf, internal, bounds = _tree(b)
     prm = O.params(theta=0.3, dt=1.0, softening=0.01)
     plain = O.force(b, sidx, n, n, leaf, internal, prm, 1.0, 1.0, bounds, stack_cap=256)
-    zero = O.force_quad(b, sidx, n, n, leaf, internal, prm, 1.0, 1.0, bounds, np.zeros((n - 1, 8), np.float32))
-    np.testing.assert_array_equal(plain["accel"].view(np.uint32), zero["accel"].view(np.uint32))
+    zero = O.force_corrected(leaf, internal, n, n, np.zeros((n - 1, 8), np.float32), 0.3, 0.01)
+    assert zero["rc"] == 0 and zero["quad_terms"] > 0
+    same = plain["interactions"] == zero["interactions"]
+    assert same.mean() >= 0.999
+    err = np.linalg.norm(plain["accel"].astype(np.float64) - zero["accel"], axis=1)
+    assert (err[same] <= 1e-5 * zero["absum"][same]).all()
+    assert np.median(err / np.linalg.norm(zero["accel"], axis=1)) < 1e-6
     # and the records reduce the force error against the FP64 direct sum
     q = _records(O.node_moments(leaf, internal, n, n))
-    corr = O.force_quad(b, sidx, n, n, leaf, internal, prm, 1.0, 1.0, bounds, q)
+    corr = O.force_corrected(leaf, internal, n, n, q, 0.3, 0.01)
     samp = np.arange(0, n, 7)
     exact = O.direct_sum_f64(b, sidx[samp].astype(np.uint32), G=1.0, softening=0.01)
     ep = np.linalg.norm(plain["accel"][samp] - exact, axis=1) / np.linalg.norm(exact, axis=1)
     eq = np.linalg.norm(corr["accel"][samp] - exact, axis=1) / np.linalg.norm(exact, axis=1)
     assert np.percentile(eq, 50) < 0.33 * np.percentile(ep, 50)
     assert np.percentile(eq, 99) < 0.5 * np.percentile(ep, 99)
+
+
+def test_corrected_walk_self_and_counts():
+    """Two bodies: each walk opens the root and takes the other's leaf (never
+    its own); with records, an accepted node's term is the closed form of the
+    monopole plus quadrupole."""
+    b = np.zeros((2, 8), np.float32)
+    b[0, :4] = [0.0, 0.0, 0.0, 1.0]
+    b[1, :4] = [1.0, 0.0, 0.0, 3.0]
+    sidx, leaf, internal, _ = _tree(b)
+    r = O.force_corrected(leaf, internal, 2, 2, np.zeros((1, 8), np.float32), 0.5, 0.0 + 1e-12)
+    assert list(r["interactions"]) == [1, 1] and r["opened"] == 2 and r["quad_terms"] == 0
+    a = r["accel"][np.argsort(sidx)]            # by body id
+    s3 = (1.0 + 1e-12) ** 1.5
+    np.testing.assert_allclose(a[0], [3.0 / s3, 0, 0], rtol=1e-6)
+    np.testing.assert_allclose(a[1], [-1.0 / s3, 0, 0], rtol=1e-6)
