@@ -172,10 +172,12 @@ def timed_loop(bh, prm, steps: int, warmup: int) -> dict:
             "stage_ms": {k: v / steps for k, v in stage.items()}, "counters": c, "num": num}
 
 
-def train_network(soft: float, theta: float, seed: int) -> tuple:
+def train_network(soft: float, theta: float, tolerance: float, seed: int) -> tuple:
     """The correction's network trained on the GPU: (body, accepted node)
-    pairs of the walk at `theta` on a 1M-body Plummer and a 1M-body two-galaxy
-    snapshot, FP64 labels (the residual after monopole and exact quadrupole)."""
+    pairs of the corrected walk's geometric test at `theta` on a 1M-body
+    Plummer and a 1M-body two-galaxy snapshot, FP64 labels (the residual after
+    monopole and exact quadrupole); its output scale calibrated with the
+    relative test at `tolerance` on a third snapshot."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
     from gravity_amd import learned as Lr
@@ -203,30 +205,38 @@ def train_network(soft: float, theta: float, seed: int) -> tuple:
     loss1, _ = mlp.loss_grad(x[hold], t[hold])
     log(f"learned: trained on {train.size} pairs in {t_train:.1f} s, held-out loss {loss0:.4g} -> {loss1:.4g}")
     t2 = time.perf_counter()
-    alpha, cal = Lr.calibrate(mlp, ic.plummer(1 << 20, seed + 13), theta, soft)
+    alpha, cal = Lr.calibrate(mlp, ic.plummer(1 << 20, seed + 13), 0.5, soft, force_tolerance=tolerance)
     log(f"learned: output scale {alpha} on a validation snapshot ({time.perf_counter() - t2:.1f} s): {cal}")
     return mlp, {"train_pairs": int(train.size), "held_out_pairs": int(hold.size), "data_s": t_data,
                  "train_s": t_train, "held_out_loss_zero_network": loss0, "held_out_loss_trained": loss1,
-                 "train_theta": theta, "output_scale": alpha,
+                 "train_theta": theta, "calibration_force_tolerance": tolerance, "output_scale": alpha,
                  "calibration": [{"alpha": a, "p50": p50, "p99": p99} for a, p50, p99 in cal],
                  "calibration_snapshot": "Plummer 1,048,576 (a seed of its own), 4,096 sampled bodies"}
 
 
-def sweep(bh, saved, sample, exact, soft: float, method: str, thetas, flags: int) -> list:
-    """p50/p99 on the frozen state `saved` and step time of each angle, in
-    order, until the first angle whose p99 is at the target."""
+def sweep(bh, saved, sample, exact, soft: float, method: str, values, flags: int, relative: bool = False) -> list:
+    """p50/p99 on the frozen state `saved` and step time of each value of the
+    opening parameter (theta, or with `relative` the relative test's
+    tolerance at theta 0.5 for its first walk), in order, until the first
+    value whose p99 is at the target."""
     from gravity_amd import barnes_hut as B
     rows = []
-    for th in thetas:
-        prm = B.make_params(theta=th, softening=soft, flags=flags)
+    for v in values:
+        if relative:
+            prm = B.make_params(theta=0.5, softening=soft, flags=flags | B.FLAG_RELATIVE_MAC, force_tolerance=v)
+        else:
+            prm = B.make_params(theta=v, softening=soft, flags=flags)
         bh.upload(B.BODIES_IN, saved)
         err = error_stats(bh.accelerations(prm)[sample], exact)
         ms = variant_time(bh, prm)
         c = bh.counters()
-        row = {"method": method, "theta": th, "ms_per_step": ms, "body_steps_per_s": bh.n * 1e3 / ms,
-               "p50": err["p50"], "p99": err["p99"], "interactions_per_body": c["interactions"] / bh.n}
+        row = {"method": method, "mac": "relative" if relative else "geometric",
+               ("force_tolerance" if relative else "theta"): v, "ms_per_step": ms,
+               "body_steps_per_s": bh.n * 1e3 / ms, "p50": err["p50"], "p99": err["p99"],
+               "interactions_per_body": c["interactions"] / bh.n}
         rows.append(row)
-        log(f"sweep {method} theta {th}: {ms:.1f} ms, p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+        log(f"sweep {method} {row['mac']} {v}: {ms:.1f} ms, p50 {err['p50']:.3g} p99 {err['p99']:.3g}, "
+            f"{row['interactions_per_body']:.0f} interactions/body")
         if err["p99"] <= P99_TARGET:
             break
     return rows
@@ -395,9 +405,10 @@ def main() -> int:
     ap.add_argument("--ensemble-sims", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-walk-sample", type=int, default=1048576)
-    ap.add_argument("--profile-step", type=float, default=None, metavar="THETA",
-                    help="only the headline's timed loop: the learned-corrected step at THETA with an untrained "
-                         "(zero-output) network, which runs the same kernels; for rocprofv3 profiles")
+    ap.add_argument("--profile-step", type=float, default=None, metavar="ALPHA",
+                    help="only the headline's timed loop: the learned-corrected step with the relative opening "
+                         "test at tolerance ALPHA and an untrained (zero-output) network, which runs the same "
+                         "kernels; for rocprofv3 profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -423,7 +434,8 @@ def main() -> int:
     if args.profile_step is not None:
         mlp = Lr.Mlp(params=Lr.init_params(args.seed))      # zero output layer: no residual
         Lr.attach(bh, mlp)
-        prm = B.make_params(theta=args.profile_step, dt=dt, softening=soft, flags=B.FLAG_LEARNED)
+        prm = B.make_params(theta=0.5, dt=dt, softening=soft, flags=B.FLAG_LEARNED | B.FLAG_RELATIVE_MAC,
+                            force_tolerance=args.profile_step)
         run = timed_loop(bh, prm, args.steps, args.warmup)
         c = run["counters"]
         walk_ms = run["stage_ms"]["force"]
@@ -443,7 +455,7 @@ def main() -> int:
     log(f"plain theta 0.5: {ref['ms_per_step']:.2f} ms/step, {ref['body_steps_per_s']:.4g} body-steps/s")
 
     # 2. the network
-    mlp, train_info = train_network(soft, 0.3, args.seed)
+    mlp, train_info = train_network(soft, 0.6, 1.5e-3, args.seed)
     Lr.attach(bh, mlp)
 
     # 3. sweeps on one frozen state against one FP64 judge
@@ -451,22 +463,25 @@ def main() -> int:
     sample = sample_ids(saved)
     exact = bh.direct_sum_f64(sample, soft)
     plain_rows = sweep(bh, saved, sample, exact, soft, "plain", (0.5, 0.3, 0.2, 0.15, 0.12, 0.1, 0.08, 0.06), 0)
-    quad_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", (0.5, 0.4, 0.35, 0.3, 0.27, 0.25, 0.22, 0.2),
-                      B.FLAG_QUADRUPOLE)
-    learned_rows = sweep(bh, saved, sample, exact, soft, "learned", (0.5, 0.4, 0.35, 0.3, 0.27, 0.25, 0.22, 0.2),
-                         B.FLAG_LEARNED)
+    geo_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", (0.8, 0.7, 0.6, 0.5, 0.45, 0.4, 0.35),
+                     B.FLAG_QUADRUPOLE)
+    tolerances = (0.004, 0.003, 0.002, 0.0015, 0.001, 0.0007, 0.0005)
+    quad_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", tolerances, B.FLAG_QUADRUPOLE, relative=True)
+    learned_rows = sweep(bh, saved, sample, exact, soft, "learned", tolerances, B.FLAG_LEARNED, relative=True)
     plain_best = fastest_at_target(plain_rows)
     quad_best = fastest_at_target(quad_rows)
     learned_best = fastest_at_target(learned_rows)
 
-    # 4. the timed loop: learned-corrected at its widest angle within the target
+    # 4. the timed loop: learned-corrected at its largest tolerance within the target
     pick = learned_best or learned_rows[-1]
-    headline = B.make_params(theta=pick["theta"], dt=dt, softening=soft, flags=B.FLAG_LEARNED)
+    headline = B.make_params(theta=0.5, dt=dt, softening=soft, flags=B.FLAG_LEARNED | B.FLAG_RELATIVE_MAC,
+                             force_tolerance=pick["force_tolerance"])
     bh.upload(B.BODIES_IN, saved)
     run = timed_loop(bh, headline, args.steps, args.warmup)
     c = run["counters"]
     num = run["num"]
-    log(f"timed {args.steps} learned-corrected steps (theta {pick['theta']}): {run['ms_per_step']:.2f} ms/step; "
+    log(f"timed {args.steps} learned-corrected steps (tolerance {pick['force_tolerance']}): "
+        f"{run['ms_per_step']:.2f} ms/step; "
         f"counters {c}")
     walk_ms = run["stage_ms"]["force"]
     flops = c["interactions"] * FLOP_PER_INTERACTION + c["quad_terms"] * FLOP_PER_QUAD_TERM
@@ -484,11 +499,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plummer_{n}_learned_corrected_at_p99_1e-3", "n_bodies": n, "theta": pick["theta"],
-                   "dt": dt, "softening": soft, "merging": "off", "bounds": "dynamic",
+        "config": {"workload": f"plummer_{n}_learned_corrected_at_p99_1e-3", "n_bodies": n,
+                   "force_tolerance": pick["force_tolerance"], "dt": dt, "softening": soft, "merging": "off",
+                   "bounds": "dynamic",
                    "method": "learned-corrected Barnes-Hut: monopole + softened quadrupole of the exact second "
                              "moment plus the network's residual, one network evaluation per node and step",
-                   "mac": "half longest AABB side x inverse softened distance < theta (the reference's)",
+                   "mac": "relative: G M b^3 / s^5 < alpha |a_prev| and s > b (b: centre of mass to the farthest "
+                          "box corner), decided for 64 bodies together (a node taken only if all accept it)",
                    "p50": pick["p50"], "p99": pick["p99"], "error_sample": int(sample.size),
                    "p99_target": P99_TARGET, "p99_met": bool(pick["p99"] <= P99_TARGET)},
         "roofline": {"bound": "valu", "kernel": "k_walk_q (corrected)", "achieved": achieved,
@@ -512,7 +529,8 @@ def main() -> int:
                                             if quad_best and plain_best else None),
             "speedup_learned_vs_plain_theta_0_5": (learned_best["body_steps_per_s"] / ref["body_steps_per_s"]
                                                    if learned_best else None),
-            "sweeps": plain_rows + quad_rows + learned_rows},
+            "quadrupole_geometric": fastest_at_target(geo_rows),
+            "sweeps": plain_rows + geo_rows + quad_rows + learned_rows},
         "learned_training": train_info,
         "plain_theta_0_5": {"ms_per_step": ref["ms_per_step"], "body_steps_per_s": ref["body_steps_per_s"],
                             "stage_ms": ref["stage_ms"], "p50": plain_rows[0]["p50"], "p99": plain_rows[0]["p99"],

@@ -29,6 +29,7 @@ NODE_QUADRUPOLES = 16   # the corrected walk's records (no reference counterpart
 NEITHER, MERGING, COLLISION = 0, 1, 2
 STATIC, DYNAMIC = 0, 1
 FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK, FLAG_QUADRUPOLE = 1, 2, 4, 8, 16
+FLAG_RELATIVE_MAC = 32
 FLAG_CORRECTED = FLAG_LEARNED | FLAG_QUADRUPOLE
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
 NOALLOC_MAX_SAMPLE = 65536   # BH_NOALLOC_MAX_SAMPLE
@@ -45,7 +46,7 @@ class Params(C.Structure):
                 ("static_or_dynamic", C.c_uint32), ("wrap_around", C.c_uint32), ("unit_mass", C.c_float),
                 ("unit_density", C.c_float), ("unit_len", C.c_float), ("unit_time", C.c_float),
                 ("gravitational_constant", C.c_float), ("body_length_constant", C.c_float),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("force_tolerance", C.c_float)]
 
 
 class Counters(C.Structure):
@@ -96,12 +97,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 
 def make_params(theta=0.5, dt=1e-3, softening=0.01, elasticity=0.5, restitution=0.0, merging=NEITHER,
-                dynamic=DYNAMIC, wrap_around=0, units=None, G=0.0, body_length_constant=0.0, flags=0) -> Params:
+                dynamic=DYNAMIC, wrap_around=0, units=None, G=0.0, body_length_constant=0.0, flags=0,
+                force_tolerance=0.0) -> Params:
     if units is None:
         from .initial_conditions import g_one_units
         units = g_one_units()
     return Params(theta, dt, softening, elasticity, restitution, merging, dynamic, wrap_around, units.mass,
-                  units.density, units.len, units.time, G, body_length_constant, flags)
+                  units.density, units.len, units.time, G, body_length_constant, flags, force_tolerance)
 
 
 class BarnesHut:
@@ -242,7 +244,7 @@ class BarnesHut:
         z[:, 4:7] = 0.0
         p1 = Params.from_buffer_copy(params or self.params)
         p1.dt = 1.0
-        p1.flags &= FLAG_CORRECTED | FLAG_GENERAL_WALK
+        p1.flags &= FLAG_CORRECTED | FLAG_GENERAL_WALK | FLAG_RELATIVE_MAC
         p1.merging_collision_or_neither = NEITHER
         p1.static_or_dynamic = DYNAMIC
         self.upload(BODIES_IN, z)

@@ -80,12 +80,16 @@ struct bh_context {
     uint32_t* sort_pos[2] = {nullptr, nullptr};
     bool colors = false;
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
-    // corrected walk (allocated on first use): the fused 64-byte records of the
-    // internal nodes (walk record, then quadrupole terms) and the FP64 moment
-    // prefix sums they are made from. quad_valid: the records belong to the
-    // current tree (made by its tree stage under a corrected-walk flag).
-    float4* qrec = nullptr;                   // [4(n-1)]
+    // corrected walk (allocated on first use): the 64-byte records of every
+    // node id (layout with corr_records) and the FP64 moment prefix sums they
+    // are made from. quad_valid: the records belong to the current tree (made
+    // by its tree stage under a corrected-walk flag). acc_mag: |a| by body id
+    // from the last corrected walk, for the relative opening test (accmag_valid:
+    // a corrected walk has run since the bodies were last set from the host).
+    float4* qrec = nullptr;                   // [4 corr_records(n)]
     bool quad_valid = false;
+    float* acc_mag = nullptr;                 // [n]
+    bool accmag_valid = false;
     double* mom_prefix = nullptr;             // [mom_prefix_doubles(n)]
     double* mom_tiles = nullptr;              // [mom_tile_doubles(n)]
 
@@ -375,7 +379,7 @@ void bh_destroy(bh_context* c) {
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
-                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles};
+                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles, c->acc_mag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -426,13 +430,14 @@ static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
     return BH_OK;
 }
 
-// the corrected walk's arrays, allocated on first use (about 210 bytes per body)
+// the corrected walk's arrays, allocated on first use (about 215 bytes per body)
 static int ensure_quad(bh_context* c) {
     if (c->qrec) return BH_OK;
     int rc;
     if ((rc = dalloc(c, &c->mom_prefix, mom_prefix_doubles(c->n)))) return rc;
     if ((rc = dalloc(c, &c->mom_tiles, mom_tile_doubles(c->n)))) return rc;
     if ((rc = dalloc(c, &c->qrec, 4 * corr_records(c->n)))) return rc;
+    if ((rc = dalloc(c, &c->acc_mag, c->n))) return rc;
     // the quadrupole halves of the leaf records stay zero
     HIPCHECK(c, hipMemsetAsync(c->qrec, 0, 64 * corr_records(c->n), c->stream));
     return BH_OK;
@@ -455,6 +460,7 @@ static int quad_binding_copy(bh_context* c, void* host, size_t off, size_t bytes
 int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) {
     if (!c || (!host && bytes)) return BH_ERR_INVALID_ARGUMENT;
     c->step_pos_valid = false;
+    if (b == BH_BINDING_BODIES_IN || b == BH_BINDING_BODIES_OUT) c->accmag_valid = false;
     int rc = check_range(c, b, off, bytes);
     if (rc) return rc;
     HIPCHECK(c, hipSetDevice(c->device));
@@ -546,6 +552,7 @@ int bh_swap_morton_and_index_buffers(bh_context* c) {
 int bh_init(bh_context* c) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
     c->step_pos_valid = false;
+    c->accmag_valid = false;
     HIPCHECK(c, hipSetDevice(c->device));
     launch_init(c->stream, c->index[c->mi_in], c->n);
     return check_launch(c, "init");
@@ -701,9 +708,23 @@ int compute_force(bh_context* c, const bh_params* p, float4* step_pos, bool* wro
             return fail(c, BH_ERR_INVALID_ARGUMENT,
                         "no records of the current tree: build it with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED first");
         a.qrec = c->qrec;
+        a.acc_mag = c->acc_mag;
+        if (p->flags & BH_FLAG_RELATIVE_MAC) {
+            if (!(p->force_tolerance > 0.0f))
+                return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_RELATIVE_MAC needs force_tolerance > 0");
+            if (!c->accmag_valid) {
+                // no |a| yet: a walk with the geometric test at theta first,
+                // whose counts are then cleared (its output is overwritten)
+                launch_walk(c->stream, c->n, a);
+                launch_walk_counters_reset(c->stream, c->sim, c->ctr);
+            }
+            a.rel = 1;
+            a.alpha = p->force_tolerance;
+        }
     }
     launch_walk(c->stream, c->n, a);
     const int rc = check_launch(c, "walk");
+    if (rc == BH_OK && a.qrec) c->accmag_valid = true;
     *wrote = rc == BH_OK && step_pos != nullptr;
     return rc;
 }
@@ -970,8 +991,8 @@ int bh_collect_training_pairs(bh_context* c, float theta, uint32_t stride, float
     HIPCHECK(c, hipMalloc((void**)&count, 4));
     hipError_t e = hipMemsetAsync(count, 0, 4, c->stream);
     if (e == hipSuccess) {
-        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, theta, softening, stride, pairs,
-                             cap, count);
+        launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb, theta,
+                             softening, stride, pairs, cap, count);
         e = hipGetLastError();
     }
     uint32_t found = 0;

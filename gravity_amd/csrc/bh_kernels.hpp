@@ -79,12 +79,17 @@ struct WalkArgs {
     uint32_t* queues = nullptr;      // [kWalkQueues * 32] per-CU work queues (persistent walk), or nullptr
     // the corrected walk: its records [4 corr_records(n)], or nullptr
     const float4* qrec = nullptr;
+    // the corrected walk's |a| by body id [n]: read by the relative opening
+    // test (rel), written by every corrected walk
+    float* acc_mag = nullptr;
+    uint32_t rel = 0;
+    float alpha = 0.0f;
 };
 
 // The corrected walk's records, 64 bytes per node id (four float4):
-//   (centre of mass, G M), (half, childA, childB, half^2),
+//   (centre of mass, G M), (K of corr_rel_k, childA, childB, b^2 of corr_size2),
 //   G (T_xx, T_xy, T_xz, T_yy), (G T_yz, G T_zz, -1.5 G tr T, 0).
-// Leaf j is record j (its position and G m, half 0, half^2 -1, no children,
+// Leaf j is record j (its position and G m, K -1, b^2 -1, no children,
 // zero quadrupole terms); internal node k is record n + kCorrGap + k, and the
 // child ids are in the same numbering, so every internal id is at least 64
 // above every leaf id. The kCorrGap records in between are never read.
@@ -93,6 +98,34 @@ __host__ __device__ inline uint32_t corr_internal_id(uint32_t n, uint32_t k) { r
 __host__ __device__ inline size_t corr_records(uint32_t n) { return 2 * (size_t)n - 1 + kCorrGap; }
 // every record's byte offset fits 32 bits
 __host__ __device__ inline bool corr_offsets_32bit(uint32_t n) { return corr_records(n) <= (1ull << 26); }
+
+// The size in the corrected walk's opening test (record word 7): the squared
+// distance b^2 from the node's centre of mass to the farthest corner of its
+// box, which bounds the distance to every body of the node; the walk accepts
+// the node when fl(fl(s^2) theta^2) > b^2, s^2 = fma(x, x, fma(y, y, fma(z, z,
+// eps^2))). A node whose centre of mass sits off the box centre counts as
+// larger than its half side, in step with the error of its expansion.
+__device__ __forceinline__ float corr_size2(float4 com, float4 mn, float4 mx) {
+    const float dx = fmaxf(com.x - mn.x, mx.x - com.x);
+    const float dy = fmaxf(com.y - mn.y, mx.y - com.y);
+    const float dz = fmaxf(com.z - mn.z, mx.z - com.z);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+// The relative opening test: a node's next-order (octupole) error is about
+// G M b^3 / s^5; the walk accepts it where that is below alpha |a| (|a| the
+// body's acceleration in the previous corrected walk), raised to the power
+// 2/5: fl(s^2) A > K with K = (G M b^3)^(2/5) per node and A = (alpha |a|)^(2/5)
+// per body, both in FP64 rounded to FP32, together with s^2 > b^2 (the body
+// outside the sphere around the centre of mass that holds the node's bodies).
+__device__ __forceinline__ float corr_rel_k(float gm, float b2) {
+    return (float)(pow((double)gm, 0.4) * pow((double)b2, 0.6));
+}
+__device__ __forceinline__ float corr_rel_a(float alpha, float amag) {
+    return (float)pow((double)alpha * (double)amag, 0.4);
+}
+__device__ __forceinline__ float corr_dist2(float rx, float ry, float rz, float soft) {
+    return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, soft)));
+}
 
 // Second moments of the corrected walk: positions per tile of the FP64
 // prefix sums (m, m x, m x x^T: 10 components per position).
@@ -174,6 +207,7 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos = nullptr, const uint32_t* from = nullptr, uint32_t* ids_out = nullptr);
+void launch_walk_counters_reset(hipStream_t s, const SimValues* sim, DevCounters* ctr);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 // The corrected walk's node records on the current tree: FP64 prefix sums of
 // the moments (prefix: mom_prefix_doubles(n), tiles: mom_tile_doubles(n)),
@@ -185,8 +219,8 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
                              double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, float theta, float soft, uint32_t stride, uint2* pairs, uint32_t cap,
-                          uint32_t* count);
+                          const WalkNode* nodes, const float4* node_aabb, float theta, float soft, uint32_t stride,
+                          uint2* pairs, uint32_t cap, uint32_t* count);
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,
                         float soft, uint32_t max_bodies, float* feat, float* targ);

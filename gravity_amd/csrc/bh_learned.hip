@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
     for (uint32_t j = (blockIdx.x * 4 + w) * 64 + lane; j < num; j += waves * 64) {
         const WalkNode lf = walk[j];
         qrec[4 * (size_t)j] = lf.comMass;
-        qrec[4 * (size_t)j + 1] = make_float4(lf.half, __uint_as_float(0u), __uint_as_float(0u), lf.half2);
+        qrec[4 * (size_t)j + 1] = make_float4(-1.0f, __uint_as_float(0u), __uint_as_float(0u), lf.half2);
     }
     MlpRegs R;
     if (wg) mlp_load(R, wg, bg);
@@ -193,7 +193,8 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
             const uint32_t b = nd.childB >= n ? nd.childB + kCorrGap : nd.childB;
             float4* rec = qrec + 4 * (size_t)corr_internal_id(n, k);
             rec[0] = nd.comMass;
-            rec[1] = make_float4(nd.half, __uint_as_float(a), __uint_as_float(b), nd.half2);
+            const float b2 = corr_size2(com, mn, mx);
+            rec[1] = make_float4(corr_rel_k(nd.comMass.w, b2), __uint_as_float(a), __uint_as_float(b), b2);
             rec[2] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
             rec[3] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
         }
@@ -214,16 +215,19 @@ void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, De
 
 // ------------------------------------------------------------------ pairs
 // The pairs of the corrected walk: every stride-th sorted body walks with the
-// reference's opening test at theta and records each internal node it accepts.
+// corrected walk's opening test at theta (corr_size2) and records each
+// internal node it accepts.
 __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restrict__ sim,
                                                        const float4* __restrict__ leaf_pos,
-                                                       const WalkNode* __restrict__ nodes, float theta, float soft,
+                                                       const WalkNode* __restrict__ nodes,
+                                                       const float4* __restrict__ node_aabb, float theta, float soft,
                                                        uint32_t stride, uint2* __restrict__ pairs, uint32_t cap,
                                                        uint32_t* __restrict__ count) {
     const uint32_t n = sim->initialNumBodies, num = sim->numBodies;
     const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) * stride;
     if (g >= num || num < 2) return;
     const float4 pm = leaf_pos[g];
+    const float th2 = theta * theta;
     uint32_t stack[kStackDepth];
     int sp = 0;
     stack[sp++] = n;
@@ -231,9 +235,9 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
         const uint32_t node = stack[--sp];
         if (node < n) continue;
         const WalkNode nd = nodes[node - n];
-        const float rx = fsub(nd.comMass.x, pm.x), ry = fsub(nd.comMass.y, pm.y), rz = fsub(nd.comMass.z, pm.z);
-        const float inv = inv_sqrt_exact(fadd(dot3(rx, ry, rz), soft));
-        if (fmul(nd.half, inv) < theta) {
+        const float rx = nd.comMass.x - pm.x, ry = nd.comMass.y - pm.y, rz = nd.comMass.z - pm.z;
+        const float b2 = corr_size2(nd.comMass, node_aabb[2 * (node - n)], node_aabb[2 * (node - n) + 1]);
+        if (corr_dist2(rx, ry, rz, soft) * th2 > b2) {
             uint32_t slot = atomicAdd(count, 1u);
             if (slot < cap) pairs[slot] = make_uint2(g, node - n);
         } else if (sp + 2 <= kStackDepth) {
@@ -515,10 +519,11 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(const float* __restric
 static inline uint32_t cdiv2(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
-                          const WalkNode* nodes, float theta, float soft, uint32_t stride, uint2* pairs, uint32_t cap,
-                          uint32_t* count) {
+                          const WalkNode* nodes, const float4* node_aabb, float theta, float soft, uint32_t stride,
+                          uint2* pairs, uint32_t cap, uint32_t* count) {
     uint32_t threads = (n + stride - 1) / stride;
-    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, theta, soft, stride, pairs, cap, count);
+    k_collect_pairs<<<cdiv2(threads, 256), 256, 0, s>>>(sim, leaf_pos, nodes, node_aabb, theta, soft, stride, pairs,
+                                                        cap, count);
 }
 void launch_pair_labels(hipStream_t s, const uint2* pairs, uint32_t npairs, const float4* leaf_pos,
                         const WalkNode* nodes, const float4* node_aabb, const uint2* node_range, double G,

@@ -321,59 +321,92 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
     o.err = err;
 }
 
-// The corrected walk (no reference counterpart): the same masked-stack walk
-// and visiting order, in which every accepted internal node adds its softened
-// monopole and quadrupole,
+// The corrected walk (no reference counterpart). Every node a body takes adds
+// its softened monopole and quadrupole,
 //     a += G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5,
-// s^2 = |r|^2 + eps^2, and every leaf its monopole. It reads one 64-byte
-// record per node id (see corr_records in bh_kernels.hpp: leaves have zero
-// quadrupole terms and half^2 = -1, so one sequence serves both kinds). It
-// needs no bit parity with the reference, so it takes the shortest FP32
-// sequence: s^2 by fused multiply-adds, the acceptance test half < theta s as
-// theta^2 s^2 > half^2 (no root; exactly reproducible on the host), the
-// hardware 1/sqrt (within 1 ulp) and one fused update of both terms. The
-// lane whose own leaf a record is skips it (its r is 0, so it would add
-// exactly 0, but it does not count as an interaction). kLds: stack in LDS
-// (any tree height, overflow checked, 64-bit record offsets) instead of lanes
-// of three registers (trees at most kLeanHeight high, offsets below 2^32).
+// s^2 = |r|^2 + eps^2 (a leaf: its monopole), from one 64-byte record per node
+// id (corr_records in bh_kernels.hpp: leaves have zero quadrupole terms and
+// b^2 = K = -1). It needs no bit parity with the reference, so it takes the
+// shortest FP32 sequence: s^2 by fused multiply-adds, opening tests without a
+// root (exactly reproducible on the host), the hardware 1/sqrt (within 1 ulp)
+// and one fused update of both terms.
+//
+// Decisions are wave-wide: the 64 bodies of a wave take a node together when
+// every lane's test accepts it (the geometric fl(s^2) theta^2 > b^2, or the
+// relative s^2 > b^2 and fl(s^2) A > K), and open it together otherwise. The
+// lane that needs the finer nodes decides for its neighbours, which then add
+// more, smaller terms (never less accurate than their own test asks); the
+// mask never changes, so the stack holds node ids only and every internal
+// node taken adds a term in all lanes. Order: childB before childA, as the
+// reference's walk. The lane whose own leaf a record is skips it (r = 0 would
+// add exactly 0; it does not count as an interaction). kLds: stack in LDS (any
+// tree height, overflow checked, 64-bit record offsets) instead of one lane
+// register (trees at most kLeanHeight high, offsets below 2^32).
 // Needs eps^2 >= 1e-14 (1/s^5 stays finite) and the fast domain's bounds.
 __device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
     return soft >= 1e-14f && inv_sqrt_fast_domain(root_half, soft);
 }
 
-template <bool kLds>
-__device__ __forceinline__ void walk_corrected(uint32_t n_ids, uint32_t base, unsigned long long mask,
-                                               const float4& pm, const float4* __restrict__ crec, float theta,
-                                               float soft, uint32_t* __restrict__ s_node,
-                                               unsigned long long* __restrict__ s_mask, WalkAcc& o) {
+template <bool kLds, bool kRel>
+__device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t base, unsigned long long mask,
+                                                     const float4& pm, const float4* __restrict__ crec, float afac,
+                                                     float soft, uint32_t* __restrict__ s_node, WalkAcc& o) {
     typedef typename std::conditional<kLds, unsigned long long, uint32_t>::type Count;
     float ax = o.ax, ay = o.ay, az = o.az;
-    Count inter = 0, leaves = 0, opened = 0;
+    Count takes = 0, opens = 0, leafint = 0;
     uint32_t maxsp = o.maxsp, err = o.err;
-    const float th2 = theta * theta;
     const char* __restrict__ cbase = reinterpret_cast<const char*>(crec);
+    auto rec = [&](uint32_t id) {
+        return kLds ? *reinterpret_cast<const u32x16*>(cbase + ((size_t)id << 6))
+                    : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(id << 6));
+    };
+    auto touch = [&](uint32_t id) {
+        return kLds ? *reinterpret_cast<const uint32_t*>(cbase + ((size_t)id << 6))
+                    : *reinterpret_cast<const uint32_t*>(cbase + (uint32_t)(id << 6));
+    };
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
     const uint32_t first_internal = corr_internal_id(n_ids, 0);
-    int st_node = 0, st_lo = 0, st_hi = 0;   // lane stack: entry i in lane i
+    // lane stack: entry i in lane i; every value it ever holds is a node id,
+    // so a read above the top still names a record inside the array
+    int st_node = 0;
     uint32_t node = first_internal;          // the root
     int sp = 0;
+    // The next node is the stack top if this one is taken, its childB if it
+    // is opened. One word of each is loaded while this node is evaluated, so
+    // that the next visit's whole record comes from the scalar cache (a
+    // 64-byte line: one record); the words only feed `sink`, which keeps the
+    // loads. (A leaf's childB is 0, a leaf. On an empty lane stack the top is
+    // lane 63, which starts at 0 and is only ever written with node ids; an
+    // empty LDS stack gives the root.)
+    uint32_t sink = 0, t_b = 0;
     for (;;) {
-        const u32x16 r = kLds ? *reinterpret_cast<const u32x16*>(cbase + ((size_t)node << 6))
-                              : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(node << 6));
+        // (the LDS stack is not initialised: with sp == 0 its top is not read)
+        uint32_t top_node;
+        if constexpr (kLds) top_node = sp > 0 ? uniform(s_node[sp - 1]) : first_internal;
+        else top_node = (uint32_t)__builtin_amdgcn_readlane(st_node, sp - 1);
+        const u32x16 r = rec(node);
+        sink += touch(top_node) ^ t_b;
+        t_b = touch(r[6]);
         const float rx = __uint_as_float(r[0]) - pm.x;
         const float ry = __uint_as_float(r[1]) - pm.y;
         const float rz = __uint_as_float(r[2]) - pm.z;
-        const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, soft)));
-        const unsigned long long acc = ballot(d2 * th2 > __uint_as_float(r[7]));
-        unsigned long long take = acc & mask;
-        const unsigned long long open = mask & ~acc;
-        if (node < first_internal) {
-            // a leaf (rare): not for the lane whose own body it is
-            const uint32_t rel = node - wbase;
-            if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
-            leaves += (Count)__popcll(take);
-        }
-        {
+        const float d2 = corr_dist2(rx, ry, rz, soft);
+        // afac: theta^2 (the geometric test) or A (the relative one); a leaf
+        // is always taken (never opened, whatever its distance holds)
+        const bool leaf = node < first_internal;
+        const float b2 = __uint_as_float(r[7]);
+        const bool ok = kRel ? (d2 > b2 && d2 * afac > __uint_as_float(r[4])) : d2 * afac > b2;
+        const bool all = leaf || (ballot(!ok) & mask) == 0ull;
+        if (all) {
+            unsigned long long take = mask;
+            if (leaf) {
+                // a leaf: not for the lane whose own body it is (bh_force.comp:77-79)
+                const uint32_t rel = node - wbase;
+                if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
+                leafint += (Count)__popcll(take);
+            } else {
+                takes += 1;
+            }
             if (__builtin_amdgcn_inverse_ballot_w64(take)) {
                 const float inv = __builtin_amdgcn_rsqf(d2);
                 const float i2 = inv * inv;
@@ -392,60 +425,53 @@ __device__ __forceinline__ void walk_corrected(uint32_t n_ids, uint32_t base, un
                 ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
                 az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
             }
-            inter += (Count)__popcll(take);
+        } else {
+            opens += 1;
         }
         if constexpr (kLds) {
-            if (open) {
+            if (!all) {
                 if (sp + 1 > kStackDepth) {
                     err |= 1u;
                     break;
                 }
-                if (lane_id() == 0) {
-                    s_node[sp] = r[5];
-                    s_mask[sp] = open;
-                }
+                if (lane_id() == 0) s_node[sp] = r[5];
                 sp += 1;
-                opened += (Count)__popcll(open);
                 maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
-                node = r[6];
-                mask = open;
             } else {
                 if (sp == 0) break;
                 sp -= 1;
-                node = uniform(s_node[sp]);
-                const unsigned long long mraw = s_mask[sp];
-                mask = ((unsigned long long)uniform((uint32_t)(mraw >> 32)) << 32) | uniform((uint32_t)mraw);
             }
         } else {
-            // push (childA, open) and go to childB if any lane opened the node,
-            // else pop; both without a branch (as in walk_lean)
-            const int top = sp - 1;
-            const uint32_t pop_node = (uint32_t)__builtin_amdgcn_readlane(st_node, top);
-            const unsigned long long pop_mask =
-                ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(st_hi, top) << 32) |
-                (uint32_t)__builtin_amdgcn_readlane(st_lo, top);
+            // push childA if the node was opened, else pop; without a branch
             st_node = lane_write((int)r[5], sp, st_node);
-            st_lo = lane_write((int)(uint32_t)open, sp, st_lo);
-            st_hi = lane_write((int)(uint32_t)(open >> 32), sp, st_hi);
-            opened += (Count)__popcll(open);
-            const bool descend = open != 0ull;
-            node = descend ? r[6] : pop_node;
-            mask = descend ? open : pop_mask;
-            sp = descend ? sp + 1 : top;
+            sp = all ? sp - 1 : sp + 1;
             maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
             if (sp < 0) break;
         }
+        node = all ? top_node : r[6];
     }
-    const Count quads = inter - leaves;
+    // an empty statement that takes the touched words as input: the loads stay
+    asm volatile("" ::"s"(sink + t_b));
+    const Count lanes = (Count)__popcll(mask);
     o.ax = ax;
     o.ay = ay;
     o.az = az;
-    o.quads += quads;
-    o.inter += inter;
-    o.opened += opened;
+    o.quads += (unsigned long long)takes * lanes;
+    o.inter += (unsigned long long)takes * lanes + leafint;
+    o.opened += (unsigned long long)opens * lanes;
     o.maxsp = maxsp;
     o.err = err;
 }
+
+// The corrected walk's inputs for the walk kernels besides its records (a
+// __restrict__ parameter of their own, so that they come in through scalar
+// loads): the |a| of the previous corrected walk by body id (read by the
+// relative test, rewritten by every corrected walk) and the relative test's
+// alpha.
+struct CorrArgs {
+    float* acc_mag;
+    float alpha;
+};
 
 // Totals a wave accumulates over its 64-body groups, added to the counters once.
 struct WalkTotals {
@@ -455,14 +481,14 @@ struct WalkTotals {
 };
 
 // Walk and integrate the 64 sorted bodies [base, base + 64) of one wave.
-template <uint32_t kMode, bool kQuad>
+template <uint32_t kMode, int kCorr>
 __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsigned long long* s_mask,
                                            SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                            const float4* __restrict__ src, float4* __restrict__ dst,
                                            const uint32_t* __restrict__ index_in,
                                            const float4* __restrict__ leaf_pos,
                                            const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
-                                           uint2* __restrict__ tasks,
+                                           const CorrArgs corr, uint2* __restrict__ tasks,
                                            uint32_t task_cap, float theta_, float dt_, float softening,
                                            uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                            const float* __restrict__ leaf_radius, uint32_t general,
@@ -480,13 +506,18 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
 
     WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0ull};
     if (num > 1) {
-        if constexpr (kQuad) {
+        if constexpr (kCorr != 0) {
+            // kCorr 1: geometric test (theta^2), 2: relative (A of the body)
+            const float afac =
+                kCorr == 2 ? corr_rel_a(corr.alpha, active ? corr.acc_mag[bodyIdx] : 0.0f) : theta_ * theta_;
             if (!walk_corrected_ok(nodes[0].half, softening))
                 o.err |= 4u;   // outside the corrected walk's range: no forces
             else if (!general && ctr->treeHeight <= kLeanHeight && corr_offsets_32bit(n))
-                walk_corrected<false>(n, base, ballot(active), pm, qrec, theta_, softening, s_node, s_mask, o);
+                walk_corrected_group<false, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                                                        s_node, o);
             else
-                walk_corrected<true>(n, base, ballot(active), pm, qrec, theta_, softening, s_node, s_mask, o);
+                walk_corrected_group<true, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                                                       s_node, o);
         } else if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num)) {
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                              elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);
@@ -530,6 +561,8 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
         dst[2 * bodyIdx] = np;
         dst[2 * bodyIdx + 1] = nv;
         if (pos_out) pos_out[g] = np;   // the same value by sorted position
+        if constexpr (kCorr != 0)       // |a| for the next relative test (correctly rounded root)
+            corr.acc_mag[bodyIdx] = fsqrt(__builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az)));
         oob = kill ? 1u : 0u;
     }
     tot.opened += o.opened;
@@ -560,13 +593,13 @@ __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* c
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
-template <uint32_t kMode, int kW, bool kQuad>
+template <uint32_t kMode, int kW, int kCorr>
 __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
                                               const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
-                                              uint2* __restrict__ tasks,
+                                              const CorrArgs corr, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                               const float* __restrict__ leaf_radius, uint32_t general,
@@ -578,7 +611,7 @@ __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, D
     const uint32_t base = (lb * kW + w) * kWave;
     if (base >= sim->numBodies) return;
     WalkTotals t = {0ull, 0ull, 1u, 0u, 0u, 0ull};
-    walk_group<kMode, kQuad>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, qrec, tasks,
+    walk_group<kMode, kCorr>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, qrec, corr, tasks,
                              task_cap, theta_, dt_, softening, dynamic, wrap, elasticity, restitution, leaf_radius,
                              general, pos_out, t);
     walk_totals_flush(sim, ctr, t);
@@ -610,13 +643,13 @@ __global__ void k_walk_queue_reset(uint32_t* __restrict__ queues) {
 // to the nearest queue (by slot, then XCD) that still holds groups. A queue
 // that a wave found drained (by its atomic or by a load) is never tried again
 // by that wave, so each wave ends after at most kWalkQueues failed takes.
-template <uint32_t kMode, bool kQuad>
+template <uint32_t kMode, int kCorr>
 __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_walk_q(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
                                               const float4* __restrict__ leaf_pos,
                                               const WalkNode* __restrict__ nodes, const float4* __restrict__ qrec,
-                                              uint2* __restrict__ tasks,
+                                              const CorrArgs corr, uint2* __restrict__ tasks,
                                               uint32_t task_cap, float theta_, float dt_, float softening,
                                               uint32_t dynamic, uint32_t wrap, float elasticity, float restitution,
                                               const float* __restrict__ leaf_radius, uint32_t general,
@@ -635,8 +668,8 @@ __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu
         k = uniform(__shfl(k, 0, 64));
         const uint32_t qb = walk_queue_begin(groups, q);
         if (k < walk_queue_begin(groups, q + 1) - qb) {
-            walk_group<kMode, kQuad>((qb + k) * kWave, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos,
-                                     nodes, qrec, tasks, task_cap, theta_, dt_, softening, dynamic, wrap, elasticity,
+            walk_group<kMode, kCorr>((qb + k) * kWave, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos,
+                                     nodes, qrec, corr, tasks, task_cap, theta_, dt_, softening, dynamic, wrap, elasticity,
                                      restitution, leaf_radius, general, pos_out, t);
             continue;
         }
@@ -665,20 +698,36 @@ __global__ __launch_bounds__(kWalkWaves * 64) __attribute__((amdgpu_waves_per_eu
     walk_totals_flush(sim, ctr, t);
 }
 
+// the walk's totals back to their state before a walk (between the two walks
+// of a first relative-test call)
+__global__ void k_walk_counters_reset(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr) {
+    if (threadIdx.x != 0) return;
+    ctr->opened = 0;
+    ctr->interactions = 0;
+    ctr->quadTerms = 0;
+    ctr->maxStack = 0;
+    sim->uintDebug[0] = 0;
+}
+
+void launch_walk_counters_reset(hipStream_t s, const SimValues* sim, DevCounters* ctr) {
+    k_walk_counters_reset<<<1, 64, 0, s>>>(const_cast<SimValues*>(sim), ctr);
+}
+
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
+    const CorrArgs corr = {a.acc_mag, a.alpha};
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
     const uint32_t nb_small = (n + kWalkSmallWaves * kWave - 1) / (kWalkSmallWaves * kWave);
 #define BH_WALK(M, Q)                                                                                            \
     k_walk<M, kWalkSmallWaves, Q><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
-                                                a.leaf_pos, a.nodes, a.qrec, a.tasks, a.task_cap, a.theta, a.dt, \
+                                                a.leaf_pos, a.nodes, a.qrec, corr, a.tasks, a.task_cap, a.theta, a.dt, \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                 a.leaf_radius, a.general, a.pos_out)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
 #define BH_WALK_Q(M, Q)                                                                                          \
     k_walk_q<M, Q><<<nq, kWalkWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
-                                                  a.leaf_pos, a.nodes, a.qrec, a.tasks, a.task_cap, a.theta, a.dt, \
+                                                  a.leaf_pos, a.nodes, a.qrec, corr, a.tasks, a.task_cap, a.theta, a.dt, \
                                                   a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                   a.leaf_radius, a.general, a.pos_out, a.queues)
     // the corrected walk (quadrupole records) runs without merging or collisions
@@ -687,14 +736,16 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
         // what is resident find the queues drained and end
         const uint32_t nq = std::min(nb, 2u * 256u);
         k_walk_queue_reset<<<1, kWalkQueues, 0, s>>>(a.queues);
-        if (a.qrec) BH_WALK_Q(0u, true);
-        else if (a.merging == 1u) BH_WALK_Q(1u, false);
-        else if (a.merging == 2u) BH_WALK_Q(2u, false);
-        else BH_WALK_Q(0u, false);
-    } else if (a.qrec) BH_WALK(0u, true);
-    else if (a.merging == 1u) BH_WALK(1u, false);
-    else if (a.merging == 2u) BH_WALK(2u, false);
-    else BH_WALK(0u, false);
+        if (a.qrec && a.rel) BH_WALK_Q(0u, 2);
+        else if (a.qrec) BH_WALK_Q(0u, 1);
+        else if (a.merging == 1u) BH_WALK_Q(1u, 0);
+        else if (a.merging == 2u) BH_WALK_Q(2u, 0);
+        else BH_WALK_Q(0u, 0);
+    } else if (a.qrec && a.rel) BH_WALK(0u, 2);
+    else if (a.qrec) BH_WALK(0u, 1);
+    else if (a.merging == 1u) BH_WALK(1u, 0);
+    else if (a.merging == 2u) BH_WALK(2u, 0);
+    else BH_WALK(0u, 0);
 #undef BH_WALK
 #undef BH_WALK_Q
 }

@@ -193,11 +193,13 @@ def scaled_params(params: np.ndarray, alpha: float) -> np.ndarray:
 
 
 def calibrate(mlp: Mlp, bodies: np.ndarray, theta: float, softening: float, n_sample: int = 4096,
-              alphas=(0.0, 0.25, 0.5, 0.75, 1.0), seed: int = 0) -> tuple[float, list]:
+              alphas=(0.0, 0.25, 0.5, 0.75, 1.0), seed: int = 0,
+              force_tolerance: float | None = None) -> tuple[float, list]:
     """Choose the output scale alpha on a validation snapshot: the alpha whose
-    corrected walk at `theta` has the lowest p99 force error against the FP64
-    direct sum (then p50). The network is left with that scale. Returns
-    (alpha, [(alpha, p50, p99), ...])."""
+    corrected walk (the geometric test at `theta`, or with `force_tolerance`
+    the relative test) has the lowest p99 force error against the FP64 direct
+    sum (then p50). The network is left with that scale. Returns (alpha,
+    [(alpha, p50, p99), ...])."""
     base = mlp.params()
     bh = B.BarnesHut(bodies, _bounds(bodies), B.make_params(theta=theta, softening=softening))
     bh.init()
@@ -208,7 +210,12 @@ def calibrate(mlp: Mlp, bodies: np.ndarray, theta: float, softening: float, n_sa
     rows = []
     for a in alphas:
         mlp.set_params(scaled_params(base, a))
-        acc = bh.accelerations(B.make_params(theta=theta, softening=softening, flags=B.FLAG_LEARNED))
+        if force_tolerance is None:
+            prm = B.make_params(theta=theta, softening=softening, flags=B.FLAG_LEARNED)
+        else:
+            prm = B.make_params(theta=theta, softening=softening, flags=B.FLAG_LEARNED | B.FLAG_RELATIVE_MAC,
+                                force_tolerance=force_tolerance)
+        acc = bh.accelerations(prm)
         e = np.linalg.norm(acc[sample].astype(np.float64) - exact, axis=1) / np.linalg.norm(exact, axis=1)
         rows.append((float(a), float(np.percentile(e, 50)), float(np.percentile(e, 99))))
     attach(bh, None)

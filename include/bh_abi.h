@@ -78,8 +78,11 @@ extern "C" {
 #define BH_FLAG_GENERAL_WALK 8u /* run the walk's general loop (LDS stack, overflow check, IEEE inverse
                                    square root outside its fast domain) even where the shorter loop
                                    applies; same results (for tests of both loops) */
-#define BH_FLAG_QUADRUPOLE 16u /* corrected walk with the exact quadrupole (T = M2, no network); the
-                                  opening test stays the reference's (no reference counterpart) */
+#define BH_FLAG_QUADRUPOLE 16u /* corrected walk with the exact quadrupole (T = M2, no network) (no
+                                  reference counterpart) */
+#define BH_FLAG_RELATIVE_MAC 32u /* with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED: the corrected walk's
+                                    relative opening test, tolerance bh_params.force_tolerance (no
+                                    reference counterpart) */
 
 #define BH_WALK_STACK_DEPTH 192
 
@@ -101,6 +104,9 @@ typedef struct {
     float gravitational_constant;
     float body_length_constant;
     uint32_t flags;
+    /* no reference counterpart: alpha of the corrected walk's relative opening
+     * test (BH_FLAG_RELATIVE_MAC), > 0; unused otherwise */
+    float force_tolerance;
 } bh_params;
 
 /* Integer counters the reference keeps in SimulationValues, plus walk totals. */

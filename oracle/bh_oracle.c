@@ -566,102 +566,126 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
 }
 
 /* The corrected walk (no reference counterpart) for the bodies at sorted
- * positions [0, num): the reference's walk order (childB before childA) with
- * the opening test half < theta s taken as fl(fl(s^2) theta^2) > half^2, s^2 =
- * fma(x, x, fma(y, y, fma(z, z, eps^2))) in FP32 with r = com - position in
- * FP32 (the GPU kernel's exact sequence, so both accept the same nodes), and
- * every accepted node's term evaluated in FP64 from those FP32 inputs:
+ * positions [0, num), in groups of `group` consecutive positions (64: one GPU
+ * wave, the GPU kernel's rule; 1: every body on its own):
+ * a group walks the tree in the reference's order (childB before childA) and
+ * takes an internal node when every body of the group accepts it, opening it
+ * for all of them otherwise; a leaf is taken by every body but its own. A
+ * body accepts node k, in FP32 with r = com - position, s^2 = fma(x, x, fma(y,
+ * y, fma(z, z, eps^2))) and b^2 = fma(dx, dx, fma(dy, dy, dz dz)), d = max(com
+ * - min, max - com) (b: distance from the centre of mass to the farthest box
+ * corner), when
+ *   geometric (amag == NULL): fl(s^2 theta^2) > b^2;
+ *   relative: s^2 > b^2 and fl(s^2 A) > K, A = (float)(alpha amag[g])^0.4 and
+ *   K = (float)((G M)^0.4 (b^2)^0.6) in FP64 (G M the FP32 product);
+ * the GPU kernel's exact sequence, so both take the same nodes. Every node
+ * taken adds, in FP64 from those FP32 inputs,
  *     G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr T) r / s^5 - 3 GT r / s^5
  * with quad[8k .. 8k+6] = G T (xx xy xz yy yz zz), -1.5 G tr T for internal
- * node k and no quadrupole for a leaf (the body's own leaf skipped). Outputs
- * per sorted position: accel[3g..] the FP64 sum, absum[g] the sum over the
- * accepted nodes of |G M r / s^3| + |quadrupole term| (the scale of the FP32
- * rounding a GPU sum may carry), inter[g] the accepted nodes. The stats get
- * the opened nodes, the quadrupole terms (in mac_tests) and the stack depth.
- * Pinned by tests/test_oracle_quadrupole.py (exact field of clusters, error
- * falling one order faster than the plain walk's, zero records = plain). */
+ * node k (a leaf: its monopole). Outputs per sorted position: accel[3g..]
+ * the FP64 sum, absum[g] the sum over the nodes taken of |G M r / s^3| +
+ * |quadrupole term| (the scale of the FP32 rounding a GPU sum may carry),
+ * inter[g] the nodes taken. The stats get the opened nodes (per body), the
+ * quadrupole terms (in mac_tests) and the deepest stack. Pinned by
+ * tests/test_oracle_quadrupole.py (exact field of clusters, the error falling
+ * one order faster than the monopole's, the taken set against an independent
+ * top-down restatement). */
 int bho_force_corrected(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init,
-                        const float *quad, float theta, float softening, float G, int stack_cap, double *accel,
-                        double *absum, uint32_t *inter, bho_walk_stats *st, int threads) {
+                        const float *quad, float theta, float softening, float G, const float *amag, float alpha,
+                        uint32_t group, int stack_cap, double *accel, double *absum, uint32_t *inter,
+                        bho_walk_stats *st, int threads) {
     memset(st, 0, sizeof *st);
+    if (group < 1 || group > 64) return -2;
     if (stack_cap < 2) stack_cap = 2;
-    if (num < 2) {
-        for (uint32_t g = 0; g < num; g++) {
-            accel[3 * g] = accel[3 * g + 1] = accel[3 * g + 2] = 0.0;
-            absum[g] = 0.0;
-            inter[g] = 0;
-        }
-        return 0;
+    for (uint32_t g = 0; g < num; g++) {
+        accel[3 * g] = accel[3 * g + 1] = accel[3 * g + 2] = 0.0;
+        absum[g] = 0.0;
+        inter[g] = 0;
     }
+    if (num < 2) return 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
     const float th2 = theta * theta;
+    const uint32_t groups = (num + group - 1) / group;
     int overflow = 0;
     uint64_t opened_all = 0, quads_all = 0;
     uint32_t maxsp_all = 0;
 #pragma omp parallel reduction(+ : opened_all, quads_all) reduction(max : maxsp_all) reduction(| : overflow)
     {
         uint32_t *stack = malloc((size_t)stack_cap * 4);
-#pragma omp for schedule(dynamic, 256)
-        for (uint32_t g = 0; g < num; g++) {
-            const float *p = leaf[g].comMass;
-            double a[3] = {0.0, 0.0, 0.0}, sabs = 0.0;
-            uint32_t sp = 0, n_acc = 0, maxsp = 1;
+        float afac[64];
+        int ok[64];
+#pragma omp for schedule(dynamic, 16)
+        for (uint32_t gi = 0; gi < groups; gi++) {
+            const uint32_t g0 = group * gi, cnt = num - g0 < group ? num - g0 : group;
+            for (uint32_t j = 0; j < cnt; j++)
+                afac[j] = amag ? (float)pow((double)alpha * (double)amag[g0 + j], 0.4) : th2;
+            uint32_t sp = 0, maxsp = 1;
             stack[sp++] = n_init;
             while (sp > 0) {
                 const uint32_t id = stack[--sp];
                 const int internal_node = id >= n_init;
                 const bho_node *nd = get_node(leaf, internal, n_init, id);
-                if (!internal_node && id == g) continue;
-                const float rf[3] = {nd->comMass[0] - p[0], nd->comMass[1] - p[1], nd->comMass[2] - p[2]};
-                const float d2f = fmaf(rf[0], rf[0], fmaf(rf[1], rf[1], fmaf(rf[2], rf[2], softening)));
+                float b2 = -1.0f, K = -1.0f;
                 if (internal_node) {
-                    float ex = nd->aabb[3] - nd->aabb[0], ey = nd->aabb[4] - nd->aabb[1], ez = nd->aabb[5] - nd->aabb[2];
-                    float half = gmax(ex, gmax(ey, ez)) / 2.0f;
-                    float half2 = half * half;
-                    float lhs = d2f * th2;
-                    if (!(lhs > half2)) {
+                    const float *c = nd->comMass, *bb = nd->aabb;
+                    const float dx = gmax(c[0] - bb[0], bb[3] - c[0]);
+                    const float dy = gmax(c[1] - bb[1], bb[4] - c[1]);
+                    const float dz = gmax(c[2] - bb[2], bb[5] - c[2]);
+                    b2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    const float gm = G * nd->comMass[3];
+                    K = (float)(pow((double)gm, 0.4) * pow((double)b2, 0.6));
+                    int all = 1;
+                    for (uint32_t j = 0; j < cnt; j++) {
+                        const float *p = leaf[g0 + j].comMass;
+                        const float rx = c[0] - p[0], ry = c[1] - p[1], rz = c[2] - p[2];
+                        const float d2f = fmaf(rx, rx, fmaf(ry, ry, fmaf(rz, rz, softening)));
+                        ok[j] = amag ? (d2f > b2 && d2f * afac[j] > K) : (d2f * afac[j] > b2);
+                        all &= ok[j];
+                    }
+                    if (!all) {
                         if ((int)sp + 2 > stack_cap) { overflow = 1; break; }
                         stack[sp++] = nd->childA;
                         stack[sp++] = nd->childB;
                         if (sp > maxsp) maxsp = sp;
-                        opened_all++;
+                        opened_all += cnt;
                         continue;
                     }
                 }
-                const double r[3] = {rf[0], rf[1], rf[2]};
-                const double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + (double)softening;
-                const double inv = 1.0 / sqrt(s2);
-                const double i2 = inv * inv, i3 = i2 * inv, i5 = i3 * i2;
-                const double gm = (double)(G * nd->comMass[3]);
-                double term[3], mono = 0.0, qabs = 0.0;
-                for (int k = 0; k < 3; k++) { term[k] = gm * i3 * r[k]; mono += term[k] * term[k]; }
-                if (internal_node) {
-                    const float *Q = quad + 8 * (size_t)(id - n_init);
-                    const double t[3] = {Q[0] * r[0] + Q[1] * r[1] + Q[2] * r[2],
-                                         Q[1] * r[0] + Q[3] * r[1] + Q[4] * r[2],
-                                         Q[2] * r[0] + Q[4] * r[1] + Q[5] * r[2]};
-                    const double rtr = r[0] * t[0] + r[1] * t[1] + r[2] * t[2];
-                    const double c = (7.5 * rtr * i2 + (double)Q[6]) * i5;
-                    for (int k = 0; k < 3; k++) {
-                        const double qk = c * r[k] - 3.0 * i5 * t[k];
-                        term[k] += qk;
-                        qabs += qk * qk;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    const uint32_t g = g0 + j;
+                    if (!internal_node && id == g) continue;
+                    const float *p = leaf[g].comMass;
+                    const double r[3] = {(float)(nd->comMass[0] - p[0]), (float)(nd->comMass[1] - p[1]),
+                                         (float)(nd->comMass[2] - p[2])};
+                    const double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + (double)softening;
+                    const double inv = 1.0 / sqrt(s2);
+                    const double i2 = inv * inv, i3 = i2 * inv, i5 = i3 * i2;
+                    const double gm = (double)(G * nd->comMass[3]);
+                    double term[3], mono = 0.0, qabs = 0.0;
+                    for (int k = 0; k < 3; k++) { term[k] = gm * i3 * r[k]; mono += term[k] * term[k]; }
+                    if (internal_node) {
+                        const float *Q = quad + 8 * (size_t)(id - n_init);
+                        const double t[3] = {Q[0] * r[0] + Q[1] * r[1] + Q[2] * r[2],
+                                             Q[1] * r[0] + Q[3] * r[1] + Q[4] * r[2],
+                                             Q[2] * r[0] + Q[4] * r[1] + Q[5] * r[2]};
+                        const double rtr = r[0] * t[0] + r[1] * t[1] + r[2] * t[2];
+                        const double cq = (7.5 * rtr * i2 + (double)Q[6]) * i5;
+                        for (int k = 0; k < 3; k++) {
+                            const double qk = cq * r[k] - 3.0 * i5 * t[k];
+                            term[k] += qk;
+                            qabs += qk * qk;
+                        }
+                        quads_all++;
                     }
-                    quads_all++;
+                    for (int k = 0; k < 3; k++) accel[3 * g + k] += term[k];
+                    absum[g] += sqrt(mono) + sqrt(qabs);
+                    inter[g]++;
                 }
-                for (int k = 0; k < 3; k++) a[k] += term[k];
-                sabs += sqrt(mono) + sqrt(qabs);
-                n_acc++;
             }
-            accel[3 * g] = a[0];
-            accel[3 * g + 1] = a[1];
-            accel[3 * g + 2] = a[2];
-            absum[g] = sabs;
-            inter[g] = n_acc;
             if (maxsp > maxsp_all) maxsp_all = maxsp;
         }
         free(stack);

@@ -112,14 +112,18 @@ int bho_force_range(const float *src, float *dst, const uint32_t *index, uint32_
                     float G, float body_len_const, const float *bounds, int stack_cap,
                     float *accel, uint32_t *interactions, uint32_t *tasks, uint32_t task_cap,
                     bho_walk_stats *stats, int threads, uint32_t g_begin, uint32_t g_end);
-/* the corrected walk (no reference counterpart) in FP64 over the node set the
- * GPU's FP32 opening test accepts; quad[8k .. 8k+6] = G T xx xy xz yy yz zz,
- * -1.5 G tr T of internal node k. accel [num][3], absum [num] (sum of the
- * accepted terms' magnitudes), inter [num] by sorted position; the stats'
- * mac_tests field counts the quadrupole terms. Returns -1 on stack overflow. */
+/* the corrected walk (no reference counterpart) in FP64 over the nodes the
+ * GPU's FP32 tests take, in groups of `group` (1..64; the GPU: 64) sorted
+ * positions that take a node only together: geometric test at
+ * theta (amag NULL) or relative test at alpha with amag [num] (|a| by sorted
+ * position); quad[8k .. 8k+6] = G T xx xy xz yy yz zz, -1.5 G tr T of internal
+ * node k. accel [num][3], absum [num] (sum of the taken terms' magnitudes),
+ * inter [num] by sorted position; the stats' mac_tests field counts the
+ * quadrupole terms. Returns -1 on stack overflow. */
 int bho_force_corrected(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init,
-                        const float *quad, float theta, float softening, float G, int stack_cap, double *accel,
-                        double *absum, uint32_t *inter, bho_walk_stats *st, int threads);
+                        const float *quad, float theta, float softening, float G, const float *amag, float alpha,
+                        uint32_t group, int stack_cap, double *accel, double *absum, uint32_t *inter,
+                        bho_walk_stats *st, int threads);
 /* FP64 second moment of each internal node's bodies about its centre of mass
  * (m2 [num-1][6]: xx xy xz yy yz zz), by direct summation */
 void bho_node_moments(const bho_node *leaf, const bho_node *internal, uint32_t num, uint32_t n_init, double *m2,

@@ -96,8 +96,8 @@ def _declare(L: C.CDLL) -> None:
     L.bho_force.restype = C.c_int
     L.bho_force_range.argtypes = L.bho_force.argtypes + [U32, U32]
     L.bho_force_range.restype = C.c_int
-    L.bho_force_corrected.argtypes = [P, P, U32, U32, P, C.c_float, C.c_float, C.c_float, C.c_int, P, P, P,
-                                      C.POINTER(WalkStats), C.c_int]
+    L.bho_force_corrected.argtypes = [P, P, U32, U32, P, C.c_float, C.c_float, C.c_float, P, C.c_float, U32,
+                                      C.c_int, P, P, P, C.POINTER(WalkStats), C.c_int]
     L.bho_force_corrected.restype = C.c_int
     L.bho_node_moments.argtypes = [P, P, U32, U32, P, C.c_int]
     L.bho_node_moments.restype = None
@@ -198,20 +198,27 @@ def force(src, index, num, n_init, leaf, internal, prm: Params, G, blc, bounds, 
     return {"rc": rc, "accel": accel, "interactions": inter, "tasks": tasks[: st.n_tasks].copy(), "stats": st}
 
 
-def force_corrected(leaf, internal, num, n_init, quad, theta, softening, G=1.0, stack_cap=256, threads=0):
-    """The corrected walk in FP64 over the node set that the GPU's FP32 opening
-    test accepts: per sorted position the acceleration (float64), the sum of
-    the accepted terms' magnitudes and the accepted-node count; quad[k] holds
-    internal node k's G T xx xy xz yy yz zz, -1.5 G tr T (8 floats, the last
-    unused)."""
+def force_corrected(leaf, internal, num, n_init, quad, theta, softening, G=1.0, amag=None, alpha=0.0, group=64,
+                    stack_cap=256, threads=0):
+    """The corrected walk in FP64 over the nodes that the GPU's FP32 tests
+    take, in groups of `group` sorted positions that take a node only
+    together (64: the GPU's rule, 1: each body alone) (geometric test at theta, or with
+    amag (|a| by sorted position) the relative test at alpha): per sorted
+    position the acceleration (float64), the sum of the taken terms'
+    magnitudes and the count of nodes taken; quad[k] holds internal node k's
+    G T xx xy xz yy yz zz, -1.5 G tr T (8 floats, the last unused)."""
     q = np.ascontiguousarray(quad, dtype=np.float32)
     assert q.shape == (n_init - 1, 8)
+    am = None if amag is None else np.ascontiguousarray(amag, dtype=np.float32)
+    if am is not None:
+        assert am.shape == (num,)
     accel = np.zeros((max(num, 1), 3), np.float64)
     absum = np.zeros(max(num, 1), np.float64)
     inter = np.zeros(max(num, 1), np.uint32)
     st = WalkStats()
     rc = lib().bho_force_corrected(_p(leaf), _p(internal), num, n_init, _p(q), float(theta), float(softening),
-                                   float(G), stack_cap, _p(accel), _p(absum), _p(inter), C.byref(st), threads)
+                                   float(G), None if am is None else _p(am), float(alpha), group, stack_cap, _p(accel),
+                                   _p(absum), _p(inter), C.byref(st), threads)
     return {"rc": rc, "accel": accel[:num], "absum": absum[:num], "interactions": inter[:num],
             "opened": int(st.opened), "quad_terms": int(st.mac_tests), "max_stack": int(st.max_stack)}
 

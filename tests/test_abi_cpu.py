@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 def test_params_struct_matches_header():
     from gravity_amd import barnes_hut
     # 12 four-byte fields before flags... sizes must agree with the C struct
-    assert ctypes.sizeof(barnes_hut.Params) == 15 * 4
+    assert ctypes.sizeof(barnes_hut.Params) == 16 * 4
     assert ctypes.sizeof(barnes_hut.Counters) == 4 * 4 + 2 * 8 + 6 * 4 + 8 + 8
 
 

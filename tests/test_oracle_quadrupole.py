@@ -9,10 +9,12 @@ and the quadrupole term, pinned by closed forms and by physics.
   and the corrected walk's error falls one order faster with distance than
   the plain walk's (a wrong coefficient, sign or trace term would leave an
   error of the monopole's order).
-* With all-zero records the corrected walk accepts the plain walk's nodes and
-  gives its accelerations within FP32 rounding.
+* The corrected walk's accepted set (opening test b < theta s, b the
+  distance from the centre of mass to the farthest box corner) against an
+  independent top-down restatement: same counts, same FP64 monopole sums.
 """
 import numpy as np
+import pytest
 
 from oracle import bh_oracle as O
 
@@ -131,7 +133,7 @@ def test_corrected_walk_error_falls_one_order_faster():
         t = int(np.nonzero(sidx == 63)[0][0])
         exact = O.direct_sum_f64(b, np.array([63], np.uint32), G=1.0, softening=1e-6)[0]
         plain = O.force(b, sidx, 64, 64, leaf, internal, prm, 1.0, 1.0, bounds, stack_cap=256)
-        corr = O.force_corrected(leaf, internal, 64, 64, q, 0.3, 1e-6)
+        corr = O.force_corrected(leaf, internal, 64, 64, q, 0.3, 1e-6, group=1)
         assert plain["rc"] == 0 and corr["rc"] == 0
         assert plain["interactions"][t] == 1 and corr["interactions"][t] == 1   # one node: the whole cluster
         ep = np.linalg.norm(plain["accel"][t] - exact) / np.linalg.norm(exact)
@@ -143,32 +145,110 @@ def test_corrected_walk_error_falls_one_order_faster():
     assert 0.07 < q2 / q1 < 0.18          # corrected: ~ (b / s)^3
 
 
-def test_zero_records_give_the_plain_walk():
-    """With zero records the corrected walk's opening test (theta^2 s^2 >
-    half^2 in FP32) accepts the nodes of the reference's (half / s < theta)
-    except at ties, and its FP64 sum is the plain walk's FP32 sum within the
-    FP32 rounding of the terms."""
+def _accepted_sets(leaf, internal, n, theta, soft, group, amag=None, alpha=0.0):
+    """The corrected walk's taken nodes per body, top-down over the tree for
+    all bodies at once (an independent restatement of the traversal): the
+    bodies of a group of `group` consecutive sorted positions reach the same
+    nodes; they take an internal node if every one of them accepts it (the
+    geometric test fl(s^2 theta^2) > b^2, or the relative s^2 > b^2 and
+    fl(s^2 A) > K, in FP32 with fused multiply-adds emulated in FP64), else
+    all reach both children; every leaf reached is taken but a body's own."""
+    f32 = np.float32
+    lf = O.node_fields(leaf)
+    fi = O.node_fields(internal)
+    p = lf["com"].astype(np.float32)
+    th2 = f32(f32(theta) * f32(theta))
+    gid = np.arange(n) // group
+    ngroups = int(gid[-1]) + 1
+    if amag is not None:
+        A = np.power(float(alpha) * amag.astype(np.float64), 0.4).astype(np.float32)
+
+    def fma(a, b, c):
+        return (np.asarray(a, np.float64) * b + c).astype(np.float32)
+
+    reach = {n: np.ones(n, bool)}
+    taken = []          # (node id, mask of bodies)
+    order = [n]
+    while order:
+        k = order.pop()
+        m = reach.pop(k)
+        if k < n:
+            m = m.copy()
+            m[k] = False
+            taken.append((k, m))
+            continue
+        c = fi["com"][k - n].astype(np.float32)
+        bb = fi["aabb"][k - n].astype(np.float32)
+        d = np.maximum(c - bb[0:3], bb[3:6] - c).astype(np.float32)
+        b2 = fma(d[0], d[0], fma(d[1], d[1], f32(d[2] * d[2])))
+        r = (c[None, :] - p).astype(np.float32)
+        s2 = fma(r[:, 0], r[:, 0], fma(r[:, 1], r[:, 1], fma(r[:, 2], r[:, 2], np.full(n, soft, np.float32))))
+        if amag is None:
+            ok = (s2 * th2).astype(np.float32) > b2
+        else:
+            gm = f32(f32(1.0) * f32(fi["mass"][k - n]))
+            K = f32(np.power(float(gm), 0.4) * np.power(float(b2), 0.6))
+            ok = (s2 > b2) & ((s2 * A).astype(np.float32) > K)
+        fails = np.bincount(gid[m & ~ok], minlength=ngroups)
+        all_ok = (fails == 0)[gid]
+        taken.append((k, m & all_ok))
+        rest = m & ~all_ok
+        if rest.any():
+            for ch in (int(fi["childA"][k - n]), int(fi["childB"][k - n])):
+                reach[ch] = reach[ch] | rest if ch in reach else rest
+                order.append(ch)
+    return taken, lf, fi
+
+
+@pytest.mark.parametrize("group", [1, 64])
+@pytest.mark.parametrize("relative", [False, True])
+def test_corrected_walk_accepts_what_the_traversal_restates(group, relative):
+    """With zero records the oracle's corrected walk is the FP64 monopole sum
+    over the nodes taken; the same set built independently top-down gives
+    the same counts per body and the same sums; every body's nodes cover the
+    other bodies once (or all of them, where a node holding the body itself
+    was taken)."""
     rng = np.random.default_rng(5)
     n = 3000
     b = np.zeros((n, 8), np.float32)
     b[:, :3] = rng.standard_normal((n, 3))
-    b[:, 3] = 1.0 / n
+    b[:, 3] = rng.uniform(0.5, 1.5, n) / n
     sidx, leaf, internal, bounds = _tree(b)
-    prm = O.params(theta=0.3, dt=1.0, softening=0.01)
-    plain = O.force(b, sidx, n, n, leaf, internal, prm, 1.0, 1.0, bounds, stack_cap=256)
-    zero = O.force_corrected(leaf, internal, n, n, np.zeros((n - 1, 8), np.float32), 0.3, 0.01)
+    theta, soft, alpha = 0.45, 0.01, 2e-3
+    amag = rng.uniform(0.2, 2.0, n).astype(np.float32) if relative else None
+    zero = O.force_corrected(leaf, internal, n, n, np.zeros((n - 1, 8), np.float32), theta, soft, amag=amag,
+                             alpha=alpha, group=group)
     assert zero["rc"] == 0 and zero["quad_terms"] > 0
-    same = plain["interactions"] == zero["interactions"]
-    assert same.mean() >= 0.999
-    err = np.linalg.norm(plain["accel"].astype(np.float64) - zero["accel"], axis=1)
-    assert (err[same] <= 1e-5 * zero["absum"][same]).all()
-    assert np.median(err / np.linalg.norm(zero["accel"], axis=1)) < 1e-6
-    # and the records reduce the force error against the FP64 direct sum
+    taken, lf, fi = _accepted_sets(leaf, internal, n, theta, soft, group, amag, alpha)
+    count = np.zeros(n, np.int64)
+    covered = np.zeros(n, np.int64)
+    acc = np.zeros((n, 3))
+    p = lf["com"].astype(np.float64)
+    for k, m in taken:
+        if k < n:
+            com, M, nb = p[k], float(lf["mass"][k]), 1
+        else:
+            com, M, nb = fi["com"][k - n].astype(np.float64), float(fi["mass"][k - n]), int(fi["bodiesContained"][k - n])
+        count += m
+        covered += m * nb
+        r = (com.astype(np.float32)[None, :] - p.astype(np.float32)).astype(np.float64)
+        s2 = (r * r).sum(1) + float(np.float32(soft))
+        acc += m[:, None] * (float(np.float32(M)) * r / s2[:, None] ** 1.5)
+    np.testing.assert_array_equal(count, zero["interactions"])
+    assert ((covered == n - 1) | (covered == n)).all()
+    np.testing.assert_allclose(zero["accel"], acc, rtol=1e-12, atol=1e-12 * np.abs(acc).max())
+    if group == 64 and not relative:
+        # a group takes no node that one of its bodies alone would open
+        solo = O.force_corrected(leaf, internal, n, n, np.zeros((n - 1, 8), np.float32), theta, soft, group=1)
+        assert (zero["interactions"] >= solo["interactions"]).all()
+    if relative:
+        return          # (|a| here is random: the gain below is the geometric test's)
+    # and the records reduce the force error of the same walk against the FP64 direct sum
     q = _records(O.node_moments(leaf, internal, n, n))
-    corr = O.force_corrected(leaf, internal, n, n, q, 0.3, 0.01)
+    corr = O.force_corrected(leaf, internal, n, n, q, theta, soft, group=group)
     samp = np.arange(0, n, 7)
-    exact = O.direct_sum_f64(b, sidx[samp].astype(np.uint32), G=1.0, softening=0.01)
-    ep = np.linalg.norm(plain["accel"][samp] - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    exact = O.direct_sum_f64(b, sidx[samp].astype(np.uint32), G=1.0, softening=soft)
+    ep = np.linalg.norm(zero["accel"][samp] - exact, axis=1) / np.linalg.norm(exact, axis=1)
     eq = np.linalg.norm(corr["accel"][samp] - exact, axis=1) / np.linalg.norm(exact, axis=1)
     assert np.percentile(eq, 50) < 0.33 * np.percentile(ep, 50)
     assert np.percentile(eq, 99) < 0.5 * np.percentile(ep, 99)
@@ -182,7 +262,7 @@ def test_corrected_walk_self_and_counts():
     b[0, :4] = [0.0, 0.0, 0.0, 1.0]
     b[1, :4] = [1.0, 0.0, 0.0, 3.0]
     sidx, leaf, internal, _ = _tree(b)
-    r = O.force_corrected(leaf, internal, 2, 2, np.zeros((1, 8), np.float32), 0.5, 0.0 + 1e-12)
+    r = O.force_corrected(leaf, internal, 2, 2, np.zeros((1, 8), np.float32), 0.5, 0.0 + 1e-12, group=1)
     assert list(r["interactions"]) == [1, 1] and r["opened"] == 2 and r["quad_terms"] == 0
     a = r["accel"][np.argsort(sidx)]            # by body id
     s3 = (1.0 + 1e-12) ** 1.5
