@@ -353,8 +353,12 @@ __device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t ba
                                                      float soft, uint32_t* __restrict__ s_node, WalkAcc& o) {
     typedef typename std::conditional<kLds, unsigned long long, uint32_t>::type Count;
     float ax = o.ax, ay = o.ay, az = o.az;
-    Count takes = 0, opens = 0, leafint = 0;
-    uint32_t maxsp = o.maxsp, err = o.err;
+    // Only the visits are counted on the way: a walk of a binary tree that
+    // opens k nodes visits 1 + 2k, and takes 1 + k - (leaf visits) internal
+    // nodes (the leaves are counted where they are visited).
+    Count visits = 0, leafvis = 0, leafint = 0;
+    uint32_t err = o.err;
+    int maxsp = 0;
     const char* __restrict__ cbase = reinterpret_cast<const char*>(crec);
     auto rec = [&](uint32_t id) {
         return kLds ? *reinterpret_cast<const u32x16*>(cbase + ((size_t)id << 6))
@@ -391,43 +395,43 @@ __device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t ba
         const float ry = __uint_as_float(r[1]) - pm.y;
         const float rz = __uint_as_float(r[2]) - pm.z;
         const float d2 = corr_dist2(rx, ry, rz, soft);
-        // afac: theta^2 (the geometric test) or A (the relative one); a leaf
-        // is always taken (never opened, whatever its distance holds)
-        const bool leaf = node < first_internal;
+        // the lanes whose test rejects the node (afac: theta^2 for the
+        // geometric test, A for the relative one); a leaf is always taken,
+        // never opened, whatever its distance holds
         const float b2 = __uint_as_float(r[7]);
-        const bool ok = kRel ? (d2 > b2 && d2 * afac > __uint_as_float(r[4])) : d2 * afac > b2;
-        const bool all = leaf || (ballot(!ok) & mask) == 0ull;
-        if (all) {
-            unsigned long long take = mask;
-            if (leaf) {
-                // a leaf: not for the lane whose own body it is (bh_force.comp:77-79)
-                const uint32_t rel = node - wbase;
-                if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
-                leafint += (Count)__popcll(take);
-            } else {
-                takes += 1;
-            }
-            if (__builtin_amdgcn_inverse_ballot_w64(take)) {
-                const float inv = __builtin_amdgcn_rsqf(d2);
-                const float i2 = inv * inv;
-                const float i3 = i2 * inv;
-                const float i5 = i3 * i2;
-                const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
-                const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
-                const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
-                const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
-                const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
-                const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
-                const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
-                const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
-                const float m3 = -3.0f * i5;
-                ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
-                ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
-                az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
-            }
-        } else {
-            opens += 1;
+        const unsigned long long fail =
+            kRel ? ballot(!(d2 > b2)) | ballot(!(d2 * afac > __uint_as_float(r[4]))) : ballot(!(d2 * afac > b2));
+        const bool leaf = node < first_internal;
+        const bool all = leaf || (fail & mask) == 0ull;
+        unsigned long long take = all ? mask : 0ull;
+        if (leaf) {
+            // not for the lane whose own body it is (bh_force.comp:77-79)
+            const uint32_t rel = node - wbase;
+            if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
+            leafvis += 1;
+            leafint += (Count)__popcll(take);
         }
+        if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            const float i2 = inv * inv;
+            const float i3 = i2 * inv;
+            const float i5 = i3 * i2;
+            const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
+            const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
+            const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
+            const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
+            const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
+            const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
+            const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
+            const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
+            const float m3 = -3.0f * i5;
+            ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
+            ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
+            az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
+        }
+        visits += 1;
+        // push childA if the node was opened, else pop (the lane stack:
+        // without a branch; its write above the top is never read)
         if constexpr (kLds) {
             if (!all) {
                 if (sp + 1 > kStackDepth) {
@@ -435,31 +439,27 @@ __device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t ba
                     break;
                 }
                 if (lane_id() == 0) s_node[sp] = r[5];
-                sp += 1;
-                maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
-            } else {
-                if (sp == 0) break;
-                sp -= 1;
             }
         } else {
-            // push childA if the node was opened, else pop; without a branch
             st_node = lane_write((int)r[5], sp, st_node);
-            sp = all ? sp - 1 : sp + 1;
-            maxsp = (uint32_t)sp + 1 > maxsp ? (uint32_t)sp + 1 : maxsp;
-            if (sp < 0) break;
         }
+        sp = all ? sp - 1 : sp + 1;
+        maxsp = sp > maxsp ? sp : maxsp;
         node = all ? top_node : r[6];
+        if (sp < 0) break;
     }
     // an empty statement that takes the touched words as input: the loads stay
     asm volatile("" ::"s"(sink + t_b));
     const Count lanes = (Count)__popcll(mask);
+    const Count opens = (visits - 1) / 2;
+    const Count takes = 1 + opens - leafvis;
     o.ax = ax;
     o.ay = ay;
     o.az = az;
     o.quads += (unsigned long long)takes * lanes;
     o.inter += (unsigned long long)takes * lanes + leafint;
     o.opened += (unsigned long long)opens * lanes;
-    o.maxsp = maxsp;
+    o.maxsp = (uint32_t)maxsp + 1 > o.maxsp ? (uint32_t)maxsp + 1 : o.maxsp;
     o.err = err;
 }
 
