@@ -988,16 +988,23 @@ int bh_collect_training_pairs(bh_context* c, float theta, uint32_t stride, float
     SimValues h;
     HIPCHECK(c, hipMemcpyAsync(&h, c->sim, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMalloc((void**)&pairs, (size_t)cap * 8));
-    HIPCHECK(c, hipMalloc((void**)&count, 4));
-    hipError_t e = hipMemsetAsync(count, 0, 4, c->stream);
+    HIPCHECK(c, hipMalloc((void**)&count, 8));   // pairs found, stack overflow flag
+    hipError_t e = hipMemsetAsync(count, 0, 8, c->stream);
     if (e == hipSuccess) {
         launch_collect_pairs(c->stream, c->n, c->sim, c->tree.leaf_pos, c->tree.nodes, c->tree.node_aabb, theta,
                              softening, stride, pairs, cap, count);
         e = hipGetLastError();
     }
-    uint32_t found = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&found, count, 4, hipMemcpyDeviceToHost, c->stream);
+    uint32_t counts[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, count, 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const uint32_t found = counts[0];
+    if (e == hipSuccess && counts[1]) {
+        (void)hipFree(pairs);
+        (void)hipFree(count);
+        return fail(c, BH_ERR_STACK_OVERFLOW, "collect pairs: a walk needed more than %d stack entries",
+                    (int)kStackDepth);
+    }
     const uint32_t np = found > cap ? cap : found;
     if (e == hipSuccess) e = hipMalloc((void**)&f, (size_t)(np ? np : 1) * kMlpIn * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&t, (size_t)(np ? np : 1) * kMlpTarget * 4);

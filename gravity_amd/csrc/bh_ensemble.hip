@@ -149,20 +149,30 @@ int enqueue_step(bh_ensemble* e) {
 int capture_graphs(bh_ensemble* e) {
     drop_graphs(e);
     const int b0 = e->body_in, m0 = e->mi_in;
+    // enqueueing a step advances the host's buffer roles although nothing
+    // runs: every return restores them
+    auto bail = [&](int rc) {
+        e->body_in = b0;
+        e->mi_in = m0;
+        drop_graphs(e);
+        return rc;
+    };
     for (int k = 0; k < 2; k++) {
         e->start_body[k] = e->body_in;
         e->start_mi[k] = e->mi_in;
-        EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t r = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+        if (r != hipSuccess) return bail(efail(e, BH_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(r)));
         int rc = enqueue_step(e);
         hipGraph_t g = nullptr;
-        hipError_t r = hipStreamEndCapture(e->stream, &g);
+        r = hipStreamEndCapture(e->stream, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
-            return rc;
+            return bail(rc);
         }
-        if (r != hipSuccess) return efail(e, BH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(r));
+        if (r != hipSuccess) return bail(efail(e, BH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(r)));
         e->graph[k] = g;
-        EHIP(e, hipGraphInstantiate(&e->exec[k], g, nullptr, nullptr, 0));
+        r = hipGraphInstantiate(&e->exec[k], g, nullptr, nullptr, 0);
+        if (r != hipSuccess) return bail(efail(e, BH_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(r)));
     }
     // nothing ran: the current buffers are those before the two captured steps
     e->body_in = b0;

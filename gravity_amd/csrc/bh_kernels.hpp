@@ -218,6 +218,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 // evaluations.
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
                              double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
+// count [2]: pairs found, and 1 where a walk ran out of stack (its pairs are incomplete)
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
                           const WalkNode* nodes, const float4* node_aabb, float theta, float soft, uint32_t stride,
                           uint2* pairs, uint32_t cap, uint32_t* count);

@@ -243,6 +243,9 @@ __global__ __launch_bounds__(256) void k_collect_pairs(const SimValues* __restri
         } else if (sp + 2 <= kStackDepth) {
             stack[sp++] = nd.childA;
             stack[sp++] = nd.childB;
+        } else {
+            atomicOr(count + 1, 1u);   // reported: the pairs of this walk are incomplete
+            break;
         }
     }
 }

@@ -71,10 +71,11 @@ extern "C" {
 /* bh_params.flags */
 #define BH_FLAG_COLORS 1u      /* compute leaf star colours and node avgColor (render interop) */
 #define BH_FLAG_TIMING 2u      /* record per-stage HIP-event timings */
-#define BH_FLAG_LEARNED 4u     /* corrected walk with the attached network: every accepted internal node adds
-                                  the softened quadrupole of T = M2 + M h^2 net(node), M2 its exact second
-                                  moment about its centre of mass, h half its longest side, net the
-                                  network's residual (no reference counterpart) */
+#define BH_FLAG_LEARNED 4u     /* corrected walk with the attached network: every internal node taken adds
+                                  its softened monopole and the softened quadrupole of T = M2 + M h^2
+                                  net(node), M2 its exact second moment about its centre of mass, h half
+                                  its longest side, net the network's residual (no reference counterpart;
+                                  see "corrected walk" below) */
 #define BH_FLAG_GENERAL_WALK 8u /* run the walk's general loop (LDS stack, overflow check, IEEE inverse
                                    square root outside its fast domain) even where the shorter loop
                                    applies; same results (for tests of both loops) */
@@ -84,11 +85,31 @@ extern "C" {
                                     relative opening test, tolerance bh_params.force_tolerance (no
                                     reference counterpart) */
 
+/* The corrected walk (BH_FLAG_QUADRUPOLE / BH_FLAG_LEARNED; no reference
+ * counterpart). bh_build_tree under either flag also makes one 64-byte record
+ * per node (the quadrupole terms from FP64 moment prefix sums, plus the
+ * network's residual under BH_FLAG_LEARNED); bh_compute_force under the same
+ * flag then walks with them, without merging or collisions, softening >=
+ * 1e-14. Every node taken adds G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr
+ * T) r / s^5 - 3 GT r / s^5, s^2 = |r|^2 + softening (a leaf: its monopole),
+ * in FP32 with the hardware inverse square root. Opening tests, with b the
+ * distance from a node's centre of mass to the farthest corner of its box:
+ *   geometric: b < theta s;
+ *   relative (BH_FLAG_RELATIVE_MAC): G M b^3 / s^5 < force_tolerance |a|,
+ *     |a| the body's acceleration in the previous corrected walk on this
+ *     context, and b < s. A call with no previous corrected walk since
+ *     bh_init or a body upload first walks with the geometric test at theta.
+ * The 64 bodies of consecutive sorted positions [64k, 64k + 64) decide
+ * together: they take a node when every one of them accepts it, and open it
+ * for all of them otherwise (each body then adds more, smaller terms than
+ * its own test asks for). */
+
 #define BH_WALK_STACK_DEPTH 192
 
 /* The uniforms of the step (gpu/GPU.java:402-498) as one explicit struct. */
 typedef struct {
-    float theta;            /* opening angle of the half-longest-side MAC (bh_force.comp:6-20) */
+    float theta;            /* opening angle of the half-longest-side MAC (bh_force.comp:6-20); the
+                               corrected walk's geometric test (see above) */
     float dt;
     float softening;        /* added to |r|^2, i.e. epsilon^2 (bh_force.comp:27-32) */
     float elasticity;
@@ -121,7 +142,8 @@ typedef struct {
     uint32_t merges_applied;      /* merges applied by the last merge resolution */
     uint32_t tree_height;         /* height of the last tree (the reference fills at most 64 levels) */
     uint32_t max_stack;           /* deepest walk stack of the last walk */
-    uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow */
+    uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow, bit2 the corrected walk's box
+                                     beyond its range (16 h_root^2 + softening > 1e29: no forces) */
     uint32_t reserved;
     uint64_t corrected;           /* network evaluations (one per internal node) of the last tree build */
     uint64_t quad_terms;          /* quadrupole terms added by the last corrected walk (accepted internal nodes) */
@@ -233,13 +255,14 @@ int bh_mlp_loss_grad(bh_mlp *m, const float *x, const float *t, uint32_t rows, f
 int bh_mlp_train(bh_mlp *m, const float *x, const float *t, uint32_t rows, uint32_t epochs, uint32_t batch,
                  float lr, uint64_t seed, float *final_loss);
 /* On the context's current tree (after bh_build_tree): every stride-th sorted
- * body walks with the reference's opening test at theta; every internal node
- * it accepts makes a pair with FP32 node features [12] and an FP64-labelled
+ * body walks on its own with the corrected walk's geometric test at theta;
+ * every internal node it accepts makes a pair with FP32 node features [12] and an FP64-labelled
  * target [7] = (u, y, unit) as above, unit = (h / s)^2 G M / s^2 (a caller
  * turns it into the row's weight; nodes holding more than max_node_bodies bodies,
  * and nodes holding the target body itself, get NaN targets). n_out = pairs found; when it exceeds cap only the first
  * cap (in no particular order) are written, and a caller that wants an
- * unbiased sample raises stride and calls again. */
+ * unbiased sample raises stride and calls again. BH_ERR_STACK_OVERFLOW if a
+ * walk needs more than BH_WALK_STACK_DEPTH stack entries. */
 int bh_collect_training_pairs(bh_context *ctx, float theta, uint32_t stride, float softening,
                               uint32_t max_node_bodies, uint32_t cap, float *features, float *targets,
                               uint32_t *pairs_out /* optional: (sorted body, internal node) */, uint32_t *n_out);
