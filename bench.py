@@ -12,22 +12,27 @@ an FP64 direct sum (the GPU's FP64 kernel), p50 and p99.
 1. The reference's configuration, the plain step at theta 0.5, is timed.
 2. The correction's network is trained on the GPU on (body, node) pairs with
    FP64 labels from a 1M-body Plummer and a 1M-body two-galaxy snapshot.
-3. On one frozen state, sweeps of the opening angle: plain, the exact
-   quadrupole (no network) and learned-corrected (quadrupole + the network's
-   residual), each with its p50/p99 and its step time; the sweeps stop at the
-   metric's error level, p99 <= 1e-3.
-4. The timed loop: K steps of the learned-corrected method at the widest
-   angle whose p99 is <= 1e-3, bracketed by stream synchronisation, inputs
-   resident in HBM. value = alive bodies x K / wall time of the K steps.
-   equal_p99 compares it with the fastest plain angle at the same level.
+3. On one frozen state, sweeps of the opening parameter: the plain walk's
+   angle; the corrected walk's (exact quadrupole, no network) geometric
+   angle; the relative test's tolerance for the exact quadrupole and for the
+   learned-corrected walk (quadrupole + the network's residual); each with
+   its p50/p99 and its step time; the sweeps stop at the metric's error
+   level, p99 <= 1e-3.
+4. The timed loop: K steps of the learned-corrected method at the largest
+   tolerance whose p99 is <= 1e-3, bracketed by stream synchronisation,
+   inputs resident in HBM. value = alive bodies x K / wall time of the K
+   steps. equal_p99 compares it with the fastest plain angle at the same
+   level.
 5. The other workloads (the reference's CPU-runnable Plummer 65,536 x 10
    steps, cube + core merging 4,194,304 x 20 steps, two galaxies 33,554,432 x
    5 steps), the 256-simulation ensemble, and the CPU baseline (the oracle's C
    port of the plain step on this host over a bounded sample).
 
 The roofline entry is the dominant kernel of the timed step, the corrected
-walk (VALU-bound: 20 FLOP per body-node interaction plus 33 per quadrupole
-term, over its mean duration from HIP events on the library's stream).
+walk (FP32 VALU: 20 FLOP per body-node interaction plus 33 per quadrupole
+term, over its mean duration from HIP events on the library's stream; the
+walk is bound by instruction issue and record latency, its HBM traffic is
+taken from the committed rocprofv3 summary).
 """
 from __future__ import annotations
 
