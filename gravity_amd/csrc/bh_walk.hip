@@ -337,26 +337,30 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
 // lane that needs the finer nodes decides for its neighbours, which then add
 // more, smaller terms (never less accurate than their own test asks); the
 // mask never changes, so the stack holds node ids only and every internal
-// node taken adds a term in all lanes. Order: childB before childA, as the
-// reference's walk. The lane whose own leaf a record is skips it (r = 0 would
-// add exactly 0; it does not count as an interaction). kLds: stack in LDS (any
-// tree height, overflow checked, 64-bit record offsets) instead of one lane
-// register (trees at most kLeanHeight high, offsets below 2^32).
+// node taken adds a term in all lanes. The lane whose own leaf a record is
+// skips it (r = 0 would add exactly 0; it does not count as an interaction).
+// kLds: stack in LDS (any tree height, overflow checked, 64-bit record
+// offsets) instead of two lane registers (trees at most kLeanHeight high,
+// offsets below 2^32).
 // Needs eps^2 >= 1e-14 (1/s^5 stays finite) and the fast domain's bounds.
 __device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
     return soft >= 1e-14f && inv_sqrt_fast_domain(root_half, soft);
 }
 
+// The traversal evaluates the two children of an opened node together: the
+// stack holds sibling pairs (childA, childB), an iteration pops one pair,
+// loads both records at once (one latency for two nodes) and pushes the child
+// pair of each of the two that is opened (b's last: its subtree comes next).
+// Which nodes are taken does not depend on the order. Counting: the root,
+// then two nodes per pair; every opened node pushes one pair.
 template <bool kLds, bool kRel>
-__device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t base, unsigned long long mask,
+__device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t base, unsigned long long mask,
                                                      const float4& pm, const float4* __restrict__ crec, float afac,
-                                                     float soft, uint32_t* __restrict__ s_node, WalkAcc& o) {
+                                                     float soft, unsigned long long* __restrict__ s_pair,
+                                                     WalkAcc& o) {
     typedef typename std::conditional<kLds, unsigned long long, uint32_t>::type Count;
     float ax = o.ax, ay = o.ay, az = o.az;
-    // Only the visits are counted on the way: a walk of a binary tree that
-    // opens k nodes visits 1 + 2k, and takes 1 + k - (leaf visits) internal
-    // nodes (the leaves are counted where they are visited).
-    Count visits = 0, leafvis = 0, leafint = 0;
+    Count pairs = 0, leafvis = 0, leafint = 0;
     uint32_t err = o.err;
     int maxsp = 0;
     const char* __restrict__ cbase = reinterpret_cast<const char*>(crec);
@@ -364,49 +368,23 @@ __device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t ba
         return kLds ? *reinterpret_cast<const u32x16*>(cbase + ((size_t)id << 6))
                     : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(id << 6));
     };
-    auto touch = [&](uint32_t id) {
-        return kLds ? *reinterpret_cast<const uint32_t*>(cbase + ((size_t)id << 6))
-                    : *reinterpret_cast<const uint32_t*>(cbase + (uint32_t)(id << 6));
-    };
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
     const uint32_t first_internal = corr_internal_id(n_ids, 0);
-    // lane stack: entry i in lane i; every value it ever holds is a node id,
-    // so a read above the top still names a record inside the array
-    int st_node = 0;
-    uint32_t node = first_internal;          // the root
-    int sp = 0;
-    // The next node is the stack top if this one is taken, its childB if it
-    // is opened. One word of each is loaded while this node is evaluated, so
-    // that the next visit's whole record comes from the scalar cache (a
-    // 64-byte line: one record); the words only feed `sink`, which keeps the
-    // loads. (A leaf's childB is 0, a leaf. On an empty lane stack the top is
-    // lane 63, which starts at 0 and is only ever written with node ids; an
-    // empty LDS stack gives the root.)
-    uint32_t sink = 0, t_b = 0;
-    for (;;) {
-        // (the LDS stack is not initialised: with sp == 0 its top is not read)
-        uint32_t top_node;
-        if constexpr (kLds) top_node = sp > 0 ? uniform(s_node[sp - 1]) : first_internal;
-        else top_node = (uint32_t)__builtin_amdgcn_readlane(st_node, sp - 1);
-        const u32x16 r = rec(node);
-        sink += touch(top_node) ^ t_b;
-        t_b = touch(r[6]);
+    // One node for the whole wave: whether every lane takes it (a leaf: always,
+    // but not for the lane whose own body it is), adding its terms if so.
+    auto visit = [&](uint32_t id, const u32x16& r) -> bool {
         const float rx = __uint_as_float(r[0]) - pm.x;
         const float ry = __uint_as_float(r[1]) - pm.y;
         const float rz = __uint_as_float(r[2]) - pm.z;
         const float d2 = corr_dist2(rx, ry, rz, soft);
-        // the lanes whose test rejects the node (afac: theta^2 for the
-        // geometric test, A for the relative one); a leaf is always taken,
-        // never opened, whatever its distance holds
         const float b2 = __uint_as_float(r[7]);
         const unsigned long long fail =
             kRel ? ballot(!(d2 > b2)) | ballot(!(d2 * afac > __uint_as_float(r[4]))) : ballot(!(d2 * afac > b2));
-        const bool leaf = node < first_internal;
+        const bool leaf = id < first_internal;
         const bool all = leaf || (fail & mask) == 0ull;
         unsigned long long take = all ? mask : 0ull;
         if (leaf) {
-            // not for the lane whose own body it is (bh_force.comp:77-79)
-            const uint32_t rel = node - wbase;
+            const uint32_t rel = id - wbase;     // (bh_force.comp:77-79)
             if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
             leafvis += 1;
             leafint += (Count)__popcll(take);
@@ -429,30 +407,68 @@ __device__ __forceinline__ void walk_corrected_group(uint32_t n_ids, uint32_t ba
             ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
             az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
         }
-        visits += 1;
-        // push childA if the node was opened, else pop (the lane stack:
-        // without a branch; its write above the top is never read)
+        return all;
+    };
+    // the root; then pairs until the stack is empty
+    const uint32_t root = first_internal;
+    const u32x16 rr = rec(root);
+    if (!visit(root, rr)) {
+        // lane stack: pair i in lane i of st_a / st_b; every value they ever
+        // hold is a node id, so reads above the top name records in the array
+        int st_a = 0, st_b = 0;
+        int sp;
         if constexpr (kLds) {
-            if (!all) {
-                if (sp + 1 > kStackDepth) {
+            if (lane_id() == 0) s_pair[0] = ((unsigned long long)rr[6] << 32) | rr[5];
+        } else {
+            st_a = lane_write((int)rr[5], 0, st_a);
+            st_b = lane_write((int)rr[6], 0, st_b);
+        }
+        sp = 1;
+        for (;;) {
+            uint32_t a, b;
+            if constexpr (kLds) {
+                const unsigned long long e = s_pair[sp - 1];
+                a = uniform((uint32_t)e);
+                b = uniform((uint32_t)(e >> 32));
+            } else {
+                a = (uint32_t)__builtin_amdgcn_readlane(st_a, sp - 1);
+                b = (uint32_t)__builtin_amdgcn_readlane(st_b, sp - 1);
+            }
+            sp -= 1;
+            const u32x16 ra = rec(a);
+            const u32x16 rb = rec(b);
+            const bool take_a = visit(a, ra);
+            const bool take_b = visit(b, rb);
+            pairs += 1;
+            // push a's children if a was opened, then b's
+            if constexpr (kLds) {
+                const int need = (take_a ? 0 : 1) + (take_b ? 0 : 1);
+                if (sp + need > kStackDepth) {
                     err |= 1u;
                     break;
                 }
-                if (lane_id() == 0) s_node[sp] = r[5];
+                if (lane_id() == 0) {
+                    if (!take_a) s_pair[sp] = ((unsigned long long)ra[6] << 32) | ra[5];
+                    if (!take_b) s_pair[sp + (take_a ? 0 : 1)] = ((unsigned long long)rb[6] << 32) | rb[5];
+                }
+            } else {
+                // without a branch: a's pair goes to sp, b's to sp or sp + 1
+                // (a write above the new top is never read)
+                st_a = lane_write((int)ra[5], sp, st_a);
+                st_b = lane_write((int)ra[6], sp, st_b);
+                const int pb = sp + (take_a ? 0 : 1);
+                st_a = lane_write((int)rb[5], pb, st_a);
+                st_b = lane_write((int)rb[6], pb, st_b);
             }
-        } else {
-            st_node = lane_write((int)r[5], sp, st_node);
+            sp += (take_a ? 0 : 1) + (take_b ? 0 : 1);
+            maxsp = sp > maxsp ? sp : maxsp;
+            if (sp == 0) break;
         }
-        sp = all ? sp - 1 : sp + 1;
-        maxsp = sp > maxsp ? sp : maxsp;
-        node = all ? top_node : r[6];
-        if (sp < 0) break;
     }
-    // an empty statement that takes the touched words as input: the loads stay
-    asm volatile("" ::"s"(sink + t_b));
     const Count lanes = (Count)__popcll(mask);
-    const Count opens = (visits - 1) / 2;
-    const Count takes = 1 + opens - leafvis;
+    const Count evaluated = 1 + 2 * pairs;
+    const Count opens = pairs;                 // (each opened node pushed one pair)
+    const Count takes = evaluated - opens - leafvis;
     o.ax = ax;
     o.ay = ay;
     o.az = az;
@@ -513,11 +529,11 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
             if (!walk_corrected_ok(nodes[0].half, softening))
                 o.err |= 4u;   // outside the corrected walk's range: no forces
             else if (!general && ctr->treeHeight <= kLeanHeight && corr_offsets_32bit(n))
-                walk_corrected_group<false, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
-                                                        s_node, o);
+                walk_corrected_pairs<false, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                                                        s_mask, o);
             else
-                walk_corrected_group<true, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
-                                                       s_node, o);
+                walk_corrected_pairs<true, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                                                       s_mask, o);
         } else if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num)) {
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
                              elasticity, restitution, index_in, src, ctr, tasks, task_cap, o);

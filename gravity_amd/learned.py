@@ -164,7 +164,10 @@ def collect_training_pairs(bh: B.BarnesHut, theta: float, stride: int, softening
         raise RuntimeError("collect_training_pairs: pairs do not fit in cap")
     f, t, pr = f[: n.value], t[: n.value], pr[: n.value]
     keep = np.isfinite(t).all(axis=1) & np.isfinite(f).all(axis=1)
-    f, t, pr = f[keep], t[keep], pr[keep]
+    # the walkers append in whatever order their atomics land: sorted by
+    # (body, node), the rows (and a training run on them) are reproducible
+    order = np.lexsort((pr[keep, 1], pr[keep, 0]))
+    f, t, pr = f[keep][order], t[keep][order], pr[keep][order]
     # |a| of every target (sorted positions -> body ids of the current order)
     targets, inverse = np.unique(pr[:, 0], return_inverse=True)
     ids = bh.index()[targets]
