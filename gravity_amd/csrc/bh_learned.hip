@@ -37,11 +37,48 @@ __device__ __forceinline__ void body_moments(float4 pm, double* v) {
     v[9] = v[3] * z;
 }
 
-// Exclusive scan of 10 doubles per thread over a block of kMomTile threads
-// (fixed order: lanes, then waves); *total receives the block's sums.
-__device__ __forceinline__ void block_excl_scan10(const double* v, double* ex, double* s_w, double* total) {
+// Sum of 10 doubles over a block of kMomTile threads (wave shuffles, then
+// one thread per component over the wave totals); valid in threads 0..9 as
+// total[threadIdx.x].
+__device__ __forceinline__ double block_sum10(const double* v, double* s_w) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     constexpr uint32_t kW = kMomTile / 64;
+    for (int c = 0; c < 10; c++) {
+        double x = v[c];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) s_w[w * 10 + c] = x;
+    }
+    __syncthreads();
+    double all = 0.0;
+    if (threadIdx.x < 10)
+        for (uint32_t k = 0; k < kW; k++) all += s_w[k * 10 + threadIdx.x];
+    return all;
+}
+
+// tile t: sums of the moments of positions [t kMomTile, (t + 1) kMomTile) below numBodies
+__global__ __launch_bounds__(kMomTile) void k_mom_tile(const SimValues* __restrict__ sim,
+                                                       const float4* __restrict__ leaf_pos,
+                                                       double* __restrict__ tiles) {
+    __shared__ double s_w[(kMomTile / 64) * 10];
+    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < sim->numBodies) body_moments(leaf_pos[g], v);
+    const double t = block_sum10(v, s_w);
+    if (threadIdx.x < 10) tiles[blockIdx.x * 10 + threadIdx.x] = t;
+}
+
+// one block: the tile sums become exclusive tile offsets (in place)
+__global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tiles, uint32_t ntiles) {
+    __shared__ double s_w[(kMomTile / 64) * 10];
+    __shared__ double s_off[(kMomTile / 64) * 10];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    constexpr uint32_t kW = kMomTile / 64;
+    const uint32_t per = (ntiles + kMomTile - 1) / kMomTile;
+    const uint32_t t0 = threadIdx.x * per, t1 = min(ntiles, t0 + per);
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t t = t0; t < t1; t++)
+        for (int c = 0; c < 10; c++) v[c] += tiles[t * 10 + c];
+    double ex[10];
     for (int c = 0; c < 10; c++) {
         double x = v[c];
         for (int o = 1; o < 64; o <<= 1) {
@@ -53,39 +90,15 @@ __device__ __forceinline__ void block_excl_scan10(const double* v, double* ex, d
         ex[c] = lane == 0 ? 0.0 : e;
     }
     __syncthreads();
-    for (int c = 0; c < 10; c++) {
-        double off = 0.0, all = 0.0;
+    if (threadIdx.x < 10) {
+        double acc = 0.0;
         for (uint32_t k = 0; k < kW; k++) {
-            const double t = s_w[k * 10 + c];
-            if (k < w) off += t;
-            all += t;
+            s_off[k * 10 + threadIdx.x] = acc;
+            acc += s_w[k * 10 + threadIdx.x];
         }
-        ex[c] += off;
-        total[c] = all;
     }
-}
-
-// tile t: sums of the moments of positions [t kMomTile, (t + 1) kMomTile) below numBodies
-__global__ __launch_bounds__(kMomTile) void k_mom_tile(const SimValues* __restrict__ sim,
-                                                       const float4* __restrict__ leaf_pos,
-                                                       double* __restrict__ tiles) {
-    __shared__ double s_w[(kMomTile / 64) * 10];
-    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
-    if (g < sim->numBodies) body_moments(leaf_pos[g], v);
-    block_excl_scan10(v, ex, s_w, tot);
-    if (threadIdx.x < 10) tiles[blockIdx.x * 10 + threadIdx.x] = tot[threadIdx.x];
-}
-
-// one block: the tile sums become exclusive tile offsets (in place)
-__global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tiles, uint32_t ntiles) {
-    __shared__ double s_w[(kMomTile / 64) * 10];
-    const uint32_t per = (ntiles + kMomTile - 1) / kMomTile;
-    const uint32_t t0 = threadIdx.x * per, t1 = min(ntiles, t0 + per);
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
-    for (uint32_t t = t0; t < t1; t++)
-        for (int c = 0; c < 10; c++) v[c] += tiles[t * 10 + c];
-    block_excl_scan10(v, ex, s_w, tot);
+    __syncthreads();
+    for (int c = 0; c < 10; c++) ex[c] += s_off[w * 10 + c];
     for (uint32_t t = t0; t < t1; t++)
         for (int c = 0; c < 10; c++) {
             const double x = tiles[t * 10 + c];
@@ -94,31 +107,58 @@ __global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tile
         }
 }
 
-// prefix[g] = the moments of positions [0, g), for g = 0 .. numBodies
+// prefix[c P + g] = component c of the moments of positions [0, g), for
+// g = 0 .. numBodies (component-major, P = n + 1: each component's store of
+// a wave is 512 contiguous bytes)
 __global__ __launch_bounds__(kMomTile) void k_mom_prefix(const SimValues* __restrict__ sim,
                                                          const float4* __restrict__ leaf_pos,
                                                          const double* __restrict__ tiles,
                                                          double* __restrict__ prefix) {
     __shared__ double s_w[(kMomTile / 64) * 10];
+    __shared__ double s_off[(kMomTile / 64) * 10];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    constexpr uint32_t kW = kMomTile / 64;
     const uint32_t num = sim->numBodies;
+    const size_t P = (size_t)sim->initialNumBodies + 1;
     const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10], tot[10];
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10];
     if (g < num) body_moments(leaf_pos[g], v);
-    block_excl_scan10(v, ex, s_w, tot);
+    for (int c = 0; c < 10; c++) {
+        double x = v[c];
+        for (int o = 1; o < 64; o <<= 1) {
+            const double y = __shfl_up(x, o, 64);
+            if ((int)lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w * 10 + c] = x;
+        ex[c] = x - v[c];                      // exclusive within the wave
+    }
+    __syncthreads();
+    // the offset of every wave: the tile's offset plus the totals of the waves before
+    if (threadIdx.x < 10) {
+        double acc = tiles[blockIdx.x * 10 + threadIdx.x];
+        for (uint32_t k = 0; k < kW; k++) {
+            s_off[k * 10 + threadIdx.x] = acc;
+            acc += s_w[k * 10 + threadIdx.x];
+        }
+    }
+    __syncthreads();
     if (g <= num)
-        for (int c = 0; c < 10; c++) prefix[(size_t)g * 10 + c] = tiles[blockIdx.x * 10 + c] + ex[c];
+        for (int c = 0; c < 10; c++) prefix[c * P + g] = s_off[w * 10 + c] + ex[c];
 }
 
-// Second moment about the point c of the bodies whose prefix entries are a
-// (first) and b (one past the last): M2 = S2 - c S1^T - S1 c^T + S0 c c^T.
-__device__ __forceinline__ void range_m2(const double* __restrict__ a, const double* __restrict__ b, double cx,
-                                         double cy, double cz, double* m2) {
-    const double S0 = b[0] - a[0], S1x = b[1] - a[1], S1y = b[2] - a[2], S1z = b[3] - a[3];
-    const double c[3] = {cx, cy, cz}, S1[3] = {S1x, S1y, S1z};
+// Second moment about the point c of the bodies of sorted positions [lo, hi)
+// from the component-major prefix sums (stride P): M2 = S2 - c S1^T - S1 c^T
+// + S0 c c^T.
+__device__ __forceinline__ void range_m2(const double* __restrict__ prefix, size_t P, uint32_t lo, uint32_t hi,
+                                         double cx, double cy, double cz, double* m2) {
+    double d[10];
+    for (int q = 0; q < 10; q++) d[q] = prefix[q * P + hi] - prefix[q * P + lo];
+    const double S0 = d[0];
+    const double c[3] = {cx, cy, cz}, S1[3] = {d[1], d[2], d[3]};
     const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
     for (int q = 0; q < 6; q++) {
         const int i = I[q], j = J[q];
-        m2[q] = (b[4 + q] - a[4 + q]) - c[i] * S1[j] - S1[i] * c[j] + S0 * c[i] * c[j];
+        m2[q] = d[4 + q] - c[i] * S1[j] - S1[i] * c[j] + S0 * c[i] * c[j];
     }
 }
 
@@ -166,7 +206,7 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
             mn = node_aabb[2 * k];
             mx = node_aabb[2 * k + 1];
             M = mx.w;
-            range_m2(prefix + (size_t)rg.x * 10, prefix + ((size_t)rg.x + rg.y) * 10, com.x, com.y, com.z, m2);
+            range_m2(prefix, (size_t)n + 1, rg.x, rg.x + rg.y, com.x, com.y, com.z, m2);
         }
         const bool ok = active && M > 0.0f && h > 0.0f;
         double o[6] = {0, 0, 0, 0, 0, 0};
