@@ -13,6 +13,7 @@
 #include <mutex>
 #include <new>
 #include <utility>
+#include <vector>
 
 #include "../../include/bh_abi.h"
 #include "bh_device.hpp"
@@ -90,6 +91,25 @@ struct bh_context {
     bool quad_valid = false;
     float* acc_mag = nullptr;                 // [n]
     bool accmag_valid = false;
+
+    // bh_step under BH_FLAG_GRAPH: one instantiated graph per (host-side
+    // buffer roles before the step, parameters), with the roles after it
+    struct Roles {
+        int body_in, mi_in;
+        uint32_t *index0, *index1, *sort0, *sort1;
+        bool step_pos_valid, quad_valid, accmag_valid, colors;
+        bool operator==(const Roles& o) const {
+            return body_in == o.body_in && mi_in == o.mi_in && index0 == o.index0 && index1 == o.index1 &&
+                   sort0 == o.sort0 && sort1 == o.sort1 && step_pos_valid == o.step_pos_valid &&
+                   quad_valid == o.quad_valid && accmag_valid == o.accmag_valid && colors == o.colors;
+        }
+    };
+    struct StepGraph {
+        Roles pre, post;
+        bh_params prm;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> graphs;
     double* mom_prefix = nullptr;             // [mom_prefix_doubles(n)]
     double* mom_tiles = nullptr;              // [mom_tile_doubles(n)]
 
@@ -374,6 +394,8 @@ void bh_destroy(bh_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
@@ -744,12 +766,11 @@ int bh_merge_bodies(bh_context* c) {
     return check_launch(c, "merge");
 }
 
-int bh_step(bh_context* c, const bh_params* p) {
-    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
-    HIPCHECK(c, hipSetDevice(c->device));
-    c->timing = (p->flags & BH_FLAG_TIMING) != 0;
+namespace {
+// The stages of one step on c->stream (no host synchronisation); advances
+// the host's record of the buffer roles.
+int step_enqueue(bh_context* c, const bh_params* p) {
     int rc;
-    if (c->timing && (rc = ensure_events(c))) return rc;
     // the entry points below end the validity; this step's encode may use
     // what the previous step's walk left, if nothing else ran in between
     const bool use_pos = c->step_pos_valid;
@@ -777,6 +798,78 @@ int bh_step(bh_context* c, const bh_params* p) {
     rc = bh_swap_body_buffers(c);
     c->step_pos_valid = rc == BH_OK && wrote;
     return rc;
+}
+
+bh_context::Roles roles_of(const bh_context* c) {
+    return {c->body_in,  c->mi_in,          c->index[0],   c->index[1],      c->sort_pos[0],
+            c->sort_pos[1], c->step_pos_valid, c->quad_valid, c->accmag_valid, c->colors};
+}
+
+void set_roles(bh_context* c, const bh_context::Roles& r) {
+    c->body_in = r.body_in;
+    c->mi_in = r.mi_in;
+    c->index[0] = r.index0;
+    c->index[1] = r.index1;
+    c->sort_pos[0] = r.sort0;
+    c->sort_pos[1] = r.sort1;
+    c->step_pos_valid = r.step_pos_valid;
+    c->quad_valid = r.quad_valid;
+    c->accmag_valid = r.accmag_valid;
+    c->colors = r.colors;
+}
+
+// The step as an instantiated graph: the one captured for the current roles
+// and parameters, captured now if there is none (capturing runs nothing; the
+// roles are restored, the graph launched, and the roles it leaves applied).
+int graph_step(bh_context* c, const bh_params* p) {
+    const bh_context::Roles cur = roles_of(c);
+    for (const auto& g : c->graphs)
+        if (g.pre == cur && memcmp(&g.prm, p, sizeof *p) == 0) {
+            HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+            set_roles(c, g.post);
+            return BH_OK;
+        }
+    if (c->graphs.size() >= 8) {
+        for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+        c->graphs.clear();
+    }
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = step_enqueue(c, p);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    const bh_context::Roles post = roles_of(c);
+    set_roles(c, cur);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : fail(c, BH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    bh_context::StepGraph g;
+    g.pre = cur;
+    g.post = post;
+    g.prm = *p;
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, BH_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    c->graphs.push_back(g);
+    HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+    set_roles(c, post);
+    return BH_OK;
+}
+}  // namespace
+
+int bh_step(bh_context* c, const bh_params* p) {
+    if (!c || !p) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    c->timing = (p->flags & BH_FLAG_TIMING) != 0;
+    int rc;
+    if (c->timing && (rc = ensure_events(c))) return rc;
+    // a graph only in the steady state: the previous step's positions are
+    // valid, the records' arrays exist, and (relative test) a previous |a|
+    const bool quad = (p->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED)) != 0;
+    if ((p->flags & BH_FLAG_GRAPH) && !c->timing && c->step_pos_valid && (!quad || c->qrec) &&
+        (!(p->flags & BH_FLAG_RELATIVE_MAC) || c->accmag_valid))
+        return graph_step(c, p);
+    return step_enqueue(c, p);
 }
 
 int bh_sync(bh_context* c) {
@@ -1029,6 +1122,11 @@ int bh_collect_training_pairs(bh_context* c, float theta, uint32_t stride, float
 
 int bh_attach_correction(bh_context* c, const bh_mlp* m) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
+    if (c->correction != m) {
+        // captured steps hold the previous network's weight pointers
+        for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+        c->graphs.clear();
+    }
     c->correction = m;
     return BH_OK;
 }

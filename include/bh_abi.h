@@ -84,6 +84,12 @@ extern "C" {
 #define BH_FLAG_RELATIVE_MAC 32u /* with BH_FLAG_QUADRUPOLE or BH_FLAG_LEARNED: the corrected walk's
                                     relative opening test, tolerance bh_params.force_tolerance (no
                                     reference counterpart) */
+#define BH_FLAG_GRAPH 64u      /* bh_step replays the step as a hipGraph, captured on first use for each
+                                  state of the buffer roles (they cycle every few steps) and each
+                                  parameter set (at most 8 kept). Same results as the direct step. A
+                                  step after anything but bh_step (an upload, a stage call, bh_init)
+                                  or with BH_FLAG_TIMING runs directly. Attaching another network
+                                  drops the graphs. */
 
 /* The corrected walk (BH_FLAG_QUADRUPOLE / BH_FLAG_LEARNED; no reference
  * counterpart). bh_build_tree under either flag also makes one 64-byte record
