@@ -1120,6 +1120,15 @@ int bh_collect_training_pairs(bh_context* c, float theta, uint32_t stride, float
     return e == hipSuccess ? BH_OK : fail(c, BH_ERR_HIP, "collect pairs: %s", hipGetErrorString(e));
 }
 
+extern "C++" {
+namespace bh {
+void mlp_device_weights(const bh_mlp* m, const __bf16** w, const float** bias) {
+    *w = m ? m->w : nullptr;
+    *bias = m ? m->bias : nullptr;
+}
+}  // namespace bh
+}
+
 int bh_attach_correction(bh_context* c, const bh_mlp* m) {
     if (!c) return BH_ERR_INVALID_ARGUMENT;
     if (c->correction != m) {

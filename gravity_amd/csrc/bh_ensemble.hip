@@ -42,6 +42,18 @@ struct bh_ensemble {
 
     float theta = 0.5f, dt = 1e-3f, soft = 0.01f, G = 1.0f;
     uint32_t general = 0;      // BH_FLAG_GENERAL_WALK: always the walk's general loop
+
+    // the corrected walk (BH_FLAG_QUADRUPOLE / BH_FLAG_LEARNED, optionally
+    // BH_FLAG_RELATIVE_MAC): records over all simulations' trees, moment
+    // prefix sums over all sorted positions, |a| by body id
+    bool corrected = false, learned = false, relative = false;
+    float alpha = 0.0f;
+    const bh_mlp* correction = nullptr;
+    float4* qrec = nullptr;
+    double* mom_prefix = nullptr;
+    double* mom_tiles = nullptr;
+    float* acc_mag = nullptr;
+    bool accmag_valid = false;   // a corrected walk has run on these bodies
     bh_ensemble_config cfg{8, 4, 192, 1};
     uint32_t walk_blocks = 0;
     int cus = 0;
@@ -125,9 +137,13 @@ int enqueue_sort(bh_ensemble* e, uint32_t bits) {
 
 int enqueue_walk(bh_ensemble* e, int waves, int depth, uint32_t blocks) {
     const int b = e->body_in;
+    // the relative test reads the |a| of a previous corrected walk: the first
+    // walk uses the geometric test at theta
+    const uint32_t rel = e->relative && e->accmag_valid ? 1u : 0u;
     if (launch_ens_walk(e->stream, waves, depth, blocks, e->sims, e->tasks, e->queues, e->ctr,
                         e->bodies[b], e->bodies[1 - b], e->index[e->mi_in], e->tree.leaf_pos, e->tree.nodes, e->N,
-                        e->G, e->theta, e->dt, e->soft, e->general))
+                        e->G, e->theta, e->dt, e->soft, e->general, e->corrected ? e->qrec : nullptr, e->acc_mag,
+                        rel, e->alpha))
         return efail(e, BH_ERR_INVALID_ARGUMENT, "walk shape %d x %d", waves, depth);
     return BH_OK;
 }
@@ -141,6 +157,15 @@ int enqueue_step(bh_ensemble* e) {
     if (rc) return rc;
     launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b], e->index[e->mi_in],
                     e->keys[e->mi_in], e->tree, e->G);
+    if (e->corrected) {
+        if (e->learned && !e->correction)
+            return efail(e, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
+        const __bf16* w = nullptr;
+        const float* bias = nullptr;
+        if (e->learned) mlp_device_weights(e->correction, &w, &bias);
+        launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, e->mom_prefix, e->mom_tiles,
+                                    w, bias, e->qrec);
+    }
     if ((rc = enqueue_walk(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, e->walk_blocks))) return rc;
     e->body_in = 1 - b;
     return BH_OK;
@@ -220,7 +245,7 @@ void bh_ensemble_destroy(bh_ensemble* e) {
                     e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.walk, e->tree.node_aabb,
                     e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->tree.frontier,
                     e->tree.frontier_count, e->ctr, e->queues,
-                    e->partials, e->hist};
+                    e->partials, e->hist, e->qrec, e->mom_prefix, e->mom_tiles, e->acc_mag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -233,9 +258,12 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
     if (!out || !counts || !bodies || !bounds || !params || n_sims == 0) return BH_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     if (cfg && !valid_config(*cfg)) return BH_ERR_INVALID_ARGUMENT;
-    if (params->merging_collision_or_neither != BH_NEITHER || params->static_or_dynamic != BH_DYNAMIC ||
-        (params->flags & BH_FLAG_LEARNED))
+    if (params->merging_collision_or_neither != BH_NEITHER || params->static_or_dynamic != BH_DYNAMIC)
         return BH_ERR_UNSUPPORTED;
+    const bool corrected = (params->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED)) != 0;
+    const bool relative = corrected && (params->flags & BH_FLAG_RELATIVE_MAC);
+    if (corrected && !(params->softening >= 1e-14f)) return BH_ERR_UNSUPPORTED;
+    if (relative && !(params->force_tolerance > 0.0f)) return BH_ERR_INVALID_ARGUMENT;
     uint64_t total = 0;
     for (uint32_t s = 0; s < n_sims; s++) {
         if (counts[s] < 2) return BH_ERR_INVALID_ARGUMENT;
@@ -254,6 +282,10 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
     e->dt = params->dt;
     e->soft = params->softening;
     e->general = (params->flags & BH_FLAG_GENERAL_WALK) ? 1u : 0u;
+    e->corrected = corrected;
+    e->learned = (params->flags & BH_FLAG_LEARNED) != 0;
+    e->relative = relative;
+    e->alpha = params->force_tolerance;
     float G = params->gravitational_constant;
     if (G == 0.0f && params->unit_mass != 0.0f) {
         // common/common.glsl:346-352, in FP32 as the shader does
@@ -350,6 +382,15 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &e->queues, kEnsQueues));
         TRY(ealloc(e, &e->partials, 6 * (size_t)e->ntiles));
         TRY(ealloc(e, &e->hist, 256 * (size_t)e->ntiles));
+        if (corrected) {
+            TRY(ealloc(e, &e->qrec, 4 * corr_records(N)));
+            TRY(ealloc(e, &e->mom_prefix, mom_prefix_doubles(N)));
+            TRY(ealloc(e, &e->mom_tiles, mom_tile_doubles(N)));
+            TRY(ealloc(e, &e->acc_mag, (size_t)N));
+            // the quadrupole halves of the leaf records stay zero
+            r = hipMemsetAsync(e->qrec, 0, 64 * corr_records(N), e->stream);
+            if (r != hipSuccess) { rc = efail(e, BH_ERR_HIP, "records: %s", hipGetErrorString(r)); goto bad; }
+        }
         TRY(walk_grid(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, &e->walk_blocks));
         const hipMemcpyKind H2D = hipMemcpyHostToDevice;
         if (hipMemcpyAsync(e->sims, hs.data(), S * sizeof(EnsSim), H2D, e->stream) != hipSuccess ||
@@ -400,6 +441,17 @@ int bh_ensemble_step(bh_ensemble* e, uint32_t steps) {
     if (!e) return BH_ERR_INVALID_ARGUMENT;
     EHIP(e, hipSetDevice(e->device));
     for (uint32_t k = 0; k < steps; k++) {
+        if (e->relative && !e->accmag_valid) {
+            // the first corrected walk uses the geometric test (no |a| yet);
+            // run directly, and capture graphs from the next step on
+            int rc = enqueue_step(e);
+            if (rc) return rc;
+            hipError_t r = hipGetLastError();
+            if (r != hipSuccess) return efail(e, BH_ERR_HIP, "step launch: %s", hipGetErrorString(r));
+            e->accmag_valid = true;
+            drop_graphs(e);
+            continue;
+        }
         int rc = e->cfg.use_graph ? graph_step(e) : enqueue_step(e);
         if (rc) return rc;
         if (!e->cfg.use_graph) {
@@ -470,6 +522,18 @@ int bh_ensemble_get_counters(bh_ensemble* e, bh_counters* out) {
     out->tree_height = d.treeHeight;
     out->max_stack = d.maxStack;
     out->error_flags = d.errorFlags;
+    out->corrected = d.corrected;
+    out->quad_terms = d.quadTerms;
+    return BH_OK;
+}
+
+int bh_ensemble_attach_correction(bh_ensemble* e, const bh_mlp* m) {
+    if (!e) return BH_ERR_INVALID_ARGUMENT;
+    if (!e->learned) return efail(e, BH_ERR_INVALID_ARGUMENT, "the ensemble was created without BH_FLAG_LEARNED");
+    EHIP(e, hipSetDevice(e->device));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    drop_graphs(e);   // captured steps hold the network's weight pointers
+    e->correction = m;
     return BH_OK;
 }
 
@@ -483,6 +547,8 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
     // the state every trial starts from: bounds and the previous sorted order
     EnsSim* saved_sims = nullptr;
     uint32_t* saved_index = nullptr;
+    float* saved_amag = nullptr;   // the relative test's |a| (every walk trial rewrites it)
+    const bool keep_amag = e->relative && e->accmag_valid;
     hipEvent_t ev[2] = {nullptr, nullptr};
     const int b0 = e->body_in, m0 = e->mi_in;
     int rc = BH_OK;
@@ -503,6 +569,14 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
     };
     do {
         if ((rc = ealloc(e, &saved_sims, e->S)) || (rc = ealloc(e, &saved_index, (size_t)e->N))) break;
+        if (keep_amag) {
+            if ((rc = ealloc(e, &saved_amag, (size_t)e->N))) break;
+            if (hipMemcpyAsync(saved_amag, e->acc_mag, 4 * (size_t)e->N, hipMemcpyDeviceToDevice, e->stream) !=
+                hipSuccess) {
+                rc = efail(e, BH_ERR_HIP, "autotune save");
+                break;
+            }
+        }
         if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
             rc = efail(e, BH_ERR_HIP, "events");
             break;
@@ -538,6 +612,19 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
         if ((rc = prepare()) || (rc = enqueue_sort(e, best.radix_bits))) break;
         launch_ens_tree(e->stream, e->sims, e->tiles, e->ntiles, e->N, e->ctr, e->bodies[b0], e->index[e->mi_in],
                         e->keys[e->mi_in], e->tree, e->G);
+        if (e->corrected) {   // the node records of this tree
+            const __bf16* w = nullptr;
+            const float* bias = nullptr;
+            if (e->learned) {
+                if (!e->correction) {
+                    rc = efail(e, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
+                    break;
+                }
+                mlp_device_weights(e->correction, &w, &bias);
+            }
+            launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, e->mom_prefix,
+                                        e->mom_tiles, w, bias, e->qrec);
+        }
         DevCounters d;
         if (hipMemcpyAsync(&d, e->ctr, sizeof d, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
             hipStreamSynchronize(e->stream) != hipSuccess) {
@@ -554,6 +641,12 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
                 if ((rc = walk_grid(e, waves_c[wi], depth_c[di], &blocks))) break;
                 float t_min = 1e30f;
                 for (int rep = 0; rep < 3 && !rc; rep++) {
+                    if (saved_amag &&
+                        hipMemcpyAsync(e->acc_mag, saved_amag, 4 * (size_t)e->N, hipMemcpyDeviceToDevice,
+                                       e->stream) != hipSuccess) {
+                        rc = efail(e, BH_ERR_HIP, "autotune restore");
+                        break;
+                    }
                     launch_ens_reset(e->stream, e->ctr, e->queues);
                     (void)hipEventRecord(ev[0], e->stream);
                     if ((rc = enqueue_walk(e, waves_c[wi], depth_c[di], blocks))) break;
@@ -577,9 +670,12 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
         (void)hipMemcpyAsync(e->sims, saved_sims, e->S * sizeof(EnsSim), hipMemcpyDeviceToDevice, e->stream);
         (void)hipMemcpyAsync(e->index[m0], saved_index, 4 * (size_t)e->N, hipMemcpyDeviceToDevice, e->stream);
     }
+    if (saved_amag)
+        (void)hipMemcpyAsync(e->acc_mag, saved_amag, 4 * (size_t)e->N, hipMemcpyDeviceToDevice, e->stream);
     (void)hipStreamSynchronize(e->stream);
     if (saved_sims) (void)hipFree(saved_sims);
     if (saved_index) (void)hipFree(saved_index);
+    if (saved_amag) (void)hipFree(saved_amag);
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
     if (rc) return rc;

@@ -6,6 +6,8 @@
 
 #include "bh_device.hpp"
 
+struct bh_mlp;   // include/bh_abi.h
+
 namespace bh {
 
 constexpr uint32_t kPartTile = 1024;   // dead partition tile (sorted positions per block)
@@ -177,7 +179,8 @@ void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, ui
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
-                    float theta, float dt, float soft, uint32_t general);
+                    float theta, float dt, float soft, uint32_t general, const float4* qrec = nullptr,
+                    float* acc_mag = nullptr, uint32_t rel = 0, float alpha = 0.0f);
 int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu);
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out);
 
@@ -218,6 +221,13 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 // evaluations.
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
                              double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
+// the packed BF16 weights and biases of a network (include/bh_abi.h's bh_mlp), or nullptr
+void mlp_device_weights(const bh_mlp* m, const __bf16** w, const float** bias);
+// The same for an ensemble's trees (N bodies, n_internal = N - S internal
+// nodes, node ids global).
+void launch_ens_node_quadrupoles(hipStream_t s, uint32_t N, uint32_t n_internal, float G, DevCounters* ctr,
+                                 const TreeArrays& T, double* prefix, double* tiles, const __bf16* w,
+                                 const float* bias, float4* qrec);
 // count [2]: pairs found, and 1 where a walk ran out of stack (its pairs are incomplete)
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
                           const WalkNode* nodes, const float4* node_aabb, float theta, float soft, uint32_t stride,

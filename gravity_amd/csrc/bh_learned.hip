@@ -21,6 +21,23 @@
 namespace bh {
 
 // ------------------------------------------------------------------ second moments
+// The counts the record kernels work on: one simulation's from its device
+// values (bodies alive, internal nodes = alive - 1), or an ensemble's given
+// on the host (N bodies, all alive; N - S internal nodes).
+struct RecScope {
+    const SimValues* sim;
+    uint32_t n, num, ninternal;
+    float G;
+    __device__ __forceinline__ uint32_t n_ids() const { return sim ? sim->initialNumBodies : n; }
+    __device__ __forceinline__ uint32_t alive() const { return sim ? sim->numBodies : num; }
+    __device__ __forceinline__ uint32_t internal() const {
+        if (!sim) return ninternal;
+        const uint32_t m = sim->numBodies;
+        return m > 1 ? m - 1 : 0;
+    }
+    __device__ __forceinline__ float grav() const { return sim ? sim->gravitationalConstant : G; }
+};
+
 // The 10 moment components of a body: m, m x, m y, m z, m xx, m xy, m xz,
 // m yy, m yz, m zz, in double precision.
 __device__ __forceinline__ void body_moments(float4 pm, double* v) {
@@ -56,13 +73,12 @@ __device__ __forceinline__ double block_sum10(const double* v, double* s_w) {
 }
 
 // tile t: sums of the moments of positions [t kMomTile, (t + 1) kMomTile) below numBodies
-__global__ __launch_bounds__(kMomTile) void k_mom_tile(const SimValues* __restrict__ sim,
-                                                       const float4* __restrict__ leaf_pos,
+__global__ __launch_bounds__(kMomTile) void k_mom_tile(const RecScope sc, const float4* __restrict__ leaf_pos,
                                                        double* __restrict__ tiles) {
     __shared__ double s_w[(kMomTile / 64) * 10];
     const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (g < sim->numBodies) body_moments(leaf_pos[g], v);
+    if (g < sc.alive()) body_moments(leaf_pos[g], v);
     const double t = block_sum10(v, s_w);
     if (threadIdx.x < 10) tiles[blockIdx.x * 10 + threadIdx.x] = t;
 }
@@ -110,16 +126,15 @@ __global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tile
 // prefix[c P + g] = component c of the moments of positions [0, g), for
 // g = 0 .. numBodies (component-major, P = n + 1: each component's store of
 // a wave is 512 contiguous bytes)
-__global__ __launch_bounds__(kMomTile) void k_mom_prefix(const SimValues* __restrict__ sim,
-                                                         const float4* __restrict__ leaf_pos,
+__global__ __launch_bounds__(kMomTile) void k_mom_prefix(const RecScope sc, const float4* __restrict__ leaf_pos,
                                                          const double* __restrict__ tiles,
                                                          double* __restrict__ prefix) {
     __shared__ double s_w[(kMomTile / 64) * 10];
     __shared__ double s_off[(kMomTile / 64) * 10];
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     constexpr uint32_t kW = kMomTile / 64;
-    const uint32_t num = sim->numBodies;
-    const size_t P = (size_t)sim->initialNumBodies + 1;
+    const uint32_t num = sc.alive();
+    const size_t P = (size_t)sc.n_ids() + 1;
     const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10];
     if (g < num) body_moments(leaf_pos[g], v);
@@ -168,7 +183,7 @@ __device__ __forceinline__ void range_m2(const double* __restrict__ prefix, size
 // features (o = 0 without a network), children renumbered. Each wave takes 64
 // consecutive nodes per turn; the grid strides over the nodes so that every
 // wave loads the network's fragments once.
-__global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
+__global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* __restrict__ ctr,
                                                 const WalkNode* __restrict__ walk,
                                                 const float4* __restrict__ node_aabb,
                                                 const uint2* __restrict__ node_range,
@@ -177,10 +192,10 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
     __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
     __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t num = sim->numBodies;
-    const uint32_t n = sim->initialNumBodies;
-    const uint32_t ninternal = num > 1 ? num - 1 : 0;
-    const float G = sim->gravitationalConstant;
+    const uint32_t num = sc.alive();
+    const uint32_t n = sc.n_ids();
+    const uint32_t ninternal = sc.internal();
+    const float G = sc.grav();
     const WalkNode* __restrict__ nodes = walk + n;
     const uint32_t waves = gridDim.x * 4;
     for (uint32_t j = (blockIdx.x * 4 + w) * 64 + lane; j < num; j += waves * 64) {
@@ -242,15 +257,28 @@ __global__ __launch_bounds__(256) void k_qnodes(const SimValues* __restrict__ si
     if (evals && lane == 0) atomicAdd(&ctr->corrected, (unsigned long long)evals);
 }
 
-void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
+static void node_quadrupoles(hipStream_t s, uint32_t n, const RecScope& sc, DevCounters* ctr, const TreeArrays& T,
                              double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec) {
     const uint32_t ntiles = n / kMomTile + 1;   // positions 0 .. n
-    k_mom_tile<<<ntiles, kMomTile, 0, s>>>(sim, T.leaf_pos, tiles);
+    k_mom_tile<<<ntiles, kMomTile, 0, s>>>(sc, T.leaf_pos, tiles);
     k_mom_scan<<<1, kMomTile, 0, s>>>(tiles, ntiles);
-    k_mom_prefix<<<ntiles, kMomTile, 0, s>>>(sim, T.leaf_pos, tiles, prefix);
+    k_mom_prefix<<<ntiles, kMomTile, 0, s>>>(sc, T.leaf_pos, tiles, prefix);
     const uint32_t groups = (n + 63) / 64;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048u);
-    k_qnodes<<<blocks, 256, 0, s>>>(sim, ctr, T.walk, T.node_aabb, T.node_range, prefix, w, bias, qrec);
+    k_qnodes<<<blocks, 256, 0, s>>>(sc, ctr, T.walk, T.node_aabb, T.node_range, prefix, w, bias, qrec);
+}
+
+void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
+                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec) {
+    const RecScope sc = {sim, 0u, 0u, 0u, 0.0f};
+    node_quadrupoles(s, n, sc, ctr, T, prefix, tiles, w, bias, qrec);
+}
+
+void launch_ens_node_quadrupoles(hipStream_t s, uint32_t N, uint32_t n_internal, float G, DevCounters* ctr,
+                                 const TreeArrays& T, double* prefix, double* tiles, const __bf16* w,
+                                 const float* bias, float4* qrec) {
+    const RecScope sc = {nullptr, N, N, n_internal, G};
+    node_quadrupoles(s, N, sc, ctr, T, prefix, tiles, w, bias, qrec);
 }
 
 // ------------------------------------------------------------------ pairs

@@ -353,11 +353,13 @@ __device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
 // pair of each of the two that is opened (b's last: its subtree comes next).
 // Which nodes are taken does not depend on the order. Counting: the root,
 // then two nodes per pair; every opened node pushes one pair.
-template <bool kLds, bool kRel>
-__device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t base, unsigned long long mask,
-                                                     const float4& pm, const float4* __restrict__ crec, float afac,
-                                                     float soft, unsigned long long* __restrict__ s_pair,
-                                                     WalkAcc& o) {
+// root_k: the root's internal storage index (0 for one simulation; an
+// ensemble's simulations have their own); kCap: entries of the LDS stack.
+template <bool kLds, bool kRel, int kCap = kStackDepth>
+__device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t root_k, uint32_t base,
+                                                     unsigned long long mask, const float4& pm,
+                                                     const float4* __restrict__ crec, float afac, float soft,
+                                                     unsigned long long* __restrict__ s_pair, WalkAcc& o) {
     typedef typename std::conditional<kLds, unsigned long long, uint32_t>::type Count;
     float ax = o.ax, ay = o.ay, az = o.az;
     Count pairs = 0, leafvis = 0, leafint = 0;
@@ -410,7 +412,7 @@ __device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t ba
         return all;
     };
     // the root; then pairs until the stack is empty
-    const uint32_t root = first_internal;
+    const uint32_t root = corr_internal_id(n_ids, root_k);
     const u32x16 rr = rec(root);
     if (!visit(root, rr)) {
         // lane stack: pair i in lane i of st_a / st_b; every value they ever
@@ -443,7 +445,7 @@ __device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t ba
             // push a's children if a was opened, then b's
             if constexpr (kLds) {
                 const int need = (take_a ? 0 : 1) + (take_b ? 0 : 1);
-                if (sp + need > kStackDepth) {
+                if (sp + need > kCap) {
                     err |= 1u;
                     break;
                 }
@@ -529,10 +531,10 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
             if (!walk_corrected_ok(nodes[0].half, softening))
                 o.err |= 4u;   // outside the corrected walk's range: no forces
             else if (!general && ctr->treeHeight <= kLeanHeight && corr_offsets_32bit(n))
-                walk_corrected_pairs<false, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                walk_corrected_pairs<false, kCorr == 2>(n, 0u, base, ballot(active), pm, qrec, afac, softening,
                                                         s_mask, o);
             else
-                walk_corrected_pairs<true, kCorr == 2>(n, base, ballot(active), pm, qrec, afac, softening,
+                walk_corrected_pairs<true, kCorr == 2>(n, 0u, base, ballot(active), pm, qrec, afac, softening,
                                                        s_mask, o);
         } else if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num)) {
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
@@ -773,7 +775,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
 // per XCD holding a contiguous, equal-cost share of the task list (tasks in
 // simulation order), so an XCD's waves walk few trees and keep them in its own
 // L2; a wave whose queue is empty takes tasks from the next queues.
-template <int kWaves, int kDepth>
+template <int kWaves, int kDepth, int kCorr>
 __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restrict__ sims,
                                                           const EnsTask* __restrict__ tasks,
                                                           EnsQueue* __restrict__ queues, DevCounters* __restrict__ ctr,
@@ -781,14 +783,15 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
                                                           const uint32_t* __restrict__ index_in,
                                                           const float4* __restrict__ leaf_pos,
                                                           const WalkNode* __restrict__ nodes, uint32_t N, float G,
-                                                          float theta, float dt, float soft, uint32_t general) {
+                                                          float theta, float dt, float soft, uint32_t general,
+                                                          const float4* __restrict__ qrec, const CorrArgs corr) {
     __shared__ uint32_t s_node[kWaves][kDepth];
     __shared__ unsigned long long s_mask[kWaves][kDepth];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
     // workgroups are dealt to the 8 XCDs round robin: this block's XCD
     const uint32_t home = blockIdx.x % kEnsQueues;
     const uint32_t height = ctr->treeHeight;   // the highest tree of the ensemble
-    unsigned long long opened = 0, inter = 0;
+    unsigned long long opened = 0, inter = 0, quads = 0;
     uint32_t maxsp = 1, err = 0;
     for (uint32_t qi = 0; qi < kEnsQueues; qi++) {
         EnsQueue* q = queues + (home + qi) % kEnsQueues;
@@ -806,7 +809,21 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
             float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
             WalkAcc o = {0.0f, 0.0f, 0.0f, 0ull, 0ull, 1u, 0u, 0ull};
-            if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
+            if constexpr (kCorr != 0) {
+                // the corrected walk (as walk_group's), from this simulation's root
+                const float afac =
+                    kCorr == 2 ? corr_rel_a(corr.alpha, active ? corr.acc_mag[bodyIdx] : 0.0f) : theta * theta;
+                // (every simulation of an ensemble holds at least two bodies)
+                if (!walk_corrected_ok(nodes[root - N].half, soft)) {
+                    o.err |= 4u;
+                } else if (!general && height <= kLeanHeight && corr_offsets_32bit(N)) {
+                    walk_corrected_pairs<false, kCorr == 2, kDepth>(N, root - N, base, ballot(active), pm, qrec, afac,
+                                                                    soft, s_mask[w], o);
+                } else {
+                    walk_corrected_pairs<true, kCorr == 2, kDepth>(N, root - N, base, ballot(active), pm, qrec, afac,
+                                                                   soft, s_mask[w], o);
+                }
+            } else if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
                 walk_lean<0u>(root, N, g, ballot(active), pm, vd, nodes, G, theta, soft, 0.0f, nullptr,
                               bodyIdx, 0.0f, 0.0f, index_in, src, ctr, nullptr, 0u, o);
             else if (inv_sqrt_fast_domain(nodes[root - N].half, soft))
@@ -824,9 +841,12 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
                 dst[2 * bodyIdx] = make_float4(fadd(pm.x, fmul(nvx, dt)), fadd(pm.y, fmul(nvy, dt)),
                                                fadd(pm.z, fmul(nvz, dt)), pm.w);
                 dst[2 * bodyIdx + 1] = make_float4(nvx, nvy, nvz, vd.w);
+                if constexpr (kCorr != 0)
+                    corr.acc_mag[bodyIdx] = fsqrt(__builtin_fmaf(o.ax, o.ax, __builtin_fmaf(o.ay, o.ay, o.az * o.az)));
             }
             opened += o.opened;
             inter += o.inter;
+            quads += o.quads;
             maxsp = o.maxsp > maxsp ? o.maxsp : maxsp;
             err |= o.err;
         }
@@ -834,6 +854,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
     if (lane == 0) {
         atomicAdd(&ctr->opened, opened);
         atomicAdd(&ctr->interactions, inter);
+        if (quads) atomicAdd(&ctr->quadTerms, quads);
         atomicMax(&ctr->maxStack, maxsp);
         if (err) atomicOr(&ctr->errorFlags, err);
     }
@@ -845,11 +866,20 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
 int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const EnsSim* sims, const EnsTask* tasks,
                     EnsQueue* queues, DevCounters* ctr, const float4* src, float4* dst,
                     const uint32_t* index_in, const float4* leaf_pos, const WalkNode* nodes, uint32_t N, float G,
-                    float theta, float dt, float soft, uint32_t general) {
+                    float theta, float dt, float soft, uint32_t general, const float4* qrec, float* acc_mag,
+                    uint32_t rel, float alpha) {
+    const CorrArgs corr = {acc_mag, alpha};
 #define BH_LAUNCH(W, D)                                                                                            \
     if (waves == W && depth == D) {                                                                                \
-        k_ens_walk<W, D><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos, nodes,  \
-                                                   N, G, theta, dt, soft, general);                                \
+        if (qrec && rel)                                                                                           \
+            k_ens_walk<W, D, 2><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos,  \
+                                                          nodes, N, G, theta, dt, soft, general, qrec, corr);      \
+        else if (qrec)                                                                                             \
+            k_ens_walk<W, D, 1><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos,  \
+                                                          nodes, N, G, theta, dt, soft, general, qrec, corr);      \
+        else                                                                                                       \
+            k_ens_walk<W, D, 0><<<blocks, W * 64, 0, s>>>(sims, tasks, queues, ctr, src, dst, index_in, leaf_pos,  \
+                                                          nodes, N, G, theta, dt, soft, general, qrec, corr);      \
         return 0;                                                                                                  \
     }
     BH_ENS_WALK_SHAPES(BH_LAUNCH)
@@ -860,7 +890,7 @@ int launch_ens_walk(hipStream_t s, int waves, int depth, uint32_t blocks, const 
 int ens_walk_occupancy(int waves, int depth, int* blocks_per_cu) {
 #define BH_OCC(W, D)                                                                                               \
     if (waves == W && depth == D)                                                                                  \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_ens_walk<W, D>, W * 64, 0) ==         \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_ens_walk<W, D, 0>, W * 64, 0) ==      \
                        hipSuccess                                                                                  \
                    ? 0                                                                                             \
                    : 2;

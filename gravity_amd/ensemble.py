@@ -51,6 +51,7 @@ def _lib():
     L.bh_ensemble_download_internal_nodes.argtypes = [P, P]
     L.bh_ensemble_get_counters.argtypes = [P, C.POINTER(B.Counters)]
     L.bh_ensemble_autotune.argtypes = [P, C.POINTER(Config), P]
+    L.bh_ensemble_attach_correction.argtypes = [P, P]
     L._ensemble_declared = True
     return L
 
@@ -158,6 +159,11 @@ class Ensemble:
         c = B.Counters()
         self._check(self.L.bh_ensemble_get_counters(self.h, C.byref(c)), "counters")
         return c.as_dict()
+
+    def attach(self, mlp) -> None:
+        """The network (gravity_amd.learned.Mlp) of an ensemble created with FLAG_LEARNED."""
+        self._check(self.L.bh_ensemble_attach_correction(self.h, mlp.h if mlp is not None else None), "attach")
+        self._mlp = mlp
 
     def autotune(self):
         """Measure the candidate shapes on the current state (which is left

@@ -22,8 +22,16 @@
  * of the sort, walk workgroup size and walk stack depth are chosen by
  * bh_ensemble_autotune from measurements on the ensemble itself.
  *
- * Scope: the plain method with dynamic bounds, merging off and every body
- * alive (non-zero mass); each simulation has at least 2 bodies.
+ * Scope: dynamic bounds, merging off and every body alive (non-zero mass);
+ * each simulation has at least 2 bodies. The walk is the plain method, or
+ * the corrected walk of include/bh_abi.h (params->flags BH_FLAG_QUADRUPOLE
+ * or BH_FLAG_LEARNED, optionally BH_FLAG_RELATIVE_MAC with
+ * params->force_tolerance): node records over all simulations' trees,
+ * evaluated by the attached network once per node under BH_FLAG_LEARNED,
+ * each simulation walked from its own root with the same opening tests and
+ * wave-wide decisions (a wave's 64 bodies belong to one simulation). With the
+ * relative test the first step runs directly with the geometric test at
+ * theta; the graph of the step is captured from the second on.
  */
 #ifndef BH_ENSEMBLE_H
 #define BH_ENSEMBLE_H
@@ -75,6 +83,11 @@ int bh_ensemble_download_sorted(bh_ensemble *e, uint64_t *keys, uint32_t *index)
 int bh_ensemble_download_internal_nodes(bh_ensemble *e, void *nodes);
 /* totals over all simulations for the last step (tree_height: the highest) */
 int bh_ensemble_get_counters(bh_ensemble *e, bh_counters *out);
+
+/* the network of an ensemble created with BH_FLAG_LEARNED (no reference
+ * counterpart; include/bh_abi.h's bh_mlp); it must outlive the steps that
+ * use it. Drops the captured graphs. */
+int bh_ensemble_attach_correction(bh_ensemble *e, const bh_mlp *m);
 
 /* Time the candidate digit widths (sort) and walk shapes (workgroup x stack
  * depth, depths below the current tree height + 2 excluded) on the current
