@@ -25,8 +25,11 @@ an FP64 direct sum (the GPU's FP64 kernel), p50 and p99.
    level.
 5. The other workloads (the reference's CPU-runnable Plummer 65,536 x 10
    steps, cube + core merging 4,194,304 x 20 steps, two galaxies 33,554,432 x
-   5 steps), the 256-simulation ensemble, and the CPU baseline (the oracle's C
-   port of the plain step on this host over a bounded sample).
+   5 steps), the 256-simulation ensemble (plain at theta 0.5; the
+   learned-corrected walk at theta 0.9 and with the relative test at the
+   headline's tolerance; batched against sequential), and the CPU baseline
+   (the oracle's C port of the plain step on this host over a bounded
+   sample).
 
 The roofline entry is the dominant kernel of the timed step, the corrected
 walk (FP32 VALU: 20 FLOP per body-node interaction plus 33 per quadrupole
@@ -293,26 +296,25 @@ def other_workload(name: str, steps: int, seed: int) -> dict:
     return out
 
 
-def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, seed: int) -> dict:
-    """Batched vs sequential: n_sims independent Plummer spheres (N per
-    simulation log-uniform in [4096, 524288], virial ratio in [0.3, 0.7]) for
-    `steps` steps, once as one ensemble (autotuned, hipGraph replay) and once
-    one simulation after another through the single-simulation path. Both are
-    aggregate body-steps/s over the timed steps with inputs resident."""
+def _ensemble_method(sims, prm, mlp, steps: int, soft: float) -> dict:
+    """One method over the ensemble: batched (autotuned, hipGraph replay) and
+    sequential (one single-simulation run after another), aggregate
+    body-steps/s over `steps` timed steps each with inputs resident; the force
+    error of the first 4 simulations from their single-simulation runs, whose
+    walk takes the same nodes as the ensemble's."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import ensemble as E
     from gravity_amd import initial_conditions as ic
-    sims = ic.ensemble(n_sims, seed)
+    from gravity_amd import learned as Lr
     total = int(sum(s.shape[0] for s in sims))
-    prm = B.make_params(theta=theta, dt=dt, softening=soft)
-    log(f"ensemble: {n_sims} simulations, {total} bodies")
     ens = E.Ensemble(sims, prm)
+    if mlp is not None:
+        ens.attach(mlp)
     ens.step(1)
     ens.sync()
     t0 = time.perf_counter()
     cfg, timings = ens.autotune()
     t_tune = time.perf_counter() - t0
-    log(f"ensemble: autotune {t_tune:.1f} s chose {cfg.as_dict()}")
     ens.step(1)
     ens.sync()
     t0 = time.perf_counter()
@@ -321,12 +323,13 @@ def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, 
     t_batched = time.perf_counter() - t0
     c = ens.counters()
     ens.close()
-    log(f"ensemble: batched {steps} steps in {t_batched:.3f} s")
     t_seq = 0.0
     errs = []
     for k, b in enumerate(sims):
         bh = B.BarnesHut(b, ic.tight_bounds(b), prm)
         bh.init()
+        if mlp is not None:
+            Lr.attach(bh, mlp)
         bh.step()
         bh.sync()
         t0 = time.perf_counter()
@@ -335,20 +338,59 @@ def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, 
         bh.sync()
         t_seq += time.perf_counter() - t0
         if k < 4:
-            errs.append(force_error(bh, soft, n_sample=1024))
+            errs.append(force_error(bh, soft, n_sample=1024, prm=prm))
+        if mlp is not None:
+            Lr.attach(bh, None)
         bh.close()
-    log(f"ensemble: sequential {steps} steps of each simulation in {t_seq:.3f} s")
     batched = total * steps / t_batched
     sequential = total * steps / t_seq
-    return {"n_sims": n_sims, "n_bodies": total, "steps": steps,
-            "sizes": {"min": int(min(s.shape[0] for s in sims)), "max": int(max(s.shape[0] for s in sims))},
-            "batched_body_steps_per_s": batched, "batched_ms_per_step": t_batched * 1e3 / steps,
+    return {"batched_body_steps_per_s": batched, "batched_ms_per_step": t_batched * 1e3 / steps,
             "sequential_body_steps_per_s": sequential, "sequential_ms_per_step": t_seq * 1e3 / steps,
             "speedup_batched_vs_sequential": batched / sequential,
             "autotuned_config": cfg.as_dict(), "autotune_s": t_tune, "autotune_ms": timings,
-            "interactions_per_body": c["interactions"] / total, "tree_height_max": int(c["tree_height"]),
+            "interactions_per_body": c["interactions"] / total, "quad_terms_per_body": c["quad_terms"] / total,
+            "network_evaluations_per_step": int(c["corrected"]), "tree_height_max": int(c["tree_height"]),
             "error_flags": int(c["error_flags"]),
             "force_error_first_4_sims": [{"p50": e["p50"], "p99": e["p99"]} for e in errs]}
+
+
+def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, seed: int, mlp=None,
+                 tolerance: float | None = None) -> dict:
+    """Batched vs sequential: n_sims independent Plummer spheres (N per
+    simulation log-uniform in [4096, 524288], virial ratio in [0.3, 0.7]) for
+    `steps` steps of the plain method at `theta`; with a network, also the
+    learned-corrected method with the geometric test at theta 0.9 and with
+    the relative test at `tolerance` (its first step geometric at 0.5)."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    sims = ic.ensemble(n_sims, seed)
+    total = int(sum(s.shape[0] for s in sims))
+    log(f"ensemble: {n_sims} simulations, {total} bodies")
+    prm = B.make_params(theta=theta, dt=dt, softening=soft)
+    out = {"n_sims": n_sims, "n_bodies": total, "steps": steps,
+           "sizes": {"min": int(min(s.shape[0] for s in sims)), "max": int(max(s.shape[0] for s in sims))},
+           "method": f"plain, theta {theta}"}
+    r = _ensemble_method(sims, prm, None, steps, soft)
+    log(f"ensemble plain: batched {r['batched_ms_per_step']:.2f} ms/step, sequential "
+        f"{r['sequential_ms_per_step']:.2f} ms/step, chose {r['autotuned_config']}")
+    out.update(r)
+    if mlp is None:
+        return out
+    learned = []
+    variants = [("learned-corrected, geometric test, theta 0.9",
+                 B.make_params(theta=0.9, dt=dt, softening=soft, flags=B.FLAG_LEARNED))]
+    if tolerance is not None:
+        variants.append((f"learned-corrected, relative test, tolerance {tolerance}",
+                         B.make_params(theta=0.5, dt=dt, softening=soft,
+                                       flags=B.FLAG_LEARNED | B.FLAG_RELATIVE_MAC, force_tolerance=tolerance)))
+    for name, p in variants:
+        r = _ensemble_method(sims, p, mlp, steps, soft)
+        e = r["force_error_first_4_sims"]
+        log(f"ensemble {name}: batched {r['batched_ms_per_step']:.2f} ms/step, sequential "
+            f"{r['sequential_ms_per_step']:.2f} ms/step, p99 of the first sims {[round(x['p99'], 5) for x in e]}")
+        learned.append(dict(r, method=name))
+    out["learned"] = learned
+    return out
 
 
 def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float, softening: float,
@@ -555,14 +597,15 @@ def main() -> int:
            for k, b in alg.items() if sm.get(k, 0.0) > 0.0}}
     Lr.attach(bh, None)
     bh.close()
-    mlp.close()
     if not args.no_configs:
         result["configs"] = {name: other_workload(name, steps, args.seed)
                              for name, steps in (("plummer_65536", 10), ("cube_core_merging", 20),
                                                  ("two_galaxy", 5))}
     if not args.no_ensemble:
         log("ensemble leg")
-        result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, 0.5, dt, args.seed)
+        result["ensemble"] = ensemble_leg(args.ensemble_sims, args.steps, soft, 0.5, dt, args.seed, mlp,
+                                          pick["force_tolerance"])
+    mlp.close()
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle port)")
         result["cpu_baseline"] = cpu_baseline(bodies, bounds, 0.5, dt, soft, args.cpu_threads,
