@@ -89,8 +89,8 @@ struct WalkArgs {
 };
 
 // The corrected walk's records, 64 bytes per node id (four float4):
-//   (centre of mass, G M), (K of corr_rel_k, childA, childB, b^2 of corr_size2),
-//   G (T_xx, T_xy, T_xz, T_yy), (G T_yz, G T_zz, -1.5 G tr T, 0).
+//   (centre of mass, G M), (K of corr_rel_k, children word, childB, b^2 of
+//   corr_size2), G (T_xx, T_xy, T_xz, T_yy), (G T_yz, G T_zz, -1.5 G tr T, 0).
 // Leaf j is record j (its position and G m, K -1, b^2 -1, no children,
 // zero quadrupole terms); internal node k is record n + kCorrGap + k, and the
 // child ids are in the same numbering, so every internal id is at least 64
@@ -98,6 +98,23 @@ struct WalkArgs {
 constexpr uint32_t kCorrGap = 64;
 __host__ __device__ inline uint32_t corr_internal_id(uint32_t n, uint32_t k) { return n + kCorrGap + k; }
 __host__ __device__ inline size_t corr_records(uint32_t n) { return 2 * (size_t)n - 1 + kCorrGap; }
+// The children word names both children in 32 bits: childA's id, bit 30 set
+// if childA is internal, bit 31 if childB is. The tree splits a node's sorted
+// range [lo, hi] at one position g: childA is leaf g (g = lo) or the internal
+// node of index g, childB leaf g + 1 (g + 1 = hi) or the internal node of
+// index g + 1 (bh_tree.comp:96-113). So childB's id is childA's plus
+// 1, minus ioff if childA is internal, plus ioff if childB is: ioff is the
+// difference between the id of internal node g and that of leaf g (n +
+// kCorrGap for one simulation; an ensemble's simulation s, whose leaves
+// start at sorted position off_s and internal nodes at index off_s - s, has
+// N + kCorrGap - s).
+__host__ __device__ inline uint32_t corr_children_word(uint32_t a_id, bool a_internal, bool b_internal) {
+    return a_id | (a_internal ? 1u << 30 : 0u) | (b_internal ? 1u << 31 : 0u);
+}
+__host__ __device__ inline uint32_t corr_child_a(uint32_t w) { return w & 0x3FFFFFFFu; }
+__host__ __device__ inline uint32_t corr_child_b(uint32_t w, uint32_t ioff) {
+    return (w & 0x3FFFFFFFu) + 1u + ((w >> 31) ? ioff : 0u) - (((w >> 30) & 1u) ? ioff : 0u);
+}
 // every record's byte offset fits 32 bits
 __host__ __device__ inline bool corr_offsets_32bit(uint32_t n) { return corr_records(n) <= (1ull << 26); }
 

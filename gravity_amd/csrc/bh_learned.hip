@@ -249,7 +249,9 @@ __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* 
             float4* rec = qrec + 4 * (size_t)corr_internal_id(n, k);
             rec[0] = nd.comMass;
             const float b2 = corr_size2(com, mn, mx);
-            rec[1] = make_float4(corr_rel_k(nd.comMass.w, b2), __uint_as_float(a), __uint_as_float(b), b2);
+            rec[1] = make_float4(corr_rel_k(nd.comMass.w, b2),
+                                 __uint_as_float(corr_children_word(a, nd.childA >= n, nd.childB >= n)),
+                                 __uint_as_float(b), b2);
             rec[2] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
             rec[3] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
         }

@@ -348,15 +348,17 @@ __device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
 }
 
 // The traversal evaluates the two children of an opened node together: the
-// stack holds sibling pairs (childA, childB), an iteration pops one pair,
-// loads both records at once (one latency for two nodes) and pushes the child
-// pair of each of the two that is opened (b's last: its subtree comes next).
-// Which nodes are taken does not depend on the order. Counting: the root,
-// then two nodes per pair; every opened node pushes one pair.
+// stack holds sibling pairs (one children word each, corr_children_word),
+// an iteration pops one pair, loads both records at once (one latency for two
+// nodes) and pushes the child pair of each of the two that is opened (b's
+// last: its subtree comes next). Which nodes are taken does not depend on the
+// order. Counting: the root, then two nodes per pair; every opened node
+// pushes one pair.
 // root_k: the root's internal storage index (0 for one simulation; an
-// ensemble's simulations have their own); kCap: entries of the LDS stack.
+// ensemble's simulations have their own); ioff: the children word's offset
+// (corr_child_b); kCap: entries of the LDS stack.
 template <bool kLds, bool kRel, int kCap = kStackDepth>
-__device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t root_k, uint32_t base,
+__device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t root_k, uint32_t ioff, uint32_t base,
                                                      unsigned long long mask, const float4& pm,
                                                      const float4* __restrict__ crec, float afac, float soft,
                                                      unsigned long long* __restrict__ s_pair, WalkAcc& o) {
@@ -371,98 +373,117 @@ __device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t ro
                     : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(id << 6));
     };
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
-    const uint32_t first_internal = corr_internal_id(n_ids, 0);
-    // One node for the whole wave: whether every lane takes it (a leaf: always,
-    // but not for the lane whose own body it is), adding its terms if so.
-    auto visit = [&](uint32_t id, const u32x16& r) -> bool {
+    // the node's terms (record r, offset r, s^2 d2) added to this lane's sum
+    auto terms = [&](const u32x16& r, float rx, float ry, float rz, float d2) {
+        const float inv = __builtin_amdgcn_rsqf(d2);
+        const float i2 = inv * inv;
+        const float i3 = i2 * inv;
+        const float i5 = i3 * i2;
+        const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
+        const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
+        const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
+        const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
+        const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
+        const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
+        const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
+        const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
+        const float m3 = -3.0f * i5;
+        ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
+        ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
+        az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
+    };
+    // the lanes of the mask that fail the opening test for record r
+    auto fails = [&](const u32x16& r, float d2) -> unsigned long long {
+        const float b2 = __uint_as_float(r[7]);
+        const unsigned long long f =
+            kRel ? ballot(!(d2 > b2)) | ballot(!(d2 * afac > __uint_as_float(r[4]))) : ballot(!(d2 * afac > b2));
+        return f & mask;
+    };
+    // One node for the whole wave: 1 if some lane of the mask fails the
+    // opening test (the wave opens the node), else 0 and the node's terms are
+    // added in every lane of the wave, under a scalar branch (a lane outside
+    // the mask holds no body: its sum is never read). A leaf (b^2 = K = -1)
+    // passes every test; the lane whose own leaf it is adds exactly zero
+    // (r = 0, no quadrupole terms, s^2 = eps^2 > 0). The decision is kept an
+    // integer (a count of 0 .. 64 bits is nonzero when (count + 63) >> 6 is
+    // 1): the compiler keeps a condition carried on as a lane mask and turns
+    // it back into an integer with vector instructions.
+    auto visit = [&](const u32x16& r) -> uint32_t {
         const float rx = __uint_as_float(r[0]) - pm.x;
         const float ry = __uint_as_float(r[1]) - pm.y;
         const float rz = __uint_as_float(r[2]) - pm.z;
         const float d2 = corr_dist2(rx, ry, rz, soft);
-        const float b2 = __uint_as_float(r[7]);
-        const unsigned long long fail =
-            kRel ? ballot(!(d2 > b2)) | ballot(!(d2 * afac > __uint_as_float(r[4]))) : ballot(!(d2 * afac > b2));
-        const bool leaf = id < first_internal;
-        const bool all = leaf || (fail & mask) == 0ull;
-        unsigned long long take = all ? mask : 0ull;
-        if (leaf) {
-            const uint32_t rel = id - wbase;     // (bh_force.comp:77-79)
-            if (rel < (uint32_t)kWave) take &= ~(1ull << rel);
-            leafvis += 1;
-            leafint += (Count)__popcll(take);
-        }
-        if (__builtin_amdgcn_inverse_ballot_w64(take)) {
-            const float inv = __builtin_amdgcn_rsqf(d2);
-            const float i2 = inv * inv;
-            const float i3 = i2 * inv;
-            const float i5 = i3 * i2;
-            const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
-            const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
-            const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
-            const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
-            const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
-            const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
-            const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
-            const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
-            const float m3 = -3.0f * i5;
-            ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
-            ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
-            az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
-        }
-        return all;
+        const uint32_t opn = ((uint32_t)__popcll(fails(r, d2)) + 63u) >> 6;
+        if (opn == 0u) terms(r, rx, ry, rz, d2);
+        return opn;
+    };
+    // The interactions of leaf id x with the mask's lanes: all but the lane
+    // whose own body it is (bh_force.comp:77-79).
+    const uint32_t mask_lanes = (uint32_t)__popcll(mask);
+    auto leaf_interactions = [&](uint32_t x) -> uint32_t {
+        const uint32_t rel = x - wbase;
+        const unsigned long long own = rel < (uint32_t)kWave ? (1ull << rel) & mask : 0ull;
+        return mask_lanes - (uint32_t)__popcll(own);
     };
     // the root; then pairs until the stack is empty
     const uint32_t root = corr_internal_id(n_ids, root_k);
     const u32x16 rr = rec(root);
-    if (!visit(root, rr)) {
-        // lane stack: pair i in lane i of st_a / st_b; every value they ever
-        // hold is a node id, so reads above the top name records in the array
-        int st_a = 0, st_b = 0;
+    if (visit(rr)) {
+        // lane stack: pair i in lane i of st; every value it ever holds is a
+        // children word, so a read above the top names records in the array
+        int st = 0;
         int sp;
         if constexpr (kLds) {
-            if (lane_id() == 0) s_pair[0] = ((unsigned long long)rr[6] << 32) | rr[5];
+            if (lane_id() == 0) s_pair[0] = rr[5];
         } else {
-            st_a = lane_write((int)rr[5], 0, st_a);
-            st_b = lane_write((int)rr[6], 0, st_b);
+            st = lane_write((int)rr[5], 0, st);
         }
         sp = 1;
         for (;;) {
-            uint32_t a, b;
-            if constexpr (kLds) {
-                const unsigned long long e = s_pair[sp - 1];
-                a = uniform((uint32_t)e);
-                b = uniform((uint32_t)(e >> 32));
-            } else {
-                a = (uint32_t)__builtin_amdgcn_readlane(st_a, sp - 1);
-                b = (uint32_t)__builtin_amdgcn_readlane(st_b, sp - 1);
-            }
+            uint32_t w;
+            if constexpr (kLds)
+                w = uniform((uint32_t)s_pair[sp - 1]);
+            else
+                w = (uint32_t)__builtin_amdgcn_readlane(st, sp - 1);
+            const uint32_t a = corr_child_a(w), b = corr_child_b(w, ioff);
             sp -= 1;
-            const u32x16 ra = rec(a);
-            const u32x16 rb = rec(b);
-            const bool take_a = visit(a, ra);
-            const bool take_b = visit(b, rb);
+            u32x16 ra = rec(a);
+            u32x16 rb = rec(b);
+            // both records before either is used: the two loads are issued
+            // back to back and waited for once (every use of either record
+            // comes after this statement, which needs both)
+            asm volatile("" : "+s"(ra), "+s"(rb));
+            // a leaf is never opened (the internal bit of its side is 0)
+            const uint32_t open_a = visit(ra) & ((w >> 30) & 1u);
+            const uint32_t open_b = visit(rb) & (w >> 31);
+            if (w < 0xC0000000u) {   // a leaf in the pair (rare): its counts
+                if (!((w >> 30) & 1u)) {
+                    leafvis += 1;
+                    leafint += (Count)leaf_interactions(a);
+                }
+                if (!(w >> 31)) {
+                    leafvis += 1;
+                    leafint += (Count)leaf_interactions(b);
+                }
+            }
             pairs += 1;
             // push a's children if a was opened, then b's
             if constexpr (kLds) {
-                const int need = (take_a ? 0 : 1) + (take_b ? 0 : 1);
-                if (sp + need > kCap) {
+                if (sp + (int)(open_a + open_b) > kCap) {
                     err |= 1u;
                     break;
                 }
                 if (lane_id() == 0) {
-                    if (!take_a) s_pair[sp] = ((unsigned long long)ra[6] << 32) | ra[5];
-                    if (!take_b) s_pair[sp + (take_a ? 0 : 1)] = ((unsigned long long)rb[6] << 32) | rb[5];
+                    if (open_a) s_pair[sp] = ra[5];
+                    if (open_b) s_pair[sp + open_a] = rb[5];
                 }
             } else {
                 // without a branch: a's pair goes to sp, b's to sp or sp + 1
                 // (a write above the new top is never read)
-                st_a = lane_write((int)ra[5], sp, st_a);
-                st_b = lane_write((int)ra[6], sp, st_b);
-                const int pb = sp + (take_a ? 0 : 1);
-                st_a = lane_write((int)rb[5], pb, st_a);
-                st_b = lane_write((int)rb[6], pb, st_b);
+                st = lane_write((int)ra[5], sp, st);
+                st = lane_write((int)rb[5], sp + (int)open_a, st);
             }
-            sp += (take_a ? 0 : 1) + (take_b ? 0 : 1);
+            sp += (int)(open_a + open_b);
             maxsp = sp > maxsp ? sp : maxsp;
             if (sp == 0) break;
         }
@@ -531,10 +552,10 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
             if (!walk_corrected_ok(nodes[0].half, softening))
                 o.err |= 4u;   // outside the corrected walk's range: no forces
             else if (!general && ctr->treeHeight <= kLeanHeight && corr_offsets_32bit(n))
-                walk_corrected_pairs<false, kCorr == 2>(n, 0u, base, ballot(active), pm, qrec, afac, softening,
+                walk_corrected_pairs<false, kCorr == 2>(n, 0u, n + kCorrGap, base, ballot(active), pm, qrec, afac, softening,
                                                         s_mask, o);
             else
-                walk_corrected_pairs<true, kCorr == 2>(n, 0u, base, ballot(active), pm, qrec, afac, softening,
+                walk_corrected_pairs<true, kCorr == 2>(n, 0u, n + kCorrGap, base, ballot(active), pm, qrec, afac, softening,
                                                        s_mask, o);
         } else if (!general && walk_lean_ok(nodes[0].half, softening, ctr->treeHeight, num)) {
             walk_lean<kMode>(n, n, g, ballot(active), pm, vd, nodes, G, theta_, softening, Ri, leaf_radius, bodyIdx,
@@ -817,10 +838,10 @@ __global__ __launch_bounds__(kWaves * 64) void k_ens_walk(const EnsSim* __restri
                 if (!walk_corrected_ok(nodes[root - N].half, soft)) {
                     o.err |= 4u;
                 } else if (!general && height <= kLeanHeight && corr_offsets_32bit(N)) {
-                    walk_corrected_pairs<false, kCorr == 2, kDepth>(N, root - N, base, ballot(active), pm, qrec, afac,
+                    walk_corrected_pairs<false, kCorr == 2, kDepth>(N, root - N, N + kCorrGap - tk.sim, base, ballot(active), pm, qrec, afac,
                                                                     soft, s_mask[w], o);
                 } else {
-                    walk_corrected_pairs<true, kCorr == 2, kDepth>(N, root - N, base, ballot(active), pm, qrec, afac,
+                    walk_corrected_pairs<true, kCorr == 2, kDepth>(N, root - N, N + kCorrGap - tk.sim, base, ballot(active), pm, qrec, afac,
                                                                    soft, s_mask[w], o);
                 }
             } else if (!general && walk_lean_ok(nodes[root - N].half, soft, height, N))
