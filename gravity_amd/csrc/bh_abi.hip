@@ -466,16 +466,27 @@ static int ensure_quad(bh_context* c) {
 }
 
 // Binding 16 is the quadrupole half (bytes 32..63) of every internal node's
-// record: copied whole, with a 64-byte pitch on the device.
+// record: copied whole, with a 64-byte pitch on the device. The records hold
+// S = -3 G T; the binding holds G T (converted in FP32 on the host).
 static int quad_binding_copy(bh_context* c, void* host, size_t off, size_t bytes, bool upload) {
     if (off != 0 || bytes != bh_binding_size(c, BH_BINDING_NODE_QUADRUPOLES))
         return fail(c, BH_ERR_INVALID_ARGUMENT, "the quadrupole records are copied whole");
     char* dev = reinterpret_cast<char*>(c->qrec + 4 * (size_t)corr_internal_id(c->n, 0)) + 32;
-    if (upload)
-        HIPCHECK(c, hipMemcpy2DAsync(dev, 64, host, 32, 32, (size_t)c->n - 1, hipMemcpyHostToDevice, c->stream));
-    else
-        HIPCHECK(c, hipMemcpy2DAsync(host, 32, dev, 64, 32, (size_t)c->n - 1, hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const size_t count = (size_t)c->n - 1;
+    if (upload) {
+        std::vector<float> s(8 * count);
+        const float* in = static_cast<const float*>(host);
+        for (size_t k = 0; k < count; k++)
+            for (int q = 0; q < 8; q++) s[8 * k + q] = q < 6 ? -3.0f * in[8 * k + q] : in[8 * k + q];
+        HIPCHECK(c, hipMemcpy2DAsync(dev, 64, s.data(), 32, 32, count, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    } else {
+        HIPCHECK(c, hipMemcpy2DAsync(host, 32, dev, 64, 32, count, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        float* out = static_cast<float*>(host);
+        for (size_t k = 0; k < count; k++)
+            for (int q = 0; q < 6; q++) out[8 * k + q] = out[8 * k + q] / -3.0f;
+    }
     return BH_OK;
 }
 

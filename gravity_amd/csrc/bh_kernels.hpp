@@ -90,7 +90,8 @@ struct WalkArgs {
 
 // The corrected walk's records, 64 bytes per node id (four float4):
 //   (centre of mass, G M), (K of corr_rel_k, children word, childB, b^2 of
-//   corr_size2), G (T_xx, T_xy, T_xz, T_yy), (G T_yz, G T_zz, -1.5 G tr T, 0).
+//   corr_size2), -3 G (T_xx, T_xy, T_xz, T_yy), (-3 G T_yz, -3 G T_zz,
+//   -1.5 G tr T, 0).
 // Leaf j is record j (its position and G m, K -1, b^2 -1, no children,
 // zero quadrupole terms); internal node k is record n + kCorrGap + k, and the
 // child ids are in the same numbering, so every internal id is at least 64
@@ -117,6 +118,12 @@ __host__ __device__ inline uint32_t corr_child_b(uint32_t w, uint32_t ioff) {
 }
 // every record's byte offset fits 32 bits
 __host__ __device__ inline bool corr_offsets_32bit(uint32_t n) { return corr_records(n) <= (1ull << 26); }
+// The corrected walk's range: softening at least 1e-14 and every s^2 at most
+// 16 h_root^2 + softening <= 1e15, so that 1/s^5 stays a normal FP32 number
+// between 1e-37.5 and 1e35 (the walk forms the monopole as G M s^2 / s^5).
+__host__ __device__ inline bool corr_range_ok(float root_half, float soft) {
+    return soft >= 1e-14f && 16.0f * root_half * root_half + soft <= 1e15f;
+}
 
 // The size in the corrected walk's opening test (record word 7): the squared
 // distance b^2 from the node's centre of mass to the farthest corner of its

@@ -7,7 +7,7 @@
 //    two prefix entries, with no bottom-up pass.
 //  * k_qnodes: per internal node, the second moment about its centre of mass,
 //    the network's residual (BF16 MFMA, one evaluation per node), and the
-//    32-byte quadrupole record the walk reads.
+//    64-byte record the walk reads.
 //  * k_collect_pairs / k_pair_labels: (target, accepted node) pairs of the walk
 //    and their FP64 labels: the exact field of the node's bodies minus its
 //    monopole and exact quadrupole, in units of (h / s)^2 G M / s^2.
@@ -252,8 +252,9 @@ __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* 
             rec[1] = make_float4(corr_rel_k(nd.comMass.w, b2),
                                  __uint_as_float(corr_children_word(a, nd.childA >= n, nd.childB >= n)),
                                  __uint_as_float(b), b2);
-            rec[2] = make_float4(G * T[0], G * T[1], G * T[2], G * T[3]);
-            rec[3] = make_float4(G * T[4], G * T[5], -1.5f * (G * tr), 0.0f);
+            const float g3 = -3.0f * G;
+            rec[2] = make_float4(g3 * T[0], g3 * T[1], g3 * T[2], g3 * T[3]);
+            rec[3] = make_float4(g3 * T[4], g3 * T[5], -1.5f * (G * tr), 0.0f);
         }
     }
     if (evals && lane == 0) atomicAdd(&ctr->corrected, (unsigned long long)evals);

@@ -342,9 +342,9 @@ __device__ __forceinline__ void walk_lean(uint32_t root, uint32_t n_ids, uint32_
 // kLds: stack in LDS (any tree height, overflow checked, 64-bit record
 // offsets) instead of two lane registers (trees at most kLeanHeight high,
 // offsets below 2^32).
-// Needs eps^2 >= 1e-14 (1/s^5 stays finite) and the fast domain's bounds.
+// Needs eps^2 >= 1e-14 and 16 h_root^2 + eps^2 <= 1e15 (corr_range_ok).
 __device__ __forceinline__ bool walk_corrected_ok(float root_half, float soft) {
-    return soft >= 1e-14f && inv_sqrt_fast_domain(root_half, soft);
+    return corr_range_ok(root_half, soft);
 }
 
 // The traversal evaluates the two children of an opened node together: the
@@ -373,24 +373,25 @@ __device__ __forceinline__ void walk_corrected_pairs(uint32_t n_ids, uint32_t ro
                     : *reinterpret_cast<const u32x16*>(cbase + (uint32_t)(id << 6));
     };
     const uint32_t wbase = __builtin_amdgcn_readfirstlane(base);
-    // the node's terms (record r, offset r, s^2 d2) added to this lane's sum
+    // The node's terms (record r, offset r, s^2 d2) added to this lane's sum,
+    // from the record's S = -3 G T:
+    //     a += (W r + S r) / s^5,  W = G M s^2 - 2.5 (r.S.r) / s^2 - 1.5 G tr T
+    // (25 operations; within the range of walk_corrected_ok 1/s^5 is normal).
     auto terms = [&](const u32x16& r, float rx, float ry, float rz, float d2) {
         const float inv = __builtin_amdgcn_rsqf(d2);
         const float i2 = inv * inv;
-        const float i3 = i2 * inv;
-        const float i5 = i3 * i2;
+        const float i5 = (i2 * i2) * inv;
+        const float h = -2.5f * i2;
         const float q0 = __uint_as_float(r[8]), q1 = __uint_as_float(r[9]), q2 = __uint_as_float(r[10]);
         const float q3 = __uint_as_float(r[11]), q4 = __uint_as_float(r[12]), q5 = __uint_as_float(r[13]);
         const float tx = __builtin_fmaf(q0, rx, __builtin_fmaf(q1, ry, q2 * rz));
         const float ty = __builtin_fmaf(q1, rx, __builtin_fmaf(q3, ry, q4 * rz));
         const float tz = __builtin_fmaf(q2, rx, __builtin_fmaf(q4, ry, q5 * rz));
-        const float rtr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
-        const float c = __builtin_fmaf(rtr, 7.5f * i2, __uint_as_float(r[14]));
-        const float A = __builtin_fmaf(c, i5, __uint_as_float(r[3]) * i3);
-        const float m3 = -3.0f * i5;
-        ax = __builtin_fmaf(A, rx, __builtin_fmaf(m3, tx, ax));
-        ay = __builtin_fmaf(A, ry, __builtin_fmaf(m3, ty, ay));
-        az = __builtin_fmaf(A, rz, __builtin_fmaf(m3, tz, az));
+        const float rsr = __builtin_fmaf(rx, tx, __builtin_fmaf(ry, ty, rz * tz));
+        const float W = __builtin_fmaf(__uint_as_float(r[3]), d2, __builtin_fmaf(rsr, h, __uint_as_float(r[14])));
+        ax = __builtin_fmaf(i5, __builtin_fmaf(W, rx, tx), ax);
+        ay = __builtin_fmaf(i5, __builtin_fmaf(W, ry, ty), ay);
+        az = __builtin_fmaf(i5, __builtin_fmaf(W, rz, tz), az);
     };
     // the lanes of the mask that fail the opening test for record r
     auto fails = [&](const u32x16& r, float d2) -> unsigned long long {

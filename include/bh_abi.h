@@ -96,7 +96,9 @@ extern "C" {
  * per node (the quadrupole terms from FP64 moment prefix sums, plus the
  * network's residual under BH_FLAG_LEARNED); bh_compute_force under the same
  * flag then walks with them, without merging or collisions, softening >=
- * 1e-14. Every node taken adds G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr
+ * 1e-14 and 16 h_root^2 + softening <= 1e15 (h_root: half the root box's
+ * longest side; beyond it the walk adds no forces and sets error bit 2).
+ * Every node taken adds G M r / s^3 + (7.5 (r.GT.r) / s^2 - 1.5 G tr
  * T) r / s^5 - 3 GT r / s^5, s^2 = |r|^2 + softening (a leaf: its monopole),
  * in FP32 with the hardware inverse square root. Opening tests, with b the
  * distance from a node's centre of mass to the farthest corner of its box:
@@ -149,7 +151,7 @@ typedef struct {
     uint32_t tree_height;         /* height of the last tree (the reference fills at most 64 levels) */
     uint32_t max_stack;           /* deepest walk stack of the last walk */
     uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow, bit2 the corrected walk's box
-                                     beyond its range (16 h_root^2 + softening > 1e29: no forces) */
+                                     beyond its range (16 h_root^2 + softening > 1e15: no forces) */
     uint32_t reserved;
     uint64_t corrected;           /* network evaluations (one per internal node) of the last tree build */
     uint64_t quad_terms;          /* quadrupole terms added by the last corrected walk (accepted internal nodes) */

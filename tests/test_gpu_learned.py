@@ -379,3 +379,28 @@ def test_correction_gains_on_held_out_data():
     assert errs["learned"][1] <= 1.03 * errs["quadrupole"][1]
     bh.close()
     m.close()
+
+
+def test_corrected_walk_outside_its_range_reports_and_adds_nothing():
+    """16 h_root^2 + softening above 1e15 (1/s^5 would leave the normal FP32
+    range): the walk adds no forces and sets error bit 2; the plain walk of
+    the same bodies is unaffected."""
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    print("[gpu] corrected walk outside its range", flush=True)
+    bodies = ic.plummer(4096, 11)
+    bodies[:, 4:7] = 0.0
+    bodies[0, 0:3] = (3e7, 0.0, 0.0)          # the root box's half side ~1.5e7: 16 h^2 ~ 3.6e15
+    prm = B.make_params(theta=0.5, dt=1.0, softening=0.01, G=1.0, flags=B.FLAG_QUADRUPOLE)
+    bh = B.BarnesHut(bodies, ic.tight_bounds(bodies), prm)
+    bh.init()
+    bh.build_current_tree(prm)
+    bh.compute_force(prm)
+    assert bh.counters()["error_flags"] & 4
+    assert not bh.bodies(B.BODIES_OUT)[:, 4:7].any()
+    plain = B.make_params(theta=0.5, dt=1.0, softening=0.01, G=1.0)
+    bh.build_current_tree(plain)
+    bh.compute_force(plain)
+    assert bh.counters()["error_flags"] == 0
+    assert bh.bodies(B.BODIES_OUT)[:, 4:7].any()
+    bh.close()
