@@ -31,6 +31,7 @@ STATIC, DYNAMIC = 0, 1
 FLAG_COLORS, FLAG_TIMING, FLAG_LEARNED, FLAG_GENERAL_WALK, FLAG_QUADRUPOLE = 1, 2, 4, 8, 16
 FLAG_RELATIVE_MAC = 32
 FLAG_GRAPH = 64
+FLAG_FILL_CAP = 128
 FLAG_CORRECTED = FLAG_LEARNED | FLAG_QUADRUPOLE
 STAGES = ("reset", "dead", "bounds", "morton", "sort", "tree", "force", "merge")
 NOALLOC_MAX_SAMPLE = 65536   # BH_NOALLOC_MAX_SAMPLE

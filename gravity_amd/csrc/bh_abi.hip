@@ -80,6 +80,7 @@ struct bh_context {
     // IndexIn (the buffers trade places).
     uint32_t* sort_pos[2] = {nullptr, nullptr};
     bool colors = false;
+    uint32_t fill_cap = 0;   // the last tree's fill cap (BH_FLAG_FILL_CAP), for its std430 download
     const bh_mlp* correction = nullptr;       // attached network of the corrected walk
     // corrected walk (allocated on first use): the 64-byte records of every
     // node id (layout with corr_records) and the FP64 moment prefix sums they
@@ -98,10 +99,12 @@ struct bh_context {
         int body_in, mi_in;
         uint32_t *index0, *index1, *sort0, *sort1;
         bool step_pos_valid, quad_valid, accmag_valid, colors;
+        uint32_t fill_cap;
         bool operator==(const Roles& o) const {
             return body_in == o.body_in && mi_in == o.mi_in && index0 == o.index0 && index1 == o.index1 &&
                    sort0 == o.sort0 && sort1 == o.sort1 && step_pos_valid == o.step_pos_valid &&
-                   quad_valid == o.quad_valid && accmag_valid == o.accmag_valid && colors == o.colors;
+                   quad_valid == o.quad_valid && accmag_valid == o.accmag_valid && colors == o.colors &&
+                   fill_cap == o.fill_cap;
         }
     };
     struct StepGraph {
@@ -538,7 +541,7 @@ int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {
     }
     if (b == BH_BINDING_LEAF_NODES || b == BH_BINDING_INTERNAL_NODES) {
         launch_nodes_to_std430(c->stream, c->n, c->sim, c->tree, c->colors, b == BH_BINDING_LEAF_NODES ? 0 : 1,
-                               c->staging_nodes);
+                               c->fill_cap, c->staging_nodes);
         rc = check_launch(c, "nodes_to_std430");
         if (rc) return rc;
         HIPCHECK(c, hipMemcpyAsync(host, (uint8_t*)c->staging_nodes + off, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -669,11 +672,14 @@ int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* 
     const bool quad = p && (p->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED));
     const bool learned = p && (p->flags & BH_FLAG_LEARNED);
     if (learned && !c->correction) return fail(c, BH_ERR_INVALID_ARGUMENT, "BH_FLAG_LEARNED without an attached network");
+    const uint32_t cap = p && (p->flags & BH_FLAG_FILL_CAP) ? kReferenceFillPasses : 0u;
+    if (quad && cap) return fail(c, BH_ERR_UNSUPPORTED, "BH_FLAG_FILL_CAP is for the plain method");
     int rc;
     if (quad && (rc = ensure_quad(c))) return rc;
+    c->fill_cap = cap;
     launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
                 c->tree, c->colors ? 1 : 0, positions ? step_pos : nullptr,
-                positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr);
+                positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr, cap);
     rc = check_launch(c, "tree");
     if (rc == BH_OK && positions) std::swap(c->index[c->mi_in], c->sort_pos[1 - c->mi_in]);
     c->quad_valid = false;
@@ -813,7 +819,7 @@ int step_enqueue(bh_context* c, const bh_params* p) {
 
 bh_context::Roles roles_of(const bh_context* c) {
     return {c->body_in,  c->mi_in,          c->index[0],   c->index[1],      c->sort_pos[0],
-            c->sort_pos[1], c->step_pos_valid, c->quad_valid, c->accmag_valid, c->colors};
+            c->sort_pos[1], c->step_pos_valid, c->quad_valid, c->accmag_valid, c->colors, c->fill_cap};
 }
 
 void set_roles(bh_context* c, const bh_context::Roles& r) {
@@ -827,6 +833,7 @@ void set_roles(bh_context* c, const bh_context::Roles& r) {
     c->quad_valid = r.quad_valid;
     c->accmag_valid = r.accmag_valid;
     c->colors = r.colors;
+    c->fill_cap = r.fill_cap;
 }
 
 // The step as an instantiated graph: the one captured for the current roles

@@ -258,7 +258,8 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
     if (!out || !counts || !bodies || !bounds || !params || n_sims == 0) return BH_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     if (cfg && !valid_config(*cfg)) return BH_ERR_INVALID_ARGUMENT;
-    if (params->merging_collision_or_neither != BH_NEITHER || params->static_or_dynamic != BH_DYNAMIC)
+    if (params->merging_collision_or_neither != BH_NEITHER || params->static_or_dynamic != BH_DYNAMIC ||
+        (params->flags & BH_FLAG_FILL_CAP))
         return BH_ERR_UNSUPPORTED;
     const bool corrected = (params->flags & (BH_FLAG_QUADRUPOLE | BH_FLAG_LEARNED)) != 0;
     const bool relative = corrected && (params->flags & BH_FLAG_RELATIVE_MAC);

@@ -706,8 +706,29 @@ __device__ __forceinline__ void get_state(const float4* x, NodeState& s, int col
 // segment off = 0, ibase = 0, n_ids = n; an ensemble has one per simulation.
 struct TreeSeg {
     uint32_t off, num, ibase, n_ids;
-    float G;   // the walk records carry G m
+    float G;                // the walk records carry G m
+    uint32_t fill_cap = 0;  // nonzero: nodes higher than this are left unfilled
 };
+
+// The reference fills the tree in PROPAGATE_NODES_ITERATIONS = 64 passes of
+// one height each (gpu/GPU.java:41, tree/bh_tree.comp:183-259): a node
+// higher than that keeps what the build gave it (tree/bh_tree.comp:114-117):
+// no mass, the empty box (1e38, -1e38), no colour. Its walk record has the
+// half side of that box, -1e38, which every body accepts at once (no force),
+// and a half2 of -1 (never opened by the wide-margin test).
+__device__ __forceinline__ void unfilled_node(uint32_t k, uint32_t height, const NodeState& A, const NodeState& B,
+                                              const TreeArrays& T, int colors) {
+    WalkNode nd;
+    nd.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
+    nd.half = fdiv(fsub(-1e38f, 1e38f), 2.0f);
+    nd.childA = A.id;
+    nd.childB = B.id;
+    nd.half2 = -1.0f;
+    T.nodes[k] = nd;
+    T.node_aabb[2 * k] = make_float4(1e38f, 1e38f, 1e38f, __uint_as_float(height));
+    T.node_aabb[2 * k + 1] = make_float4(-1e38f, -1e38f, -1e38f, 0.0f);
+    if (colors) T.node_color[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
 
 // The summary of leaf (sorted position) j as its own thread starts it.
 __device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
@@ -826,10 +847,14 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
     nd.childB = B.id;
     nd.half2 = fmul(nd.half, nd.half);
     // these arrays are read only by later kernels
-    T.nodes[k] = nd;
-    T.node_aabb[2 * k] = make_float4(p.mn.x, p.mn.y, p.mn.z, __uint_as_float(p.height));
-    T.node_aabb[2 * k + 1] = make_float4(p.mx.x, p.mx.y, p.mx.z, M);
-    if (colors) T.node_color[k] = p.col;
+    if (S.fill_cap && p.height > S.fill_cap) {
+        unfilled_node(k, p.height, A, B, T, colors);   // (its ancestors are higher still)
+    } else {
+        T.nodes[k] = nd;
+        T.node_aabb[2 * k] = make_float4(p.mn.x, p.mn.y, p.mn.z, __uint_as_float(p.height));
+        T.node_aabb[2 * k + 1] = make_float4(p.mx.x, p.mx.y, p.mx.z, M);
+        if (colors) T.node_color[k] = p.col;
+    }
     T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
     if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
     if (B.id < S.n_ids) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - S.n_ids] = p.id;
@@ -1073,7 +1098,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
                                                      const uint64_t* __restrict__ keys, TreeArrays T,
                                                      const float4* __restrict__ step_pos,
                                                      const uint32_t* __restrict__ from,
-                                                     uint32_t* __restrict__ ids_out) {
+                                                     uint32_t* __restrict__ ids_out, uint32_t fill_cap) {
     __shared__ TreeBlockLds<kColors> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
@@ -1107,7 +1132,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
         ctr->treeHeight = 0;
         return;
     }
-    const TreeSeg S = {0u, (uint32_t)num, 0u, n, sim->gravitationalConstant};
+    const TreeSeg S = {0u, (uint32_t)num, 0u, n, sim->gravitationalConstant, fill_cap};
     // the previous walk's positions hold this step's bodies unless the
     // partition before the sort removed some (then their order is another)
     const bool by_pos = from && step_pos && ctr->died == 0;
@@ -1133,14 +1158,15 @@ __device__ __forceinline__ NodeState frontier_state(const FrontierEntry& e) {
 __global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                                     const float4* __restrict__ src,
                                                     const uint32_t* __restrict__ index_in,
-                                                    const uint64_t* __restrict__ keys, TreeArrays T, int colors) {
+                                                    const uint64_t* __restrict__ keys, TreeArrays T, int colors,
+                                                    uint32_t fill_cap) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *T.frontier_count) return;
     const uint32_t n = sim->initialNumBodies;
     const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
     NodeState me = frontier_state(e);
     if (colors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
-    const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant};
+    const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant, fill_cap};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
     tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors);
 }
@@ -1346,8 +1372,12 @@ __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const 
 }
 
 // ------------------------------------------------------------------ boundary layouts
+// fill_cap: the cap of the tree's build (0: none); a node higher than it is
+// unfilled and shows the build's nodeDepth, the start of its range
+// (tree/bh_tree.comp:127), and in readyChildren the number of its children
+// that were filled (leaves count as filled).
 __global__ void k_nodes_to_std430(const SimValues* __restrict__ sim, TreeArrays T, int colors, int which,
-                                  NodeStd430* __restrict__ out) {
+                                  uint32_t fill_cap, NodeStd430* __restrict__ out) {
     const uint32_t n = sim->initialNumBodies, num = sim->numBodies;
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     NodeStd430 o = {};
@@ -1376,6 +1406,13 @@ __global__ void k_nodes_to_std430(const SimValues* __restrict__ sim, TreeArrays 
             o.nodeDepth = __float_as_uint(mn.w);
             o.bodiesContained = T.node_range[k].y;
             o.readyChildren = kEmpty;
+            if (fill_cap && o.nodeDepth > fill_cap) {
+                auto filled = [&](uint32_t c) -> uint32_t {
+                    return c < n || __float_as_uint(T.node_aabb[2 * (c - n)].w) <= fill_cap ? 1u : 0u;
+                };
+                o.nodeDepth = T.node_range[k].x;
+                o.readyChildren = filled(w.childA) + filled(w.childB);
+            }
             o.parentId = T.node_parent[k];
             if (colors) { float4 c = T.node_color[k]; o.avgColor[0] = c.x; o.avgColor[1] = c.y; o.avgColor[2] = c.z; }
         }
@@ -1403,7 +1440,9 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
         w.half = fdiv(ext, 2.0f);
         w.childA = o.childA;
         w.childB = o.childB;
-        w.half2 = fmul(w.half, w.half);
+        // (an unfilled node's box is empty: a negative half side, never
+        // opened by the wide-margin test)
+        w.half2 = w.half < 0.0f ? -1.0f : fmul(w.half, w.half);
         T.nodes[k] = w;
         T.node_aabb[2 * k] = make_float4(o.aabb[0], o.aabb[1], o.aabb[2], __uint_as_float(o.nodeDepth));
         T.node_aabb[2 * k + 1] = make_float4(o.aabb[3], o.aabb[4], o.aabb[5], o.comMass[3]);
@@ -1476,19 +1515,19 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
-                 const float4* step_pos, const uint32_t* from, uint32_t* ids_out) {
+                 const float4* step_pos, const uint32_t* from, uint32_t* ids_out, uint32_t fill_cap) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
     if (colors)
         k_tree<true><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
-                                                                ids_out);
+                                                                ids_out, fill_cap);
     else
         k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
-                                                                 ids_out);
+                                                                 ids_out, fill_cap);
     if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
     // the second launch meets leaves by their ids in the new order
     k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src,
                                                                                   from ? ids_out : index_in, keys,
-                                                                                  T, colors);
+                                                                                  T, colors, fill_cap);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {
@@ -1499,8 +1538,8 @@ void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, 
     k_merge<<<1, 1024, 0, s>>>(dst, tasks, cap, ctr, reserve, done);
 }
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
-                            int which, NodeStd430* out) {
-    k_nodes_to_std430<<<cdiv(n, 256), 256, 0, s>>>(sim, T, colors, which, out);
+                            int which, uint32_t fill_cap, NodeStd430* out) {
+    k_nodes_to_std430<<<cdiv(n, 256), 256, 0, s>>>(sim, T, colors, which, fill_cap, out);
 }
 void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
                             const NodeStd430* leaf_in, const NodeStd430* int_in, int which) {

@@ -231,9 +231,14 @@ void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uin
 // order before the sort and the tree writes the ids in the new order to
 // ids_out. step_pos (with from): the previous walk's positions by its sorted
 // positions, read instead of the bodies when the partition removed nobody.
+// fill_cap: 0, or the reference's pass count (kReferenceFillPasses): nodes
+// higher than it are left unfilled.
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
-                 const float4* step_pos = nullptr, const uint32_t* from = nullptr, uint32_t* ids_out = nullptr);
+                 const float4* step_pos = nullptr, const uint32_t* from = nullptr, uint32_t* ids_out = nullptr,
+                 uint32_t fill_cap = 0);
+// the reference's tree-fill passes, one height each (gpu/GPU.java:41)
+constexpr uint32_t kReferenceFillPasses = 64;
 void launch_walk_counters_reset(hipStream_t s, const SimValues* sim, DevCounters* ctr);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
 // The corrected walk's node records on the current tree: FP64 prefix sums of
@@ -277,7 +282,7 @@ void launch_mlp_forward_mfma(hipStream_t s, const float* x, uint32_t n, const __
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done);
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
-                            int which, NodeStd430* out);
+                            int which, uint32_t fill_cap, NodeStd430* out);
 void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
                             const NodeStd430* leaf_in, const NodeStd430* int_in, int which);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);

@@ -90,6 +90,12 @@ extern "C" {
                                   step after anything but bh_step (an upload, a stage call, bh_init)
                                   or with BH_FLAG_TIMING runs directly. Attaching another network
                                   drops the graphs. */
+#define BH_FLAG_FILL_CAP 128u  /* fill the tree as the reference does, in 64 passes of one height each
+                                  (gpu/GPU.java:41, tree/bh_tree.comp:183-259): a node higher than 64
+                                  keeps the build's state (no mass, the empty box, nodeDepth the start
+                                  of its range, readyChildren its filled children), so in a taller tree
+                                  the root is empty and every body feels no force. Without it every
+                                  level is filled. Plain method only. */
 
 /* The corrected walk (BH_FLAG_QUADRUPOLE / BH_FLAG_LEARNED; no reference
  * counterpart). bh_build_tree under either flag also makes one 64-byte record

@@ -22,8 +22,9 @@
  * of the sort, walk workgroup size and walk stack depth are chosen by
  * bh_ensemble_autotune from measurements on the ensemble itself.
  *
- * Scope: dynamic bounds, merging off and every body alive (non-zero mass);
- * each simulation has at least 2 bodies. The walk is the plain method, or
+ * Scope: dynamic bounds, merging off, every level of each tree filled (no
+ * BH_FLAG_FILL_CAP) and every body alive (non-zero mass); each simulation
+ * has at least 2 bodies. The walk is the plain method, or
  * the corrected walk of include/bh_abi.h (params->flags BH_FLAG_QUADRUPOLE
  * or BH_FLAG_LEARNED, optionally BH_FLAG_RELATIVE_MAC with
  * params->force_tolerance): node records over all simulations' trees,

@@ -149,10 +149,32 @@ def scenario_list():
     # theta = 0 opens every internal node: the walk is a direct sum in walk order
     sc.append(dict(name="theta0_512", bodies=ic.plummer(512, 4), steps=1, theta=0.0, dt=1e-3, soft=0.01,
                    merging=0, dynamic=1, wrap=0))
+    # a tree higher than the 64 fill passes (GPU.java:41): 32 bodies at one
+    # point (a subtree of height 5) under a chain of 63 bodies whose Morton
+    # codes each hold one bit (height 63 above it); the top nodes stay
+    # unfilled and the walk takes the empty root at once
+    sc.append(dict(name="tall_tree_95", bodies=tall_tree_bodies(), steps=2, theta=0.5, dt=1e-3, soft=0.01,
+                   merging=0, dynamic=0, wrap=0, bounds=np.array([0, 0, 0, 1, 1, 1], np.float32)))
     # the two-galaxy workload (disks with bulges, 10 apart, parabolic orbit)
     sc.append(dict(name="two_galaxy_4096", bodies=ic.two_galaxy(2048, 23), steps=2, theta=0.5, dt=1e-3,
                    soft=0.01, merging=0, dynamic=1, wrap=0))
     return sc
+
+
+def tall_tree_bodies(dup=32):
+    """Bodies in the box [0, 1]^3 whose Morton codes are 0 (dup bodies) and
+    2^k for k = 0 .. 62: a quantised coordinate 2^j on one axis (x: bit 3j +
+    2, y: 3j + 1, z: 3j) and 0 on the others, each at the middle of its cell
+    (q = floor(p * 2097151))."""
+    cells = [(0, 0, 0)] * dup
+    for j in range(21):
+        cells += [(0, 0, 1 << j), (0, 1 << j, 0), (1 << j, 0, 0)]
+    q = np.array(cells, np.float64)
+    b = np.zeros((len(cells), 8), np.float32)
+    b[:, 0:3] = (q + 0.5) / 2097151.0
+    b[:, 3] = 1.0 / len(cells)
+    b[:, 7] = 1.0
+    return b
 
 
 def unit_sets():

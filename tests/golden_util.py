@@ -5,7 +5,7 @@ import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FULL = ["plummer_4096", "merge_4096", "collision_4096", "edge_dead_nan_1000", "edge_duplicates_512",
-        "static_kill_1024", "static_wrap_1024", "theta0_512", "two_galaxy_4096"]
+        "static_kill_1024", "static_wrap_1024", "theta0_512", "two_galaxy_4096", "tall_tree_95"]
 
 
 def load(name):

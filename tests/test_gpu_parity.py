@@ -56,6 +56,9 @@ def make_bh(fx, s=0, flags=0):
 
 
 CASES = [(name, s) for name in FULL for s in range(steps(load(name)))]
+# fixtures whose tree is higher than the reference's 64 fill passes: the
+# reference's behaviour is BH_FLAG_FILL_CAP
+CAPPED = {"tall_tree_95"}
 
 
 @pytest.mark.parametrize("name,s", CASES)
@@ -64,7 +67,8 @@ def test_gpu_stage_parity(name, s):
     fx = load(name)
     n = fx["bodies0"].shape[0]
     print(f"[gpu] stage parity {name} step {s} n={n}", flush=True)
-    bh, prm, units, G, blc = make_bh(fx, s, flags=B.FLAG_COLORS)
+    capped = name in CAPPED
+    bh, prm, units, G, blc = make_bh(fx, s, flags=B.FLAG_COLORS | (B.FLAG_FILL_CAP if capped else 0))
     src, idx, num_before = stage_inputs(fx, s)
     oob_before = int(fx[f"s{s - 1}_out_of_bounds"]) if s else 0
     bh.upload(B.SIMULATION_VALUES, sim_array(n, num_before, fx[f"s{s}_bounds_prev"], G, blc, units, oob_before))
@@ -102,7 +106,10 @@ def test_gpu_stage_parity(name, s):
     assert_bits(internal[:, 0:10].view(np.float32), ref_i[:, 0:10].view(np.float32), "internal fill")
     assert_bits(leaf[:, 0:10].view(np.float32), ref_l[:, 0:10].view(np.float32), "leaf fill")
     np.testing.assert_allclose(internal[:, 16:20].view(np.float32), ref_i[:, 16:20].view(np.float32), atol=1e-3)
-    assert bh.counters()["tree_height"] == int(ref_i[0, 12])
+    if capped:   # (the reference's root keeps the build's nodeDepth; the counter has the height)
+        assert bh.counters()["tree_height"] > 64 and ref_i[0, 3] == 0
+    else:
+        assert bh.counters()["tree_height"] == int(ref_i[0, 12])
 
     # accelerations: the reference's own extraction (velocities 0, dt 1 -> vel' = a)
     start = fx[f"s{s}_bodies_start"]
@@ -363,4 +370,32 @@ def test_gpu_plummer_65536_ten_steps():
         assert c["tree_height"] == int(fx[f"s{s}_height"])
         assert_bits(bh.bounds(), fx[f"s{s}_bounds"], f"bounds step {s}")
     assert_bits(bh.bodies()[fx["sample"]], fx["final_sample"], "final sample")
+    bh.close()
+
+
+def test_gpu_tall_tree_filled_without_the_cap():
+    """Without BH_FLAG_FILL_CAP every level of a tree higher than 64 is
+    filled: nodes and accelerations equal the oracle's with no fill cap (its
+    walk stack raised to the tree's height)."""
+    from gravity_amd import barnes_hut as B
+    fx = load("tall_tree_95")
+    n = fx["bodies0"].shape[0]
+    print(f"[gpu] tall tree without the fill cap n={n}", flush=True)
+    bh, prm, units, G, blc = make_bh(fx)
+    bodies = fx["bodies0"].copy()
+    assert not bodies[:, 4:7].any()          # at rest: one step at dt 1 gives the accelerations as velocities
+    p1 = B.Params.from_buffer_copy(prm)
+    p1.dt = 1.0
+    bh.init()
+    bh.step(p1)
+    c = bh.counters()
+    assert c["tree_height"] == 68 and c["error_flags"] == 0
+    sim = O.Simulation(bodies, fx["bounds0"], units.mass, units.density, G=G, body_len_const=blc, fill_cap=0,
+                       stack_cap=256)
+    assert sim.step(O.params(theta=float(fx["params"][0]), dt=1.0, softening=float(fx["params"][2]),
+                             dynamic=0)) == 0
+    assert c["opened"] == sim.st.last.opened and c["opened"] > 0
+    got = bh.bodies()
+    assert_bits(got, sim.bodies(), "bodies after one uncapped step")
+    assert np.abs(got[:, 4:7]).max() > 0
     bh.close()
