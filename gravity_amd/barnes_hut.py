@@ -94,8 +94,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.bh_direct_sum_f64.argtypes = [ctx, P, C.c_uint32, C.c_double, P]
     L.bh_direct_sum_f64_noalloc.argtypes = [ctx, P, C.c_uint32, C.c_double, P]
     L.bh_selftest_math.argtypes = [ctx, P]
+    L.bh_render_nodes.argtypes = [ctx, C.c_uint32, C.c_uint32, P, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.bh_render_bodies.argtypes = [ctx, P, C.c_uint32, C.POINTER(C.c_uint32)]
     _lib = L
     return L
+
+
+# the render feed's records (include/bh_abi.h bh_render_node, bh_render_body)
+RENDER_NODE = np.dtype([("com", "<f4", 3), ("mass", "<f4"), ("aabb_min", "<f4", 3), ("depth", "<u4"),
+                        ("aabb_max", "<f4", 3), ("rgba", "<u4")])
+RENDER_BODY = np.dtype([("pos", "<f4", 3), ("radius", "<f4"), ("rgba", "<u4")])
 
 
 def make_params(theta=0.5, dt=1e-3, softening=0.01, elasticity=0.5, restitution=0.0, merging=NEITHER,
@@ -188,6 +196,26 @@ class BarnesHut:
         """The corrected walk's records of the internal nodes, (n - 1, 8) float32:
         G T xx xy xz yy yz zz, -1.5 G tr T, 0."""
         return self.download(NODE_QUADRUPOLES, np.float32).reshape(self.n - 1, 8)
+
+    def render_nodes(self, depth_min: int = 0, depth_max: int = 0xFFFFFFFF, cap: int | None = None) -> np.ndarray:
+        """The render feed's internal nodes (RENDER_NODE records) of the last tree."""
+        cap = self.n if cap is None else cap
+        out = np.zeros(max(cap, 1), RENDER_NODE)
+        count = C.c_uint32()
+        self._check(self.L.bh_render_nodes(self.ctx, depth_min, depth_max, out.ctypes.data_as(C.c_void_p), cap,
+                                           C.byref(count)), "render_nodes")
+        self.render_total = int(count.value)
+        return out[: min(cap, count.value)]
+
+    def render_bodies(self, cap: int | None = None) -> np.ndarray:
+        """The render feed's alive bodies (RENDER_BODY records) of BodiesIn."""
+        cap = self.n if cap is None else cap
+        out = np.zeros(max(cap, 1), RENDER_BODY)
+        count = C.c_uint32()
+        self._check(self.L.bh_render_bodies(self.ctx, out.ctypes.data_as(C.c_void_p), cap, C.byref(count)),
+                    "render_bodies")
+        self.render_total = int(count.value)
+        return out[: min(cap, count.value)]
 
     def merge_tasks(self) -> np.ndarray:
         raw = self.download(MERGE_TASKS, np.uint32)

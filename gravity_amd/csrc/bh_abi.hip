@@ -64,7 +64,8 @@ struct bh_context {
     uint32_t* digit_total = nullptr;
     unsigned long long* reserve = nullptr;
     uint32_t* task_done = nullptr;
-    NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1
+    NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1 (and the render feed)
+    uint32_t* feed_counts = nullptr;          // [render_feed_blocks(n) + 1] render feed offsets
     float* leaf_radius = nullptr;             // body radius per sorted position (merging/collision walks)
     // Positions the last bh_step's walk wrote, by sorted position (the same
     // values it wrote to the body buffer by id). While valid, the next step's
@@ -352,6 +353,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->reserve, (size_t)n));
         TRY(dalloc(c, &c->task_done, (size_t)n));
         TRY(dalloc(c, &c->staging_nodes, (size_t)n));
+        TRY(dalloc(c, &c->feed_counts, (size_t)render_feed_blocks(n) + 1));
         TRY(dalloc(c, &c->leaf_radius, (size_t)n));
         TRY(dalloc(c, &c->step_pos, (size_t)n));
         // zero everything the reference zero-initialises, then pack sim
@@ -404,7 +406,8 @@ void bh_destroy(bh_context* c) {
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
                     c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
-                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles, c->acc_mag};
+                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles, c->acc_mag,
+                    c->feed_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev) {
@@ -528,6 +531,48 @@ int bh_upload(bh_context* c, int b, const void* host, size_t off, size_t bytes) 
     HIPCHECK(c, hipMemcpyAsync((uint8_t*)binding_ptr(c, b) + off, host, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return BH_OK;
+}
+
+extern "C++" {
+namespace {
+// the render feed's records, staged in the node conversion buffer (80 bytes
+// per body: more than either kind needs), then copied in one piece
+template <class R>
+int feed_copy(bh_context* c, R* out, uint32_t cap, uint32_t* count, const char* what) {
+    int rc = check_launch(c, what);
+    if (rc) return rc;
+    uint32_t total = 0;
+    HIPCHECK(c, hipMemcpyAsync(&total, c->feed_counts + render_feed_blocks(c->n), 4, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const uint32_t m = total < cap ? total : cap;
+    if (m) {
+        HIPCHECK(c, hipMemcpyAsync(out, c->staging_nodes, (size_t)m * sizeof(R), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    *count = total;
+    return BH_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int bh_render_nodes(bh_context* c, uint32_t depth_min, uint32_t depth_max, bh_render_node* out, uint32_t cap,
+                    uint32_t* count) {
+    if (!c || !count || (!out && cap)) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    static_assert(sizeof(bh_render_node) <= sizeof(NodeStd430), "staging");
+    launch_render_nodes(c->stream, c->n, c->sim, c->tree, c->colors ? 1 : 0, depth_min, depth_max, c->feed_counts,
+                        reinterpret_cast<bh_render_node*>(c->staging_nodes), cap < c->n ? cap : c->n);
+    return feed_copy(c, out, cap, count, "render nodes");
+}
+
+int bh_render_bodies(bh_context* c, bh_render_body* out, uint32_t cap, uint32_t* count) {
+    if (!c || !count || (!out && cap)) return BH_ERR_INVALID_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    static_assert(sizeof(bh_render_body) <= sizeof(NodeStd430), "staging");
+    launch_render_bodies(c->stream, c->n, c->sim, c->bodies[c->body_in], c->feed_counts,
+                         reinterpret_cast<bh_render_body*>(c->staging_nodes), cap < c->n ? cap : c->n);
+    return feed_copy(c, out, cap, count, "render bodies");
 }
 
 int bh_download(bh_context* c, int b, void* host, size_t off, size_t bytes) {

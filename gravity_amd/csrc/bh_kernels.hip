@@ -1371,6 +1371,109 @@ __global__ __launch_bounds__(1024) void k_merge(float4* __restrict__ dst, const 
     if (applied && lane_id() == 0) atomicAdd(&ctr->mergesApplied, applied);
 }
 
+// ------------------------------------------------------------------ render feed
+// Stable compaction in three launches: per-block counts, one block turning
+// them into offsets (and the total), then each block writes its records at
+// its offset in item order.
+__device__ __forceinline__ uint32_t pack_rgba8(float4 c) {
+    auto q = [](float x) -> uint32_t { return __float2uint_rn(gmin(gmax(x, 0.0f), 1.0f) * 255.0f); };
+    return q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (255u << 24);
+}
+
+struct FeedNodes {
+    const SimValues* sim;
+    TreeArrays T;
+    int colors;
+    uint32_t dmin, dmax;
+    __device__ uint32_t items() const { const uint32_t m = sim->numBodies; return m > 1 ? m - 1 : 0; }
+    // the static culls of the node-glow and region shaders (emptyAABB: min.x > max.x)
+    __device__ bool keep(uint32_t k) const {
+        const float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
+        const uint32_t d = __float_as_uint(mn.w);
+        return !(mn.x > mx.x) && mx.w > 0.0f && T.node_range[k].y > 0 && d >= dmin && d <= dmax;
+    }
+    __device__ void write(uint32_t k, bh_render_node* out) const {
+        const float4 mn = T.node_aabb[2 * k], mx = T.node_aabb[2 * k + 1];
+        const float4 com = T.nodes[k].comMass;
+        bh_render_node r;
+        r.com[0] = com.x; r.com[1] = com.y; r.com[2] = com.z; r.mass = mx.w;
+        r.aabb_min[0] = mn.x; r.aabb_min[1] = mn.y; r.aabb_min[2] = mn.z; r.depth = __float_as_uint(mn.w);
+        r.aabb_max[0] = mx.x; r.aabb_max[1] = mx.y; r.aabb_max[2] = mx.z;
+        r.rgba = colors ? pack_rgba8(T.node_color[k]) : 0u;
+        *out = r;
+    }
+};
+
+struct FeedBodies {
+    const SimValues* sim;
+    const float4* bodies;
+    __device__ uint32_t items() const { return sim->initialNumBodies; }
+    __device__ bool keep(uint32_t i) const { return bodies[2 * i].w != 0.0f; }   // alive (common.glsl:203-205)
+    __device__ void write(uint32_t i, bh_render_body* out) const {
+        const float4 pm = bodies[2 * i], vd = bodies[2 * i + 1];
+        bh_render_body r;
+        r.pos[0] = pm.x; r.pos[1] = pm.y; r.pos[2] = pm.z;
+        // common/common.glsl:228-230 and :235-276, as the leaves' colours
+        r.radius = fmul(sim->bodyLengthConstant, powf(fdiv(pm.w, vd.w), 1.0f / 3.0f));
+        float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (pm.w > 0.0f && vd.w > 0.0f) c = star_color(fmul(pm.w, sim->unitMass), fmul(vd.w, sim->unitDensity));
+        r.rgba = pack_rgba8(c);
+        *out = r;
+    }
+};
+
+template <class F>
+__global__ __launch_bounds__(kFeedBlock) void k_feed_count(const F f, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_w[kFeedBlock / 64];
+    const uint32_t i = blockIdx.x * kFeedBlock + threadIdx.x;
+    const uint64_t b = ballot(i < f.items() && f.keep(i));
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kFeedBlock / 64; w++) t += s_w[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// one block of 1024: counts[0 .. nb) -> exclusive offsets, counts[nb] = total
+__global__ __launch_bounds__(1024) void k_feed_scan(uint32_t* __restrict__ counts, uint32_t nb) {
+    __shared__ uint32_t s[1024];
+    const uint32_t per = (nb + 1023) / 1024, t0 = threadIdx.x * per, t1 = min(nb, t0 + per);
+    uint32_t sum = 0;
+    for (uint32_t b = t0; b < t1; b++) sum += counts[b];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {   // inclusive scan (Hillis-Steele)
+        const uint32_t v = threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t b = t0; b < t1; b++) {
+        const uint32_t c = counts[b];
+        counts[b] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) counts[nb] = s[1023];
+}
+
+template <class F, class R>
+__global__ __launch_bounds__(kFeedBlock) void k_feed_write(const F f, const uint32_t* __restrict__ offsets,
+                                                           R* __restrict__ out, uint32_t cap) {
+    __shared__ uint32_t s_w[kFeedBlock / 64];
+    const uint32_t i = blockIdx.x * kFeedBlock + threadIdx.x, w = threadIdx.x >> 6;
+    const bool k = i < f.items() && f.keep(i);
+    const uint64_t b = ballot(k);
+    if (lane_id() == 0) s_w[w] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t pos = offsets[blockIdx.x];
+    for (uint32_t v = 0; v < w; v++) pos += s_w[v];
+    pos += (uint32_t)__popcll(b & ((1ull << lane_id()) - 1ull));
+    if (k && pos < cap) f.write(i, out + pos);
+}
+
 // ------------------------------------------------------------------ boundary layouts
 // fill_cap: the cap of the tree's build (0: none); a node higher than it is
 // unfilled and shows the build's nodeDepth, the start of its range
@@ -1540,6 +1643,22 @@ void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, 
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
                             int which, uint32_t fill_cap, NodeStd430* out) {
     k_nodes_to_std430<<<cdiv(n, 256), 256, 0, s>>>(sim, T, colors, which, fill_cap, out);
+}
+void launch_render_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
+                         uint32_t depth_min, uint32_t depth_max, uint32_t* counts, bh_render_node* out, uint32_t cap) {
+    const FeedNodes f = {sim, T, colors, depth_min, depth_max};
+    const uint32_t nb = render_feed_blocks(n);   // (at most n - 1 internal nodes)
+    k_feed_count<<<nb, kFeedBlock, 0, s>>>(f, counts);
+    k_feed_scan<<<1, 1024, 0, s>>>(counts, nb);
+    k_feed_write<<<nb, kFeedBlock, 0, s>>>(f, counts, out, cap);
+}
+void launch_render_bodies(hipStream_t s, uint32_t n, const SimValues* sim, const float4* bodies, uint32_t* counts,
+                          bh_render_body* out, uint32_t cap) {
+    const FeedBodies f = {sim, bodies};
+    const uint32_t nb = render_feed_blocks(n);
+    k_feed_count<<<nb, kFeedBlock, 0, s>>>(f, counts);
+    k_feed_scan<<<1, 1024, 0, s>>>(counts, nb);
+    k_feed_write<<<nb, kFeedBlock, 0, s>>>(f, counts, out, cap);
 }
 void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
                             const NodeStd430* leaf_in, const NodeStd430* int_in, int which) {

@@ -6,7 +6,7 @@
 
 #include "bh_device.hpp"
 
-struct bh_mlp;   // include/bh_abi.h
+#include "../../include/bh_abi.h"
 
 namespace bh {
 
@@ -283,6 +283,15 @@ void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, 
                   unsigned long long* reserve, uint32_t* done);
 void launch_nodes_to_std430(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
                             int which, uint32_t fill_cap, NodeStd430* out);
+// Render feed: compaction of the nodes / bodies that pass (counts: the
+// per-block counts and offsets, render_feed_blocks(items) + 1 entries; the
+// last receives the total). out: device records, at most cap written.
+constexpr uint32_t kFeedBlock = 256;
+inline uint32_t render_feed_blocks(uint32_t items) { return (items + kFeedBlock - 1) / kFeedBlock; }
+void launch_render_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T, int colors,
+                         uint32_t depth_min, uint32_t depth_max, uint32_t* counts, bh_render_node* out, uint32_t cap);
+void launch_render_bodies(hipStream_t s, uint32_t n, const SimValues* sim, const float4* bodies, uint32_t* counts,
+                          bh_render_body* out, uint32_t cap);
 void launch_std430_to_nodes(hipStream_t s, uint32_t n, const SimValues* sim, const TreeArrays& T,
                             const NodeStd430* leaf_in, const NodeStd430* int_in, int which);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);

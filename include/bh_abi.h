@@ -216,6 +216,38 @@ int bh_get_counters(bh_context *ctx, bh_counters *out);
  * steps summed. Waits for the stream. */
 int bh_get_stage_times(bh_context *ctx, float *ms, int n);
 
+/* Render feed (no reference counterpart): what the reference's renderer
+ * reads each frame, compacted on the device and copied in one piece, instead
+ * of the 80-byte std430 nodes of every binding. The node-glow and region
+ * shaders draw InternalNodes' comMass, aabb, nodeDepth and avgColor and cull
+ * empty boxes, massless nodes and depths outside their range
+ * (render/nodeglow/nodeglow.vert:55-123, render/regions/regions.vert:20-42);
+ * the impostor shader draws each body with its radius and star colour
+ * (render/impostor/impostor.vert:29-33, common/common.glsl:228-276).
+ *
+ * bh_render_nodes: the internal nodes of the last tree with a non-empty box,
+ * mass > 0 and depth_min <= nodeDepth <= depth_max, in node order, 48 bytes
+ * each; colours are those of the last tree built with BH_FLAG_COLORS (else
+ * 0). bh_render_bodies: the alive bodies of BodiesIn (mass != 0) in id
+ * order, 20 bytes each: position, radius bodyLengthConstant (m / rho)^(1/3),
+ * star colour. Colours are packed R | G << 8 | B << 16 | 255 << 24 with each
+ * channel round(clamp(c, 0, 1) 255). *count receives the number of records
+ * that pass; at most cap are written. Wait for the stream. */
+typedef struct {
+    float com[3], mass;
+    float aabb_min[3];
+    uint32_t depth;   /* nodeDepth: the node's height */
+    float aabb_max[3];
+    uint32_t rgba;
+} bh_render_node;
+typedef struct {
+    float pos[3], radius;
+    uint32_t rgba;
+} bh_render_body;
+int bh_render_nodes(bh_context *ctx, uint32_t depth_min, uint32_t depth_max, bh_render_node *out, uint32_t cap,
+                    uint32_t *count);
+int bh_render_bodies(bh_context *ctx, bh_render_body *out, uint32_t cap, uint32_t *count);
+
 /* FP64 direct summation over the bodies of BodiesIn for n_sample sampled
  * body ids: accel_out[3k..3k+2] = sum_{j != i, m_j != 0} G m_j r_ij
  * (|r_ij|^2 + softening)^-3/2 with i = sample[k], in double precision, G and

@@ -105,7 +105,14 @@ def test_gpu_stage_parity(name, s):
     np.testing.assert_array_equal(leaf[:, 10:16], ref_l[:, 10:16])
     assert_bits(internal[:, 0:10].view(np.float32), ref_i[:, 0:10].view(np.float32), "internal fill")
     assert_bits(leaf[:, 0:10].view(np.float32), ref_l[:, 0:10].view(np.float32), "leaf fill")
-    np.testing.assert_allclose(internal[:, 16:20].view(np.float32), ref_i[:, 16:20].view(np.float32), atol=1e-3)
+    # star colours (common.glsl:235-276: pow and log of the HIP math library
+    # against the reference's) and their mass-weighted averages
+    for what, got_c, ref_c in (("leaf", leaf, ref_l), ("internal", internal, ref_i)):
+        d = np.abs(got_c[:, 16:20].view(np.float32) - ref_c[:, 16:20].view(np.float32))
+        exact = float((got_c[:, 16:20] == ref_c[:, 16:20]).all(axis=1).mean()) if d.size else 1.0
+        print(f"[gpu] {what} colours: max difference {float(d.max()) if d.size else 0.0:.3g}, "
+              f"exact fraction {exact:.4f}", flush=True)
+        assert_bits(got_c[:, 16:20].view(np.float32), ref_c[:, 16:20].view(np.float32), f"{what} colours")
     if capped:   # (the reference's root keeps the build's nodeDepth; the counter has the height)
         assert bh.counters()["tree_height"] > 64 and ref_i[0, 3] == 0
     else:
