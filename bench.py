@@ -393,6 +393,50 @@ def ensemble_leg(n_sims: int, steps: int, soft: float, theta: float, dt: float, 
     return out
 
 
+def render_feed_leg(bh, prm) -> dict:
+    """Per-frame host feed for a renderer on the headline state: one step
+    with colours, then the compacted feed (internal nodes of depth 1 .. 8,
+    the node-glow default nodeGlowMaxDepth, and the alive bodies) against
+    downloading the std430 InternalNodes and LeafNodes bindings and the
+    bodies; best of 3 wall times, host buffers pageable."""
+    from gravity_amd import barnes_hut as B
+    p = B.Params.from_buffer_copy(prm)
+    p.flags |= B.FLAG_COLORS
+    bh.step(p)
+    bh.sync()
+
+    def best(fn):
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = fn()
+            t.append(time.perf_counter() - t0)
+        return min(t), r
+
+    t_full, full = best(lambda: (bh.nodes(B.INTERNAL_NODES), bh.nodes(B.LEAF_NODES), bh.bodies(B.BODIES_IN)))
+    full_bytes = sum(a.nbytes for a in full)
+    del full
+    t_nodes, nodes = best(lambda: bh.render_nodes(1, 8))
+    t_bodies, bodies = best(lambda: bh.render_bodies())
+    feed_bytes = nodes.nbytes + bodies.nbytes
+    pn = B.PinnedBuffer((bh.n,), B.RENDER_NODE)
+    pb = B.PinnedBuffer((bh.n,), B.RENDER_BODY)
+    t_pn, _ = best(lambda: bh.render_nodes(1, 8, out=pn.array))
+    t_pb, _ = best(lambda: bh.render_bodies(out=pb.array))
+    pn.close()
+    pb.close()
+    out = {"std430_ms": t_full * 1e3, "std430_bytes": int(full_bytes),
+           "feed_ms": (t_nodes + t_bodies) * 1e3, "feed_bytes": int(feed_bytes),
+           "feed_pinned_ms": (t_pn + t_pb) * 1e3,
+           "feed_nodes": int(nodes.shape[0]), "feed_bodies": int(bodies.shape[0]),
+           "feed_nodes_ms": t_nodes * 1e3, "feed_bodies_ms": t_bodies * 1e3,
+           "depth_range": [1, 8], "host_memory": "pageable (numpy); feed_pinned_ms: page-locked (bh_host_alloc)"}
+    log(f"render feed: {out['feed_ms']:.1f} ms ({out['feed_pinned_ms']:.1f} ms into page-locked memory) for "
+        f"{feed_bytes / 1e9:.2f} GB ({nodes.shape[0]} nodes, {bodies.shape[0]} bodies) vs std430 bindings "
+        f"{out['std430_ms']:.1f} ms for {full_bytes / 1e9:.2f} GB")
+    return out
+
+
 def cpu_baseline(bodies: np.ndarray, bounds: np.ndarray, theta: float, dt: float, softening: float,
                  threads: int, walk_sample: int) -> dict:
     """The oracle's C port of one step on this host: the O(N) stages at full
@@ -595,6 +639,7 @@ def main() -> int:
         **{k: {"ms": sm[k], "alg_bytes_per_body": b, "alg_GBps": b * n / (sm[k] * 1e-3) / 1e9,
                "frac_of_peak": b * n / (sm[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
            for k, b in alg.items() if sm.get(k, 0.0) > 0.0}}
+    result["render_feed"] = render_feed_leg(bh, plain_ref)
     Lr.attach(bh, None)
     bh.close()
     if not args.no_configs:

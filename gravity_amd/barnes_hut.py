@@ -96,6 +96,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.bh_selftest_math.argtypes = [ctx, P]
     L.bh_render_nodes.argtypes = [ctx, C.c_uint32, C.c_uint32, P, C.c_uint32, C.POINTER(C.c_uint32)]
     L.bh_render_bodies.argtypes = [ctx, P, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.bh_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    L.bh_host_free.argtypes = [P]
+    L.bh_host_free.restype = None
     _lib = L
     return L
 
@@ -104,6 +107,33 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 RENDER_NODE = np.dtype([("com", "<f4", 3), ("mass", "<f4"), ("aabb_min", "<f4", 3), ("depth", "<u4"),
                         ("aabb_max", "<f4", 3), ("rgba", "<u4")])
 RENDER_BODY = np.dtype([("pos", "<f4", 3), ("radius", "<f4"), ("rgba", "<u4")])
+
+
+class PinnedBuffer:
+    """Page-locked host memory from bh_host_alloc, viewed as a numpy array
+    (`array`); freed by close() or when collected."""
+
+    def __init__(self, shape, dtype):
+        self.L = load_library()
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        self.ptr = C.c_void_p()
+        if self.L.bh_host_alloc(nbytes, C.byref(self.ptr)) != 0:
+            raise MemoryError(f"bh_host_alloc({nbytes})")
+        buf = (C.c_char * max(nbytes, 1)).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.array = None
+            self.L.bh_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_params(theta=0.5, dt=1e-3, softening=0.01, elasticity=0.5, restitution=0.0, merging=NEITHER,
@@ -197,20 +227,29 @@ class BarnesHut:
         G T xx xy xz yy yz zz, -1.5 G tr T, 0."""
         return self.download(NODE_QUADRUPOLES, np.float32).reshape(self.n - 1, 8)
 
-    def render_nodes(self, depth_min: int = 0, depth_max: int = 0xFFFFFFFF, cap: int | None = None) -> np.ndarray:
-        """The render feed's internal nodes (RENDER_NODE records) of the last tree."""
-        cap = self.n if cap is None else cap
-        out = np.zeros(max(cap, 1), RENDER_NODE)
+    def render_nodes(self, depth_min: int = 0, depth_max: int = 0xFFFFFFFF, cap: int | None = None,
+                     out: np.ndarray | None = None) -> np.ndarray:
+        """The render feed's internal nodes (RENDER_NODE records) of the last
+        tree, into `out` (RENDER_NODE array, e.g. a PinnedBuffer's) if given."""
+        if out is None:
+            cap = self.n if cap is None else cap
+            out = np.zeros(max(cap, 1), RENDER_NODE)
+        else:
+            cap = out.shape[0] if cap is None else min(cap, out.shape[0])
         count = C.c_uint32()
         self._check(self.L.bh_render_nodes(self.ctx, depth_min, depth_max, out.ctypes.data_as(C.c_void_p), cap,
                                            C.byref(count)), "render_nodes")
         self.render_total = int(count.value)
         return out[: min(cap, count.value)]
 
-    def render_bodies(self, cap: int | None = None) -> np.ndarray:
-        """The render feed's alive bodies (RENDER_BODY records) of BodiesIn."""
-        cap = self.n if cap is None else cap
-        out = np.zeros(max(cap, 1), RENDER_BODY)
+    def render_bodies(self, cap: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
+        """The render feed's alive bodies (RENDER_BODY records) of BodiesIn,
+        into `out` (RENDER_BODY array) if given."""
+        if out is None:
+            cap = self.n if cap is None else cap
+            out = np.zeros(max(cap, 1), RENDER_BODY)
+        else:
+            cap = out.shape[0] if cap is None else min(cap, out.shape[0])
         count = C.c_uint32()
         self._check(self.L.bh_render_bodies(self.ctx, out.ctypes.data_as(C.c_void_p), cap, C.byref(count)),
                     "render_bodies")

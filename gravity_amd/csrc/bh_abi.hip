@@ -556,6 +556,20 @@ int feed_copy(bh_context* c, R* out, uint32_t cap, uint32_t* count, const char* 
 }  // namespace
 }  // extern "C++"
 
+int bh_host_alloc(size_t bytes, void** out) {
+    if (!out) return BH_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return BH_ERR_OUT_OF_MEMORY;
+    }
+    return BH_OK;
+}
+
+void bh_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 int bh_render_nodes(bh_context* c, uint32_t depth_min, uint32_t depth_max, bh_render_node* out, uint32_t cap,
                     uint32_t* count) {
     if (!c || !count || (!out && cap)) return BH_ERR_INVALID_ARGUMENT;

@@ -247,6 +247,11 @@ typedef struct {
 int bh_render_nodes(bh_context *ctx, uint32_t depth_min, uint32_t depth_max, bh_render_node *out, uint32_t cap,
                     uint32_t *count);
 int bh_render_bodies(bh_context *ctx, bh_render_body *out, uint32_t cap, uint32_t *count);
+/* Page-locked host memory (hipHostMalloc; no reference counterpart): copies
+ * into it, a per-frame render feed or a binding download, run at the link's
+ * rate instead of through the runtime's staging of pageable memory. */
+int bh_host_alloc(size_t bytes, void **out);
+void bh_host_free(void *p);
 
 /* FP64 direct summation over the bodies of BodiesIn for n_sample sampled
  * body ids: accel_out[3k..3k+2] = sum_{j != i, m_j != 0} G m_j r_ij

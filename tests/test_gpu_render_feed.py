@@ -106,3 +106,22 @@ def test_render_feed_without_colours_has_none():
     got = bh.render_nodes()
     assert got.shape[0] == bh.render_total > 0 and not got["rgba"].any()
     bh.close()
+
+
+def test_render_feed_into_page_locked_memory():
+    from gravity_amd import barnes_hut as B
+    from gravity_amd import initial_conditions as ic
+    print("[gpu] render feed into page-locked memory", flush=True)
+    b = ic.plummer(20000, 5)
+    bh = _stepped(b, ic.tight_bounds(b), steps=1)
+    want_n, want_b = bh.render_nodes(1, 8), bh.render_bodies()
+    pn = B.PinnedBuffer((bh.n,), B.RENDER_NODE)
+    pb = B.PinnedBuffer((bh.n,), B.RENDER_BODY)
+    got_n = bh.render_nodes(1, 8, out=pn.array)
+    got_b = bh.render_bodies(out=pb.array)
+    np.testing.assert_array_equal(got_n.view(np.uint8), want_n.view(np.uint8))
+    np.testing.assert_array_equal(got_b.view(np.uint8), want_b.view(np.uint8))
+    del got_n, got_b
+    pn.close()
+    pb.close()
+    bh.close()
