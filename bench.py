@@ -558,7 +558,7 @@ def main() -> int:
     log(f"plain theta 0.5: {ref['ms_per_step']:.2f} ms/step, {ref['body_steps_per_s']:.4g} body-steps/s")
 
     # 2. the network
-    mlp, train_info = train_network(soft, 0.6, 1.2e-3, args.seed)
+    mlp, train_info = train_network(soft, 0.6, 1.8e-3, args.seed)
     Lr.attach(bh, mlp)
 
     # 3. sweeps on one frozen state against one FP64 judge
@@ -568,7 +568,7 @@ def main() -> int:
     plain_rows = sweep(bh, saved, sample, exact, soft, "plain", (0.5, 0.3, 0.2, 0.15, 0.12, 0.1, 0.08, 0.06), 0)
     geo_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", (0.8, 0.7, 0.6, 0.5, 0.45, 0.4, 0.35),
                      B.FLAG_QUADRUPOLE)
-    tolerances = (0.004, 0.003, 0.002, 0.0015, 0.0012, 0.001, 0.0008, 0.0006)
+    tolerances = (0.004, 0.003, 0.0025, 0.002, 0.0018, 0.0016, 0.0015, 0.0012, 0.001, 0.0008, 0.0006)
     quad_rows = sweep(bh, saved, sample, exact, soft, "quadrupole", tolerances, B.FLAG_QUADRUPOLE, relative=True)
     learned_rows = sweep(bh, saved, sample, exact, soft, "learned", tolerances, B.FLAG_LEARNED, relative=True)
     plain_best = fastest_at_target(plain_rows)
