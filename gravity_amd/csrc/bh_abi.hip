@@ -59,9 +59,11 @@ struct bh_context {
     uint32_t* part_scratch = nullptr;         // per-tile alive counts / offsets
     float* bounds_partials = nullptr;
     uint32_t bounds_partial_cap = 0;
-    uint32_t* tile_hist = nullptr;
-    uint32_t tiles_cap = 0;
-    uint32_t* digit_total = nullptr;
+    uint32_t* sort_partials = nullptr;        // [sort_digit_blocks(n)][8][256] digit counts
+    uint32_t tiles_cap = 0;                   // sort tiles of n keys
+    uint32_t* digit_base = nullptr;           // [8][256]
+    unsigned long long* sort_status = nullptr;   // [8][tiles_cap][256] look-back words
+    SortState* sort_state = nullptr;
     unsigned long long* reserve = nullptr;
     uint32_t* task_done = nullptr;
     NodeStd430* staging_nodes = nullptr;      // boundary conversion of bindings 0 and 1 (and the render feed)
@@ -348,8 +350,10 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
         c->tiles_cap = (n + kSortTileOne - 1) / kSortTileOne;
-        TRY(dalloc(c, &c->tile_hist, 256 * (size_t)c->tiles_cap));
-        TRY(dalloc(c, &c->digit_total, 256));
+        TRY(dalloc(c, &c->sort_partials, (size_t)sort_digit_blocks(n) * kSortPasses * 256));
+        TRY(dalloc(c, &c->digit_base, kSortPasses * 256));
+        TRY(dalloc(c, &c->sort_status, (size_t)kSortPasses * c->tiles_cap * 256));
+        TRY(dalloc(c, &c->sort_state, 1));
         TRY(dalloc(c, &c->reserve, (size_t)n));
         TRY(dalloc(c, &c->task_done, (size_t)n));
         TRY(dalloc(c, &c->staging_nodes, (size_t)n));
@@ -364,6 +368,9 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
             (void)hipMemsetAsync(c->index[k], 0, 4 * (size_t)n, c->stream);
         }
         (void)hipMemsetAsync(c->ctr, 0, sizeof(DevCounters), c->stream);
+        // epoch 0 marks look-back words no sort has written
+        (void)hipMemsetAsync(c->sort_status, 0, 8 * (size_t)kSortPasses * c->tiles_cap * 256, c->stream);
+        (void)hipMemsetAsync(c->sort_state, 0, sizeof(SortState), c->stream);
         (void)hipMemsetAsync(T.meet, 0xFF, 4 * ((size_t)n - 1), c->stream);
         (void)hipMemsetAsync(c->reserve, 0xFF, 8 * (size_t)n, c->stream);
         (void)hipMemsetAsync(T.leaf_color, 0, 16 * (size_t)n, c->stream);
@@ -404,7 +411,7 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->tile_hist, c->digit_total,
+                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_base, c->sort_status, c->sort_state,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
                     c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles, c->acc_mag,
                     c->feed_counts};
@@ -708,20 +715,10 @@ namespace {
 // 0..num-1 instead, into sort_pos[1], IndexIn untouched).
 static_assert(kSortPasses % 2 == 0, "the sort ends in the In buffers");
 int radix_sort(bh_context* c, bool positions) {
-    for (uint32_t pass = 0; pass < kSortPasses; pass++) {
-        if (positions)
-            launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], pass ? c->sort_pos[c->mi_in] : nullptr,
-                             c->morton[1 - c->mi_in], c->sort_pos[1 - c->mi_in], pass * kSortBits, c->tile_hist,
-                             c->tiles_cap, c->digit_total);
-        else
-            launch_sort_pass(c->stream, c->n, c->sim, c->morton[c->mi_in], c->index[c->mi_in],
-                             c->morton[1 - c->mi_in], c->index[1 - c->mi_in], pass * kSortBits, c->tile_hist,
-                             c->tiles_cap, c->digit_total);
-        int rc = check_launch(c, "sort pass");
-        if (rc) return rc;
-        c->mi_in = 1 - c->mi_in;
-    }
-    return BH_OK;
+    uint32_t* vals[2] = {positions ? c->sort_pos[0] : c->index[0], positions ? c->sort_pos[1] : c->index[1]};
+    launch_sort(c->stream, c->n, c->sim, c->ctr, c->morton, vals, c->mi_in, positions, c->sort_partials,
+                c->digit_base, c->sort_status, c->tiles_cap, c->sort_state);
+    return check_launch(c, "sort");
 }
 
 // positions: the sort before sorted positions (radix_sort(c, true)); the

@@ -7,7 +7,7 @@
 //                          [COMPUTE_DEAD_*, dead/bh_dead.comp:13-112 + UPDATE flag=0]
 //   k_bounds               [COMPUTE_MORTON_AABB_REPOPULATE + COLLAPSE, morton/bh_morton.comp:50-231]
 //   k_encode               [COMPUTE_MORTON_ENCODE, morton/bh_morton.comp:6-34,238-262]
-//   k_sort_hist/scan/scatter
+//   k_sort_digits/bases/onesweep
 //                          [COMPUTE_RADIX_*, radix/bh_radix.comp:12-166]
 //   k_tree/k_tree_upper    [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
 //   (walk: bh_walk.hip)    [COMPUTE_FORCE_COMPUTE]
@@ -397,8 +397,13 @@ __global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ si
 
 // ------------------------------------------------------------------ radix sort
 // Stable LSD sort of (key, index) pairs over positions [0, numBodies), eight
-// passes of eight bits. Per pass: tile histograms, per-digit scan over tiles,
-// stable scatter with in-wave ballot ranking.
+// passes of eight bits. One simulation: one histogram launch reads every key
+// once and counts all eight digits, a one-block-per-pass scan turns them into
+// digit bases, then each pass is one launch: tiles take their ids in launch
+// order, rank their keys in-wave by ballots, publish their digit counts and
+// find the counts of all earlier tiles by a decoupled look-back, then write
+// their keys out in coalesced digit runs. The ensembles keep tile histograms
+// and a scan per simulation (k_ens_sort_*).
 // Digit histogram of one tile of up to kSortTile keys, per wave without
 // atomics: keys of a tile share their high digits (the input is nearly in
 // order), so per-key LDS atomics would serialize on one bin. Each row of 64
@@ -437,49 +442,6 @@ __device__ __forceinline__ uint32_t tile_histogram(uint32_t (&s_h)[W][256], cons
     return c;
 }
 
-__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_hist(const SimValues* __restrict__ sim,
-                                                                   const uint64_t* __restrict__ keys, uint32_t shift,
-                                                                   uint32_t* __restrict__ tile_hist,
-                                                                   uint32_t tiles_cap) {
-    __shared__ uint32_t s_h[kSortWavesOne][256];
-    const uint32_t num = sim->numBodies;
-    const uint32_t base = blockIdx.x * kSortTileOne;
-    if (base >= num) return;
-    const uint32_t count = num - base < kSortTileOne ? num - base : kSortTileOne;
-    const uint32_t c = tile_histogram<8, kSortTileOne, kSortWavesOne>(s_h, keys, base, count, shift);
-    if (threadIdx.x < 256) tile_hist[threadIdx.x * tiles_cap + blockIdx.x] = c;
-}
-
-// one block per digit: exclusive scan of that digit's tile counts
-__global__ __launch_bounds__(256) void k_sort_scan(const SimValues* __restrict__ sim, uint32_t* __restrict__ tile_hist,
-                                                   uint32_t tiles_cap, uint32_t* __restrict__ digit_total) {
-    __shared__ uint32_t s_w[4];
-    const uint32_t num = sim->numBodies;
-    const uint32_t T = (num + kSortTileOne - 1) / kSortTileOne;
-    uint32_t* row = tile_hist + (size_t)blockIdx.x * tiles_cap;
-    const uint32_t per = (T + 255) / 256;
-    const uint32_t t = threadIdx.x;
-    uint32_t start = t * per, end = min(start + per, T);
-    uint32_t sum = 0;
-    for (uint32_t k = start; k < end; k++) sum += row[k];
-    uint32_t incl = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t v = __shfl_up(incl, off, 64);
-        if (lane_id() >= (uint32_t)off) incl += v;
-    }
-    if (lane_id() == 63) s_w[t >> 6] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t k = 0; k < (t >> 6); k++) before += s_w[k];
-    uint32_t run = before + incl - sum;
-    for (uint32_t k = start; k < end; k++) {
-        uint32_t v = row[k];
-        row[k] = run;
-        run += v;
-    }
-    if (t == 255) digit_total[blockIdx.x] = before + incl;
-}
-
 // Stable scatter of one tile of up to TILE keys. Wave w owns the tile's
 // elements [w TILE/4, (w + 1) TILE/4) and ranks them by digit on its own (in-wave
 // ballot matching, a per-wave running count per digit in LDS, no block
@@ -504,12 +466,22 @@ struct ScatterLds {
 constexpr uint32_t kPosScramble = 0x9E3779B1u, kPosUnscramble = 0x0E8B2F51u;
 static_assert(kPosScramble * kPosUnscramble == 1u, "inverse mod 2^32");
 
+// The tile's digit counts before they turn into global positions: a scatter
+// whose bases are known up front (the ensembles) passes NoLookback; the
+// one-simulation sort passes a TileLookback, which publishes the counts and
+// adds those of all earlier tiles.
+struct NoLookback {
+    static constexpr bool kActive = false;
+    __device__ void publish(uint32_t, uint32_t) const {}
+    __device__ uint32_t resolve(uint32_t, uint32_t) const { return 0; }
+};
+
 // idx_in nullptr: the values start as the scrambled positions start + e.
-template <int BITS, int TILE, int W = 4>
+template <int BITS, int TILE, int W = 4, class Lookback = NoLookback>
 __device__ __forceinline__ void scatter_tile(ScatterLds<TILE, W>& L, const uint64_t* __restrict__ keys_in,
                                              const uint32_t* __restrict__ idx_in, uint32_t start, uint32_t count,
                                              uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
-                                             uint32_t shift) {
+                                             uint32_t shift, const Lookback& lb = Lookback()) {
     constexpr uint32_t R = 1u << BITS;
     constexpr int kPer = TILE / W;         // elements per wave
     constexpr int kRows = kPer / 64;       // rows of 64 per wave
@@ -539,13 +511,15 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE, W>& L, const uint6
         if (valid && in_row == 0) L.cnt[w][d] = before + __popcll(peers);
     }
     __syncthreads();
+    uint32_t tot = 0;   // thread t < 256: the tile's count of digit t
     {   // thread t < 256 = digit t: per-wave offsets and the tile's digit starts
-        uint32_t c[W], tot = 0, incl = 0;
+        uint32_t c[W], incl = 0;
         if (t < 256) {
             for (int k = 0; k < W; k++) {
                 c[k] = L.cnt[k][t];
                 tot += c[k];
             }
+            lb.publish(t, tot);
             incl = tot;
             for (int off = 1; off < 64; off <<= 1) {
                 const uint32_t v = __shfl_up(incl, off, 64);
@@ -575,6 +549,8 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE, W>& L, const uint6
             L.idx[pos] = idx[r];
         }
     }
+    // the earlier tiles' counts arrive while the tile sits reordered in LDS
+    if (Lookback::kActive && t < 256) L.base[t] += lb.resolve(t, tot);
     __syncthreads();
     for (uint32_t j = t; j < count; j += W * 64) {
         const uint64_t k = L.key[j];
@@ -585,38 +561,164 @@ __device__ __forceinline__ void scatter_tile(ScatterLds<TILE, W>& L, const uint6
     }
 }
 
-__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_scatter(const SimValues* __restrict__ sim,
-                                                      const uint64_t* __restrict__ keys_in,
-                                                      const uint32_t* __restrict__ idx_in,
-                                                      uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
-                                                      uint32_t shift, const uint32_t* __restrict__ tile_hist,
-                                                      uint32_t tiles_cap, const uint32_t* __restrict__ digit_total) {
-    __shared__ ScatterLds<kSortTileOne, kSortWavesOne> L;
-    __shared__ uint32_t s_w[4];
+// ---- one simulation: all digit counts in one read, one launch per pass.
+// Counts of all eight digits of the keys at [0, numBodies): each block counts
+// a contiguous chunk in LDS (a row of 64 keys whose digit is the same in
+// every lane, the common case for the high digits of nearly ordered keys,
+// adds with one atomic; other rows with one per key) and writes its 8 x 256
+// counts; block 0 also starts a new sort: the next epoch, the passes' tile
+// ids back to zero.
+__global__ __launch_bounds__(512) void k_sort_digits(const SimValues* __restrict__ sim,
+                                                     const uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ partials, SortState* __restrict__ st) {
+    __shared__ uint32_t h[kSortPasses * 256];
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    for (uint32_t i = t; i < kSortPasses * 256; i += 512) h[i] = 0;
+    __syncthreads();
     const uint32_t num = sim->numBodies;
-    const uint32_t base = blockIdx.x * kSortTileOne;
-    if (base >= num) return;
-    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
-    // digit bases: exclusive scan of the 256 digit totals (threads 0..255)
-    uint32_t tot = 0, incl = 0;
-    if (t < 256) {
-        tot = digit_total[t];
-        incl = tot;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t v = __shfl_up(incl, off, 64);
-            if (lane >= (uint32_t)off) incl += v;
+    const uint32_t chunk = ((num + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
+    const uint32_t begin = min(blockIdx.x * chunk, num), end = min(begin + chunk, num);
+    for (uint32_t r0 = begin + w * 64; r0 < end; r0 += 512) {
+        const uint32_t e = r0 + lane;
+        const bool valid = e < end;                  // a prefix of the row's lanes
+        const uint64_t key = valid ? keys[e] : 0ull;
+        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+        for (uint32_t p = 0; p < kSortPasses; p++) {
+            const uint32_t d = (uint32_t)(key >> (p * kSortBits)) & 255u;
+            const uint32_t d0 = uniform(d);          // lane 0 is valid
+            if (ballot(valid && d != d0) == 0ull) {
+                if (lane == 0) atomicAdd(&h[p * 256 + d0], nvalid);
+            } else if (valid) {
+                atomicAdd(&h[p * 256 + d], 1u);
+            }
         }
-        if (lane == 63) s_w[w] = incl;
     }
+    __syncthreads();
+    for (uint32_t i = t; i < kSortPasses * 256; i += 512) partials[(size_t)blockIdx.x * kSortPasses * 256 + i] = h[i];
+    if (blockIdx.x == 0 && t == 0) {
+        const uint32_t e = st->epoch + 1u;
+        st->epoch = e ? e : 1u;                      // 0 marks a status word never written
+        for (uint32_t p = 0; p < kSortPasses; p++) st->tile[p] = 0;
+    }
+}
+
+// One block per pass: each digit's total over the partial counts (four
+// threads per digit, each over every fourth count set, 8 loads in flight),
+// then the exclusive scan over the 256 digits: where each digit's run starts.
+__global__ __launch_bounds__(1024) void k_sort_bases(const uint32_t* __restrict__ partials, uint32_t nparts,
+                                                     uint32_t* __restrict__ digit_base) {
+    __shared__ uint32_t s_q[4][256];
+    __shared__ uint32_t s_w[4];
+    const uint32_t p = blockIdx.x, t = threadIdx.x, d = t & 255u, q = t >> 8, lane = lane_id();
+    constexpr uint32_t kU = 8;
+    uint32_t part = 0;
+    for (uint32_t b0 = q; b0 < nparts; b0 += 4 * kU) {
+        uint32_t v[kU];
+        for (uint32_t j = 0; j < kU; j++)
+            v[j] = b0 + 4 * j < nparts ? partials[(size_t)(b0 + 4 * j) * kSortPasses * 256 + p * 256 + d] : 0u;
+        for (uint32_t j = 0; j < kU; j++) part += v[j];
+    }
+    s_q[q][d] = part;
+    __syncthreads();
+    const uint32_t tot = t < 256 ? s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d] : 0u;
+    uint32_t incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += v;
+    }
+    if (t < 256 && lane == 63) s_w[t >> 6] = incl;
     __syncthreads();
     if (t < 256) {
         uint32_t before = 0;
-        for (uint32_t k = 0; k < w; k++) before += s_w[k];
-        L.base[t] = before + incl - tot + tile_hist[t * tiles_cap + blockIdx.x];
+        for (uint32_t k = 0; k < (t >> 6); k++) before += s_w[k];
+        digit_base[p * 256 + d] = before + incl - tot;
     }
-    scatter_tile<8, kSortTileOne, kSortWavesOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out,
-                                                 idx_out, shift);
 }
+
+// Decoupled look-back over the tiles of one pass. Status word of (tile,
+// digit): epoch << 32 | inclusive << 31 | count, where count is the tile's
+// own count of the digit (an aggregate) or, with the inclusive bit, the
+// count over this and every earlier tile. A word of another epoch is a
+// previous sort's: not yet written. Tile ids follow the order in which the
+// blocks started, so every tile a block waits for is running or done; the
+// wait is bounded all the same (kLookbackSpins polls, then error bit 3).
+constexpr uint32_t kLookbackInclusive = 0x80000000u;
+constexpr uint32_t kLookbackSpins = 1u << 20;
+// earlier tiles' words read at once: 4 gave 1.33 ms for a 16M sort of 4,096-key
+// tiles, 1 gave 1.42 ms and 8 1.36 ms
+constexpr int kLookbackWindow = 4;
+struct TileLookback {
+    static constexpr bool kActive = true;
+    unsigned long long* status;   // [tiles][256] of this pass
+    uint32_t tile, epoch;
+    DevCounters* ctr;
+    __device__ void publish(uint32_t d, uint32_t cnt) const {
+        const unsigned long long v = ((unsigned long long)epoch << 32) | (tile ? 0u : kLookbackInclusive) | cnt;
+        __hip_atomic_store(&status[(size_t)tile * 256 + d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the words of kLookbackWindow earlier tiles are read at once; counts are
+    // taken in order up to the first inclusive word or the first word not yet
+    // written, which is read again (after a short sleep) with the next window
+    __device__ uint32_t resolve(uint32_t d, uint32_t cnt) const {
+        if (tile == 0) return 0;
+        uint32_t excl = 0, spins = 0;
+        int j = (int)tile - 1;
+        for (;;) {
+            unsigned long long s[kLookbackWindow];
+            for (int q = 0; q < kLookbackWindow; q++)
+                s[q] = j - q >= 0 ? __hip_atomic_load(&status[(size_t)(j - q) * 256 + d], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0ull;
+            int q = 0;
+            bool done = false;
+            for (; q < kLookbackWindow; q++) {
+                if ((uint32_t)(s[q] >> 32) != epoch) break;
+                excl += (uint32_t)s[q] & ~kLookbackInclusive;
+                if (((uint32_t)s[q] & kLookbackInclusive) || j - q == 0) {
+                    done = true;
+                    break;
+                }
+            }
+            if (done) break;
+            j -= q;
+            if (q < kLookbackWindow) {
+                if (++spins > kLookbackSpins) {
+                    atomicOr(&ctr->errorFlags, 8u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        const unsigned long long v = ((unsigned long long)epoch << 32) | kLookbackInclusive | (excl + cnt);
+        __hip_atomic_store(&status[(size_t)tile * 256 + d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return excl;
+    }
+};
+
+__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimValues* __restrict__ sim,
+                                                                      DevCounters* __restrict__ ctr,
+                                                                      const uint64_t* __restrict__ keys_in,
+                                                                      const uint32_t* __restrict__ idx_in,
+                                                                      uint64_t* __restrict__ keys_out,
+                                                                      uint32_t* __restrict__ idx_out, uint32_t pass,
+                                                                      const uint32_t* __restrict__ digit_base,
+                                                                      unsigned long long* __restrict__ status,
+                                                                      SortState* __restrict__ st) {
+    __shared__ ScatterLds<kSortTileOne, kSortWavesOne> L;
+    __shared__ uint32_t s_tile;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) s_tile = atomicAdd(&st->tile[pass], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t num = sim->numBodies;
+    const uint32_t base = tile * kSortTileOne;
+    if (base >= num) return;                          // ids are dense: the tiles below num are all taken
+    if (t < 256) L.base[t] = digit_base[pass * 256 + t];
+    const TileLookback lb = {status, tile, st->epoch, ctr};
+    scatter_tile<8, kSortTileOne, kSortWavesOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out,
+                                                 idx_out, pass * kSortBits, lb);
+}
+
 
 // ------------------------------------------------------------------ tree (build + fill)
 // star colour, common/common.glsl:235-276
@@ -1607,14 +1709,22 @@ void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCou
                    const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in) {
     k_encode<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, step_pos, morton_in);
 }
-void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
-                      const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
-                      uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total) {
-    uint32_t nt = cdiv(n, kSortTileOne);
-    k_sort_hist<<<nt, kSortWavesOne * 64, 0, s>>>(sim, keys_in, shift, tile_hist, tiles_cap);
-    k_sort_scan<<<256, 256, 0, s>>>(sim, tile_hist, tiles_cap, digit_total);
-    k_sort_scatter<<<nt, kSortWavesOne * 64, 0, s>>>(sim, keys_in, idx_in, keys_out, idx_out, shift, tile_hist,
-                                                     tiles_cap, digit_total);
+uint32_t sort_digit_blocks(uint32_t n) {
+    const uint32_t b = cdiv(n, 16384);
+    return b < 1 ? 1 : (b > 256 ? 256 : b);
+}
+void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, uint64_t* keys[2],
+                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* digit_base,
+                 unsigned long long* status, uint32_t tiles_cap, SortState* st) {
+    const uint32_t nb = sort_digit_blocks(n), nt = cdiv(n, kSortTileOne);
+    k_sort_digits<<<nb, 512, 0, s>>>(sim, keys[in], partials, st);
+    k_sort_bases<<<kSortPasses, 1024, 0, s>>>(partials, nb, digit_base);
+    for (uint32_t pass = 0; pass < kSortPasses; pass++) {
+        const int i = (in + (int)pass) & 1;
+        k_sort_onesweep<<<nt, kSortWavesOne * 64, 0, s>>>(sim, ctr, keys[i], (positions && pass == 0) ? nullptr : vals[i],
+                                                          keys[1 - i], vals[1 - i], pass, digit_base,
+                                                          status + (size_t)pass * tiles_cap * 256, st);
+    }
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,

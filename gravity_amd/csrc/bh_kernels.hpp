@@ -12,8 +12,13 @@ namespace bh {
 
 constexpr uint32_t kPartTile = 1024;   // dead partition tile (sorted positions per block)
 constexpr uint32_t kSortTile = 2048;   // radix sort tile of the ensembles (keys per block)
-constexpr uint32_t kSortTileOne = 4096; // radix sort tile of one simulation
-constexpr int kSortWavesOne = 8;        // waves per block of its histogram and scatter kernels
+// radix sort tile of one simulation: 8,192 keys per block of 16 waves (8 rows
+// of 64 keys each), one block per CU (114 KB of LDS). Larger tiles shorten
+// the look-back and lengthen the digit runs written out: at 16M keys the
+// sort takes 1.24 ms with these tiles, 1.33 ms with 4,096 and 1.78 ms with
+// 2,048 (MI355X, same box)
+constexpr uint32_t kSortTileOne = 8192;
+constexpr int kSortWavesOne = kSortTileOne / 512;
 constexpr uint32_t kSortBits = 8;
 constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
 constexpr int kWalkWaves = 16;         // waves per walk block (adjacent body groups share a CU's scalar cache)
@@ -223,9 +228,19 @@ void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCou
                    const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in);
 // One LSD pass over (key, value) at positions [0, numBodies); idx_in nullptr:
 // the values start as the positions (scrambled: times an odd constant).
-void launch_sort_pass(hipStream_t s, uint32_t n, const SimValues* sim, const uint64_t* keys_in,
-                      const uint32_t* idx_in, uint64_t* keys_out, uint32_t* idx_out, uint32_t shift,
-                      uint32_t* tile_hist, uint32_t tiles_cap, uint32_t* digit_total);
+// launch order of the one-simulation sort's tiles and the epoch that tells
+// this sort's look-back words from an earlier sort's
+struct SortState {
+    uint32_t epoch;
+    uint32_t tile[kSortPasses];
+};
+uint32_t sort_digit_blocks(uint32_t n);   // histogram blocks (partial count sets of 8 x 256)
+// Sorts the keys of keys[in] with the values of vals[in] (positions: the
+// values start as the positions 0..num-1 instead) in kSortPasses passes; the
+// result lands back in keys[in], vals[in].
+void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, uint64_t* keys[2],
+                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* digit_base,
+                 unsigned long long* status, uint32_t tiles_cap, SortState* st);
 // from: when the sort sorted positions instead of ids, the (scrambled)
 // position before the sort of the body at each sorted position; index_in is then in the
 // order before the sort and the tree writes the ids in the new order to

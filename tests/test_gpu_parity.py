@@ -270,10 +270,10 @@ def test_gpu_steps_with_uploads_between_match_oracle():
         bh.close()
 
 
-@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 1023, 1025, 4095, 4097, 8193])
+@pytest.mark.parametrize("n", [2, 3, 63, 64, 65, 1023, 1025, 4095, 4097, 8191, 8193, 16385])
 def test_gpu_step_sizes_match_oracle(n):
     """bh_step at sizes around the wave (64), tree block (1,024) and sort
-    tile (4,096) boundaries equals the oracle's step bit for bit, with the
+    tile (8,192) boundaries equals the oracle's step bit for bit, with the
     positions carried from step to step."""
     from gravity_amd import barnes_hut as B
     from gravity_amd import initial_conditions as ic
