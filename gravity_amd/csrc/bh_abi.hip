@@ -61,7 +61,7 @@ struct bh_context {
     uint32_t bounds_partial_cap = 0;
     uint32_t* sort_partials = nullptr;        // [sort_digit_blocks(n)][8][256] digit counts
     uint32_t tiles_cap = 0;                   // sort tiles of n keys
-    uint32_t* digit_base = nullptr;           // [8][256]
+    uint32_t* digit_totals = nullptr;         // [8][256] keys per digit value of each pass
     unsigned long long* sort_status = nullptr;   // [8][tiles_cap][256] look-back words
     SortState* sort_state = nullptr;
     unsigned long long* reserve = nullptr;
@@ -116,8 +116,11 @@ struct bh_context {
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> graphs;
-    double* mom_prefix = nullptr;             // [mom_prefix_doubles(n)]
-    double* mom_tiles = nullptr;              // [mom_tile_doubles(n)]
+    // the tree's second moments for the corrected walk (TreeArrays' node_mom,
+    // xchg_mom, frontier_mom), set in the tree's arrays only for its builds
+    float4* node_mom = nullptr;
+    float4* xchg_mom = nullptr;
+    float4* frontier_mom = nullptr;
 
     // per-stage HIP events of up to kTimedSteps steps, read without
     // synchronising inside the stepping loop
@@ -351,7 +354,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
         c->tiles_cap = (n + kSortTileOne - 1) / kSortTileOne;
         TRY(dalloc(c, &c->sort_partials, (size_t)sort_digit_blocks(n) * kSortPasses * 256));
-        TRY(dalloc(c, &c->digit_base, kSortPasses * 256));
+        TRY(dalloc(c, &c->digit_totals, kSortPasses * 256));
         TRY(dalloc(c, &c->sort_status, (size_t)kSortPasses * c->tiles_cap * 256));
         TRY(dalloc(c, &c->sort_state, 1));
         TRY(dalloc(c, &c->reserve, (size_t)n));
@@ -411,9 +414,9 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_base, c->sort_status, c->sort_state,
+                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_totals, c->sort_status, c->sort_state,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
-                    c->sort_pos[1], c->walk_queues, c->qrec, c->mom_prefix, c->mom_tiles, c->acc_mag,
+                    c->sort_pos[1], c->walk_queues, c->qrec, c->node_mom, c->xchg_mom, c->frontier_mom, c->acc_mag,
                     c->feed_counts};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -465,12 +468,14 @@ static int check_range(bh_context* c, int b, size_t off, size_t bytes) {
     return BH_OK;
 }
 
-// the corrected walk's arrays, allocated on first use (about 215 bytes per body)
+// the corrected walk's arrays, allocated on first use (about 260 bytes per body)
 static int ensure_quad(bh_context* c) {
     if (c->qrec) return BH_OK;
     int rc;
-    if ((rc = dalloc(c, &c->mom_prefix, mom_prefix_doubles(c->n)))) return rc;
-    if ((rc = dalloc(c, &c->mom_tiles, mom_tile_doubles(c->n)))) return rc;
+    if ((rc = dalloc(c, &c->node_mom, 2 * ((size_t)c->n - 1)))) return rc;
+    if ((rc = dalloc(c, &c->xchg_mom, 6 * ((size_t)c->n - 1)))) return rc;
+    if ((rc = dalloc(c, &c->frontier_mom, 3 * (size_t)frontier_cap(c->n, (c->n + kTreeBlock - 1) / kTreeBlock))))
+        return rc;
     if ((rc = dalloc(c, &c->qrec, 4 * corr_records(c->n)))) return rc;
     if ((rc = dalloc(c, &c->acc_mag, c->n))) return rc;
     // the quadrupole halves of the leaf records stay zero
@@ -717,7 +722,7 @@ static_assert(kSortPasses % 2 == 0, "the sort ends in the In buffers");
 int radix_sort(bh_context* c, bool positions) {
     uint32_t* vals[2] = {positions ? c->sort_pos[0] : c->index[0], positions ? c->sort_pos[1] : c->index[1]};
     launch_sort(c->stream, c->n, c->sim, c->ctr, c->morton, vals, c->mi_in, positions, c->sort_partials,
-                c->digit_base, c->sort_status, c->tiles_cap, c->sort_state);
+                c->digit_totals, c->sort_status, c->tiles_cap, c->sort_state);
     return check_launch(c, "sort");
 }
 
@@ -733,14 +738,20 @@ int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* 
     int rc;
     if (quad && (rc = ensure_quad(c))) return rc;
     c->fill_cap = cap;
+    TreeArrays T = c->tree;
+    if (quad) {   // the tree also builds every node's second moment
+        T.node_mom = c->node_mom;
+        T.xchg_mom = c->xchg_mom;
+        T.frontier_mom = c->frontier_mom;
+    }
     launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
-                c->tree, c->colors ? 1 : 0, positions ? step_pos : nullptr,
+                T, c->colors ? 1 : 0, positions ? step_pos : nullptr,
                 positions ? c->sort_pos[c->mi_in] : nullptr, positions ? c->sort_pos[1 - c->mi_in] : nullptr, cap);
     rc = check_launch(c, "tree");
     if (rc == BH_OK && positions) std::swap(c->index[c->mi_in], c->sort_pos[1 - c->mi_in]);
     c->quad_valid = false;
     if (rc == BH_OK && quad) {
-        launch_node_quadrupoles(c->stream, c->n, c->sim, c->ctr, c->tree, c->mom_prefix, c->mom_tiles,
+        launch_node_quadrupoles(c->stream, c->n, c->sim, c->ctr, T,
                                 learned ? c->correction->w : nullptr, learned ? c->correction->bias : nullptr,
                                 c->qrec);
         rc = check_launch(c, "node quadrupoles");

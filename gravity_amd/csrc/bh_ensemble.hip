@@ -50,8 +50,7 @@ struct bh_ensemble {
     float alpha = 0.0f;
     const bh_mlp* correction = nullptr;
     float4* qrec = nullptr;
-    double* mom_prefix = nullptr;
-    double* mom_tiles = nullptr;
+
     float* acc_mag = nullptr;
     bool accmag_valid = false;   // a corrected walk has run on these bodies
     bh_ensemble_config cfg{8, 4, 192, 1};
@@ -163,8 +162,7 @@ int enqueue_step(bh_ensemble* e) {
         const __bf16* w = nullptr;
         const float* bias = nullptr;
         if (e->learned) mlp_device_weights(e->correction, &w, &bias);
-        launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, e->mom_prefix, e->mom_tiles,
-                                    w, bias, e->qrec);
+        launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, w, bias, e->qrec);
     }
     if ((rc = enqueue_walk(e, (int)e->cfg.walk_waves, (int)e->cfg.stack_depth, e->walk_blocks))) return rc;
     e->body_in = 1 - b;
@@ -245,7 +243,8 @@ void bh_ensemble_destroy(bh_ensemble* e) {
                     e->index[1], e->tree.leaf_pos, e->tree.leaf_parent, e->tree.walk, e->tree.node_aabb,
                     e->tree.node_parent, e->tree.node_range, e->tree.meet, e->tree.xchg, e->tree.frontier,
                     e->tree.frontier_count, e->ctr, e->queues,
-                    e->partials, e->hist, e->qrec, e->mom_prefix, e->mom_tiles, e->acc_mag};
+                    e->partials, e->hist, e->qrec, e->tree.node_mom, e->tree.xchg_mom, e->tree.frontier_mom,
+                    e->acc_mag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -385,8 +384,10 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &e->hist, 256 * (size_t)e->ntiles));
         if (corrected) {
             TRY(ealloc(e, &e->qrec, 4 * corr_records(N)));
-            TRY(ealloc(e, &e->mom_prefix, mom_prefix_doubles(N)));
-            TRY(ealloc(e, &e->mom_tiles, mom_tile_doubles(N)));
+            // the trees also build every node's second moment
+            TRY(ealloc(e, &T.node_mom, 2 * ni));
+            TRY(ealloc(e, &T.xchg_mom, 6 * ni));
+            TRY(ealloc(e, &T.frontier_mom, 3 * (size_t)frontier_cap(N, 2 * e->ntiles)));
             TRY(ealloc(e, &e->acc_mag, (size_t)N));
             // the quadrupole halves of the leaf records stay zero
             r = hipMemsetAsync(e->qrec, 0, 64 * corr_records(N), e->stream);
@@ -623,8 +624,7 @@ int bh_ensemble_autotune(bh_ensemble* e, bh_ensemble_config* chosen, float* timi
                 }
                 mlp_device_weights(e->correction, &w, &bias);
             }
-            launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, e->mom_prefix,
-                                        e->mom_tiles, w, bias, e->qrec);
+            launch_ens_node_quadrupoles(e->stream, e->N, e->N - e->S, e->G, e->ctr, e->tree, w, bias, e->qrec);
         }
         DevCounters d;
         if (hipMemcpyAsync(&d, e->ctr, sizeof d, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||

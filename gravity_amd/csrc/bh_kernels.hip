@@ -398,9 +398,9 @@ __global__ __launch_bounds__(256) void k_encode(const SimValues* __restrict__ si
 // ------------------------------------------------------------------ radix sort
 // Stable LSD sort of (key, index) pairs over positions [0, numBodies), eight
 // passes of eight bits. One simulation: one histogram launch reads every key
-// once and counts all eight digits, a one-block-per-pass scan turns them into
-// digit bases, then each pass is one launch: tiles take their ids in launch
-// order, rank their keys in-wave by ballots, publish their digit counts and
+// once and counts all eight digits, a small launch sums the blocks' counts,
+// then each pass is one launch: tiles take their ids in launch order, scan the
+// pass's digit totals, rank their keys in-wave by ballots, publish their digit counts and
 // find the counts of all earlier tiles by a decoupled look-back, then write
 // their keys out in coalesced digit runs. The ensembles keep tile histograms
 // and a scan per simulation (k_ens_sort_*).
@@ -578,18 +578,25 @@ __global__ __launch_bounds__(512) void k_sort_digits(const SimValues* __restrict
     const uint32_t num = sim->numBodies;
     const uint32_t chunk = ((num + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
     const uint32_t begin = min(blockIdx.x * chunk, num), end = min(begin + chunk, num);
-    for (uint32_t r0 = begin + w * 64; r0 < end; r0 += 512) {
-        const uint32_t e = r0 + lane;
-        const bool valid = e < end;                  // a prefix of the row's lanes
-        const uint64_t key = valid ? keys[e] : 0ull;
-        const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
-        for (uint32_t p = 0; p < kSortPasses; p++) {
-            const uint32_t d = (uint32_t)(key >> (p * kSortBits)) & 255u;
-            const uint32_t d0 = uniform(d);          // lane 0 is valid
-            if (ballot(valid && d != d0) == 0ull) {
-                if (lane == 0) atomicAdd(&h[p * 256 + d0], nvalid);
-            } else if (valid) {
-                atomicAdd(&h[p * 256 + d], 1u);
+    constexpr uint32_t kRows = 4;                    // rows of 64 keys a wave has in flight
+    for (uint32_t r0 = begin + w * 64 * kRows; r0 < end; r0 += 512 * kRows) {
+        uint64_t key[kRows];
+        for (uint32_t r = 0; r < kRows; r++) {
+            const uint32_t e = r0 + r * 64 + lane;
+            key[r] = e < end ? keys[e] : 0ull;
+        }
+        for (uint32_t r = 0; r < kRows; r++) {
+            if (r0 + r * 64 >= end) break;           // (wave-uniform)
+            const bool valid = r0 + r * 64 + lane < end;   // a prefix of the row's lanes
+            const uint32_t nvalid = (uint32_t)__popcll(ballot(valid));
+            for (uint32_t p = 0; p < kSortPasses; p++) {
+                const uint32_t d = (uint32_t)(key[r] >> (p * kSortBits)) & 255u;
+                const uint32_t d0 = uniform(d);      // lane 0 is valid
+                if (ballot(valid && d != d0) == 0ull) {
+                    if (lane == 0) atomicAdd(&h[p * 256 + d0], nvalid);
+                } else if (valid) {
+                    atomicAdd(&h[p * 256 + d], 1u);
+                }
             }
         }
     }
@@ -602,36 +609,29 @@ __global__ __launch_bounds__(512) void k_sort_digits(const SimValues* __restrict
     }
 }
 
-// One block per pass: each digit's total over the partial counts (four
-// threads per digit, each over every fourth count set, 8 loads in flight),
-// then the exclusive scan over the 256 digits: where each digit's run starts.
-__global__ __launch_bounds__(1024) void k_sort_bases(const uint32_t* __restrict__ partials, uint32_t nparts,
-                                                     uint32_t* __restrict__ digit_base) {
-    __shared__ uint32_t s_q[4][256];
-    __shared__ uint32_t s_w[4];
-    const uint32_t p = blockIdx.x, t = threadIdx.x, d = t & 255u, q = t >> 8, lane = lane_id();
-    constexpr uint32_t kU = 8;
+// Each digit's total over the partial counts: block (pass, group of 32
+// digits), 32 threads per digit over every 32nd count set, then summed in LDS.
+constexpr uint32_t kTotalsDigits = 32;
+__global__ __launch_bounds__(1024) void k_sort_totals(const uint32_t* __restrict__ partials, uint32_t nparts,
+                                                      uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_part[32][kTotalsDigits + 1];
+    const uint32_t p = blockIdx.x / (256 / kTotalsDigits);
+    const uint32_t d0 = (blockIdx.x % (256 / kTotalsDigits)) * kTotalsDigits;
+    const uint32_t t = threadIdx.x, dl = t % kTotalsDigits, q = t / kTotalsDigits;
+    constexpr uint32_t kU = 4;
     uint32_t part = 0;
-    for (uint32_t b0 = q; b0 < nparts; b0 += 4 * kU) {
+    for (uint32_t b0 = q; b0 < nparts; b0 += 32 * kU) {
         uint32_t v[kU];
         for (uint32_t j = 0; j < kU; j++)
-            v[j] = b0 + 4 * j < nparts ? partials[(size_t)(b0 + 4 * j) * kSortPasses * 256 + p * 256 + d] : 0u;
+            v[j] = b0 + 32 * j < nparts ? partials[(size_t)(b0 + 32 * j) * kSortPasses * 256 + p * 256 + d0 + dl] : 0u;
         for (uint32_t j = 0; j < kU; j++) part += v[j];
     }
-    s_q[q][d] = part;
+    s_part[q][dl] = part;
     __syncthreads();
-    const uint32_t tot = t < 256 ? s_q[0][d] + s_q[1][d] + s_q[2][d] + s_q[3][d] : 0u;
-    uint32_t incl = tot;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off, 64);
-        if (lane >= (uint32_t)off) incl += v;
-    }
-    if (t < 256 && lane == 63) s_w[t >> 6] = incl;
-    __syncthreads();
-    if (t < 256) {
-        uint32_t before = 0;
-        for (uint32_t k = 0; k < (t >> 6); k++) before += s_w[k];
-        digit_base[p * 256 + d] = before + incl - tot;
+    if (t < kTotalsDigits) {
+        uint32_t tot = 0;
+        for (uint32_t k = 0; k < 32; k++) tot += s_part[k][t];
+        totals[p * 256 + d0 + t] = tot;
     }
 }
 
@@ -701,7 +701,7 @@ __global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimV
                                                                       const uint32_t* __restrict__ idx_in,
                                                                       uint64_t* __restrict__ keys_out,
                                                                       uint32_t* __restrict__ idx_out, uint32_t pass,
-                                                                      const uint32_t* __restrict__ digit_base,
+                                                                      const uint32_t* __restrict__ totals,
                                                                       unsigned long long* __restrict__ status,
                                                                       SortState* __restrict__ st) {
     __shared__ ScatterLds<kSortTileOne, kSortWavesOne> L;
@@ -713,7 +713,23 @@ __global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimV
     const uint32_t num = sim->numBodies;
     const uint32_t base = tile * kSortTileOne;
     if (base >= num) return;                          // ids are dense: the tiles below num are all taken
-    if (t < 256) L.base[t] = digit_base[pass * 256 + t];
+    {   // where each digit's run starts: the exclusive scan of the pass's digit totals
+        __shared__ uint32_t s_w[4];
+        const uint32_t lane = lane_id();
+        const uint32_t tot = t < 256 ? totals[pass * 256 + t] : 0u;
+        uint32_t incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += v;
+        }
+        if (t < 256 && lane == 63) s_w[t >> 6] = incl;
+        __syncthreads();
+        if (t < 256) {
+            uint32_t before = 0;
+            for (uint32_t k = 0; k < (t >> 6); k++) before += s_w[k];
+            L.base[t] = before + incl - tot;
+        }
+    }
     const TileLookback lb = {status, tile, st->epoch, ctr};
     scatter_tile<8, kSortTileOne, kSortWavesOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out,
                                                  idx_out, pass * kSortBits, lb);
@@ -774,7 +790,51 @@ struct NodeState {
     float4 mn, mx;       // AABB
     float4 col;          // mass-weighted colour
     uint32_t height, id; // height, global node id
+    // (trees with second moments) about com: sum m (x - c)(x - c)^T as xx, xy,
+    // xz, yy, yz, zz, and the first moment sum m (x - c), which the rounding of
+    // the FP32 centre of mass leaves slightly off zero
+    float m2[6];
+    float dm[3];
 };
+
+// The moments of the parent p of A and B about p's centre of mass, from the
+// children's about theirs: with d = c_child - c_p, a child's bodies add
+// sum m (x - c_p)(x - c_p)^T = M2 + D d^T + d D^T + m d d^T and
+// sum m (x - c_p) = D + m d. Carrying D keeps this exact up to FP32 rounding
+// of terms of the node's own size, however far the FP32 centre lies from the
+// true one (the parallel-axis form without D is off by D d^T, of relative size
+// ulp(|x|) / h for a node of half side h).
+__device__ __forceinline__ void merge_moments(const NodeState& A, const NodeState& B, NodeState& p) {
+    float m2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, dm[3] = {0.f, 0.f, 0.f};
+    const NodeState* ch[2] = {&A, &B};
+    for (int c = 0; c < 2; c++) {
+        const NodeState& C = *ch[c];
+        const float d[3] = {C.com.x - p.com.x, C.com.y - p.com.y, C.com.z - p.com.z};
+        const float hm = 0.5f * C.com.w;
+        float u[3];   // D + m d / 2: then D d^T + d D^T + m d d^T = u d^T + d u^T
+        for (int i = 0; i < 3; i++) {
+            u[i] = __builtin_fmaf(hm, d[i], C.dm[i]);
+            dm[i] += __builtin_fmaf(C.com.w, d[i], C.dm[i]);
+        }
+        int q = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = i; j < 3; j++, q++) m2[q] += __builtin_fmaf(u[i], d[j], __builtin_fmaf(d[i], u[j], C.m2[q]));
+    }
+    for (int q = 0; q < 6; q++) p.m2[q] = m2[q];
+    for (int i = 0; i < 3; i++) p.dm[i] = dm[i];
+}
+
+// the three 16-byte words of a node's moments in the exchange and frontier records
+__device__ __forceinline__ void moment_words(const NodeState& s, float4* w) {
+    w[0] = make_float4(s.m2[0], s.m2[1], s.m2[2], s.m2[3]);
+    w[1] = make_float4(s.m2[4], s.m2[5], s.dm[0], s.dm[1]);
+    w[2] = make_float4(s.dm[2], 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ void moments_from_words(const float4* w, NodeState& s) {
+    s.m2[0] = w[0].x; s.m2[1] = w[0].y; s.m2[2] = w[0].z; s.m2[3] = w[0].w;
+    s.m2[4] = w[1].x; s.m2[5] = w[1].y; s.dm[0] = w[1].z; s.dm[1] = w[1].w;
+    s.dm[2] = w[2].x;
+}
 
 // record of one side of a split: 3 (4 with colours) 16-byte words
 __device__ __forceinline__ void put_state(float4* x, const NodeState& s, int colors) {
@@ -782,6 +842,17 @@ __device__ __forceinline__ void put_state(float4* x, const NodeState& s, int col
     store_coherent(x + 1, make_float4(s.mn.x, s.mn.y, s.mn.z, s.mx.x));
     store_coherent(x + 2, make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id)));
     if (colors) store_coherent(x + 3, s.col);
+}
+
+__device__ __forceinline__ void put_moments(float4* xm, const NodeState& s) {
+    float4 w[3];
+    moment_words(s, w);
+    for (int k = 0; k < 3; k++) store_coherent(xm + k, w[k]);
+}
+__device__ __forceinline__ void get_moments(const float4* xm, NodeState& s) {
+    float4 w[3];
+    for (int k = 0; k < 3; k++) w[k] = load_coherent(xm + k);
+    moments_from_words(w, s);
 }
 
 __device__ __forceinline__ void get_state(const float4* x, NodeState& s, int colors) {
@@ -840,6 +911,8 @@ __device__ __forceinline__ NodeState leaf_state(float4 pm, uint32_t id) {
     me.height = 0;
     me.id = id;
     me.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < 6; q++) me.m2[q] = 0.f;
+    for (int i = 0; i < 3; i++) me.dm[i] = 0.f;
     return me;
 }
 
@@ -916,7 +989,8 @@ struct FrontierEntry {
 // [plo, phi] with Karras index pid, stored in the node arrays (bh_tree.comp:183-259).
 __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeState& B, int plo, int phi,
                                                  uint32_t pid, const TreeSeg& S, const TreeArrays& T,
-                                                 uint32_t* __restrict__ height_out, bool height_max, int colors) {
+                                                 uint32_t* __restrict__ height_out, bool height_max, int colors,
+                                                 bool moments) {
     NodeState p;
     const float M = fadd(A.com.w, B.com.w);
     if (M > 0.0f) {
@@ -957,6 +1031,11 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
         T.node_aabb[2 * k + 1] = make_float4(p.mx.x, p.mx.y, p.mx.z, M);
         if (colors) T.node_color[k] = p.col;
     }
+    if (moments) {
+        merge_moments(A, B, p);
+        T.node_mom[2 * k] = make_float4(p.m2[0], p.m2[1], p.m2[2], p.m2[3]);
+        T.node_mom[2 * k + 1] = make_float4(p.m2[4], p.m2[5], 0.f, 0.f);
+    }
     T.node_range[k] = make_uint2(S.off + (uint32_t)plo, (uint32_t)(phi - plo + 1));
     if (A.id < S.n_ids) T.leaf_parent[A.id] = p.id; else T.node_parent[A.id - S.n_ids] = p.id;
     if (B.id < S.n_ids) T.leaf_parent[B.id] = p.id; else T.node_parent[B.id - S.n_ids] = p.id;
@@ -971,7 +1050,7 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
 __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, NodeState me,
                                            const uint64_t* __restrict__ keys, const LeafSource& L,
                                            const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
-                                           int colors) {
+                                           int colors, bool moments) {
     const int num = (int)S.num;
     for (;;) {
         if (lo == 0 && hi == num - 1) break;
@@ -992,13 +1071,16 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
         } else {
             if (lo == hi) return;                         // the internal sibling's thread builds the parent
             float4* x = T.xchg + 8 * ((size_t)S.ibase + split);
+            float4* xm = moments ? T.xchg_mom + 6 * ((size_t)S.ibase + split) : nullptr;
             put_state(x + (left ? 0 : 4), me, colors);
+            if (moments) put_moments(xm + (left ? 0 : 3), me);
             stores_done();
             other = __hip_atomic_exchange(&T.meet[S.ibase + split], (uint32_t)(left ? lo : hi), __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
             if (other == kEmpty) return;                  // first to arrive: the sibling continues
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             get_state(x + (left ? 4 : 0), sib, colors);
+            if (moments) get_moments(xm + (left ? 3 : 0), sib);
             T.meet[S.ibase + split] = kEmpty;             // only this thread touches the slot now
         }
         const int plo = left ? lo : (int)other;
@@ -1010,8 +1092,8 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
             const bool pl = (plo == 0) || (phi != num - 1 && delta_at(keys, num, phi) > delta_at(keys, num, plo - 1));
             pid = (uint32_t)(pl ? phi : plo);
         }
-        me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, colors)
-                  : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, colors);
+        me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, colors, moments)
+                  : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, colors, moments);
         lo = plo;
         hi = phi;
     }
@@ -1035,9 +1117,10 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
 // right child.
 constexpr uint32_t kSideRight = 0x80000000u;
 
-template <bool kColors>
+template <bool kColors, bool kMoments>
 struct TreeBlockLds {
-    static constexpr int kRec = kColors ? 4 : 3;   // float4 words of one record
+    static constexpr int kMom = kColors ? 4 : 3;   // first moment word of a record
+    static constexpr int kRec = kMom + (kMoments ? 3 : 0);   // float4 words of one record
     int d[kTreeBlock + 1];                         // delta(blo - 1 + j)
     uint32_t meet[kTreeBlock];                     // arrival slot of split blo + j
     float4 pm[kTreeBlock];                         // posMass of leaf blo + j
@@ -1048,15 +1131,16 @@ struct TreeBlockLds {
     uint32_t nf, base;
 };
 
-template <bool kColors>
+template <bool kColors, bool kMoments>
 __device__ __forceinline__ void lds_put(float4* x, const NodeState& s) {
     x[0] = s.com;
     x[1] = make_float4(s.mn.x, s.mn.y, s.mn.z, s.mx.x);
     x[2] = make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id));
     if (kColors) x[3] = s.col;
+    if (kMoments) moment_words(s, x + TreeBlockLds<kColors, kMoments>::kMom);
 }
 
-template <bool kColors>
+template <bool kColors, bool kMoments>
 __device__ __forceinline__ NodeState lds_get(const float4* x) {
     NodeState s;
     s.com = x[0];
@@ -1066,6 +1150,7 @@ __device__ __forceinline__ NodeState lds_get(const float4* x) {
     s.height = __float_as_uint(b.z);
     s.id = __float_as_uint(b.w);
     s.col = kColors ? x[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (kMoments) moments_from_words(x + TreeBlockLds<kColors, kMoments>::kMom, s);
     return s;
 }
 
@@ -1081,12 +1166,12 @@ __device__ __forceinline__ FrontierEntry frontier_entry(const NodeState& me, int
 // Positions [blo, blo + cnt) of segment S (local positions; keys given from
 // the segment's first key); thread t holds leaf blo + t. Called by every
 // thread of the block (it has barriers).
-template <bool kColors>
-__device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeSeg& S, int blo, int cnt,
+template <bool kColors, bool kMoments>
+__device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, const TreeSeg& S, int blo, int cnt,
                                            const uint64_t* __restrict__ keys, const LeafSource& L,
                                            const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
                                            uint32_t segment) {
-    constexpr int R = TreeBlockLds<kColors>::kRec;
+    constexpr int R = TreeBlockLds<kColors, kMoments>::kRec;
     const int num = (int)S.num;
     const int t = threadIdx.x;
     const int bhi = blo + cnt - 1;
@@ -1124,7 +1209,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
             const int split = left ? hi : lo - 1;
             if (split < blo || split >= bhi) {             // the parent crosses the block's edge
                 const int side = left ? 0 : 1;
-                lds_put<kColors>(s.edge[side], me);
+                lds_put<kColors, kMoments>(s.edge[side], me);
                 s.edge_range[side] = make_int2(lo, hi);
                 break;
             }
@@ -1141,7 +1226,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
                 if (lo == hi) break;                       // the internal sibling's thread builds the parent
                 // the exchange orders LDS only: the fences are for the local
                 // address space, so they do not wait for this wave's node stores
-                lds_put<kColors>(&s.rec[R * ((left ? hi : lo) - blo)], me);
+                lds_put<kColors, kMoments>(&s.rec[R * ((left ? hi : lo) - blo)], me);
                 const uint32_t v = left ? (uint32_t)lo : ((uint32_t)hi | kSideRight);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
                 const uint32_t old =
@@ -1150,7 +1235,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 s.meet[split - blo] = kEmpty;              // only this thread touches the slot now
                 other = (int)(old & ~kSideRight);
-                sib = lds_get<kColors>(&s.rec[R * ((left ? split + 1 : split) - blo)]);
+                sib = lds_get<kColors, kMoments>(&s.rec[R * ((left ? split + 1 : split) - blo)]);
             }
             const int plo = left ? lo : other;
             const int phi = left ? other : hi;
@@ -1161,8 +1246,8 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
                 const bool pl = (plo == 0) || (phi != num - 1 && D(phi) > D(plo - 1));
                 pid = (uint32_t)(pl ? phi : plo);
             }
-            me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, kColors)
-                      : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, kColors);
+            me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, kColors, kMoments)
+                      : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, kColors, kMoments);
             lo = plo;
             hi = phi;
         }
@@ -1170,6 +1255,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
     __syncthreads();
     // first arrivals left waiting, and the edge nodes
     FrontierEntry e0, e1;
+    NodeState n0, n1;   // (their moments)
     int slot0 = -1, slot1 = -1;
     if (t < cnt - 1) {
         const uint32_t v = s.meet[t];
@@ -1177,12 +1263,14 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
             const int split = blo + t, far = (int)(v & ~kSideRight);
             const bool right = (v & kSideRight) != 0;
             const int lo = right ? split + 1 : far, hi = right ? far : split;
-            e0 = frontier_entry(lds_get<kColors>(&s.rec[R * ((right ? lo : hi) - blo)]), lo, hi, segment);
+            n0 = lds_get<kColors, kMoments>(&s.rec[R * ((right ? lo : hi) - blo)]);
+            e0 = frontier_entry(n0, lo, hi, segment);
             slot0 = (int)atomicAdd(&s.nf, 1u);
         }
     }
     if (t < 2 && s.edge_range[t].x >= 0) {
-        e1 = frontier_entry(lds_get<kColors>(s.edge[t]), s.edge_range[t].x, s.edge_range[t].y, segment);
+        n1 = lds_get<kColors, kMoments>(s.edge[t]);
+        e1 = frontier_entry(n1, s.edge_range[t].x, s.edge_range[t].y, segment);
         slot1 = (int)atomicAdd(&s.nf, 1u);
     }
     __syncthreads();
@@ -1191,9 +1279,13 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors>& s, const TreeS
     FrontierEntry* F = reinterpret_cast<FrontierEntry*>(T.frontier);
     if (slot0 >= 0) F[s.base + slot0] = e0;
     if (slot1 >= 0) F[s.base + slot1] = e1;
+    if (kMoments) {
+        if (slot0 >= 0) moment_words(n0, T.frontier_mom + 3 * (size_t)(s.base + slot0));
+        if (slot1 >= 0) moment_words(n1, T.frontier_mom + 3 * (size_t)(s.base + slot1));
+    }
 }
 
-template <bool kColors>
+template <bool kColors, bool kMoments>
 __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict__ sim,
                                                      DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                      const uint32_t* __restrict__ index_in,
@@ -1201,7 +1293,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
                                                      const float4* __restrict__ step_pos,
                                                      const uint32_t* __restrict__ from,
                                                      uint32_t* __restrict__ ids_out, uint32_t fill_cap) {
-    __shared__ TreeBlockLds<kColors> s;
+    __shared__ TreeBlockLds<kColors, kMoments> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
     const int blo = (int)(blockIdx.x * kTreeBlock);
@@ -1241,7 +1333,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity, by_pos ? step_pos : nullptr, from,
                           ids_out};
     const int cnt = num - blo < (int)kTreeBlock ? num - blo : (int)kTreeBlock;
-    tree_block<kColors>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
+    tree_block<kColors, kMoments>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
 }
 
 // (the colour of a frontier node, when colours are on, is in node_color)
@@ -1253,7 +1345,16 @@ __device__ __forceinline__ NodeState frontier_state(const FrontierEntry& e) {
     me.height = __float_as_uint(e.b.z);
     me.id = __float_as_uint(e.b.w);
     me.col = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q = 0; q < 6; q++) me.m2[q] = 0.f;
+    for (int i = 0; i < 3; i++) me.dm[i] = 0.f;
     return me;
+}
+
+// entry t's moments, written by the first launch
+__device__ __forceinline__ void frontier_moments(const TreeArrays& T, uint32_t t, NodeState& me) {
+    float4 w[3];
+    for (int k = 0; k < 3; k++) w[k] = T.frontier_mom[3 * (size_t)t + k];
+    moments_from_words(w, me);
 }
 
 // The second launch: one thread per frontier node, climbing to the root.
@@ -1268,9 +1369,11 @@ __global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict_
     const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
     NodeState me = frontier_state(e);
     if (colors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
+    const bool moments = T.node_mom != nullptr;
+    if (moments) frontier_moments(T, t, me);
     const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant, fill_cap};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
-    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors);
+    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors, moments);
 }
 
 // ------------------------------------------------------------------ merge
@@ -1711,18 +1814,18 @@ void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCou
 }
 uint32_t sort_digit_blocks(uint32_t n) {
     const uint32_t b = cdiv(n, 16384);
-    return b < 1 ? 1 : (b > 256 ? 256 : b);
+    return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
 void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, uint64_t* keys[2],
-                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* digit_base,
+                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* totals,
                  unsigned long long* status, uint32_t tiles_cap, SortState* st) {
     const uint32_t nb = sort_digit_blocks(n), nt = cdiv(n, kSortTileOne);
     k_sort_digits<<<nb, 512, 0, s>>>(sim, keys[in], partials, st);
-    k_sort_bases<<<kSortPasses, 1024, 0, s>>>(partials, nb, digit_base);
+    k_sort_totals<<<kSortPasses * (256 / kTotalsDigits), 1024, 0, s>>>(partials, nb, totals);
     for (uint32_t pass = 0; pass < kSortPasses; pass++) {
         const int i = (in + (int)pass) & 1;
         k_sort_onesweep<<<nt, kSortWavesOne * 64, 0, s>>>(sim, ctr, keys[i], (positions && pass == 0) ? nullptr : vals[i],
-                                                          keys[1 - i], vals[1 - i], pass, digit_base,
+                                                          keys[1 - i], vals[1 - i], pass, totals,
                                                           status + (size_t)pass * tiles_cap * 256, st);
     }
 }
@@ -1730,12 +1833,21 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos, const uint32_t* from, uint32_t* ids_out, uint32_t fill_cap) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    if (colors)
-        k_tree<true><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
-                                                                ids_out, fill_cap);
-    else
-        k_tree<false><<<cdiv(n, kTreeBlock), kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
-                                                                 ids_out, fill_cap);
+    const dim3 grid(cdiv(n, kTreeBlock));
+    if (T.node_mom) {
+        if (colors)
+            k_tree<true, true><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
+                                                           fill_cap);
+        else
+            k_tree<false, true><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
+                                                            ids_out, fill_cap);
+    } else if (colors) {
+        k_tree<true, false><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
+                                                        fill_cap);
+    } else {
+        k_tree<false, false><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
+                                                         fill_cap);
+    }
     if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
     // the second launch meets leaves by their ids in the new order
     k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src,
@@ -1999,6 +2111,7 @@ __global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restr
 
 // Tree of every simulation: the block-local first launch over two blocks per
 // tile (tiles never span simulations).
+template <bool kMoments>
 __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restrict__ sims,
                                                          const EnsTile* __restrict__ tiles, uint32_t N,
                                                          DevCounters* __restrict__ ctr,
@@ -2006,7 +2119,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restric
                                                          const uint32_t* __restrict__ index_in,
                                                          const uint64_t* __restrict__ keys, TreeArrays T,
                                                          float G) {
-    __shared__ TreeBlockLds<false> s;
+    __shared__ TreeBlockLds<false, kMoments> s;
     const EnsTile tl = tiles[blockIdx.x >> 1];
     const uint32_t first = (blockIdx.x & 1u) * kTreeBlock;
     if (first >= tl.count) return;
@@ -2014,8 +2127,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restric
     const int cnt = tl.count - first < kTreeBlock ? (int)(tl.count - first) : (int)kTreeBlock;
     const TreeSeg seg = {S.off, S.num, S.ibase, N, G};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
-    tree_block<false>(s, seg, (int)(tl.start + first - S.off), cnt, keys + S.off, L, T, &ctr->treeHeight, true,
-                      tl.sim);
+    tree_block<false, kMoments>(s, seg, (int)(tl.start + first - S.off), cnt, keys + S.off, L, T, &ctr->treeHeight,
+                                true, tl.sim);
 }
 
 __global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict__ sims, uint32_t N,
@@ -2028,8 +2141,10 @@ __global__ __launch_bounds__(256) void k_ens_tree_upper(const EnsSim* __restrict
     const EnsSim& S = sims[e.range.z];
     const TreeSeg seg = {S.off, S.num, S.ibase, N, G};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
-    tree_climb((int)e.range.x, (int)e.range.y, seg, frontier_state(e), keys + S.off, L, T, &ctr->treeHeight,
-               true, 0);
+    NodeState me = frontier_state(e);
+    const bool moments = T.node_mom != nullptr;
+    if (moments) frontier_moments(T, t, me);
+    tree_climb((int)e.range.x, (int)e.range.y, seg, me, keys + S.off, L, T, &ctr->treeHeight, true, 0, moments);
 }
 
 __global__ void k_ens_nodes_std430(TreeArrays T, uint32_t n_internal, NodeStd430* __restrict__ out) {
@@ -2089,7 +2204,10 @@ void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, ui
                      DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
                      const TreeArrays& T, float G) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    k_ens_tree<<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
+    if (T.node_mom)
+        k_ens_tree<true><<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
+    else
+        k_ens_tree<false><<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
     k_ens_tree_upper<<<cdiv(frontier_cap(N, 2 * ntiles), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T,
                                                                                G);
 }

@@ -53,6 +53,13 @@ struct TreeArrays {
     float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
     float4* frontier;        // [4 frontier_cap] roots of the subtrees the first build launch left
     uint32_t* frontier_count;
+    // Second moments, built with the tree for the corrected walk (all null
+    // otherwise): node_mom holds each internal node's second moment about its
+    // (FP32) centre of mass, the records below carry it and the node's first
+    // moment about that centre through the exchanges.
+    float4* node_mom = nullptr;       // [2(n-1)] (xx, xy, xz, yy), (yz, zz, 0, 0)
+    float4* xchg_mom = nullptr;       // [6(n-1)] the moment words of xchg's two sides (3 words each)
+    float4* frontier_mom = nullptr;   // [3 frontier_cap] the moment words of the frontier entries
 };
 
 // The first tree launch builds each block of kTreeBlock sorted positions on
@@ -158,11 +165,6 @@ __device__ __forceinline__ float corr_dist2(float rx, float ry, float rz, float 
     return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, soft)));
 }
 
-// Second moments of the corrected walk: positions per tile of the FP64
-// prefix sums (m, m x, m x x^T: 10 components per position).
-constexpr uint32_t kMomTile = 1024;
-inline size_t mom_prefix_doubles(uint32_t n) { return ((size_t)n + 1) * 10; }
-inline size_t mom_tile_doubles(uint32_t n) { return ((size_t)n / kMomTile + 1) * 10; }
 
 // ---- ensembles: independent simulations concatenated in one set of arrays.
 // Simulation s owns sorted positions and body ids [off, off + num) and
@@ -239,7 +241,7 @@ uint32_t sort_digit_blocks(uint32_t n);   // histogram blocks (partial count set
 // values start as the positions 0..num-1 instead) in kSortPasses passes; the
 // result lands back in keys[in], vals[in].
 void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, uint64_t* keys[2],
-                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* digit_base,
+                 uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* totals,
                  unsigned long long* status, uint32_t tiles_cap, SortState* st);
 // from: when the sort sorted positions instead of ids, the (scrambled)
 // position before the sort of the body at each sorted position; index_in is then in the
@@ -256,22 +258,21 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
 constexpr uint32_t kReferenceFillPasses = 64;
 void launch_walk_counters_reset(hipStream_t s, const SimValues* sim, DevCounters* ctr);
 void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a);
-// The corrected walk's node records on the current tree: FP64 prefix sums of
-// the moments (prefix: mom_prefix_doubles(n), tiles: mom_tile_doubles(n)),
-// then per internal node the second moment about its centre of mass plus the
-// network's residual (w, bias: the packed network, or nullptr for none), into
+// The corrected walk's node records on the current tree, which was built
+// with its second moments (T.node_mom): per internal node the second moment
+// about its centre of mass plus the network's residual (w, bias: the packed
+// network, or nullptr for none), into
 // the records qrec [4 corr_records(n)] (the leaves' quadrupole halves must
 // hold zeros: they are never written); ctr->corrected counts the network
 // evaluations.
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
-                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec);
+                             const __bf16* w, const float* bias, float4* qrec);
 // the packed BF16 weights and biases of a network (include/bh_abi.h's bh_mlp), or nullptr
 void mlp_device_weights(const bh_mlp* m, const __bf16** w, const float** bias);
 // The same for an ensemble's trees (N bodies, n_internal = N - S internal
 // nodes, node ids global).
 void launch_ens_node_quadrupoles(hipStream_t s, uint32_t N, uint32_t n_internal, float G, DevCounters* ctr,
-                                 const TreeArrays& T, double* prefix, double* tiles, const __bf16* w,
-                                 const float* bias, float4* qrec);
+                                 const TreeArrays& T, const __bf16* w, const float* bias, float4* qrec);
 // count [2]: pairs found, and 1 where a walk ran out of stack (its pairs are incomplete)
 void launch_collect_pairs(hipStream_t s, uint32_t n, const SimValues* sim, const float4* leaf_pos,
                           const WalkNode* nodes, const float4* node_aabb, float theta, float soft, uint32_t stride,

@@ -1,13 +1,9 @@
 // bh_learned.hip — the quadrupole-corrected walk's node records and the
 // learned residual (no reference counterpart).
 //
-//  * second moments: FP64 prefix sums of m, m x and m x x^T over the sorted
-//    bodies (k_mom_tile, k_mom_scan, k_mom_prefix); every internal node covers
-//    a contiguous range of sorted positions, so its moments are a difference of
-//    two prefix entries, with no bottom-up pass.
-//  * k_qnodes: per internal node, the second moment about its centre of mass,
-//    the network's residual (BF16 MFMA, one evaluation per node), and the
-//    64-byte record the walk reads.
+//  * k_qnodes: per internal node, the second moment about its centre of mass
+//    (built bottom-up with the tree, T.node_mom), the network's residual (BF16
+//    MFMA, one evaluation per node), and the 64-byte record the walk reads.
 //  * k_collect_pairs / k_pair_labels: (target, accepted node) pairs of the walk
 //    and their FP64 labels: the exact field of the node's bodies minus its
 //    monopole and exact quadrupole, in units of (h / s)^2 G M / s^2.
@@ -20,7 +16,7 @@
 
 namespace bh {
 
-// ------------------------------------------------------------------ second moments
+// ------------------------------------------------------------------ node records
 // The counts the record kernels work on: one simulation's from its device
 // values (bodies alive, internal nodes = alive - 1), or an ensemble's given
 // on the host (N bodies, all alive; N - S internal nodes).
@@ -38,145 +34,6 @@ struct RecScope {
     __device__ __forceinline__ float grav() const { return sim ? sim->gravitationalConstant : G; }
 };
 
-// The 10 moment components of a body: m, m x, m y, m z, m xx, m xy, m xz,
-// m yy, m yz, m zz, in double precision.
-__device__ __forceinline__ void body_moments(float4 pm, double* v) {
-    const double m = pm.w, x = pm.x, y = pm.y, z = pm.z;
-    v[0] = m;
-    v[1] = m * x;
-    v[2] = m * y;
-    v[3] = m * z;
-    v[4] = v[1] * x;
-    v[5] = v[1] * y;
-    v[6] = v[1] * z;
-    v[7] = v[2] * y;
-    v[8] = v[2] * z;
-    v[9] = v[3] * z;
-}
-
-// Sum of 10 doubles over a block of kMomTile threads (wave shuffles, then
-// one thread per component over the wave totals); valid in threads 0..9 as
-// total[threadIdx.x].
-__device__ __forceinline__ double block_sum10(const double* v, double* s_w) {
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    constexpr uint32_t kW = kMomTile / 64;
-    for (int c = 0; c < 10; c++) {
-        double x = v[c];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) s_w[w * 10 + c] = x;
-    }
-    __syncthreads();
-    double all = 0.0;
-    if (threadIdx.x < 10)
-        for (uint32_t k = 0; k < kW; k++) all += s_w[k * 10 + threadIdx.x];
-    return all;
-}
-
-// tile t: sums of the moments of positions [t kMomTile, (t + 1) kMomTile) below numBodies
-__global__ __launch_bounds__(kMomTile) void k_mom_tile(const RecScope sc, const float4* __restrict__ leaf_pos,
-                                                       double* __restrict__ tiles) {
-    __shared__ double s_w[(kMomTile / 64) * 10];
-    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (g < sc.alive()) body_moments(leaf_pos[g], v);
-    const double t = block_sum10(v, s_w);
-    if (threadIdx.x < 10) tiles[blockIdx.x * 10 + threadIdx.x] = t;
-}
-
-// one block: the tile sums become exclusive tile offsets (in place)
-__global__ __launch_bounds__(kMomTile) void k_mom_scan(double* __restrict__ tiles, uint32_t ntiles) {
-    __shared__ double s_w[(kMomTile / 64) * 10];
-    __shared__ double s_off[(kMomTile / 64) * 10];
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    constexpr uint32_t kW = kMomTile / 64;
-    const uint32_t per = (ntiles + kMomTile - 1) / kMomTile;
-    const uint32_t t0 = threadIdx.x * per, t1 = min(ntiles, t0 + per);
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t t = t0; t < t1; t++)
-        for (int c = 0; c < 10; c++) v[c] += tiles[t * 10 + c];
-    double ex[10];
-    for (int c = 0; c < 10; c++) {
-        double x = v[c];
-        for (int o = 1; o < 64; o <<= 1) {
-            const double y = __shfl_up(x, o, 64);
-            if ((int)lane >= o) x += y;
-        }
-        if (lane == 63) s_w[w * 10 + c] = x;
-        const double e = __shfl_up(x, 1, 64);
-        ex[c] = lane == 0 ? 0.0 : e;
-    }
-    __syncthreads();
-    if (threadIdx.x < 10) {
-        double acc = 0.0;
-        for (uint32_t k = 0; k < kW; k++) {
-            s_off[k * 10 + threadIdx.x] = acc;
-            acc += s_w[k * 10 + threadIdx.x];
-        }
-    }
-    __syncthreads();
-    for (int c = 0; c < 10; c++) ex[c] += s_off[w * 10 + c];
-    for (uint32_t t = t0; t < t1; t++)
-        for (int c = 0; c < 10; c++) {
-            const double x = tiles[t * 10 + c];
-            tiles[t * 10 + c] = ex[c];
-            ex[c] += x;
-        }
-}
-
-// prefix[c P + g] = component c of the moments of positions [0, g), for
-// g = 0 .. numBodies (component-major, P = n + 1: each component's store of
-// a wave is 512 contiguous bytes)
-__global__ __launch_bounds__(kMomTile) void k_mom_prefix(const RecScope sc, const float4* __restrict__ leaf_pos,
-                                                         const double* __restrict__ tiles,
-                                                         double* __restrict__ prefix) {
-    __shared__ double s_w[(kMomTile / 64) * 10];
-    __shared__ double s_off[(kMomTile / 64) * 10];
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    constexpr uint32_t kW = kMomTile / 64;
-    const uint32_t num = sc.alive();
-    const size_t P = (size_t)sc.n_ids() + 1;
-    const uint32_t g = blockIdx.x * kMomTile + threadIdx.x;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ex[10];
-    if (g < num) body_moments(leaf_pos[g], v);
-    for (int c = 0; c < 10; c++) {
-        double x = v[c];
-        for (int o = 1; o < 64; o <<= 1) {
-            const double y = __shfl_up(x, o, 64);
-            if ((int)lane >= o) x += y;
-        }
-        if (lane == 63) s_w[w * 10 + c] = x;
-        ex[c] = x - v[c];                      // exclusive within the wave
-    }
-    __syncthreads();
-    // the offset of every wave: the tile's offset plus the totals of the waves before
-    if (threadIdx.x < 10) {
-        double acc = tiles[blockIdx.x * 10 + threadIdx.x];
-        for (uint32_t k = 0; k < kW; k++) {
-            s_off[k * 10 + threadIdx.x] = acc;
-            acc += s_w[k * 10 + threadIdx.x];
-        }
-    }
-    __syncthreads();
-    if (g <= num)
-        for (int c = 0; c < 10; c++) prefix[c * P + g] = s_off[w * 10 + c] + ex[c];
-}
-
-// Second moment about the point c of the bodies of sorted positions [lo, hi)
-// from the component-major prefix sums (stride P): M2 = S2 - c S1^T - S1 c^T
-// + S0 c c^T.
-__device__ __forceinline__ void range_m2(const double* __restrict__ prefix, size_t P, uint32_t lo, uint32_t hi,
-                                         double cx, double cy, double cz, double* m2) {
-    double d[10];
-    for (int q = 0; q < 10; q++) d[q] = prefix[q * P + hi] - prefix[q * P + lo];
-    const double S0 = d[0];
-    const double c[3] = {cx, cy, cz}, S1[3] = {d[1], d[2], d[3]};
-    const int I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
-    for (int q = 0; q < 6; q++) {
-        const int i = I[q], j = J[q];
-        m2[q] = d[4 + q] - c[i] * S1[j] - S1[i] * c[j] + S0 * c[i] * c[j];
-    }
-}
-
 // The corrected walk's records (layout with corr_records): the first half of
 // every leaf's record (its walk record), and the whole record of every
 // internal node, T = M2 + M h^2 o with o the network's output for the node's
@@ -186,8 +43,7 @@ __device__ __forceinline__ void range_m2(const double* __restrict__ prefix, size
 __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* __restrict__ ctr,
                                                 const WalkNode* __restrict__ walk,
                                                 const float4* __restrict__ node_aabb,
-                                                const uint2* __restrict__ node_range,
-                                                const double* __restrict__ prefix, const __bf16* __restrict__ wg,
+                                                const float4* __restrict__ node_mom, const __bf16* __restrict__ wg,
                                                 const float* __restrict__ bg, float4* __restrict__ qrec) {
     __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
     __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
@@ -215,13 +71,13 @@ __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* 
         WalkNode nd;
         if (active) {
             nd = nodes[k];
-            const uint2 rg = node_range[k];
             com = nd.comMass;
             h = nd.half;
             mn = node_aabb[2 * k];
             mx = node_aabb[2 * k + 1];
             M = mx.w;
-            range_m2(prefix, (size_t)n + 1, rg.x, rg.x + rg.y, com.x, com.y, com.z, m2);
+            const float4 a = node_mom[2 * k], b = node_mom[2 * k + 1];
+            m2[0] = a.x; m2[1] = a.y; m2[2] = a.z; m2[3] = a.w; m2[4] = b.x; m2[5] = b.y;
         }
         const bool ok = active && M > 0.0f && h > 0.0f;
         double o[6] = {0, 0, 0, 0, 0, 0};
@@ -261,27 +117,22 @@ __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* 
 }
 
 static void node_quadrupoles(hipStream_t s, uint32_t n, const RecScope& sc, DevCounters* ctr, const TreeArrays& T,
-                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec) {
-    const uint32_t ntiles = n / kMomTile + 1;   // positions 0 .. n
-    k_mom_tile<<<ntiles, kMomTile, 0, s>>>(sc, T.leaf_pos, tiles);
-    k_mom_scan<<<1, kMomTile, 0, s>>>(tiles, ntiles);
-    k_mom_prefix<<<ntiles, kMomTile, 0, s>>>(sc, T.leaf_pos, tiles, prefix);
+                             const __bf16* w, const float* bias, float4* qrec) {
     const uint32_t groups = (n + 63) / 64;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048u);
-    k_qnodes<<<blocks, 256, 0, s>>>(sc, ctr, T.walk, T.node_aabb, T.node_range, prefix, w, bias, qrec);
+    k_qnodes<<<blocks, 256, 0, s>>>(sc, ctr, T.walk, T.node_aabb, T.node_mom, w, bias, qrec);
 }
 
 void launch_node_quadrupoles(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const TreeArrays& T,
-                             double* prefix, double* tiles, const __bf16* w, const float* bias, float4* qrec) {
+                             const __bf16* w, const float* bias, float4* qrec) {
     const RecScope sc = {sim, 0u, 0u, 0u, 0.0f};
-    node_quadrupoles(s, n, sc, ctr, T, prefix, tiles, w, bias, qrec);
+    node_quadrupoles(s, n, sc, ctr, T, w, bias, qrec);
 }
 
 void launch_ens_node_quadrupoles(hipStream_t s, uint32_t N, uint32_t n_internal, float G, DevCounters* ctr,
-                                 const TreeArrays& T, double* prefix, double* tiles, const __bf16* w,
-                                 const float* bias, float4* qrec) {
+                                 const TreeArrays& T, const __bf16* w, const float* bias, float4* qrec) {
     const RecScope sc = {nullptr, N, N, n_internal, G};
-    node_quadrupoles(s, N, sc, ctr, T, prefix, tiles, w, bias, qrec);
+    node_quadrupoles(s, N, sc, ctr, T, w, bias, qrec);
 }
 
 // ------------------------------------------------------------------ pairs

@@ -99,8 +99,9 @@ extern "C" {
 
 /* The corrected walk (BH_FLAG_QUADRUPOLE / BH_FLAG_LEARNED; no reference
  * counterpart). bh_build_tree under either flag also makes one 64-byte record
- * per node (the quadrupole terms from FP64 moment prefix sums, plus the
- * network's residual under BH_FLAG_LEARNED); bh_compute_force under the same
+ * per node (the quadrupole terms from second moments the tree builds with
+ * its fill, plus the network's residual under BH_FLAG_LEARNED);
+ * bh_compute_force under the same
  * flag then walks with them, without merging or collisions, softening >=
  * 1e-14 and 16 h_root^2 + softening <= 1e15 (h_root: half the root box's
  * longest side; beyond it the walk adds no forces and sets error bit 2).

@@ -101,15 +101,6 @@ def _tree_ctx(n, seed, flags, theta=0.3, soft=0.01):
     return bh, bodies, prm
 
 
-def _prefix_floor(bodies):
-    """Absolute error floor of a moment taken as the difference of two FP64
-    prefix sums: ~500 units in the last place of the total second moment
-    about the origin. (A force term then changes by at most that over s^4,
-    below 1e-9 for softening 0.01.)"""
-    b = bodies.astype(np.float64)
-    return 1e-13 * float((b[:, 3] * (b[:, 0:3] ** 2).sum(1)).sum())
-
-
 def _oracle_records(bh, n):
     from gravity_amd import barnes_hut as B
     leaf = bh.nodes(B.LEAF_NODES)
@@ -129,12 +120,16 @@ def test_node_records_match_oracle_moments():
     bh.build_current_tree()
     q = bh.node_quadrupoles()
     _, _, m2, M, h = _oracle_records(bh, n)
+    # the tree builds the moments bottom-up in FP32, each merge exact up to
+    # rounding of terms of the node's own size M h^2
     scale = (M * h * h)[:, None]
-    tol = 1e-5 * np.abs(m2) + 1e-5 * scale + _prefix_floor(bodies)
-    bad = np.abs(q[:, 0:6] - m2) > tol
-    assert not bad.any(), (np.argwhere(bad)[:5], (np.abs(q[:, 0:6] - m2) - tol).max())
+    tol = 1e-5 * np.abs(m2) + 1e-5 * scale
+    err = np.abs(q[:, 0:6] - m2)
+    print(f"[gpu] node moments: worst error {(err / scale).max():.3g} of M h^2", flush=True)
+    bad = err > tol
+    assert not bad.any(), (np.argwhere(bad)[:5], (err - tol).max())
     tr = -1.5 * (m2[:, 0] + m2[:, 3] + m2[:, 5])
-    assert (np.abs(q[:, 6] - tr) <= 1e-5 * np.abs(tr) + 1e-5 * scale[:, 0] + 5 * _prefix_floor(bodies)).all()
+    assert (np.abs(q[:, 6] - tr) <= 1e-5 * np.abs(tr) + 1e-5 * scale[:, 0]).all()
     assert bh.counters()["corrected"] == 0      # no network evaluations without a network
     bh.close()
 
@@ -157,8 +152,7 @@ def test_learned_records_are_moments_plus_network():
     _, _, m2, M, h = _oracle_records(bh, n)
     want = m2 + (M * h * h)[:, None] * b[None, :].astype(np.float64)
     scale = (M * h * h)[:, None]
-    bodies = bh.bodies()
-    assert (np.abs(q[:, 0:6] - want) <= 1e-5 * np.abs(want) + 1e-5 * scale + _prefix_floor(bodies)).all()
+    assert (np.abs(q[:, 0:6] - want) <= 1e-5 * np.abs(want) + 1e-5 * scale).all()
     assert bh.counters()["corrected"] == n - 1
     # a full network: the residual is the network of the node's features
     p2 = (np.random.default_rng(1).normal(size=Lr.num_params()) * 0.1).astype(np.float32)
@@ -171,9 +165,9 @@ def test_learned_records_are_moments_plus_network():
     feats = np.concatenate([(mx - mn) / (2 * h[:, None]), (f["com"] - 0.5 * (mn + mx)) / h[:, None],
                             m2 / (M * h * h)[:, None]], 1).astype(np.float32)
     o = O.mlp_forward(p2, feats)
-    # (nodes whose M h^2 is below the prefix sums' floor are left out of this
+    # (nodes whose M h^2 is below 1e-12 of the root's are left out of this
     # relative comparison; the first part covers them)
-    big = (M * h * h) > 1e4 * _prefix_floor(bodies)
+    big = (M * h * h) > 1e-12 * (M * h * h).max()
     got = (q2[big, 0:6].astype(np.float64) - m2[big]) / (M * h * h)[big, None]
     err = np.abs(got - o[big])
     assert big.mean() > 0.9
