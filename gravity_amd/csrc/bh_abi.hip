@@ -76,6 +76,7 @@ struct bh_context {
     // the validity, and a partition that removes bodies skips them on device.
     float4* step_pos = nullptr;
     bool step_pos_valid = false;
+    bool step_merged = false;   // the step that wrote step_pos merged bodies after its walk
     // bh_step sorts positions instead of ids: sort_pos[1] ends up holding,
     // per sorted position, the position the body had before the sort; the
     // tree reads the bodies through it (from step_pos, nearly in order) and
@@ -101,11 +102,12 @@ struct bh_context {
     struct Roles {
         int body_in, mi_in;
         uint32_t *index0, *index1, *sort0, *sort1;
-        bool step_pos_valid, quad_valid, accmag_valid, colors;
+        bool step_pos_valid, step_merged, quad_valid, accmag_valid, colors;
         uint32_t fill_cap;
         bool operator==(const Roles& o) const {
             return body_in == o.body_in && mi_in == o.mi_in && index0 == o.index0 && index1 == o.index1 &&
                    sort0 == o.sort0 && sort1 == o.sort1 && step_pos_valid == o.step_pos_valid &&
+                   step_merged == o.step_merged &&
                    quad_valid == o.quad_valid && accmag_valid == o.accmag_valid && colors == o.colors &&
                    fill_cap == o.fill_cap;
         }
@@ -677,10 +679,11 @@ int bh_reset_values(bh_context* c, const bh_params* p) {
 namespace {
 // with_bounds: the partition's census pass also updates the bounds (what
 // bh_update_bounds does; the bounds skip the bodies the partition removes)
-int partition_dead_bodies(bh_context* c, bool with_bounds) {
+// step_pos: the previous walk's positions, when they are the bodies' (the census reads them)
+int partition_dead_bodies(bh_context* c, bool with_bounds, const float4* step_pos = nullptr) {
     launch_dead_partition(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->bodies[1 - c->body_in],
                           c->index[c->mi_in], c->index[1 - c->mi_in], c->part_scratch,
-                          with_bounds ? c->bounds_partials : nullptr, c->bounds_partial_cap);
+                          with_bounds ? c->bounds_partials : nullptr, c->bounds_partial_cap, step_pos);
     int rc = check_launch(c, "dead partition");
     if (rc) return rc;
     return bh_swap_morton_and_index_buffers(c);   // BarnesHut.java:117
@@ -863,7 +866,10 @@ int step_enqueue(bh_context* c, const bh_params* p) {
     stage_begin(c, 1);
     // dynamic bounds are made by the partition's census pass (one read of
     // the bodies for both); the bounds stage then records no work
-    if ((rc = partition_dead_bodies(c, p->static_or_dynamic == BH_DYNAMIC))) return rc;
+    // the walk's positions are the bodies' unless a merge followed it
+    if ((rc = partition_dead_bodies(c, p->static_or_dynamic == BH_DYNAMIC,
+                                    use_pos && !c->step_merged ? c->step_pos : nullptr)))
+        return rc;
     stage_begin(c, 2);
     stage_begin(c, 3);
     if ((rc = generate_morton_codes(c, use_pos ? c->step_pos : nullptr))) return rc;
@@ -881,12 +887,14 @@ int step_enqueue(bh_context* c, const bh_params* p) {
     if (c->timing && c->timed < bh_context::kTimedSteps) c->timed++;
     rc = bh_swap_body_buffers(c);
     c->step_pos_valid = rc == BH_OK && wrote;
+    c->step_merged = p->merging_collision_or_neither == BH_MERGING;
     return rc;
 }
 
 bh_context::Roles roles_of(const bh_context* c) {
     return {c->body_in,  c->mi_in,          c->index[0],   c->index[1],      c->sort_pos[0],
-            c->sort_pos[1], c->step_pos_valid, c->quad_valid, c->accmag_valid, c->colors, c->fill_cap};
+            c->sort_pos[1], c->step_pos_valid, c->step_merged, c->quad_valid, c->accmag_valid, c->colors,
+            c->fill_cap};
 }
 
 void set_roles(bh_context* c, const bh_context::Roles& r) {
@@ -897,6 +905,7 @@ void set_roles(bh_context* c, const bh_context::Roles& r) {
     c->sort_pos[0] = r.sort0;
     c->sort_pos[1] = r.sort1;
     c->step_pos_valid = r.step_pos_valid;
+    c->step_merged = r.step_merged;
     c->quad_valid = r.quad_valid;
     c->accmag_valid = r.accmag_valid;
     c->colors = r.colors;

@@ -300,10 +300,19 @@ __device__ void bounds_finish(SimValues* __restrict__ sim, DevCounters* __restri
 // make the bounds.
 // With kBounds the same pass also makes the bounds of k_bounds (which skips
 // the zero-mass bodies the census counts) and collapses them.
+//
+// step_pos: the previous walk's positions by sorted position, when they are
+// the bodies' (no merge since): the census then reads them, 16 bytes a body
+// in order, instead of the 32-byte bodies by id. A body of zero mass there is
+// one that died (the walk killed it); it is found by its id and set EMPTY in
+// both buffers, and the bodies behind numBodies, EMPTY already, are counted
+// without reading them.
 template <bool kBounds>
 __global__ __launch_bounds__(256) void k_dead_census(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                                      float4* __restrict__ src, float4* __restrict__ dst,
-                                                     float* __restrict__ partials) {
+                                                     float* __restrict__ partials,
+                                                     const float4* __restrict__ step_pos,
+                                                     const uint32_t* __restrict__ index_in) {
     __shared__ uint32_t s_w[4];
     const uint32_t n = sim->initialNumBodies;
     float prev[6], r[6] = {1e38f, 1e38f, 1e38f, -1e38f, -1e38f, -1e38f};
@@ -316,16 +325,32 @@ __global__ __launch_bounds__(256) void k_dead_census(SimValues* __restrict__ sim
         W = bounds_window(prev, prev + 3);
     }
     uint32_t zero = 0;
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
-        const float4 pm = src[2 * b];
-        if constexpr (kBounds) bounds_accum(W, pm, r);
-        if (pm.w == 0.0f) {
-            // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
-            src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-            src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
-            dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            zero++;
+    // bh_dead.comp:99-102: a dead body is EMPTY in both buffers
+    auto empty = [&](uint32_t b) {
+        src[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        src[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[2 * b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    if (step_pos) {
+        const uint32_t num = sim->numBodies;
+        if (blockIdx.x == 0 && threadIdx.x == 0) zero = n - num;
+        for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < num; g += gridDim.x * blockDim.x) {
+            const float4 pm = step_pos[g];
+            if constexpr (kBounds) bounds_accum(W, pm, r);
+            if (pm.w == 0.0f) {
+                empty(index_in[g]);
+                zero++;
+            }
+        }
+    } else {
+        for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+            const float4 pm = src[2 * b];
+            if constexpr (kBounds) bounds_accum(W, pm, r);
+            if (pm.w == 0.0f) {
+                empty(b);
+                zero++;
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) zero += __shfl_xor(zero, off, 64);
@@ -1785,17 +1810,22 @@ void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units
                   float len, float time, float G, float blc) {
     k_reset<<<1, 64, 0, s>>>(sim, ctr, set_units, mass, density, len, time, G, blc);
 }
+// census blocks: at most 1,024, so that the last block's reduction of the
+// bounds partials stays short (16M bodies: 61 us with 1,024 blocks, 65 with
+// 2,048, 93 with 512; 84 with 4,096)
+constexpr uint32_t kCensusBlocks = 1024;
 void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
                            const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch,
-                           float* bounds_partials, uint32_t partial_cap) {
+                           float* bounds_partials, uint32_t partial_cap, const float4* step_pos) {
     uint32_t nb = cdiv(n, kPartTile);
     uint32_t nc = cdiv(n, 256 * 8);
+    if (nc > kCensusBlocks) nc = kCensusBlocks;
     if (nc < 1) nc = 1;
     if (bounds_partials) {
         if (nc > partial_cap) nc = partial_cap;
-        k_dead_census<true><<<nc, 256, 0, s>>>(sim, ctr, src, dst, bounds_partials);
+        k_dead_census<true><<<nc, 256, 0, s>>>(sim, ctr, src, dst, bounds_partials, step_pos, index_in);
     } else {
-        k_dead_census<false><<<nc, 256, 0, s>>>(sim, ctr, src, dst, nullptr);
+        k_dead_census<false><<<nc, 256, 0, s>>>(sim, ctr, src, dst, nullptr, step_pos, index_in);
     }
     k_dead_count<<<nb, 256, 0, s>>>(sim, ctr, src, index_in, block_scratch);
     k_dead_scan<<<1, 1024, 0, s>>>(sim, ctr, block_scratch);

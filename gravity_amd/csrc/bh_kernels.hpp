@@ -219,9 +219,12 @@ void launch_init(hipStream_t s, uint32_t* index_in, uint32_t n);
 void launch_reset(hipStream_t s, SimValues* sim, DevCounters* ctr, int set_units, float mass, float density,
                   float len, float time, float G, float blc);
 // bounds_partials: the census pass also updates the bounds (as launch_bounds)
+// step_pos: the census reads the previous walk's positions by sorted
+// position instead of the bodies (only when no body changed since that walk)
 void launch_dead_partition(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, float4* src, float4* dst,
                            const uint32_t* index_in, uint32_t* index_out, uint32_t* block_scratch,
-                           float* bounds_partials = nullptr, uint32_t partial_cap = 0);
+                           float* bounds_partials = nullptr, uint32_t partial_cap = 0,
+                           const float4* step_pos = nullptr);
 void launch_bounds(hipStream_t s, uint32_t n, SimValues* sim, DevCounters* ctr, const float4* src,
                    const uint32_t* index_in, float* partials, uint32_t partial_cap);
 // step_pos: the bodies by sorted position as the previous step's walk wrote
