@@ -349,7 +349,8 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
         TRY(dalloc(c, &T.xchg, 8 * ((size_t)n - 1)));
-        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n, (n + kTreeBlock - 1) / kTreeBlock)));
+        // (room for the entries of the build with second moments, whose blocks are smaller)
+        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n, (n + kTreeBlockMoments - 1) / kTreeBlockMoments)));
         TRY(dalloc(c, &T.frontier_count, 1));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
@@ -476,7 +477,8 @@ static int ensure_quad(bh_context* c) {
     int rc;
     if ((rc = dalloc(c, &c->node_mom, 2 * ((size_t)c->n - 1)))) return rc;
     if ((rc = dalloc(c, &c->xchg_mom, 6 * ((size_t)c->n - 1)))) return rc;
-    if ((rc = dalloc(c, &c->frontier_mom, 3 * (size_t)frontier_cap(c->n, (c->n + kTreeBlock - 1) / kTreeBlock))))
+    if ((rc = dalloc(c, &c->frontier_mom,
+                     3 * (size_t)frontier_cap(c->n, (c->n + kTreeBlockMoments - 1) / kTreeBlockMoments))))
         return rc;
     if ((rc = dalloc(c, &c->qrec, 4 * corr_records(c->n)))) return rc;
     if ((rc = dalloc(c, &c->acc_mag, c->n))) return rc;

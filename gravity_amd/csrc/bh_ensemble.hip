@@ -376,7 +376,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
         TRY(ealloc(e, &T.node_range, ni));
         TRY(ealloc(e, &T.meet, ni));
         TRY(ealloc(e, &T.xchg, 8 * ni));
-        TRY(ealloc(e, &T.frontier, 4 * (size_t)frontier_cap(N, 2 * e->ntiles)));
+        TRY(ealloc(e, &T.frontier, 4 * (size_t)frontier_cap(N, ens_tree_blocks(e->ntiles, corrected))));
         TRY(ealloc(e, &T.frontier_count, 1));
         TRY(ealloc(e, &e->ctr, 1));
         TRY(ealloc(e, &e->queues, kEnsQueues));
@@ -387,7 +387,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
             // the trees also build every node's second moment
             TRY(ealloc(e, &T.node_mom, 2 * ni));
             TRY(ealloc(e, &T.xchg_mom, 6 * ni));
-            TRY(ealloc(e, &T.frontier_mom, 3 * (size_t)frontier_cap(N, 2 * e->ntiles)));
+            TRY(ealloc(e, &T.frontier_mom, 3 * (size_t)frontier_cap(N, ens_tree_blocks(e->ntiles, true))));
             TRY(ealloc(e, &e->acc_mag, (size_t)N));
             // the quadrupole halves of the leaf records stay zero
             r = hipMemsetAsync(e->qrec, 0, 64 * corr_records(N), e->stream);

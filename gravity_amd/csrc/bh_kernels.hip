@@ -1002,7 +1002,7 @@ __device__ __forceinline__ NodeState leaf_at(const LeafSource& L, const TreeArra
 // records; that is about a third of the nodes of a radix tree.
 //
 // The build runs in two launches. The first (tree_block) builds, per block of
-// kTreeBlock sorted positions, every node inside the block with the exchange
+// tree_block_size sorted positions, every node inside the block with the exchange
 // in LDS, and lists the roots of the subtrees it leaves. The second runs
 // tree_climb from each of them to the root, with the exchange in HBM.
 struct FrontierEntry {
@@ -1125,7 +1125,7 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
 }
 
 // ---- first launch: block-local build.
-// A block of kTreeBlock threads takes kTreeBlock consecutive sorted positions
+// A block of tree_block_size() threads takes as many consecutive sorted positions
 // [blo, bhi] of one segment and builds every internal node whose range lies
 // inside them, with the same agglomeration as tree_climb but with the
 // arrival slots, the records and the split lengths in LDS. A climb stops at a
@@ -1146,11 +1146,12 @@ template <bool kColors, bool kMoments>
 struct TreeBlockLds {
     static constexpr int kMom = kColors ? 4 : 3;   // first moment word of a record
     static constexpr int kRec = kMom + (kMoments ? 3 : 0);   // float4 words of one record
-    int d[kTreeBlock + 1];                         // delta(blo - 1 + j)
-    uint32_t meet[kTreeBlock];                     // arrival slot of split blo + j
-    float4 pm[kTreeBlock];                         // posMass of leaf blo + j
-    float4 col[kColors ? kTreeBlock : 1];          // colour of leaf blo + j
-    float4 rec[kRec * kTreeBlock];                 // summary of internal node blo + j
+    static constexpr uint32_t kB = tree_block_size(kMoments);   // positions per block
+    int d[kB + 1];                                 // delta(blo - 1 + j)
+    uint32_t meet[kB];                             // arrival slot of split blo + j
+    float4 pm[kB];                                 // posMass of leaf blo + j
+    float4 col[kColors ? kB : 1];                  // colour of leaf blo + j
+    float4 rec[kRec * kB];                         // summary of internal node blo + j
     float4 edge[2][kRec];                          // edge nodes: left child at split bhi, right child at blo - 1
     int2 edge_range[2];
     uint32_t nf, base;
@@ -1311,7 +1312,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
 }
 
 template <bool kColors, bool kMoments>
-__global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict__ sim,
+__global__ __launch_bounds__(tree_block_size(kMoments)) void k_tree(const SimValues* __restrict__ sim,
                                                      DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                      const uint32_t* __restrict__ index_in,
                                                      const uint64_t* __restrict__ keys, TreeArrays T,
@@ -1321,7 +1322,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
     __shared__ TreeBlockLds<kColors, kMoments> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
-    const int blo = (int)(blockIdx.x * kTreeBlock);
+    constexpr uint32_t kB = tree_block_size(kMoments);
+    const int blo = (int)(blockIdx.x * kB);
     if (blo >= num) return;
     if (num == 1) {
         if (threadIdx.x != 0) return;
@@ -1357,7 +1359,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree(const SimValues* __restrict
     const bool by_pos = from && step_pos && ctr->died == 0;
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity, by_pos ? step_pos : nullptr, from,
                           ids_out};
-    const int cnt = num - blo < (int)kTreeBlock ? num - blo : (int)kTreeBlock;
+    const int cnt = num - blo < (int)kB ? num - blo : (int)kB;
     tree_block<kColors, kMoments>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
 }
 
@@ -1863,24 +1865,21 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos, const uint32_t* from, uint32_t* ids_out, uint32_t fill_cap) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    const dim3 grid(cdiv(n, kTreeBlock));
+    const uint32_t B = tree_block_size(T.node_mom != nullptr);
+    const dim3 grid(cdiv(n, B));
     if (T.node_mom) {
         if (colors)
-            k_tree<true, true><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
-                                                           fill_cap);
+            k_tree<true, true><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
         else
-            k_tree<false, true><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from,
-                                                            ids_out, fill_cap);
+            k_tree<false, true><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     } else if (colors) {
-        k_tree<true, false><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
-                                                        fill_cap);
+        k_tree<true, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     } else {
-        k_tree<false, false><<<grid, kTreeBlock, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out,
-                                                         fill_cap);
+        k_tree<false, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     }
     if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
     // the second launch meets leaves by their ids in the new order
-    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, kTreeBlock)), 256), 256, 0, s>>>(sim, ctr, src,
+    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, B)), 256), 256, 0, s>>>(sim, ctr, src,
                                                                                   from ? ids_out : index_in, keys,
                                                                                   T, colors, fill_cap);
 }
@@ -2142,7 +2141,7 @@ __global__ __launch_bounds__(256) void k_ens_sort_scatter(const EnsTile* __restr
 // Tree of every simulation: the block-local first launch over two blocks per
 // tile (tiles never span simulations).
 template <bool kMoments>
-__global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restrict__ sims,
+__global__ __launch_bounds__(tree_block_size(kMoments)) void k_ens_tree(const EnsSim* __restrict__ sims,
                                                          const EnsTile* __restrict__ tiles, uint32_t N,
                                                          DevCounters* __restrict__ ctr,
                                                          const float4* __restrict__ src,
@@ -2150,11 +2149,12 @@ __global__ __launch_bounds__(kTreeBlock) void k_ens_tree(const EnsSim* __restric
                                                          const uint64_t* __restrict__ keys, TreeArrays T,
                                                          float G) {
     __shared__ TreeBlockLds<false, kMoments> s;
-    const EnsTile tl = tiles[blockIdx.x >> 1];
-    const uint32_t first = (blockIdx.x & 1u) * kTreeBlock;
+    constexpr uint32_t kB = tree_block_size(kMoments), kPerTile = kEnsTile / kB;
+    const EnsTile tl = tiles[blockIdx.x / kPerTile];
+    const uint32_t first = (blockIdx.x % kPerTile) * kB;
     if (first >= tl.count) return;
     const EnsSim& S = sims[tl.sim];
-    const int cnt = tl.count - first < kTreeBlock ? (int)(tl.count - first) : (int)kTreeBlock;
+    const int cnt = tl.count - first < kB ? (int)(tl.count - first) : (int)kB;
     const TreeSeg seg = {S.off, S.num, S.ibase, N, G};
     const LeafSource L = {src, index_in, 0.0f, 0.0f};
     tree_block<false, kMoments>(s, seg, (int)(tl.start + first - S.off), cnt, keys + S.off, L, T, &ctr->treeHeight,
@@ -2234,12 +2234,12 @@ void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, ui
                      DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
                      const TreeArrays& T, float G) {
     k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
+    const uint32_t B = tree_block_size(T.node_mom != nullptr), blocks = ens_tree_blocks(ntiles, T.node_mom != nullptr);
     if (T.node_mom)
-        k_ens_tree<true><<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
+        k_ens_tree<true><<<blocks, B, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
     else
-        k_ens_tree<false><<<2 * ntiles, kTreeBlock, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
-    k_ens_tree_upper<<<cdiv(frontier_cap(N, 2 * ntiles), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T,
-                                                                               G);
+        k_ens_tree<false><<<blocks, B, 0, s>>>(sims, tiles, N, ctr, src, index_in, keys, T, G);
+    k_ens_tree_upper<<<cdiv(frontier_cap(N, blocks), 256), 256, 0, s>>>(sims, N, ctr, src, index_in, keys, T, G);
 }
 
 void launch_ens_nodes_std430(hipStream_t s, const TreeArrays& T, uint32_t n_internal, NodeStd430* out) {

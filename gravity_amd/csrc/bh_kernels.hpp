@@ -68,6 +68,15 @@ struct TreeArrays {
 // 96-bit keys is at most 98 levels high); the entries are disjoint subtrees,
 // so there are also at most as many as positions.
 constexpr uint32_t kTreeBlock = 1024;
+// The build with second moments carries three more words per record in LDS,
+// so a block of 1,024 positions leaves room for one block per CU: blocks of
+// 512 positions measured faster (16M Plummer: first launch 1.85 against 2.25
+// ms, the second 0.43 against 0.30 ms); the plain build keeps 1,024 (512:
+// the same first launch, a slower second).
+constexpr uint32_t kTreeBlockMoments = 512;
+__host__ __device__ constexpr uint32_t tree_block_size(bool moments) {
+    return moments ? kTreeBlockMoments : kTreeBlock;
+}
 constexpr uint32_t kTreeBlockFrontier = 2 * 98 + 2;
 inline uint32_t frontier_cap(uint32_t n, uint32_t blocks) {
     const uint64_t b = (uint64_t)blocks * kTreeBlockFrontier;
@@ -190,7 +199,11 @@ struct EnsQueue {
 };
 static_assert(sizeof(EnsQueue) == 128, "EnsQueue layout");
 constexpr uint32_t kEnsTile = 2048;    // positions per sort/bounds/encode tile
-static_assert(kEnsTile == 2 * kTreeBlock, "an ensemble tile is two tree blocks");
+static_assert(kEnsTile % kTreeBlock == 0 && kEnsTile % kTreeBlockMoments == 0, "tiles are whole tree blocks");
+// first tree launch blocks over ntiles ensemble tiles
+inline uint32_t ens_tree_blocks(uint32_t ntiles, bool moments) {
+    return ntiles * (kEnsTile / tree_block_size(moments));
+}
 static_assert(kEnsTile == kSortTile, "ensemble tiles are histogrammed as sort tiles");
 
 void launch_ens_reset(hipStream_t s, DevCounters* ctr, EnsQueue* queues);
@@ -202,7 +215,7 @@ void launch_ens_encode(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, 
 int launch_ens_sort_pass(hipStream_t s, int bits, const EnsSim* sims, uint32_t nsims, const EnsTile* tiles,
                          uint32_t ntiles, const uint64_t* keys_in, const uint32_t* idx_in, uint64_t* keys_out,
                          uint32_t* idx_out, uint32_t shift, uint32_t* hist);
-// tree: two blocks of kTreeBlock positions per tile
+// tree: whole blocks of tree_block_size positions per tile
 void launch_ens_tree(hipStream_t s, const EnsSim* sims, const EnsTile* tiles, uint32_t ntiles, uint32_t N,
                      DevCounters* ctr, const float4* src, const uint32_t* index_in, const uint64_t* keys,
                      const TreeArrays& T, float G);
