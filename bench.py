@@ -175,19 +175,22 @@ def timed_loop(bh, prm, steps: int, warmup: int, stage_steps: int = 5) -> dict:
     wall = time.perf_counter() - t0
     c = bh.counters()
     num = int(c["num_bodies"])
-    timed = B.Params.from_buffer_copy(prm)
-    timed.flags |= B.FLAG_TIMING
-    bh.stage_times()                      # clear the event ring
-    bh.sync()
-    t1 = time.perf_counter()
-    for _ in range(stage_steps):
-        bh.step(timed)
-    bh.sync()
-    wall_timed = time.perf_counter() - t1
-    stage = bh.stage_times()
-    return {"wall_s": wall, "ms_per_step": wall * 1e3 / steps, "body_steps_per_s": num * steps / wall,
-            "stage_ms": {k: v / stage_steps for k, v in stage.items()},
-            "ms_per_step_with_stage_events": wall_timed * 1e3 / stage_steps, "counters": c, "num": num}
+    out = {"wall_s": wall, "ms_per_step": wall * 1e3 / steps, "body_steps_per_s": num * steps / wall,
+           "counters": c, "num": num}
+    if stage_steps:
+        timed = B.Params.from_buffer_copy(prm)
+        timed.flags = (timed.flags | B.FLAG_TIMING) & ~B.FLAG_GRAPH
+        bh.stage_times()                      # clear the event ring
+        bh.sync()
+        t1 = time.perf_counter()
+        for _ in range(stage_steps):
+            bh.step(timed)
+        bh.sync()
+        wall_timed = time.perf_counter() - t1
+        stage = bh.stage_times()
+        out["stage_ms"] = {k: v / stage_steps for k, v in stage.items()}
+        out["ms_per_step_with_stage_events"] = wall_timed * 1e3 / stage_steps
+    return out
 
 
 def train_network(soft: float, theta: float, tolerance: float, seed: int) -> tuple:
@@ -300,9 +303,16 @@ def other_workload(name: str, steps: int, seed: int) -> dict:
                alive_bodies=r["num"], stage_ms=r["stage_ms"], p50=err["p50"], p99=err["p99"],
                error_sample=err["sample"], error_flags=int(c["error_flags"]), tree_height=int(c["tree_height"]),
                merges_last_step=int(c["merges_applied"]))
+    # the same steps replayed as captured hipGraphs (BH_FLAG_GRAPH): one launch
+    # per step instead of ~30, what bounds the smaller workloads
+    g = B.Params.from_buffer_copy(prm)
+    g.flags |= B.FLAG_GRAPH
+    rg = timed_loop(bh, g, steps, 2, stage_steps=0)
+    out["graph"] = {"ms_per_step": rg["ms_per_step"], "body_steps_per_s": rg["body_steps_per_s"],
+                    "alive_bodies": rg["num"]}
     bh.close()
     log(f"{name}: {r['ms_per_step']:.2f} ms/step, {r['body_steps_per_s']:.3g} body-steps/s, "
-        f"p50 {err['p50']:.3g} p99 {err['p99']:.3g}")
+        f"p50 {err['p50']:.3g} p99 {err['p99']:.3g}; as graphs {rg['ms_per_step']:.2f} ms/step")
     return out
 
 
