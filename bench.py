@@ -293,21 +293,25 @@ def other_workload(name: str, steps: int, seed: int) -> dict:
         desc = {"n_bodies": int(bodies.shape[0]), "theta": 0.5, "dt": dt, "softening": soft, "merging": "off",
                 "steps": steps}
     log(f"{name}: {bodies.shape[0]} bodies, {steps} steps")
-    bh = B.BarnesHut(bodies, ic.tight_bounds(bodies), prm)
-    del bodies
+    bounds = ic.tight_bounds(bodies)
+    bh = B.BarnesHut(bodies, bounds, prm)
     bh.init()
-    r = timed_loop(bh, prm, steps, 1)
+    r = timed_loop(bh, prm, steps, 3)
     err = force_error(bh, soft)
     c = r["counters"]
     out = dict(desc, ms_per_step=r["ms_per_step"], body_steps_per_s=r["body_steps_per_s"],
                alive_bodies=r["num"], stage_ms=r["stage_ms"], p50=err["p50"], p99=err["p99"],
                error_sample=err["sample"], error_flags=int(c["error_flags"]), tree_height=int(c["tree_height"]),
                merges_last_step=int(c["merges_applied"]))
-    # the same steps replayed as captured hipGraphs (BH_FLAG_GRAPH): one launch
-    # per step instead of ~30, what bounds the smaller workloads
+    bh.close()
+    # the same steps from the same bodies, replayed as captured hipGraphs
+    # (BH_FLAG_GRAPH: one launch per step instead of ~30)
     g = B.Params.from_buffer_copy(prm)
     g.flags |= B.FLAG_GRAPH
-    rg = timed_loop(bh, g, steps, 2, stage_steps=0)
+    bh = B.BarnesHut(bodies, bounds, prm)
+    del bodies
+    bh.init()
+    rg = timed_loop(bh, g, steps, 3, stage_steps=0)   # (one direct step, then the two graphs captured)
     out["graph"] = {"ms_per_step": rg["ms_per_step"], "body_steps_per_s": rg["body_steps_per_s"],
                     "alive_bodies": rg["num"]}
     bh.close()
