@@ -748,6 +748,7 @@ int build_tree(bh_context* c, const bh_params* p, bool positions, const float4* 
         T.node_mom = c->node_mom;
         T.xchg_mom = c->xchg_mom;
         T.frontier_mom = c->frontier_mom;
+        T.corr_leaf = c->qrec;
     }
     launch_tree(c->stream, c->n, c->sim, c->ctr, c->bodies[c->body_in], c->index[c->mi_in], c->morton[c->mi_in],
                 T, c->colors ? 1 : 0, positions ? step_pos : nullptr,

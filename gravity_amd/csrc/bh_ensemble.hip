@@ -388,6 +388,7 @@ int bh_ensemble_create(int device, uint32_t n_sims, const uint32_t* counts, cons
             TRY(ealloc(e, &T.node_mom, 2 * ni));
             TRY(ealloc(e, &T.xchg_mom, 6 * ni));
             TRY(ealloc(e, &T.frontier_mom, 3 * (size_t)frontier_cap(N, ens_tree_blocks(e->ntiles, true))));
+            T.corr_leaf = e->qrec;
             TRY(ealloc(e, &e->acc_mag, (size_t)N));
             // the quadrupole halves of the leaf records stay zero
             r = hipMemsetAsync(e->qrec, 0, 64 * corr_records(N), e->stream);

@@ -1206,7 +1206,12 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
         uint32_t b;
         const float4 pm = leaf_body(L, g, b);
         T.leaf_pos[g] = pm;
-        T.walk[g] = leaf_record(pm, S.G);
+        const WalkNode lr = leaf_record(pm, S.G);
+        T.walk[g] = lr;
+        if (kMoments) {   // the corrected walk's leaf record: position and G m, b^2 = K = -1
+            T.corr_leaf[4 * (size_t)g] = lr.comMass;
+            T.corr_leaf[4 * (size_t)g + 1] = make_float4(-1.0f, __uint_as_float(0u), __uint_as_float(0u), lr.half2);
+        }
         s.pm[t] = pm;
         if (kColors) {
             const float4 vd = L.src[2 * b + 1];

@@ -34,8 +34,8 @@ struct RecScope {
     __device__ __forceinline__ float grav() const { return sim ? sim->gravitationalConstant : G; }
 };
 
-// The corrected walk's records (layout with corr_records): the first half of
-// every leaf's record (its walk record), and the whole record of every
+// The corrected walk's records (layout with corr_records; the tree's first
+// launch wrote the first half of every leaf's): the whole record of every
 // internal node, T = M2 + M h^2 o with o the network's output for the node's
 // features (o = 0 without a network), children renumbered. Each wave takes 64
 // consecutive nodes per turn; the grid strides over the nodes so that every
@@ -48,17 +48,11 @@ __global__ __launch_bounds__(256) void k_qnodes(const RecScope sc, DevCounters* 
     __shared__ __attribute__((aligned(16))) __bf16 s_x[4][64 * kMlpXStride];
     __shared__ __attribute__((aligned(16))) float s_y[4][64 * kMlpYStride];
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t num = sc.alive();
     const uint32_t n = sc.n_ids();
     const uint32_t ninternal = sc.internal();
     const float G = sc.grav();
     const WalkNode* __restrict__ nodes = walk + n;
     const uint32_t waves = gridDim.x * 4;
-    for (uint32_t j = (blockIdx.x * 4 + w) * 64 + lane; j < num; j += waves * 64) {
-        const WalkNode lf = walk[j];
-        qrec[4 * (size_t)j] = lf.comMass;
-        qrec[4 * (size_t)j + 1] = make_float4(-1.0f, __uint_as_float(0u), __uint_as_float(0u), lf.half2);
-    }
     MlpRegs R;
     if (wg) mlp_load(R, wg, bg);
     uint32_t evals = 0;
