@@ -526,6 +526,11 @@ def main() -> int:
                     help="only the headline's timed loop: the learned-corrected step with the relative opening "
                          "test at tolerance ALPHA and an untrained (zero-output) network, which runs the same "
                          "kernels; for rocprofv3 profiles")
+    ap.add_argument("--plain-step", type=int, default=None, metavar="N",
+                    help="only the plain theta 0.5 step's timed loop on an N-body Plummer sphere; for rocprofv3 "
+                         "profiles")
+    ap.add_argument("--ensemble-step", action="store_true",
+                    help="only the ensemble's plain leg (batched and sequential); for rocprofv3 profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -542,6 +547,12 @@ def main() -> int:
         return 0
 
     n, soft, dt = 16_777_216, 0.01, 1e-3
+    if args.ensemble_step:
+        print(json.dumps({"ensemble_step": ensemble_leg(args.ensemble_sims, args.steps, soft, 0.5, dt, args.seed)}),
+              flush=True)
+        return 0
+    if args.plain_step is not None:
+        n = args.plain_step
     log(f"generating Plummer N={n}")
     bodies = ic.plummer(n, args.seed)
     bounds = ic.tight_bounds(bodies)
@@ -565,6 +576,18 @@ def main() -> int:
         Lr.attach(bh, None)
         bh.close()
         mlp.close()
+        return 0
+    if args.plain_step is not None:
+        run = timed_loop(bh, plain_ref, args.steps, args.warmup)
+        c = run["counters"]
+        walk_ms = run["stage_ms"]["force"]
+        print(json.dumps({"plain_step": n, "ms_per_step": run["ms_per_step"],
+                          "body_steps_per_s": run["body_steps_per_s"], "stage_ms": run["stage_ms"],
+                          "walk_ms": walk_ms,
+                          "walk_tflops": c["interactions"] * FLOP_PER_INTERACTION / (walk_ms * 1e-3) / 1e12,
+                          "interactions_per_body": c["interactions"] / max(run["num"], 1),
+                          "tree_height": int(c["tree_height"])}), flush=True)
+        bh.close()
         return 0
 
     # 1. the reference's configuration: plain, theta 0.5
