@@ -681,7 +681,8 @@ def main() -> int:
     }
     copy = hbm_copy_gbps()
     sm = ref["stage_ms"]
-    alg = {"sort": 256, "tree": 172, "morton": 24, "dead": 16}
+    # algorithmic bytes per body: one-sweep sort 8 + 8 x 24, tree 172, encode 24, census 16 + index copy 8
+    alg = {"sort": 200, "tree": 172, "morton": 24, "dead": 24}
     result["hbm_stages_plain_theta_0_5"] = {
         "copy_GBps": copy, "peak_GBps": HBM_PEAK_GBS,
         **{k: {"ms": sm[k], "alg_bytes_per_body": b, "alg_GBps": b * n / (sm[k] * 1e-3) / 1e9,
