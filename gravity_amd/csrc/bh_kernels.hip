@@ -1142,10 +1142,13 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
 // right child.
 constexpr uint32_t kSideRight = 0x80000000u;
 
+// A record of the first launch's exchange: com, box, height and id, colour
+// (3 or 4 words); with moments the id is left out (the reader knows the
+// node's Karras index) and the 9 moment floats fill the words to 5 (6 with
+// colours), so that three blocks of 512 positions fit a CU's LDS.
 template <bool kColors, bool kMoments>
 struct TreeBlockLds {
-    static constexpr int kMom = kColors ? 4 : 3;   // first moment word of a record
-    static constexpr int kRec = kMom + (kMoments ? 3 : 0);   // float4 words of one record
+    static constexpr int kRec = kMoments ? (kColors ? 6 : 5) : (kColors ? 4 : 3);   // float4 words of one record
     static constexpr uint32_t kB = tree_block_size(kMoments);   // positions per block
     int d[kB + 1];                                 // delta(blo - 1 + j)
     uint32_t meet[kB];                             // arrival slot of split blo + j
@@ -1154,6 +1157,7 @@ struct TreeBlockLds {
     float4 rec[kRec * kB];                         // summary of internal node blo + j
     float4 edge[2][kRec];                          // edge nodes: left child at split bhi, right child at blo - 1
     int2 edge_range[2];
+    uint32_t edge_id[2];
     uint32_t nf, base;
 };
 
@@ -1161,22 +1165,36 @@ template <bool kColors, bool kMoments>
 __device__ __forceinline__ void lds_put(float4* x, const NodeState& s) {
     x[0] = s.com;
     x[1] = make_float4(s.mn.x, s.mn.y, s.mn.z, s.mx.x);
-    x[2] = make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id));
-    if (kColors) x[3] = s.col;
-    if (kMoments) moment_words(s, x + TreeBlockLds<kColors, kMoments>::kMom);
+    if (kMoments) {
+        x[2] = make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), s.m2[0]);
+        x[3] = make_float4(s.m2[1], s.m2[2], s.m2[3], s.m2[4]);
+        x[4] = make_float4(s.m2[5], s.dm[0], s.dm[1], s.dm[2]);
+        if (kColors) x[5] = s.col;
+    } else {
+        x[2] = make_float4(s.mx.y, s.mx.z, __uint_as_float(s.height), __uint_as_float(s.id));
+        if (kColors) x[3] = s.col;
+    }
 }
 
+// id: the node's global id (a record with moments does not hold it)
 template <bool kColors, bool kMoments>
-__device__ __forceinline__ NodeState lds_get(const float4* x) {
+__device__ __forceinline__ NodeState lds_get(const float4* x, uint32_t id) {
     NodeState s;
     s.com = x[0];
     const float4 a = x[1], b = x[2];
     s.mn = make_float4(a.x, a.y, a.z, 0.f);
     s.mx = make_float4(a.w, b.x, b.y, 0.f);
     s.height = __float_as_uint(b.z);
-    s.id = __float_as_uint(b.w);
-    s.col = kColors ? x[3] : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (kMoments) moments_from_words(x + TreeBlockLds<kColors, kMoments>::kMom, s);
+    if (kMoments) {
+        const float4 c = x[3], d = x[4];
+        s.id = id;
+        s.m2[0] = b.w; s.m2[1] = c.x; s.m2[2] = c.y; s.m2[3] = c.z; s.m2[4] = c.w; s.m2[5] = d.x;
+        s.dm[0] = d.y; s.dm[1] = d.z; s.dm[2] = d.w;
+        s.col = kColors ? x[5] : make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        s.id = __float_as_uint(b.w);
+        s.col = kColors ? x[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     return s;
 }
 
@@ -1242,6 +1260,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 const int side = left ? 0 : 1;
                 lds_put<kColors, kMoments>(s.edge[side], me);
                 s.edge_range[side] = make_int2(lo, hi);
+                s.edge_id[side] = me.id;
                 break;
             }
             const bool sib_leaf = left ? (split + 1 == num - 1 || D(split + 1) <= D(split))
@@ -1266,7 +1285,8 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 s.meet[split - blo] = kEmpty;              // only this thread touches the slot now
                 other = (int)(old & ~kSideRight);
-                sib = lds_get<kColors, kMoments>(&s.rec[R * ((left ? split + 1 : split) - blo)]);
+                const int sk = left ? split + 1 : split;   // the sibling's Karras index
+                sib = lds_get<kColors, kMoments>(&s.rec[R * (sk - blo)], S.n_ids + S.ibase + (uint32_t)sk);
             }
             const int plo = left ? lo : other;
             const int phi = left ? other : hi;
@@ -1294,13 +1314,14 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
             const int split = blo + t, far = (int)(v & ~kSideRight);
             const bool right = (v & kSideRight) != 0;
             const int lo = right ? split + 1 : far, hi = right ? far : split;
-            n0 = lds_get<kColors, kMoments>(&s.rec[R * ((right ? lo : hi) - blo)]);
+            const int k0 = right ? lo : hi;   // its Karras index
+            n0 = lds_get<kColors, kMoments>(&s.rec[R * (k0 - blo)], S.n_ids + S.ibase + (uint32_t)k0);
             e0 = frontier_entry(n0, lo, hi, segment);
             slot0 = (int)atomicAdd(&s.nf, 1u);
         }
     }
     if (t < 2 && s.edge_range[t].x >= 0) {
-        n1 = lds_get<kColors, kMoments>(s.edge[t]);
+        n1 = lds_get<kColors, kMoments>(s.edge[t], s.edge_id[t]);
         e1 = frontier_entry(n1, s.edge_range[t].x, s.edge_range[t].y, segment);
         slot1 = (int)atomicAdd(&s.nf, 1u);
     }
