@@ -349,8 +349,10 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &T.node_color, (size_t)n - 1));
         TRY(dalloc(c, &T.meet, (size_t)n - 1));
         TRY(dalloc(c, &T.xchg, 8 * ((size_t)n - 1)));
-        // (room for the entries of the build with second moments, whose blocks are smaller)
-        TRY(dalloc(c, &T.frontier, 4 * (size_t)frontier_cap(n, (n + kTreeBlockMoments - 1) / kTreeBlockMoments)));
+        // the entry slots and counts of every build level, for the larger of
+        // the two plans (the build with second moments has smaller blocks)
+        TRY(dalloc(c, &T.frontier, 4 * frontier_slots(n)));
+        TRY(dalloc(c, &T.frontier_cnt, frontier_counts(n)));
         TRY(dalloc(c, &T.frontier_count, 1));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
@@ -417,7 +419,7 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.xchg, c->tree.frontier, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_totals, c->sort_status, c->sort_state,
+                    c->tree.xchg, c->tree.frontier, c->tree.frontier_cnt, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_totals, c->sort_status, c->sort_state,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
                     c->sort_pos[1], c->walk_queues, c->qrec, c->node_mom, c->xchg_mom, c->frontier_mom, c->acc_mag,
                     c->feed_counts};
@@ -477,9 +479,7 @@ static int ensure_quad(bh_context* c) {
     int rc;
     if ((rc = dalloc(c, &c->node_mom, 2 * ((size_t)c->n - 1)))) return rc;
     if ((rc = dalloc(c, &c->xchg_mom, 6 * ((size_t)c->n - 1)))) return rc;
-    if ((rc = dalloc(c, &c->frontier_mom,
-                     3 * (size_t)frontier_cap(c->n, (c->n + kTreeBlockMoments - 1) / kTreeBlockMoments))))
-        return rc;
+    if ((rc = dalloc(c, &c->frontier_mom, 3 * tree_levels(c->n, kTreeBlockMoments).entries))) return rc;
     if ((rc = dalloc(c, &c->qrec, 4 * corr_records(c->n)))) return rc;
     if ((rc = dalloc(c, &c->acc_mag, c->n))) return rc;
     // the quadrupole halves of the leaf records stay zero

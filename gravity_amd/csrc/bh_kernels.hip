@@ -1158,6 +1158,7 @@ struct TreeBlockLds {
     float4 edge[2][kRec];                          // edge nodes: left child at split bhi, right child at blo - 1
     int2 edge_range[2];
     uint32_t edge_id[2];
+    uint32_t leaf_used[kB / 32];                   // bit j: leaf blo + j is a child of a node built here
     uint32_t nf, base;
 };
 
@@ -1207,14 +1208,26 @@ __device__ __forceinline__ FrontierEntry frontier_entry(const NodeState& me, int
     return e;
 }
 
+// Rank of key among the first nf of list (distinct keys): the sorted order
+// of a block's few frontier entries.
+__device__ __forceinline__ uint32_t rank_in(const int* list, uint32_t nf, int key) {
+    uint32_t r = 0;
+    for (uint32_t k = 0; k < nf; k++) r += list[k] < key;
+    return r;
+}
+
 // Positions [blo, blo + cnt) of segment S (local positions; keys given from
 // the segment's first key); thread t holds leaf blo + t. Called by every
-// thread of the block (it has barriers).
+// thread of the block (it has barriers). With T.frontier_cnt set, the
+// entries go, sorted by their first position, to slots
+// [out_block kTreeBlockFrontier, ...), and a leaf whose parent is not built
+// here is an entry too (a mid level needs every position covered).
 template <bool kColors, bool kMoments>
 __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, const TreeSeg& S, int blo, int cnt,
                                            const uint64_t* __restrict__ keys, const LeafSource& L,
                                            const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
-                                           uint32_t segment) {
+                                           uint32_t segment, uint32_t out_block = 0,
+                                           uint32_t* __restrict__ error_flags = nullptr) {
     constexpr int R = TreeBlockLds<kColors, kMoments>::kRec;
     const int num = (int)S.num;
     const int t = threadIdx.x;
@@ -1246,8 +1259,12 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
         s.edge_range[0] = s.edge_range[1] = make_int2(-1, -1);
     }
     s.meet[t] = kEmpty;
+    const bool sorted_out = T.frontier_cnt != nullptr;
+    if (t < (int)(TreeBlockLds<kColors, kMoments>::kB / 32)) s.leaf_used[t] = 0u;
+    bool leaf_edge = false;   // this thread's leaf is an edge node
     __syncthreads();
     auto D = [&](int x) { return s.d[x - blo + 1]; };   // x in [blo - 1, bhi]
+    auto use_leaf = [&](int x) { atomicOr(&s.leaf_used[(x - blo) >> 5], 1u << ((x - blo) & 31)); };
     if (t < cnt) {
         int lo = blo + t, hi = lo;
         NodeState me = leaf_state(s.pm[t], S.off + (uint32_t)lo);
@@ -1261,6 +1278,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 lds_put<kColors, kMoments>(s.edge[side], me);
                 s.edge_range[side] = make_int2(lo, hi);
                 s.edge_id[side] = me.id;
+                leaf_edge = lo == hi;
                 break;
             }
             const bool sib_leaf = left ? (split + 1 == num - 1 || D(split + 1) <= D(split))
@@ -1272,6 +1290,10 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 other = left ? split + 1 : split;
                 sib = leaf_state(s.pm[other - blo], S.off + (uint32_t)other);
                 if (kColors) sib.col = s.col[other - blo];
+                if (sorted_out) {
+                    use_leaf(other);
+                    if (lo == hi) use_leaf(lo);
+                }
             } else {
                 if (lo == hi) break;                       // the internal sibling's thread builds the parent
                 // the exchange orders LDS only: the fences are for the local
@@ -1304,10 +1326,11 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
         }
     }
     __syncthreads();
-    // first arrivals left waiting, and the edge nodes
-    FrontierEntry e0, e1;
+    // first arrivals left waiting, the edge nodes and (sorted output) the
+    // leaves no parent here took
+    FrontierEntry e0, e1, e2;
     NodeState n0, n1;   // (their moments)
-    int slot0 = -1, slot1 = -1;
+    int slot0 = -1, slot1 = -1, slot2 = -1;
     if (t < cnt - 1) {
         const uint32_t v = s.meet[t];
         if (v != kEmpty) {
@@ -1325,15 +1348,47 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
         e1 = frontier_entry(n1, s.edge_range[t].x, s.edge_range[t].y, segment);
         slot1 = (int)atomicAdd(&s.nf, 1u);
     }
+    if (sorted_out && t < cnt && !leaf_edge && !((s.leaf_used[t >> 5] >> (t & 31)) & 1u)) {
+        NodeState lf = leaf_state(s.pm[t], S.off + (uint32_t)(blo + t));
+        if (kColors) lf.col = s.col[t];
+        e2 = frontier_entry(lf, blo + t, blo + t, segment);
+        slot2 = (int)atomicAdd(&s.nf, 1u);
+    }
     __syncthreads();
-    if (t == 0 && s.nf) s.base = atomicAdd(T.frontier_count, s.nf);
-    __syncthreads();
+    if (sorted_out) {
+        // the entries in order of their first position (s.d is free now)
+        const uint32_t nf = s.nf;
+        if (slot0 >= 0) s.d[slot0] = (int)e0.range.x;
+        if (slot1 >= 0) s.d[slot1] = (int)e1.range.x;
+        if (slot2 >= 0) s.d[slot2] = (int)e2.range.x;
+        __syncthreads();
+        if (slot0 >= 0) slot0 = (int)rank_in(s.d, nf, (int)e0.range.x);
+        if (slot1 >= 0) slot1 = (int)rank_in(s.d, nf, (int)e1.range.x);
+        if (slot2 >= 0) slot2 = (int)rank_in(s.d, nf, (int)e2.range.x);
+        if (nf > kTreeBlockFrontier) {   // (the bound above: not reached)
+            if (t == 0 && error_flags) atomicOr(error_flags, 16u);
+            slot0 = slot0 < (int)kTreeBlockFrontier ? slot0 : -1;
+            slot1 = slot1 < (int)kTreeBlockFrontier ? slot1 : -1;
+            slot2 = slot2 < (int)kTreeBlockFrontier ? slot2 : -1;
+        }
+        if (t == 0) T.frontier_cnt[out_block] = nf < kTreeBlockFrontier ? nf : kTreeBlockFrontier;
+        if (t == 0) s.base = out_block * kTreeBlockFrontier;
+        __syncthreads();
+    } else {
+        if (t == 0 && s.nf) s.base = atomicAdd(T.frontier_count, s.nf);
+        __syncthreads();
+    }
     FrontierEntry* F = reinterpret_cast<FrontierEntry*>(T.frontier);
     if (slot0 >= 0) F[s.base + slot0] = e0;
     if (slot1 >= 0) F[s.base + slot1] = e1;
+    if (slot2 >= 0) F[s.base + slot2] = e2;
     if (kMoments) {
         if (slot0 >= 0) moment_words(n0, T.frontier_mom + 3 * (size_t)(s.base + slot0));
         if (slot1 >= 0) moment_words(n1, T.frontier_mom + 3 * (size_t)(s.base + slot1));
+        if (slot2 >= 0) {   // a leaf's moments about itself are zero
+            float4* w = T.frontier_mom + 3 * (size_t)(s.base + slot2);
+            w[0] = w[1] = w[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
 }
 
@@ -1350,7 +1405,10 @@ __global__ __launch_bounds__(tree_block_size(kMoments)) void k_tree(const SimVal
     const int num = (int)sim->numBodies;
     constexpr uint32_t kB = tree_block_size(kMoments);
     const int blo = (int)(blockIdx.x * kB);
-    if (blo >= num) return;
+    if (blo >= num || num == 1) {   // (no entries: the levels above read block 0's count)
+        if (T.frontier_cnt && blockIdx.x == 0 && threadIdx.x == 0) T.frontier_cnt[0] = 0u;
+        if (blo >= num) return;
+    }
     if (num == 1) {
         if (threadIdx.x != 0) return;
         // bh_tree.comp:46-62: a one-body tree is an empty root over leaf 0
@@ -1386,7 +1444,8 @@ __global__ __launch_bounds__(tree_block_size(kMoments)) void k_tree(const SimVal
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity, by_pos ? step_pos : nullptr, from,
                           ids_out};
     const int cnt = num - blo < (int)kB ? num - blo : (int)kB;
-    tree_block<kColors, kMoments>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u);
+    tree_block<kColors, kMoments>(s, S, blo, cnt, keys, L, T, &ctr->treeHeight, false, 0u, blockIdx.x,
+                                  &ctr->errorFlags);
 }
 
 // (the colour of a frontier node, when colours are on, is in node_color)
@@ -1427,6 +1486,194 @@ __global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict_
     const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant, fill_cap};
     const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
     tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors, moments);
+}
+
+// ---- the levels between: the entries of kTreeMidGroup blocks as leaves.
+// Entry u of a window covers positions [lo_u, hi_u], hi_u + 1 = lo_{u+1}, so
+// the split between entries u and u + 1 is the split at position hi_u and a
+// node over entries [cl, ch] is the node over positions [lo_cl, hi_ch]: the
+// same agglomeration as tree_block with d[u + 1] = delta(hi_u) and
+// d[0] = delta(lo_0 - 1), every entry meeting its sibling through the arrival
+// slots (an entry may be a leaf or a subtree; either one waits for its sibling
+// here). A node's id follows from its position range: the Karras index of a
+// left child is its last position, of a right child its first, and a node
+// over one position is that leaf.
+struct TreeMidArgs {
+    const float4* in;         // the level below: 4 words per entry, in_cap slots per block
+    const float4* in_mom;     // 3 moment words per entry (moments only)
+    const uint32_t* in_cnt;   // entries per block of the level below
+    float4* out;
+    float4* out_mom;
+    uint32_t* out_cnt;
+    uint32_t in_cap, out_cap;
+    uint64_t in_span;         // sorted positions per block of the level below
+};
+
+template <bool kColors, bool kMoments>
+struct TreeMidLds {
+    static constexpr int kRec = TreeBlockLds<kColors, kMoments>::kRec;
+    int d[kTreeMidUnits + 1];               // d[u + 1] = delta(hi_u); after the climb the entries' first positions
+    uint32_t meet[kTreeMidUnits];           // arrival slot of the split after entry u
+    int2 ur[kTreeMidUnits];                 // positions [lo_u, hi_u] of entry u
+    float4 rec[kRec * kTreeMidUnits];       // entry u's summary, then the node with compressed Karras index u
+    float4 edge[2][kRec];
+    int2 edge_range[2];
+    uint32_t edge_id[2];
+    uint32_t pre[kTreeMidGroup + 1];        // exclusive prefix of the group's entry counts
+    uint32_t nf;
+};
+
+template <bool kColors, bool kMoments>
+__global__ __launch_bounds__(kTreeMidUnits) void k_tree_mid(const SimValues* __restrict__ sim,
+                                                            DevCounters* __restrict__ ctr,
+                                                            const uint64_t* __restrict__ keys, TreeArrays T,
+                                                            TreeMidArgs A, uint32_t fill_cap) {
+    __shared__ TreeMidLds<kColors, kMoments> s;
+    constexpr int R = TreeMidLds<kColors, kMoments>::kRec;
+    const uint32_t n = sim->initialNumBodies;
+    const int num = (int)sim->numBodies;
+    const int t = threadIdx.x;
+    const uint32_t nb_in = num > 1 ? (uint32_t)(((uint64_t)num + A.in_span - 1) / A.in_span) : 0u;
+    const uint32_t g0 = blockIdx.x * kTreeMidGroup;
+    if (g0 >= nb_in) {
+        if (blockIdx.x == 0 && t == 0) A.out_cnt[0] = 0u;
+        return;
+    }
+    const uint32_t ng = nb_in - g0 < kTreeMidGroup ? nb_in - g0 : kTreeMidGroup;
+    if (t < 64) {   // one wave scans the group's counts
+        const uint32_t c = (uint32_t)t < ng ? A.in_cnt[g0 + t] : 0u;
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (t >= o) x += y;
+        }
+        s.pre[t + 1] = x;
+        if (t == 0) s.pre[0] = 0u;
+    }
+    __syncthreads();
+    const uint32_t U = s.pre[ng];
+    const TreeSeg S = {0u, (uint32_t)num, 0u, n, sim->gravitationalConstant, fill_cap};
+    uint32_t out_n = 0;   // entries this block has written (the same in every thread)
+    const FrontierEntry* Fin = reinterpret_cast<const FrontierEntry*>(A.in);
+    FrontierEntry* Fout = reinterpret_cast<FrontierEntry*>(A.out) + (size_t)blockIdx.x * A.out_cap;
+    for (uint32_t w0 = 0; w0 < U; w0 += kTreeMidUnits) {
+        const int W = (int)(U - w0 < kTreeMidUnits ? U - w0 : kTreeMidUnits);
+        NodeState me;
+        if (t < W) {
+            const uint32_t u = w0 + (uint32_t)t;
+            int b = 0, bh = (int)ng - 1;   // the block holding entry u: the last b with pre[b] <= u
+            while (b < bh) {
+                const int mid = (b + bh + 1) >> 1;
+                if (s.pre[mid] <= u) b = mid; else bh = mid - 1;
+            }
+            const size_t slot = (size_t)(g0 + (uint32_t)b) * A.in_cap + (u - s.pre[b]);
+            const FrontierEntry e = Fin[slot];
+            me = frontier_state(e);
+            if (kColors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
+            if (kMoments) {
+                float4 w[3];
+                for (int k = 0; k < 3; k++) w[k] = A.in_mom[3 * slot + k];
+                moments_from_words(w, me);
+            }
+            s.ur[t] = make_int2((int)e.range.x, (int)e.range.y);
+            s.d[t + 1] = delta_at(keys, num, (int)e.range.y);
+            if (t == 0) s.d[0] = delta_at(keys, num, (int)e.range.x - 1);
+        }
+        s.meet[t] = kEmpty;
+        if (t < 2) s.edge_range[t] = make_int2(-1, -1);
+        if (t == 0) s.nf = 0u;
+        __syncthreads();
+        if (t < W) {
+            int cl = t, ch = t;
+            int lo = s.ur[t].x, hi = s.ur[t].y;
+            for (;;) {
+                if (lo == 0 && hi == num - 1) break;
+                const bool left = (lo == 0) || (hi != num - 1 && s.d[ch + 1] > s.d[cl]);
+                const int split = left ? ch : cl - 1;
+                if (split < 0 || split >= W - 1) {   // the parent takes entries of another window
+                    const int side = left ? 0 : 1;
+                    lds_put<kColors, kMoments>(s.edge[side], me);
+                    s.edge_range[side] = make_int2(lo, hi);
+                    s.edge_id[side] = me.id;
+                    break;
+                }
+                lds_put<kColors, kMoments>(&s.rec[R * (left ? ch : cl)], me);
+                const uint32_t v = left ? (uint32_t)cl : ((uint32_t)ch | kSideRight);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                const uint32_t old =
+                    __hip_atomic_exchange(&s.meet[split], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old == kEmpty) break;            // first to arrive
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                s.meet[split] = kEmpty;              // only this thread touches the slot now
+                const int other = (int)(old & ~kSideRight);
+                const int slo = left ? hi + 1 : s.ur[other].x;   // the sibling's positions
+                const int shi = left ? s.ur[other].y : lo - 1;
+                const uint32_t sid = slo == shi ? S.off + (uint32_t)slo
+                                                : S.n_ids + S.ibase + (uint32_t)(left ? slo : shi);
+                const NodeState sib = lds_get<kColors, kMoments>(&s.rec[R * (left ? split + 1 : split)], sid);
+                const int pcl = left ? cl : other, pch = left ? other : ch;
+                const int plo = left ? lo : slo, phi = left ? shi : hi;
+                uint32_t pid;
+                if (plo == 0 && phi == num - 1) {
+                    pid = 0;
+                } else {
+                    const bool pl = (plo == 0) || (phi != num - 1 && s.d[pch + 1] > s.d[pcl]);
+                    pid = (uint32_t)(pl ? phi : plo);
+                }
+                me = left ? tree_parent(me, sib, plo, phi, pid, S, T, &ctr->treeHeight, false, kColors, kMoments)
+                          : tree_parent(sib, me, plo, phi, pid, S, T, &ctr->treeHeight, false, kColors, kMoments);
+                cl = pcl;
+                ch = pch;
+                lo = plo;
+                hi = phi;
+            }
+        }
+        __syncthreads();
+        // what this window leaves: first arrivals still waiting, and the edges
+        FrontierEntry e0, e1;
+        NodeState n0, n1;
+        int slot0 = -1, slot1 = -1;
+        if (t < W - 1) {
+            const uint32_t v = s.meet[t];
+            if (v != kEmpty) {
+                const bool right = (v & kSideRight) != 0;
+                const int far = (int)(v & ~kSideRight);
+                const int cl = right ? t + 1 : far, ch = right ? far : t;
+                const int lo = s.ur[cl].x, hi = s.ur[ch].y;
+                const uint32_t id = lo == hi ? S.off + (uint32_t)lo : S.n_ids + S.ibase + (uint32_t)(right ? lo : hi);
+                n0 = lds_get<kColors, kMoments>(&s.rec[R * (right ? cl : ch)], id);
+                e0 = frontier_entry(n0, lo, hi, 0u);
+                slot0 = (int)atomicAdd(&s.nf, 1u);
+            }
+        }
+        if (t < 2 && s.edge_range[t].x >= 0) {
+            n1 = lds_get<kColors, kMoments>(s.edge[t], s.edge_id[t]);
+            e1 = frontier_entry(n1, s.edge_range[t].x, s.edge_range[t].y, 0u);
+            slot1 = (int)atomicAdd(&s.nf, 1u);
+        }
+        __syncthreads();
+        const uint32_t nf = s.nf;
+        if (slot0 >= 0) s.d[slot0] = (int)e0.range.x;
+        if (slot1 >= 0) s.d[slot1] = (int)e1.range.x;
+        __syncthreads();
+        if (out_n + nf > A.out_cap) {   // (a window leaves at most kTreeBlockFrontier: not reached)
+            if (t == 0) atomicOr(&ctr->errorFlags, 16u);
+            slot0 = slot1 = -1;
+        }
+        if (slot0 >= 0) {
+            const size_t k = out_n + rank_in(s.d, nf, (int)e0.range.x);
+            Fout[k] = e0;
+            if (kMoments) moment_words(n0, A.out_mom + 3 * ((size_t)blockIdx.x * A.out_cap + k));
+        }
+        if (slot1 >= 0) {
+            const size_t k = out_n + rank_in(s.d, nf, (int)e1.range.x);
+            Fout[k] = e1;
+            if (kMoments) moment_words(n1, A.out_mom + 3 * ((size_t)blockIdx.x * A.out_cap + k));
+        }
+        out_n += nf;
+        __syncthreads();   // (the next window reuses every array)
+    }
+    if (t == 0) A.out_cnt[blockIdx.x] = out_n < A.out_cap ? out_n : A.out_cap;
 }
 
 // ------------------------------------------------------------------ merge
@@ -1890,8 +2137,9 @@ void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos, const uint32_t* from, uint32_t* ids_out, uint32_t fill_cap) {
-    k_fill_u32<<<1, 64, 0, s>>>(T.frontier_count, 0u, 1);
-    const uint32_t B = tree_block_size(T.node_mom != nullptr);
+    const bool moments = T.node_mom != nullptr;
+    const uint32_t B = tree_block_size(moments);
+    const TreeLevels Lv = tree_levels(n, B);
     const dim3 grid(cdiv(n, B));
     if (T.node_mom) {
         if (colors)
@@ -1904,10 +2152,32 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
         k_tree<false, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     }
     if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
-    // the second launch meets leaves by their ids in the new order
-    k_tree_upper<<<cdiv(frontier_cap(n, cdiv(n, B)), 256), 256, 0, s>>>(sim, ctr, src,
-                                                                                  from ? ids_out : index_in, keys,
-                                                                                  T, colors, fill_cap);
+    auto level_entries = [&](int l) { return T.frontier + 4 * Lv.entry_off[l]; };
+    auto level_mom = [&](int l) { return moments ? T.frontier_mom + 3 * Lv.entry_off[l] : nullptr; };
+    for (int l = 1; l < Lv.count; l++) {
+        const TreeMidArgs A = {level_entries(l - 1), level_mom(l - 1), T.frontier_cnt + Lv.count_off[l - 1],
+                               level_entries(l), level_mom(l), T.frontier_cnt + Lv.count_off[l],
+                               Lv.cap[l - 1], Lv.cap[l], Lv.span[l - 1]};
+        const dim3 g(Lv.blocks[l]);
+        if (moments) {
+            if (colors) k_tree_mid<true, true><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
+            else k_tree_mid<false, true><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
+        } else if (colors) {
+            k_tree_mid<true, false><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
+        } else {
+            k_tree_mid<false, false><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
+        }
+    }
+    // from what the last level left (nothing unless a window overflowed), to
+    // the root; it meets leaves by their ids in the new order
+    const int last = Lv.count - 1;
+    TreeArrays Tu = T;
+    Tu.frontier = level_entries(last);
+    Tu.frontier_mom = level_mom(last);
+    Tu.frontier_count = T.frontier_cnt + Lv.count_off[last];
+    const uint32_t cap = Lv.cap[last] < n ? Lv.cap[last] : n;
+    k_tree_upper<<<cdiv(cap > 0 ? cap : 1, 256), 256, 0, s>>>(sim, ctr, src, from ? ids_out : index_in, keys, Tu,
+                                                               colors, fill_cap);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {

@@ -53,6 +53,11 @@ struct TreeArrays {
     float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
     float4* frontier;        // [4 frontier_cap] roots of the subtrees the first build launch left
     uint32_t* frontier_count;
+    // With frontier_cnt set, the first launch writes block b's entries in
+    // sorted order to slots [b kTreeBlockFrontier, ...) of frontier and their
+    // number to frontier_cnt[b] (the levels of tree_levels() follow); unset,
+    // it appends them in any order at frontier_count (the ensemble build).
+    uint32_t* frontier_cnt = nullptr;
     // Second moments, built with the tree for the corrected walk (all null
     // otherwise): node_mom holds each internal node's second moment about its
     // (FP32) centre of mass, the records below carry it and the node's first
@@ -83,6 +88,62 @@ constexpr uint32_t kTreeBlockFrontier = 2 * 98 + 2;
 inline uint32_t frontier_cap(uint32_t n, uint32_t blocks) {
     const uint64_t b = (uint64_t)blocks * kTreeBlockFrontier;
     return (uint32_t)(b < n ? b : n) + 1;
+}
+
+// The levels of the one-simulation build. Level 0 is the first launch (one
+// block per tree_block_size positions). Each later level is one launch of
+// k_tree_mid: a block takes the entries that kTreeMidGroup consecutive
+// blocks of the level below left, in sorted order, as the leaves of a
+// smaller tree (a node covers whole entries, and the split between two
+// neighbours is the split between their position ranges), builds in LDS
+// every node whose entries are all its own, and leaves the rest, sorted, to
+// the next level. It takes kTreeMidUnits entries at a time, so a block that
+// gets more runs several windows, each leaving at most kTreeBlockFrontier
+// entries. The last level has one block; a final k_tree_upper climbs from
+// what it leaves (nothing unless a window overflowed) to the root.
+// Plummer spheres leave about 10 entries per first-launch block, so one
+// window of 1,024 takes a whole group of 64 blocks.
+constexpr uint32_t kTreeMidGroup = 64;
+constexpr uint32_t kTreeMidUnits = 1024;
+constexpr int kTreeMaxLevels = 8;
+struct TreeLevels {
+    int count = 0;
+    uint32_t blocks[kTreeMaxLevels] = {}, cap[kTreeMaxLevels] = {};   // blocks, entry slots per block
+    uint64_t span[kTreeMaxLevels] = {};                                // sorted positions per block
+    size_t entry_off[kTreeMaxLevels] = {}, count_off[kTreeMaxLevels] = {};
+    size_t entries = 0, counts = 0;                                    // totals over the levels
+};
+inline TreeLevels tree_levels(uint32_t n, uint32_t B) {
+    TreeLevels L;
+    uint64_t blocks = n > B ? ((uint64_t)n + B - 1) / B : 1;
+    uint64_t cap = kTreeBlockFrontier < B ? kTreeBlockFrontier : B;
+    uint64_t span = B;
+    for (;;) {
+        const int l = L.count++;
+        L.blocks[l] = (uint32_t)blocks;
+        L.cap[l] = (uint32_t)cap;
+        L.span[l] = span;
+        L.entry_off[l] = L.entries;
+        L.count_off[l] = L.counts;
+        L.entries += blocks * cap;
+        L.counts += blocks;
+        if (blocks == 1) break;
+        const uint64_t units = (blocks < kTreeMidGroup ? blocks : kTreeMidGroup) * cap;
+        const uint64_t windows = (units + kTreeMidUnits - 1) / kTreeMidUnits;
+        cap = units < windows * kTreeBlockFrontier ? units : windows * kTreeBlockFrontier;
+        blocks = (blocks + kTreeMidGroup - 1) / kTreeMidGroup;
+        span *= kTreeMidGroup;
+    }
+    return L;
+}
+// entry slots and counts for either plan
+inline size_t frontier_slots(uint32_t n) {
+    const size_t a = tree_levels(n, kTreeBlock).entries, b = tree_levels(n, kTreeBlockMoments).entries;
+    return a > b ? a : b;
+}
+inline size_t frontier_counts(uint32_t n) {
+    const size_t a = tree_levels(n, kTreeBlock).counts, b = tree_levels(n, kTreeBlockMoments).counts;
+    return a > b ? a : b;
 }
 
 struct WalkArgs {

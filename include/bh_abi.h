@@ -160,7 +160,8 @@ typedef struct {
     uint32_t error_flags;         /* bit0 stack overflow, bit1 task overflow, bit2 the corrected walk's box
                                      beyond its range (16 h_root^2 + softening > 1e15: no forces), bit3 a
                                      sort tile waited past its bound for an earlier tile's counts (the
-                                     order is not valid) */
+                                     order is not valid), bit4 a tree build level left more subtrees
+                                     than its slots hold (the tree is not valid) */
     uint32_t reserved;
     uint64_t corrected;           /* network evaluations (one per internal node) of the last tree build */
     uint64_t quad_terms;          /* quadrupole terms added by the last corrected walk (accepted internal nodes) */
