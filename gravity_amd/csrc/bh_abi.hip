@@ -357,7 +357,7 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
-        c->tiles_cap = (n + kSortTileOne - 1) / kSortTileOne;
+        c->tiles_cap = (n + sort_tile(n) - 1) / sort_tile(n);
         TRY(dalloc(c, &c->sort_partials, (size_t)sort_digit_blocks(n) * kSortPasses * 256));
         TRY(dalloc(c, &c->digit_totals, kSortPasses * 256));
         TRY(dalloc(c, &c->sort_status, (size_t)kSortPasses * c->tiles_cap * 256));

@@ -720,7 +720,8 @@ struct TileLookback {
     }
 };
 
-__global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimValues* __restrict__ sim,
+template <int kTile, int kWaves>
+__global__ __launch_bounds__(kWaves * 64) void k_sort_onesweep(const SimValues* __restrict__ sim,
                                                                       DevCounters* __restrict__ ctr,
                                                                       const uint64_t* __restrict__ keys_in,
                                                                       const uint32_t* __restrict__ idx_in,
@@ -729,14 +730,14 @@ __global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimV
                                                                       const uint32_t* __restrict__ totals,
                                                                       unsigned long long* __restrict__ status,
                                                                       SortState* __restrict__ st) {
-    __shared__ ScatterLds<kSortTileOne, kSortWavesOne> L;
+    __shared__ ScatterLds<kTile, kWaves> L;
     __shared__ uint32_t s_tile;
     const uint32_t t = threadIdx.x;
     if (t == 0) s_tile = atomicAdd(&st->tile[pass], 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t num = sim->numBodies;
-    const uint32_t base = tile * kSortTileOne;
+    const uint32_t base = tile * (uint32_t)kTile;
     if (base >= num) return;                          // ids are dense: the tiles below num are all taken
     {   // where each digit's run starts: the exclusive scan of the pass's digit totals
         __shared__ uint32_t s_w[4];
@@ -756,8 +757,8 @@ __global__ __launch_bounds__(kSortWavesOne * 64) void k_sort_onesweep(const SimV
         }
     }
     const TileLookback lb = {status, tile, st->epoch, ctr};
-    scatter_tile<8, kSortTileOne, kSortWavesOne>(L, keys_in, idx_in, base, min(kSortTileOne, num - base), keys_out,
-                                                 idx_out, pass * kSortBits, lb);
+    scatter_tile<8, kTile, kWaves>(L, keys_in, idx_in, base, min((uint32_t)kTile, num - base), keys_out, idx_out,
+                                   pass * kSortBits, lb);
 }
 
 
@@ -914,13 +915,13 @@ struct TreeSeg {
 // no mass, the empty box (1e38, -1e38), no colour. Its walk record has the
 // half side of that box, -1e38, which every body accepts at once (no force),
 // and a half2 of -1 (never opened by the wide-margin test).
-__device__ __forceinline__ void unfilled_node(uint32_t k, uint32_t height, const NodeState& A, const NodeState& B,
+__device__ __forceinline__ void unfilled_node(uint32_t k, uint32_t height, uint32_t idA, uint32_t idB,
                                               const TreeArrays& T, int colors) {
     WalkNode nd;
     nd.comMass = make_float4(0.f, 0.f, 0.f, 0.f);
     nd.half = fdiv(fsub(-1e38f, 1e38f), 2.0f);
-    nd.childA = A.id;
-    nd.childB = B.id;
+    nd.childA = idA;
+    nd.childB = idB;
     nd.half2 = -1.0f;
     T.nodes[k] = nd;
     T.node_aabb[2 * k] = make_float4(1e38f, 1e38f, 1e38f, __uint_as_float(height));
@@ -1010,12 +1011,18 @@ struct FrontierEntry {
     uint4 range;        // lo, hi (local positions), segment (ensemble simulation), unused
 };
 
-// The summary of the parent of A (left child) and B over local positions
-// [plo, phi] with Karras index pid, stored in the node arrays (bh_tree.comp:183-259).
-__device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeState& B, int plo, int phi,
-                                                 uint32_t pid, const TreeSeg& S, const TreeArrays& T,
+// The summary of the parent of A and B over local positions [plo, phi]
+// with Karras index pid, stored in the node arrays (bh_tree.comp:183-259);
+// a_left: A is the left child (childA). Every summary value is symmetric in
+// the two children (IEEE addition and multiplication commute, and the
+// moments add the two children's terms to zero in either order), so only
+// the child ids depend on which one is left: one inlined copy serves a wave
+// whose lanes arrive from both sides.
+__device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeState& B, bool a_left, int plo,
+                                                 int phi, uint32_t pid, const TreeSeg& S, const TreeArrays& T,
                                                  uint32_t* __restrict__ height_out, bool height_max, int colors,
                                                  bool moments) {
+    const uint32_t idA = a_left ? A.id : B.id, idB = a_left ? B.id : A.id;
     NodeState p;
     const float M = fadd(A.com.w, B.com.w);
     if (M > 0.0f) {
@@ -1044,12 +1051,12 @@ __device__ __forceinline__ NodeState tree_parent(const NodeState& A, const NodeS
     nd.comMass = make_float4(p.com.x, p.com.y, p.com.z, fmul(S.G, M));
     const float ext = gmax(fsub(p.mx.x, p.mn.x), gmax(fsub(p.mx.y, p.mn.y), fsub(p.mx.z, p.mn.z)));
     nd.half = fdiv(ext, 2.0f);
-    nd.childA = A.id;
-    nd.childB = B.id;
+    nd.childA = idA;
+    nd.childB = idB;
     nd.half2 = fmul(nd.half, nd.half);
     // these arrays are read only by later kernels
     if (S.fill_cap && p.height > S.fill_cap) {
-        unfilled_node(k, p.height, A, B, T, colors);   // (its ancestors are higher still)
+        unfilled_node(k, p.height, idA, idB, T, colors);   // (its ancestors are higher still)
     } else {
         T.nodes[k] = nd;
         T.node_aabb[2 * k] = make_float4(p.mn.x, p.mn.y, p.mn.z, __uint_as_float(p.height));
@@ -1117,8 +1124,7 @@ __device__ __forceinline__ void tree_climb(int lo, int hi, const TreeSeg& S, Nod
             const bool pl = (plo == 0) || (phi != num - 1 && delta_at(keys, num, phi) > delta_at(keys, num, plo - 1));
             pid = (uint32_t)(pl ? phi : plo);
         }
-        me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, colors, moments)
-                  : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, colors, moments);
+        me = tree_parent(me, sib, left, plo, phi, pid, S, T, height_out, height_max, colors, moments);
         lo = plo;
         hi = phi;
     }
@@ -1319,8 +1325,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 const bool pl = (plo == 0) || (phi != num - 1 && D(phi) > D(plo - 1));
                 pid = (uint32_t)(pl ? phi : plo);
             }
-            me = left ? tree_parent(me, sib, plo, phi, pid, S, T, height_out, height_max, kColors, kMoments)
-                      : tree_parent(sib, me, plo, phi, pid, S, T, height_out, height_max, kColors, kMoments);
+            me = tree_parent(me, sib, left, plo, phi, pid, S, T, height_out, height_max, kColors, kMoments);
             lo = plo;
             hi = phi;
         }
@@ -1620,8 +1625,7 @@ __global__ __launch_bounds__(kTreeMidUnits) void k_tree_mid(const SimValues* __r
                     const bool pl = (plo == 0) || (phi != num - 1 && s.d[pch + 1] > s.d[pcl]);
                     pid = (uint32_t)(pl ? phi : plo);
                 }
-                me = left ? tree_parent(me, sib, plo, phi, pid, S, T, &ctr->treeHeight, false, kColors, kMoments)
-                          : tree_parent(sib, me, plo, phi, pid, S, T, &ctr->treeHeight, false, kColors, kMoments);
+                me = tree_parent(me, sib, left, plo, phi, pid, S, T, &ctr->treeHeight, false, kColors, kMoments);
                 cl = pcl;
                 ch = pch;
                 lo = plo;
@@ -2117,21 +2121,30 @@ void launch_encode(hipStream_t s, uint32_t n, const SimValues* sim, const DevCou
                    const uint32_t* index_in, const float4* step_pos, uint64_t* morton_in) {
     k_encode<<<cdiv(n, 256), 256, 0, s>>>(sim, ctr, src, index_in, step_pos, morton_in);
 }
+// 16,384 keys per block, at most 1,024 blocks, and (from 65,536 keys) at
+// least 64 blocks: a few blocks over a small sort leave the chip idle
 uint32_t sort_digit_blocks(uint32_t n) {
-    const uint32_t b = cdiv(n, 16384);
+    const uint32_t lo = cdiv(n, 1024) < 64 ? cdiv(n, 1024) : 64;
+    uint32_t b = cdiv(n, 16384);
+    if (b < lo) b = lo;
     return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
 void launch_sort(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, uint64_t* keys[2],
                  uint32_t* vals[2], int in, bool positions, uint32_t* partials, uint32_t* totals,
                  unsigned long long* status, uint32_t tiles_cap, SortState* st) {
-    const uint32_t nb = sort_digit_blocks(n), nt = cdiv(n, kSortTileOne);
+    const uint32_t nb = sort_digit_blocks(n), tile = sort_tile(n), nt = cdiv(n, tile);
     k_sort_digits<<<nb, 512, 0, s>>>(sim, keys[in], partials, st);
     k_sort_totals<<<kSortPasses * (256 / kTotalsDigits), 1024, 0, s>>>(partials, nb, totals);
     for (uint32_t pass = 0; pass < kSortPasses; pass++) {
         const int i = (in + (int)pass) & 1;
-        k_sort_onesweep<<<nt, kSortWavesOne * 64, 0, s>>>(sim, ctr, keys[i], (positions && pass == 0) ? nullptr : vals[i],
-                                                          keys[1 - i], vals[1 - i], pass, totals,
-                                                          status + (size_t)pass * tiles_cap * 256, st);
+        const uint32_t* vin = (positions && pass == 0) ? nullptr : vals[i];
+        unsigned long long* sp = status + (size_t)pass * tiles_cap * 256;
+        if (tile == kSortTileSmall)
+            k_sort_onesweep<kSortTileSmall, kSortWavesSmall><<<nt, kSortWavesSmall * 64, 0, s>>>(
+                sim, ctr, keys[i], vin, keys[1 - i], vals[1 - i], pass, totals, sp, st);
+        else
+            k_sort_onesweep<kSortTileOne, kSortWavesOne><<<nt, kSortWavesOne * 64, 0, s>>>(
+                sim, ctr, keys[i], vin, keys[1 - i], vals[1 - i], pass, totals, sp, st);
     }
 }
 void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* ctr, const float4* src,

@@ -19,6 +19,13 @@ constexpr uint32_t kSortTile = 2048;   // radix sort tile of the ensembles (keys
 // 2,048 (MI355X, same box)
 constexpr uint32_t kSortTileOne = 8192;
 constexpr int kSortWavesOne = kSortTileOne / 512;
+// Up to kSortSmallMax keys a pass has too few 8,192-key tiles to fill the
+// chip (8 at 65,536 keys), and each takes as long as at 16M: smaller tiles
+// of 8 waves then.
+constexpr uint32_t kSortTileSmall = 2048;
+constexpr int kSortWavesSmall = 8;
+constexpr uint32_t kSortSmallMax = 1u << 18;
+inline uint32_t sort_tile(uint32_t n) { return n <= kSortSmallMax ? kSortTileSmall : kSortTileOne; }
 constexpr uint32_t kSortBits = 8;
 constexpr uint32_t kSortPasses = 8;    // 64 key bits; even, so the result lands back in the In buffers
 constexpr int kWalkWaves = 16;         // waves per walk block (adjacent body groups share a CU's scalar cache)
@@ -33,6 +40,7 @@ constexpr bool kWalkPersistent = true;
 // small blocks spreading a small walk over all CUs
 constexpr uint32_t kWalkPersistentMin = 1u << 20;
 constexpr int kWalkSmallWaves = 4;
+constexpr int kWalkSmallBodies = 64;   // bodies per wave of that walk
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 
 // Tree in HBM. Leaves are sorted positions 0..n-1, internal nodes 0..n-2

@@ -520,8 +520,9 @@ struct WalkTotals {
     unsigned long long quads;
 };
 
-// Walk and integrate the 64 sorted bodies [base, base + 64) of one wave.
-template <uint32_t kMode, int kCorr>
+// Walk and integrate the kBodies sorted bodies [base, base + kBodies) of one
+// wave (lanes kBodies.. idle).
+template <uint32_t kMode, int kCorr, int kBodies = kWave>
 __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsigned long long* s_mask,
                                            SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                            const float4* __restrict__ src, float4* __restrict__ dst,
@@ -537,7 +538,7 @@ __device__ __forceinline__ void walk_group(uint32_t base, uint32_t* s_node, unsi
     const uint32_t num = sim->numBodies;
     const uint32_t lane = lane_id();
     const uint32_t g = base + lane;
-    const bool active = g < num;
+    const bool active = (kBodies == kWave || lane < (uint32_t)kBodies) && g < num;
     const uint32_t bodyIdx = active ? index_in[g] : 0;
     float4 pm = active ? leaf_pos[g] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 vd = active ? src[2 * bodyIdx + 1] : make_float4(0.f, 0.f, 0.f, 1.f);
@@ -633,7 +634,7 @@ __device__ __forceinline__ void walk_totals_flush(SimValues* sim, DevCounters* c
 // Pointers are separate __restrict__ parameters so that the compiler can
 // prove the tree is not written during the kernel and fetch node records with
 // scalar loads.
-template <uint32_t kMode, int kW, int kCorr>
+template <uint32_t kMode, int kW, int kCorr, int kBodies>
 __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
                                               const float4* __restrict__ src, float4* __restrict__ dst,
                                               const uint32_t* __restrict__ index_in,
@@ -648,10 +649,10 @@ __global__ __launch_bounds__(kW * 64) void k_walk(SimValues* __restrict__ sim, D
     __shared__ unsigned long long s_mask[kW][kStackDepth];
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t base = (lb * kW + w) * kWave;
+    const uint32_t base = (lb * kW + w) * kBodies;
     if (base >= sim->numBodies) return;
     WalkTotals t = {0ull, 0ull, 1u, 0u, 0u, 0ull};
-    walk_group<kMode, kCorr>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, qrec, corr, tasks,
+    walk_group<kMode, kCorr, kBodies>(base, s_node[w], s_mask[w], sim, ctr, src, dst, index_in, leaf_pos, nodes, qrec, corr, tasks,
                              task_cap, theta_, dt_, softening, dynamic, wrap, elasticity, restitution, leaf_radius,
                              general, pos_out, t);
     walk_totals_flush(sim, ctr, t);
@@ -757,9 +758,9 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     const CorrArgs corr = {a.acc_mag, a.alpha};
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
-    const uint32_t nb_small = (n + kWalkSmallWaves * kWave - 1) / (kWalkSmallWaves * kWave);
+    const uint32_t nb_small = (n + kWalkSmallWaves * kWalkSmallBodies - 1) / (kWalkSmallWaves * kWalkSmallBodies);
 #define BH_WALK(M, Q)                                                                                            \
-    k_walk<M, kWalkSmallWaves, Q><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+    k_walk<M, kWalkSmallWaves, Q, kWalkSmallBodies><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.qrec, corr, a.tasks, a.task_cap, a.theta, a.dt, \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                 a.leaf_radius, a.general, a.pos_out)

@@ -1,5 +1,6 @@
 """The one-simulation radix sort (bh_radix_sort) on keys uploaded through the
-C ABI, against numpy's stable argsort: sizes around the 8,192-key tile, keys
+C ABI, against numpy's stable argsort: sizes around the 2,048-key tile of
+sorts up to 262,144 keys and the 8,192-key tile above that, keys
 with many ties, nearly ordered keys (the time-stepping case), all-equal keys,
 a sort over fewer bodies than the context holds, and sorts in a row (each
 sort's look-back words carry their own epoch).
@@ -13,6 +14,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TILE = 8192
+SMALL_TILE, SMALL_MAX = 2048, 1 << 18
 
 
 def make_bh(n):
@@ -55,7 +57,8 @@ def check_sort(bh, keys, num):
     assert bh.counters()["error_flags"] == 0
 
 
-@pytest.mark.parametrize("n", [2, 3, 1000, TILE - 1, TILE, TILE + 1, 3 * TILE + 17, (1 << 20) + 5])
+@pytest.mark.parametrize("n", [2, 3, 1000, SMALL_TILE - 1, SMALL_TILE + 1, TILE - 1, TILE, TILE + 1, 3 * TILE + 17,
+                               SMALL_MAX, SMALL_MAX + 1, (1 << 20) + 5])
 @pytest.mark.parametrize("kind", ["random", "ties", "nearly_ordered", "equal"])
 def test_gpu_sort_matches_stable_argsort(n, kind):
     print(f"[gpu] sort n={n} {kind}", flush=True)
