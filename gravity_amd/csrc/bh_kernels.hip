@@ -1410,6 +1410,11 @@ __global__ __launch_bounds__(tree_block_size(kMoments)) void k_tree(const SimVal
     const int num = (int)sim->numBodies;
     constexpr uint32_t kB = tree_block_size(kMoments);
     const int blo = (int)(blockIdx.x * kB);
+    if (from) {   // ids past numBodies (the dead) keep their positions
+        const uint32_t lo = (uint32_t)blo > (uint32_t)num ? (uint32_t)blo : (uint32_t)num;
+        const uint32_t hi = (uint32_t)blo + kB < n ? (uint32_t)blo + kB : n;
+        for (uint32_t j = lo + threadIdx.x; j < hi; j += kB) ids_out[j] = index_in[j];
+    }
     if (blo >= num || num == 1) {   // (no entries: the levels above read block 0's count)
         if (T.frontier_cnt && blockIdx.x == 0 && threadIdx.x == 0) T.frontier_cnt[0] = 0u;
         if (blo >= num) return;
@@ -2066,13 +2071,6 @@ __global__ void k_std430_to_nodes(const SimValues* __restrict__ sim, TreeArrays 
 
 // ids of the positions past numBodies (the dead bodies, which keep their
 // places) into the new index
-__global__ void k_copy_tail(const SimValues* __restrict__ sim, const uint32_t* __restrict__ index_in,
-                            uint32_t* __restrict__ ids_out) {
-    const uint32_t n = sim->initialNumBodies;
-    for (uint32_t j = sim->numBodies + blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        ids_out[j] = index_in[j];
-}
-
 // fills a u32 array (graph-safe replacement for a memset)
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2164,7 +2162,6 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
     } else {
         k_tree<false, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     }
-    if (from) k_copy_tail<<<512, 256, 0, s>>>(sim, index_in, ids_out);
     auto level_entries = [&](int l) { return T.frontier + 4 * Lv.entry_off[l]; };
     auto level_mom = [&](int l) { return moments ? T.frontier_mom + 3 * Lv.entry_off[l] : nullptr; };
     for (int l = 1; l < Lv.count; l++) {

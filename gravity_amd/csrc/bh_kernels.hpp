@@ -41,6 +41,10 @@ constexpr bool kWalkPersistent = true;
 constexpr uint32_t kWalkPersistentMin = 1u << 20;
 constexpr int kWalkSmallWaves = 4;
 constexpr int kWalkSmallBodies = 64;   // bodies per wave of that walk
+// up to this many bodies a 64-body wave per SIMD leaves the walk waiting on
+// its own latency: the plain walk (whose sums do not depend on how bodies are
+// grouped) runs waves of 32 bodies (two per SIMD) instead
+constexpr uint32_t kWalkHalfMax = 1u << 16;
 constexpr int kStackDepth = 192;       // per-wave walk stack; a tree over 96-bit keys is at most 96 high
 
 // Tree in HBM. Leaves are sorted positions 0..n-1, internal nodes 0..n-2

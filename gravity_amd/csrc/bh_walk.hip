@@ -758,12 +758,20 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     const CorrArgs corr = {a.acc_mag, a.alpha};
     const uint32_t per_block = kWalkWaves * kWave;
     const uint32_t nb = (n + per_block - 1) / per_block;
-    const uint32_t nb_small = (n + kWalkSmallWaves * kWalkSmallBodies - 1) / (kWalkSmallWaves * kWalkSmallBodies);
-#define BH_WALK(M, Q)                                                                                            \
-    k_walk<M, kWalkSmallWaves, Q, kWalkSmallBodies><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
+    // (the corrected walk decides for whole 64-body groups: its results depend on the group)
+    const bool half = n <= kWalkHalfMax && !a.qrec;
+    const uint32_t per_wave = half ? kWalkSmallBodies / 2 : kWalkSmallBodies;
+    const uint32_t nb_small = (n + kWalkSmallWaves * per_wave - 1) / (kWalkSmallWaves * per_wave);
+#define BH_WALK_B(M, Q, B)                                                                                       \
+    k_walk<M, kWalkSmallWaves, Q, B><<<nb_small, kWalkSmallWaves * kWave, 0, s>>>(const_cast<SimValues*>(a.sim), a.ctr, a.src, a.dst, a.index_in, \
                                                 a.leaf_pos, a.nodes, a.qrec, corr, a.tasks, a.task_cap, a.theta, a.dt, \
                                                 a.softening, a.dynamic, a.wrap, a.elasticity, a.restitution,     \
                                                 a.leaf_radius, a.general, a.pos_out)
+#define BH_WALK(M, Q)                                                                                            \
+    do {                                                                                                         \
+        if (half) BH_WALK_B(M, Q, kWalkSmallBodies / 2);                                                          \
+        else BH_WALK_B(M, Q, kWalkSmallBodies);                                                                   \
+    } while (0)
     if (a.merging != 0u)
         k_leaf_radius<<<(n + 255) / 256, 256, 0, s>>>(a.sim, a.src, a.index_in, a.leaf_pos, a.leaf_radius);
 #define BH_WALK_Q(M, Q)                                                                                          \
@@ -788,6 +796,7 @@ void launch_walk(hipStream_t s, uint32_t n, const WalkArgs& a) {
     else if (a.merging == 2u) BH_WALK(2u, 0);
     else BH_WALK(0u, 0);
 #undef BH_WALK
+#undef BH_WALK_B
 #undef BH_WALK_Q
 }
 
