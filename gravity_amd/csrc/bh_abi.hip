@@ -353,7 +353,6 @@ int bh_create(int device, uint32_t n, const float* bounds6, const bh_params* uni
         // the two plans (the build with second moments has smaller blocks)
         TRY(dalloc(c, &T.frontier, 4 * frontier_slots(n)));
         TRY(dalloc(c, &T.frontier_cnt, frontier_counts(n)));
-        TRY(dalloc(c, &T.frontier_count, 1));
         TRY(dalloc(c, &c->part_scratch, (size_t)(n + kPartTile - 1) / kPartTile));
         c->bounds_partial_cap = 4096;
         TRY(dalloc(c, &c->bounds_partials, 6 * (size_t)c->bounds_partial_cap));
@@ -419,7 +418,7 @@ void bh_destroy(bh_context* c) {
     void* ptrs[] = {c->bodies[0], c->bodies[1], c->morton[0], c->morton[1], c->index[0], c->index[1], c->sim,
                     c->tasks, c->ctr, c->tree.leaf_pos, c->tree.leaf_parent, c->tree.leaf_color, c->tree.walk,
                     c->tree.node_aabb, c->tree.node_parent, c->tree.node_range, c->tree.node_color, c->tree.meet,
-                    c->tree.xchg, c->tree.frontier, c->tree.frontier_cnt, c->tree.frontier_count, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_totals, c->sort_status, c->sort_state,
+                    c->tree.xchg, c->tree.frontier, c->tree.frontier_cnt, c->part_scratch, c->bounds_partials, c->sort_partials, c->digit_totals, c->sort_status, c->sort_state,
                     c->reserve, c->task_done, c->staging_nodes, c->leaf_radius, c->step_pos, c->sort_pos[0],
                     c->sort_pos[1], c->walk_queues, c->qrec, c->node_mom, c->xchg_mom, c->frontier_mom, c->acc_mag,
                     c->feed_counts};

@@ -9,7 +9,7 @@
 //   k_encode               [COMPUTE_MORTON_ENCODE, morton/bh_morton.comp:6-34,238-262]
 //   k_sort_digits/bases/onesweep
 //                          [COMPUTE_RADIX_*, radix/bh_radix.comp:12-166]
-//   k_tree/k_tree_upper    [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
+//   k_tree/k_tree_mid      [COMPUTE_TREE_BUILD + INIT_LEAVES + PROPAGATE, tree/bh_tree.comp:6-259]
 //   (walk: bh_walk.hip)    [COMPUTE_FORCE_COMPUTE]
 //   k_merge                [COMPUTE_MERGE_BODIES, merge/bh_merge.comp:5-71]
 //   k_node_to_std430 / k_std430_to_node: the 80-byte Node layout at the boundary.
@@ -1479,24 +1479,6 @@ __device__ __forceinline__ void frontier_moments(const TreeArrays& T, uint32_t t
     moments_from_words(w, me);
 }
 
-// The second launch: one thread per frontier node, climbing to the root.
-__global__ __launch_bounds__(256) void k_tree_upper(const SimValues* __restrict__ sim, DevCounters* __restrict__ ctr,
-                                                    const float4* __restrict__ src,
-                                                    const uint32_t* __restrict__ index_in,
-                                                    const uint64_t* __restrict__ keys, TreeArrays T, int colors,
-                                                    uint32_t fill_cap) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *T.frontier_count) return;
-    const uint32_t n = sim->initialNumBodies;
-    const FrontierEntry e = reinterpret_cast<const FrontierEntry*>(T.frontier)[t];
-    NodeState me = frontier_state(e);
-    if (colors) me.col = me.id < n ? T.leaf_color[me.id] : T.node_color[me.id - n];
-    const bool moments = T.node_mom != nullptr;
-    if (moments) frontier_moments(T, t, me);
-    const TreeSeg S = {0u, sim->numBodies, 0u, n, sim->gravitationalConstant, fill_cap};
-    const LeafSource L = {src, index_in, sim->unitMass, sim->unitDensity};
-    tree_climb((int)e.range.x, (int)e.range.y, S, me, keys, L, T, &ctr->treeHeight, false, colors, moments);
-}
 
 // ---- the levels between: the entries of kTreeMidGroup blocks as leaves.
 // Entry u of a window covers positions [lo_u, hi_u], hi_u + 1 = lo_{u+1}, so
@@ -1517,6 +1499,12 @@ struct TreeMidArgs {
     uint32_t* out_cnt;
     uint32_t in_cap, out_cap;
     uint64_t in_span;         // sorted positions per block of the level below
+    // The last level (one block): what a window leaves climbs to the root at
+    // once, through the arrival slots in HBM (tree_climb), meeting leaves by
+    // their ids in index_in; nothing is written out.
+    int last;
+    const float4* src;
+    const uint32_t* index_in;
 };
 
 template <bool kColors, bool kMoments>
@@ -1662,6 +1650,17 @@ __global__ __launch_bounds__(kTreeMidUnits) void k_tree_mid(const SimValues* __r
         }
         __syncthreads();
         const uint32_t nf = s.nf;
+        if (A.last) {
+            const LeafSource L = {A.src, A.index_in, sim->unitMass, sim->unitDensity};
+            if (slot0 >= 0)
+                tree_climb((int)e0.range.x, (int)e0.range.y, S, n0, keys, L, T, &ctr->treeHeight, false, kColors,
+                           kMoments);
+            if (slot1 >= 0)
+                tree_climb((int)e1.range.x, (int)e1.range.y, S, n1, keys, L, T, &ctr->treeHeight, false, kColors,
+                           kMoments);
+            __syncthreads();   // (the next window reuses every array)
+            continue;
+        }
         if (slot0 >= 0) s.d[slot0] = (int)e0.range.x;
         if (slot1 >= 0) s.d[slot1] = (int)e1.range.x;
         __syncthreads();
@@ -2162,12 +2161,15 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
     } else {
         k_tree<false, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
     }
+    // (a single first-launch block builds the whole tree: no levels above)
     auto level_entries = [&](int l) { return T.frontier + 4 * Lv.entry_off[l]; };
     auto level_mom = [&](int l) { return moments ? T.frontier_mom + 3 * Lv.entry_off[l] : nullptr; };
     for (int l = 1; l < Lv.count; l++) {
+        // the last level meets leaves by their ids in the new order
         const TreeMidArgs A = {level_entries(l - 1), level_mom(l - 1), T.frontier_cnt + Lv.count_off[l - 1],
                                level_entries(l), level_mom(l), T.frontier_cnt + Lv.count_off[l],
-                               Lv.cap[l - 1], Lv.cap[l], Lv.span[l - 1]};
+                               Lv.cap[l - 1], Lv.cap[l], Lv.span[l - 1], l == Lv.count - 1 ? 1 : 0, src,
+                               from ? ids_out : index_in};
         const dim3 g(Lv.blocks[l]);
         if (moments) {
             if (colors) k_tree_mid<true, true><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
@@ -2178,16 +2180,6 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
             k_tree_mid<false, false><<<g, kTreeMidUnits, 0, s>>>(sim, ctr, keys, T, A, fill_cap);
         }
     }
-    // from what the last level left (nothing unless a window overflowed), to
-    // the root; it meets leaves by their ids in the new order
-    const int last = Lv.count - 1;
-    TreeArrays Tu = T;
-    Tu.frontier = level_entries(last);
-    Tu.frontier_mom = level_mom(last);
-    Tu.frontier_count = T.frontier_cnt + Lv.count_off[last];
-    const uint32_t cap = Lv.cap[last] < n ? Lv.cap[last] : n;
-    k_tree_upper<<<cdiv(cap > 0 ? cap : 1, 256), 256, 0, s>>>(sim, ctr, src, from ? ids_out : index_in, keys, Tu,
-                                                               colors, fill_cap);
 }
 void launch_merge(hipStream_t s, float4* dst, const uint2* tasks, uint32_t cap, DevCounters* ctr,
                   unsigned long long* reserve, uint32_t* done) {

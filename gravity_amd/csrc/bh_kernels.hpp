@@ -111,8 +111,9 @@ inline uint32_t frontier_cap(uint32_t n, uint32_t blocks) {
 // every node whose entries are all its own, and leaves the rest, sorted, to
 // the next level. It takes kTreeMidUnits entries at a time, so a block that
 // gets more runs several windows, each leaving at most kTreeBlockFrontier
-// entries. The last level has one block; a final k_tree_upper climbs from
-// what it leaves (nothing unless a window overflowed) to the root.
+// entries. The last level has one block, which climbs from what its windows
+// leave (nothing unless there are several) to the root through the arrival
+// slots in HBM. A single first-launch block builds the whole tree.
 // Plummer spheres leave about 10 entries per first-launch block, so one
 // window of 1,024 takes a whole group of 64 blocks.
 constexpr uint32_t kTreeMidGroup = 64;
