@@ -1152,10 +1152,10 @@ constexpr uint32_t kSideRight = 0x80000000u;
 // (3 or 4 words); with moments the id is left out (the reader knows the
 // node's Karras index) and the 9 moment floats fill the words to 5 (6 with
 // colours), so that three blocks of 512 positions fit a CU's LDS.
-template <bool kColors, bool kMoments>
+template <bool kColors, bool kMoments, uint32_t kBlock = tree_block_size(kMoments)>
 struct TreeBlockLds {
     static constexpr int kRec = kMoments ? (kColors ? 6 : 5) : (kColors ? 4 : 3);   // float4 words of one record
-    static constexpr uint32_t kB = tree_block_size(kMoments);   // positions per block
+    static constexpr uint32_t kB = kBlock;         // positions per block
     int d[kB + 1];                                 // delta(blo - 1 + j)
     uint32_t meet[kB];                             // arrival slot of split blo + j
     float4 pm[kB];                                 // posMass of leaf blo + j
@@ -1164,9 +1164,14 @@ struct TreeBlockLds {
     float4 edge[2][kRec];                          // edge nodes: left child at split bhi, right child at blo - 1
     int2 edge_range[2];
     uint32_t edge_id[2];
-    uint32_t leaf_used[kB / 32];                   // bit j: leaf blo + j is a child of a node built here
     uint32_t nf, base;
 };
+
+// The arrival slot of a split whose parent has a leaf child is not used for
+// an exchange: the thread that builds such a parent marks it (sorted output),
+// and a leaf that left its parent to its internal sibling finds out there
+// whether the parent was built in the block.
+constexpr uint32_t kLeafParentBuilt = 0xFFFFFFFEu;
 
 template <bool kColors, bool kMoments>
 __device__ __forceinline__ void lds_put(float4* x, const NodeState& s) {
@@ -1228,8 +1233,8 @@ __device__ __forceinline__ uint32_t rank_in(const int* list, uint32_t nf, int ke
 // entries go, sorted by their first position, to slots
 // [out_block kTreeBlockFrontier, ...), and a leaf whose parent is not built
 // here is an entry too (a mid level needs every position covered).
-template <bool kColors, bool kMoments>
-__device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, const TreeSeg& S, int blo, int cnt,
+template <bool kColors, bool kMoments, uint32_t kBlock>
+__device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments, kBlock>& s, const TreeSeg& S, int blo, int cnt,
                                            const uint64_t* __restrict__ keys, const LeafSource& L,
                                            const TreeArrays& T, uint32_t* __restrict__ height_out, bool height_max,
                                            uint32_t segment, uint32_t out_block = 0,
@@ -1266,11 +1271,9 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
     }
     s.meet[t] = kEmpty;
     const bool sorted_out = T.frontier_cnt != nullptr;
-    if (t < (int)(TreeBlockLds<kColors, kMoments>::kB / 32)) s.leaf_used[t] = 0u;
-    bool leaf_edge = false;   // this thread's leaf is an edge node
+    int leaf_split = -1;   // this thread's leaf left its parent (at this split) to its internal sibling
     __syncthreads();
     auto D = [&](int x) { return s.d[x - blo + 1]; };   // x in [blo - 1, bhi]
-    auto use_leaf = [&](int x) { atomicOr(&s.leaf_used[(x - blo) >> 5], 1u << ((x - blo) & 31)); };
     if (t < cnt) {
         int lo = blo + t, hi = lo;
         NodeState me = leaf_state(s.pm[t], S.off + (uint32_t)lo);
@@ -1284,7 +1287,6 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 lds_put<kColors, kMoments>(s.edge[side], me);
                 s.edge_range[side] = make_int2(lo, hi);
                 s.edge_id[side] = me.id;
-                leaf_edge = lo == hi;
                 break;
             }
             const bool sib_leaf = left ? (split + 1 == num - 1 || D(split + 1) <= D(split))
@@ -1296,12 +1298,12 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
                 other = left ? split + 1 : split;
                 sib = leaf_state(s.pm[other - blo], S.off + (uint32_t)other);
                 if (kColors) sib.col = s.col[other - blo];
-                if (sorted_out) {
-                    use_leaf(other);
-                    if (lo == hi) use_leaf(lo);
-                }
+                if (sorted_out) s.meet[split - blo] = kLeafParentBuilt;
             } else {
-                if (lo == hi) break;                       // the internal sibling's thread builds the parent
+                if (lo == hi) {                            // the internal sibling's thread builds the parent
+                    leaf_split = split;
+                    break;
+                }
                 // the exchange orders LDS only: the fences are for the local
                 // address space, so they do not wait for this wave's node stores
                 lds_put<kColors, kMoments>(&s.rec[R * ((left ? hi : lo) - blo)], me);
@@ -1338,7 +1340,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
     int slot0 = -1, slot1 = -1, slot2 = -1;
     if (t < cnt - 1) {
         const uint32_t v = s.meet[t];
-        if (v != kEmpty) {
+        if (v != kEmpty && v != kLeafParentBuilt) {
             const int split = blo + t, far = (int)(v & ~kSideRight);
             const bool right = (v & kSideRight) != 0;
             const int lo = right ? split + 1 : far, hi = right ? far : split;
@@ -1353,7 +1355,7 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
         e1 = frontier_entry(n1, s.edge_range[t].x, s.edge_range[t].y, segment);
         slot1 = (int)atomicAdd(&s.nf, 1u);
     }
-    if (sorted_out && t < cnt && !leaf_edge && !((s.leaf_used[t >> 5] >> (t & 31)) & 1u)) {
+    if (sorted_out && leaf_split >= 0 && s.meet[leaf_split - blo] != kLeafParentBuilt) {
         NodeState lf = leaf_state(s.pm[t], S.off + (uint32_t)(blo + t));
         if (kColors) lf.col = s.col[t];
         e2 = frontier_entry(lf, blo + t, blo + t, segment);
@@ -1397,18 +1399,17 @@ __device__ __forceinline__ void tree_block(TreeBlockLds<kColors, kMoments>& s, c
     }
 }
 
-template <bool kColors, bool kMoments>
-__global__ __launch_bounds__(tree_block_size(kMoments)) void k_tree(const SimValues* __restrict__ sim,
+template <bool kColors, bool kMoments, uint32_t kB>
+__global__ __launch_bounds__(kB) void k_tree(const SimValues* __restrict__ sim,
                                                      DevCounters* __restrict__ ctr, const float4* __restrict__ src,
                                                      const uint32_t* __restrict__ index_in,
                                                      const uint64_t* __restrict__ keys, TreeArrays T,
                                                      const float4* __restrict__ step_pos,
                                                      const uint32_t* __restrict__ from,
                                                      uint32_t* __restrict__ ids_out, uint32_t fill_cap) {
-    __shared__ TreeBlockLds<kColors, kMoments> s;
+    __shared__ TreeBlockLds<kColors, kMoments, kB> s;
     const uint32_t n = sim->initialNumBodies;
     const int num = (int)sim->numBodies;
-    constexpr uint32_t kB = tree_block_size(kMoments);
     const int blo = (int)(blockIdx.x * kB);
     if (from) {   // ids past numBodies (the dead) keep their positions
         const uint32_t lo = (uint32_t)blo > (uint32_t)num ? (uint32_t)blo : (uint32_t)num;
@@ -2148,19 +2149,21 @@ void launch_tree(hipStream_t s, uint32_t n, const SimValues* sim, DevCounters* c
                  const uint32_t* index_in, const uint64_t* keys, const TreeArrays& T, int colors,
                  const float4* step_pos, const uint32_t* from, uint32_t* ids_out, uint32_t fill_cap) {
     const bool moments = T.node_mom != nullptr;
-    const uint32_t B = tree_block_size(moments);
+    const uint32_t B = sim_tree_block(n, moments);
     const TreeLevels Lv = tree_levels(n, B);
     const dim3 grid(cdiv(n, B));
-    if (T.node_mom) {
-        if (colors)
-            k_tree<true, true><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
-        else
-            k_tree<false, true><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
-    } else if (colors) {
-        k_tree<true, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
+#define BH_TREE(C, M, K) k_tree<C, M, K><<<grid, K, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap)
+    if (moments) {
+        if (colors) BH_TREE(true, true, kTreeBlockMoments);
+        else BH_TREE(false, true, kTreeBlockMoments);
+    } else if (B == kTreeBlock) {
+        if (colors) BH_TREE(true, false, kTreeBlock);
+        else BH_TREE(false, false, kTreeBlock);
     } else {
-        k_tree<false, false><<<grid, B, 0, s>>>(sim, ctr, src, index_in, keys, T, step_pos, from, ids_out, fill_cap);
+        if (colors) BH_TREE(true, false, kTreeBlockMoments);
+        else BH_TREE(false, false, kTreeBlockMoments);
     }
+#undef BH_TREE
     // (a single first-launch block builds the whole tree: no levels above)
     auto level_entries = [&](int l) { return T.frontier + 4 * Lv.entry_off[l]; };
     auto level_mom = [&](int l) { return moments ? T.frontier_mom + 3 * Lv.entry_off[l] : nullptr; };

@@ -90,11 +90,21 @@ constexpr uint32_t kTreeBlock = 1024;
 // The build with second moments carries three more words per record in LDS,
 // so a block of 1,024 positions leaves room for one block per CU: blocks of
 // 512 positions measured faster (16M Plummer: first launch 1.85 against 2.25
-// ms, the second 0.43 against 0.30 ms); the plain build keeps 1,024 (512:
-// the same first launch, a slower second).
+// ms, the second 0.43 against 0.30 ms). The ensembles' plain build keeps
+// 1,024 (its second launch climbs through HBM, slower from twice as many
+// entries).
 constexpr uint32_t kTreeBlockMoments = 512;
 __host__ __device__ constexpr uint32_t tree_block_size(bool moments) {
     return moments ? kTreeBlockMoments : kTreeBlock;
+}
+// The one-simulation build, whose upper levels are cheap (k_tree_mid), takes
+// blocks of 512 positions for the plain tree too above kTreeWideMax bodies
+// (four blocks per CU: MI355X, Plummer, plain tree stage 1.31 against 1.69 ms
+// at 16M and 0.149 against 0.178 ms at 1M), and 1,024 up to it (0.074 against
+// 0.085 ms at 65,536).
+constexpr uint32_t kTreeWideMax = 1u << 17;
+inline uint32_t sim_tree_block(uint32_t n, bool moments) {
+    return moments || n > kTreeWideMax ? kTreeBlockMoments : kTreeBlock;
 }
 constexpr uint32_t kTreeBlockFrontier = 2 * 98 + 2;
 inline uint32_t frontier_cap(uint32_t n, uint32_t blocks) {
