@@ -63,8 +63,10 @@ struct TreeArrays {
     float4* node_color;      // [n-1]
     uint32_t* meet;          // [n-1] bottom-up arrival slots, ~0 between builds
     float4* xchg;            // [8(n-1)] summaries of the two children meeting at each split (4 words each)
-    float4* frontier;        // [4 frontier_cap] roots of the subtrees the first build launch left
-    uint32_t* frontier_count;
+    float4* frontier;        // roots of the subtrees the first build launch left, 4 words each: the
+                             // ensembles' list (frontier_cap entries), one simulation's level slots
+                             // (frontier_slots entries)
+    uint32_t* frontier_count = nullptr;
     // With frontier_cnt set, the first launch writes block b's entries in
     // sorted order to slots [b kTreeBlockFrontier, ...) of frontier and their
     // number to frontier_cnt[b] (the levels of tree_levels() follow); unset,
