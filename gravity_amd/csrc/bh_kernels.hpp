@@ -78,7 +78,7 @@ struct TreeArrays {
     // moment about that centre through the exchanges.
     float4* node_mom = nullptr;       // [2(n-1)] (xx, xy, xz, yy), (yz, zz, 0, 0)
     float4* xchg_mom = nullptr;       // [6(n-1)] the moment words of xchg's two sides (3 words each)
-    float4* frontier_mom = nullptr;   // [3 frontier_cap] the moment words of the frontier entries
+    float4* frontier_mom = nullptr;   // the moment words of the frontier entries (3 per entry)
     float4* corr_leaf = nullptr;      // the corrected walk's records: the first launch writes the first
                                       // half of every leaf's (leaf j at 4 j, 4 j + 1)
 };
