@@ -39,7 +39,7 @@ constexpr bool kWalkPersistent = true;
 // (MI355X, Plummer: that is faster at 524,288 bodies, slower at 1,048,576),
 // small blocks spreading a small walk over all CUs
 constexpr uint32_t kWalkPersistentMin = 1u << 20;
-constexpr int kWalkSmallWaves = 4;
+constexpr int kWalkSmallWaves = 4;     // (2 and 8 measured no faster at 65,536-524,288 bodies)
 constexpr int kWalkSmallBodies = 64;   // bodies per wave of that walk
 // up to this many bodies a 64-body wave per SIMD leaves the walk waiting on
 // its own latency: the plain walk (whose sums do not depend on how bodies are
